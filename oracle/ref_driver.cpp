@@ -1,0 +1,223 @@
+// Checker-only driver around the reference's own path translation units.
+//
+// TEST INFRASTRUCTURE. This file is compiled together with the unchanged
+// reference sources (src/config.cpp, src/input.cpp, src/mesh.cpp,
+// src/math_utils.cpp, src/fluid_ideal.cpp, src/advection_{base,jst,roe}.cpp,
+// src/boundary_{base,inlet,outlet,outlet_radial_equilibrium,inviscid_wall,
+// fake}.cpp, src/source_bfm_{base,hall,lift_drag}.cpp, src/input_table.cpp)
+// into oracle/_ref/libtbfm_ref.so by oracle/Makefile.  Only tests/ and
+// tests/golden/gen_golden.py load it, to pin the CPU restatement in
+// oracle/tbfm_oracle.c and the product's host geometry.  src/solver.cpp is not
+// part of the build: it needs <Eigen/Sparse>, which this image lacks.
+//
+// Every entry point below calls a PUBLIC member of a reference class:
+//   Mesh getters                        include/mesh.hpp:20-150
+//   AdvectionBase::computeFlux          include/advection_base.hpp:21-25
+//   BoundaryBase::computeBoundaryFlux   include/boundary_base.hpp:26-32
+//   SourceBFMBase::computeTotalSource   include/source_bfm_base.hpp:20-31
+//   FluidIdeal                          include/fluid_ideal.hpp
+#include "types.hpp"
+#include "config.hpp"
+#include "mesh.hpp"
+#include "math_utils.hpp"
+#include "fluid_ideal.hpp"
+#include "advection_jst.hpp"
+#include "advection_roe.hpp"
+#include "boundary_inlet.hpp"
+#include "boundary_outlet.hpp"
+#include "boundary_outlet_radial_equilibrium.hpp"
+#include "boundary_inviscid_wall.hpp"
+#include "source_bfm_hall.hpp"
+#include "source_bfm_lift_drag.hpp"
+
+#include <memory>
+#include <cstdio>
+#include <vector>
+
+namespace {
+struct RefCase {
+    std::unique_ptr<Config> config;
+    std::unique_ptr<Mesh> mesh;
+    std::unique_ptr<FluidIdeal> fluid;
+    std::unique_ptr<AdvectionBase> advection;
+    std::unique_ptr<SourceBFMBase> bfm;
+    std::vector<FloatType> radialProfile;
+};
+RefCase g_case;
+
+StateVector sv(const double* p) { return StateVector({p[0], p[1], p[2], p[3], p[4]}); }
+void put(const StateVector& s, double* out) { for (int v = 0; v < 5; ++v) out[v] = s[v]; }
+
+void dumpVec(const Matrix3D<Vector3D>& m, double* out) {
+    const size_t n = m.sizeI() * m.sizeJ() * m.sizeK();
+    for (size_t idx = 0; idx < n; ++idx) {
+        const Vector3D& v = m[idx];
+        out[3 * idx + 0] = v.x();
+        out[3 * idx + 1] = v.y();
+        out[3 * idx + 2] = v.z();
+    }
+}
+}  // namespace
+
+extern "C" {
+
+// Loads input.ini (absolute paths inside) and builds Config, Mesh, FluidIdeal,
+// the configured advection scheme and BFM model.  Returns 0 on success.
+int ref_case_load(const char* ini_path) {
+    try {
+        std::fflush(stdout);
+        g_case = RefCase();
+        g_case.config = std::make_unique<Config>(std::string(ini_path));
+        g_case.mesh = std::make_unique<Mesh>(*g_case.config);
+        g_case.fluid = std::make_unique<FluidIdeal>(g_case.config->getFluidGamma(),
+                                                    g_case.config->getFluidGasConstant());
+        if (g_case.config->getAdvectionScheme() == AdvectionScheme::ROE)
+            g_case.advection = std::make_unique<AdvectionRoe>(*g_case.config, *g_case.fluid);
+        else
+            g_case.advection = std::make_unique<AdvectionJst>(*g_case.config, *g_case.fluid);
+        BodyForceModel m = g_case.config->getBFMModel();
+        if (m == BodyForceModel::HALL)
+            g_case.bfm = std::make_unique<SourceBFMHall>(*g_case.config, *g_case.fluid, *g_case.mesh);
+        else if (m == BodyForceModel::LIFT_DRAG)
+            g_case.bfm = std::make_unique<SourceBFMLiftDrag>(*g_case.config, *g_case.fluid, *g_case.mesh);
+        else
+            g_case.bfm = std::make_unique<SourceBFMBase>(*g_case.config, *g_case.fluid, *g_case.mesh);
+        std::fflush(stdout);
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "ref_case_load: %s\n", e.what());
+        return -1;
+    }
+    return 0;
+}
+
+void ref_dims(long* ni, long* nj, long* nk) {
+    *ni = (long)g_case.mesh->getNumberPointsI();
+    *nj = (long)g_case.mesh->getNumberPointsJ();
+    *nk = (long)g_case.mesh->getNumberPointsK();
+}
+
+// which: 0..2 surfaces I/J/K, 3..5 face centres I/J/K, 6 volumes, 7 dual nodes,
+// 8 vertices, 9 blockage Green-Gauss gradient (BFM cases only).
+// Arrays are in the reference's k-fastest order, Vector3D as xyz triples.
+int ref_get_geometry(int which, double* out) {
+    const Mesh& m = *g_case.mesh;
+    switch (which) {
+        case 0: dumpVec(m.getSurfacesI(), out); return 0;
+        case 1: dumpVec(m.getSurfacesJ(), out); return 0;
+        case 2: dumpVec(m.getSurfacesK(), out); return 0;
+        case 3: dumpVec(m.getMidPointsI(), out); return 0;
+        case 4: dumpVec(m.getMidPointsJ(), out); return 0;
+        case 5: dumpVec(m.getMidPointsK(), out); return 0;
+        case 6: {
+            const auto& v = m.getVolumes();
+            for (size_t i = 0; i < v.totalSize(); ++i) out[i] = v[i];
+            return 0;
+        }
+        case 7: dumpVec(m.getDualNodes(), out); return 0;
+        case 8: dumpVec(m.getVertices(), out); return 0;
+        case 9: {
+            const size_t ni = m.getNumberPointsI(), nj = m.getNumberPointsJ(), nk = m.getNumberPointsK();
+            size_t idx = 0;
+            for (size_t i = 0; i < ni; ++i)
+                for (size_t j = 0; j < nj; ++j)
+                    for (size_t k = 0; k < nk; ++k, ++idx) {
+                        Vector3D g = m.getInputFieldsGradient(InputField::BLOCKAGE, i, j, k);
+                        out[3 * idx + 0] = g.x();
+                        out[3 * idx + 1] = g.y();
+                        out[3 * idx + 2] = g.z();
+                    }
+            return 0;
+        }
+        default: return -1;
+    }
+}
+
+// Face flux of the configured scheme (JST or Roe[+MUSCL]), per unit area.
+void ref_face_flux(const double* Ull, const double* Ul, const double* Ur, const double* Urr,
+                   const double* S, double* out) {
+    StateVector f = g_case.advection->computeFlux(sv(Ull), sv(Ul), sv(Ur), sv(Urr),
+                                                  Vector3D(S[0], S[1], S[2]));
+    put(f, out);
+}
+
+// Boundary flux.  kind: 0 inlet, 1 outlet, 2 radial equilibrium (profile given),
+// 3 inviscid wall.  values: the patch values of input.ini.  face = {i,j,k} of the
+// boundary face as the reference passes it.
+void ref_boundary_flux(int kind, const double* values, int nvalues, const double* Uint,
+                       const double* S, const long* face, long iter, const double* profile,
+                       int nprofile, double* out) {
+    std::vector<FloatType> vals(values, values + nvalues);
+    std::array<size_t, 3> idx{(size_t)face[0], (size_t)face[1], (size_t)face[2]};
+    FlowSolution dummy;
+    Vector3D surf(S[0], S[1], S[2]);
+    Vector3D mid(0, 0, 0);
+    StateVector f;
+    if (kind == 0) {
+        BoundaryInlet bc(*g_case.config, *g_case.mesh, *g_case.fluid, vals);
+        f = bc.computeBoundaryFlux(sv(Uint), surf, mid, idx, dummy, (size_t)iter);
+    } else if (kind == 1) {
+        BoundaryOutlet bc(*g_case.config, *g_case.mesh, *g_case.fluid, vals);
+        f = bc.computeBoundaryFlux(sv(Uint), surf, mid, idx, dummy, (size_t)iter);
+    } else if (kind == 2) {
+        g_case.radialProfile.assign(profile, profile + nprofile);
+        BoundaryOutletRadialEquilibrium bc(*g_case.config, *g_case.mesh, *g_case.fluid,
+                                           g_case.radialProfile);
+        f = bc.computeBoundaryFlux(sv(Uint), surf, mid, idx, dummy, (size_t)iter);
+    } else {
+        BoundaryInviscidWall bc(*g_case.config, *g_case.mesh, *g_case.fluid);
+        f = bc.computeBoundaryFlux(sv(Uint), surf, mid, idx, dummy, (size_t)iter);
+    }
+    put(f, out);
+}
+
+// Body-force source of the configured BFM model at node (i,j,k) for the given
+// primitive state; also returns the inviscid/viscous force vectors and the
+// deviation angle the reference writes as side outputs.
+void ref_bfm_source(long i, long j, long k, const double* primitive, double time,
+                    double* out, double* finv, double* fvisc, double* dev) {
+    const Mesh& m = *g_case.mesh;
+    const size_t ni = m.getNumberPointsI(), nj = m.getNumberPointsJ(), nk = m.getNumberPointsK();
+    Matrix3D<Vector3D> fi(ni, nj, nk), fv(ni, nj, nk);
+    Matrix3D<FloatType> da(ni, nj, nk), dt(ni, nj, nk);
+    FlowSolution U(ni, nj, nk);
+    FloatType t = time, t2 = time;
+    StateVector s = g_case.bfm->computeTotalSource((size_t)i, (size_t)j, (size_t)k, sv(primitive),
+                                                   fi, fv, da, t, U, dt, t2);
+    put(s, out);
+    Vector3D a = fi(i, j, k), b = fv(i, j, k);
+    finv[0] = a.x(); finv[1] = a.y(); finv[2] = a.z();
+    fvisc[0] = b.x(); fvisc[1] = b.y(); fvisc[2] = b.z();
+    *dev = da(i, j, k);
+}
+
+// Ideal-gas helpers used on the path: out = {p, a, ht, T, pt, Tt, s}.
+void ref_fluid_state(const double* primitive, double* out) {
+    const FluidIdeal& f = *g_case.fluid;
+    Vector3D u(primitive[1], primitive[2], primitive[3]);
+    out[0] = f.computePressure_rho_u_et(primitive[0], u, primitive[4]);
+    out[1] = f.computeSoundSpeed_rho_u_et(primitive[0], u, primitive[4]);
+    out[2] = f.computeTotalEnthalpy_rho_u_et(primitive[0], u, primitive[4]);
+    out[3] = f.computeTemperature_rho_u_et(primitive[0], u, primitive[4]);
+    out[4] = f.computeTotalPressure_rho_u_et(primitive[0], u, primitive[4]);
+    out[5] = f.computeTotalTemperature_rho_u_et(primitive[0], u, primitive[4]);
+    out[6] = f.computeEntropy_rho_u_et(primitive[0], u, primitive[4]);
+}
+
+// Initial free-stream state of computeInitFields (fluid_ideal.cpp) for a direction.
+void ref_init_state(double mach, double T, double p, const double* dir, double* U) {
+    FloatType rho = 0, et = 0;
+    Vector3D vel;
+    g_case.fluid->computeInitFields(mach, T, p, Vector3D(dir[0], dir[1], dir[2]), rho, vel, et);
+    U[0] = rho; U[1] = rho * vel.x(); U[2] = rho * vel.y(); U[3] = rho * vel.z(); U[4] = rho * et;
+}
+
+// Trapezoidal radial-equilibrium integration (math_utils.cpp:141-170).
+void ref_radial_equilibrium(const double* rho, const double* ut, const double* r, int n,
+                            double hub, double* p) {
+    std::vector<FloatType> a(rho, rho + n), b(ut, ut + n), c(r, r + n), out(n);
+    FloatType h = hub;
+    integrateRadialEquilibrium(a, b, c, out, h);
+    for (int i = 0; i < n; ++i) p[i] = out[i];
+}
+
+}  // extern "C"

@@ -1,0 +1,110 @@
+/* CPU restatement of CTurboBFM's explicit residual path -- TEST INFRASTRUCTURE.
+ *
+ * This is the oracle: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it, and only as the checker.  It is never part of the
+ * product path.  Plain C11, compiled with -ffp-contract=off; OpenMP over grid lines
+ * only, with the per-node operation order of the reference kept unchanged.
+ *
+ * Layout follows the reference: scalar node arrays are k-fastest,
+ * idx = (i*NJ + j)*NK + k (include/types.hpp:411,428); conserved state is five such
+ * arrays (rho, rhoU, rhoV, rhoW, rhoE), include/types.hpp:763-1018.
+ */
+#ifndef TBFM_ORACLE_H
+#define TBFM_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { TBO_TOPO_2D = 0, TBO_TOPO_3D = 1, TBO_TOPO_AXI = 2, TBO_TOPO_1D = 3 };
+enum { TBO_JST = 0, TBO_ROE = 1 };
+enum { TBO_LIM_NONE = 0, TBO_LIM_VAN_ALBADA = 1, TBO_LIM_VAN_LEER = 2, TBO_LIM_MIN_MOD = 3 };
+enum { TBO_DT_LOCAL = 0, TBO_DT_GLOBAL = 1, TBO_DT_FIXED = 2 };
+enum { TBO_BFM_NONE = 0, TBO_BFM_HALL = 1, TBO_BFM_LIFT_DRAG = 2, TBO_BFM_BLOCKAGE = 3 };
+enum { TBO_BC_INLET = 0, TBO_BC_OUTLET = 1, TBO_BC_RADIAL_EQ = 2, TBO_BC_WALL = 3,
+       TBO_BC_PERIODIC = 4, TBO_BC_WEDGE = 5 };
+
+typedef struct {
+    long ni, nj, nk;
+    int topology, scheme, muscl, limiter;
+    int n_stages;
+    double rk[4];
+    int dt_method;
+    double cfl, fixed_dt;
+    double gamma, r_gas;
+    double init_mach, init_temperature, init_pressure, init_dir[3];
+    double ramp_iterations;          /* OUTLET_PRESSURE_RAMP_ITERATIONS (0 = none) */
+    int bfm_active, bfm_model, blockage_active;
+    long le_index, te_index;         /* LEADING/TRAILING_EDGE_INDEX (lift-drag) */
+    double omega_scaling;            /* ROTATIONAL_SPEED_SCALING_FACTOR */
+    int inlet_cylindrical;           /* INLET_REFERENCE_FRAME = cylindrical */
+    int turbo_dir;                   /* 0: i, 1: -i, 2: j, 3: -j */
+    long residual_drop;              /* RESIDUALS_DROP_CONVERGENCE */
+    int steady;
+} tbo_params;
+
+typedef struct {
+    int type;
+    int nvalues;
+    double values[8];
+    long i0, i1, j0, j1, k0, k1;
+} tbo_patch;
+
+/* Static per-node input fields (grid CSV columns); any may be NULL if absent. */
+typedef struct {
+    const double *blockage, *normal_axial, *normal_radial, *normal_tangential;
+    const double *rpm, *blade_present, *number_blades, *metal_angle;
+    const double *streamwise_length, *lift_drag_beta0, *lift_drag_kp;
+} tbo_fields;
+
+typedef struct tbo_case tbo_case;
+
+tbo_case* tbo_create(const tbo_params* p, const double* x, const double* y, const double* z,
+                     const tbo_fields* f, const tbo_patch* patches, int npatches,
+                     char* err, int errlen);
+void tbo_destroy(tbo_case* c);
+long tbo_num_nodes(const tbo_case* c);
+
+/* state: 5*N doubles, variable-major (rho[N], rhoU[N], ...), k-fastest */
+void tbo_init_state(tbo_case* c);          /* initializeSolutionFromScratch + radial profiles */
+void tbo_set_state(tbo_case* c, const double* U, int reset_profiles);
+void tbo_get_state(const tbo_case* c, double* U);
+
+/* Runs n explicit iterations of Solver::solve (solver.cpp:1337-1424).
+ * logres: n*5 log10 residual norms; turbo: n*4 {massflow, PRtt, TRtt, eta} at the
+ * top of each iteration (may be NULL).  Returns the iteration at which the
+ * convergence test passed (0 if it never did), or -1-v if residual v diverged. */
+long tbo_run(tbo_case* c, long n, double* logres, double* turbo, int nthreads);
+
+/* One computeResiduals(U) at iteration it with the current time-step array; R is
+ * 5*N.  Refreshes the radial-equilibrium profiles first when update_profiles. */
+void tbo_residual(tbo_case* c, const double* U, long it, int update_profiles, double* R);
+void tbo_timestep(tbo_case* c, const double* U, double* dt);
+
+/* Geometry: which 0..2 surfaces I/J/K, 3..5 centres I/J/K, 6 volumes, 7 dual nodes,
+ * 8 blockage gradient.  Same shapes as the reference Mesh arrays. */
+int tbo_geometry(const tbo_case* c, int which, double* out);
+
+/* Side outputs of the last source evaluation: finv, fvisc (3*N xyz), deviation (N). */
+void tbo_side_outputs(const tbo_case* c, double* finv, double* fvisc, double* dev, double* dt);
+
+/* Turbo performance of a state (solver.cpp:1633-1781): out[0..5] boundary mass flows,
+ * out[6] mass flow, out[7] PRtt, out[8] TRtt, out[9] efficiency. */
+void tbo_turbo(tbo_case* c, const double* U, double* out);
+
+/* Point functions, exposed for per-function pinning against the reference build. */
+void tbo_face_flux(const tbo_case* c, const double* Ull, const double* Ul, const double* Ur,
+                   const double* Urr, const double* S, double* out);
+void tbo_boundary_flux(const tbo_case* c, int kind, const double* values, const double* Uint,
+                       const double* S, const long* face, long iter, const double* profile,
+                       double* out);
+void tbo_bfm_source(tbo_case* c, long i, long j, long k, const double* primitive, double* out,
+                    double* finv, double* fvisc, double* dev);
+void tbo_fluid_state(const tbo_case* c, const double* primitive, double* out);
+void tbo_radial_equilibrium(const double* rho, const double* ut, const double* r, int n,
+                            double hub, double* p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
