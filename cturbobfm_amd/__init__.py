@@ -1,0 +1,228 @@
+"""MI355X-native explicit residual path of CTurboBFM (JST / Roe-MUSCL fluxes, Hall /
+lift-drag body forces, local / global time step, RK41A / RK31B).
+
+The product is the native library ``libtbfm_gpu.so`` (HIP kernels for gfx950 behind the
+C-ABI of ``include/tbfm_gpu.h``).  This module binds it with ctypes and offers a small
+host-side mirror of the reference's ``Solver`` surface (src/solver.cpp, include/solver.hpp):
+
+    s = Solver("input.ini")          # or Solver.from_case(cases.rotor3d(...))
+    s.solve()                        # N_ITERATIONS or until the convergence test passes
+
+There is no CPU fallback: without the built library every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Dict, Optional
+
+import numpy as np
+
+from . import cases  # noqa: F401  (synthetic workloads)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtbfm_gpu.so")
+
+TBFM_OK = 0
+TBFM_ERR_DIVERGED = -4
+
+
+class CaseDesc(C.Structure):
+    _fields_ = [("ni", C.c_long), ("nj", C.c_long), ("nk", C.c_long),
+                ("x", C.POINTER(C.c_double)), ("y", C.POINTER(C.c_double)), ("z", C.POINTER(C.c_double)),
+                ("nfields", C.c_int), ("field_names", C.POINTER(C.c_char_p)),
+                ("field_data", C.POINTER(C.POINTER(C.c_double))),
+                ("nconfig", C.c_int), ("config_keys", C.POINTER(C.c_char_p)),
+                ("config_values", C.POINTER(C.c_char_p)),
+                ("npatches", C.c_int), ("patch_names", C.POINTER(C.c_char_p)),
+                ("patch_ranges", C.POINTER(C.c_long))]
+
+
+EXPORTS = ["tbfm_gpu_case_create", "tbfm_gpu_case_create_from_ini", "tbfm_gpu_case_destroy",
+           "tbfm_gpu_last_error", "tbfm_gpu_dims", "tbfm_gpu_set_state", "tbfm_gpu_get_state",
+           "tbfm_gpu_advance", "tbfm_gpu_compute_residual", "tbfm_gpu_compute_timestep",
+           "tbfm_gpu_get_side_outputs", "tbfm_gpu_get_turbo", "tbfm_gpu_get_geometry",
+           "tbfm_gpu_profile", "tbfm_gpu_profile_get", "tbfm_gpu_sync"]
+
+_lib = None
+
+
+class SolverError(RuntimeError):
+    pass
+
+
+def lib():
+    """The native library; raises if it has not been built (no fallback path exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SolverError("native library missing: %s -- run __graft_entry__.build()" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        dp = C.POINTER(C.c_double)
+        vp = C.c_void_p
+        L.tbfm_gpu_case_create.argtypes = [C.POINTER(CaseDesc), C.POINTER(vp)]
+        L.tbfm_gpu_case_create_from_ini.argtypes = [C.c_char_p, C.POINTER(vp)]
+        L.tbfm_gpu_case_destroy.argtypes = [vp]
+        L.tbfm_gpu_last_error.restype = C.c_char_p
+        L.tbfm_gpu_last_error.argtypes = [vp]
+        L.tbfm_gpu_dims.argtypes = [vp, C.POINTER(C.c_long), C.POINTER(C.c_long), C.POINTER(C.c_long)]
+        L.tbfm_gpu_set_state.argtypes = [vp, dp]
+        L.tbfm_gpu_get_state.argtypes = [vp, dp]
+        L.tbfm_gpu_advance.argtypes = [vp, C.c_long, dp, C.POINTER(C.c_long)]
+        L.tbfm_gpu_compute_residual.argtypes = [vp, dp, C.c_long, C.c_int, dp]
+        L.tbfm_gpu_compute_timestep.argtypes = [vp, dp, dp]
+        L.tbfm_gpu_get_side_outputs.argtypes = [vp, dp, dp, dp, dp]
+        L.tbfm_gpu_get_turbo.argtypes = [vp, dp]
+        L.tbfm_gpu_get_geometry.argtypes = [vp, C.c_int, dp]
+        L.tbfm_gpu_profile.argtypes = [vp, C.c_int, C.c_int]
+        L.tbfm_gpu_profile_get.argtypes = [vp, dp, C.POINTER(C.c_long), dp]
+        L.tbfm_gpu_sync.argtypes = [vp]
+        _lib = L
+    return _lib
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Solver:
+    """One case resident on the GPU (mirrors Solver::Solver + Solver::solve)."""
+
+    def __init__(self, ini_path: Optional[str] = None, _handle=None):
+        self.h = None
+        if _handle is None:
+            h = C.c_void_p()
+            rc = lib().tbfm_gpu_case_create_from_ini(os.fsencode(ini_path), C.byref(h))
+            if rc != TBFM_OK:
+                raise SolverError(lib().tbfm_gpu_last_error(None).decode())
+            _handle = h
+        self.h = _handle
+        ni, nj, nk = C.c_long(), C.c_long(), C.c_long()
+        lib().tbfm_gpu_dims(self.h, C.byref(ni), C.byref(nj), C.byref(nk))
+        self.ni, self.nj, self.nk = ni.value, nj.value, nk.value
+        self.N = self.ni * self.nj * self.nk
+        self.iterations = 0
+        self.converged_at = 0
+        self.history = np.zeros((0, 5))
+
+    @classmethod
+    def from_case(cls, case: Dict, overrides: Optional[Dict[str, str]] = None) -> "Solver":
+        cfg = dict(case["config"])
+        if overrides:
+            cfg.update(overrides)
+        keep = []
+
+        def arr(a):
+            a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+            keep.append(a)
+            return _dp(a)
+
+        d = CaseDesc()
+        d.ni, d.nj, d.nk = case["ni"], case["nj"], case["nk"]
+        d.x, d.y, d.z = arr(case["x"]), arr(case["y"]), arr(case["z"])
+        names = list(case["fields"].keys())
+        d.nfields = len(names)
+        d.field_names = (C.c_char_p * max(len(names), 1))(*[n.encode() for n in names])
+        d.field_data = (C.POINTER(C.c_double) * max(len(names), 1))(*[arr(case["fields"][n]) for n in names])
+        keys = list(cfg.keys())
+        d.nconfig = len(keys)
+        d.config_keys = (C.c_char_p * len(keys))(*[k.encode() for k in keys])
+        d.config_values = (C.c_char_p * len(keys))(*[str(cfg[k]).encode() for k in keys])
+        d.npatches = len(case["patches"])
+        d.patch_names = (C.c_char_p * d.npatches)(*[p[0].encode() for p in case["patches"]])
+        ranges = np.array([p[1:7] for p in case["patches"]], dtype=np.int64).reshape(-1)
+        keep.append(ranges)
+        d.patch_ranges = ranges.ctypes.data_as(C.POINTER(C.c_long))
+        h = C.c_void_p()
+        rc = lib().tbfm_gpu_case_create(C.byref(d), C.byref(h))
+        if rc != TBFM_OK:
+            raise SolverError(lib().tbfm_gpu_last_error(None).decode())
+        return cls(_handle=h)
+
+    def close(self):
+        if self.h is not None and _lib is not None:
+            _lib.tbfm_gpu_case_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc != TBFM_OK:
+            raise SolverError(lib().tbfm_gpu_last_error(self.h).decode())
+
+    # -- state -------------------------------------------------------------------
+    def state(self) -> np.ndarray:
+        U = np.empty((5, self.ni, self.nj, self.nk))
+        self._check(lib().tbfm_gpu_get_state(self.h, _dp(U)))
+        return U
+
+    def set_state(self, U: np.ndarray) -> None:
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        self._check(lib().tbfm_gpu_set_state(self.h, _dp(U)))
+
+    # -- iterations --------------------------------------------------------------
+    def advance(self, n: int) -> np.ndarray:
+        """Runs up to n iterations; returns their log10 residual norms (rows stop at
+        convergence)."""
+        lr = np.zeros((n, 5))
+        conv = C.c_long()
+        rc = lib().tbfm_gpu_advance(self.h, n, _dp(lr), C.byref(conv))
+        first = self.iterations
+        rows = n if conv.value == 0 or conv.value <= first else conv.value - first
+        self.iterations += rows
+        self.converged_at = conv.value
+        self.history = np.vstack([self.history, lr[:rows]])
+        self._check(rc)
+        return lr[:rows]
+
+    def solve(self, n_max: Optional[int] = None, chunk: int = 500) -> np.ndarray:
+        n_max = n_max if n_max is not None else 10 ** 9
+        while self.iterations < n_max and self.converged_at == 0:
+            self.advance(min(chunk, n_max - self.iterations))
+        return self.history
+
+    def residual(self, U: np.ndarray, it: int = 1, update_profiles: bool = True) -> np.ndarray:
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        R = np.zeros_like(U)
+        self._check(lib().tbfm_gpu_compute_residual(self.h, _dp(U), it, int(update_profiles), _dp(R)))
+        return R
+
+    def timestep(self, U: np.ndarray) -> np.ndarray:
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        dt = np.zeros((self.ni, self.nj, self.nk))
+        self._check(lib().tbfm_gpu_compute_timestep(self.h, _dp(U), _dp(dt)))
+        return dt
+
+    def side_outputs(self):
+        finv = np.zeros((self.ni, self.nj, self.nk, 3))
+        fvisc = np.zeros_like(finv)
+        dev = np.zeros((self.ni, self.nj, self.nk))
+        dt = np.zeros_like(dev)
+        self._check(lib().tbfm_gpu_get_side_outputs(self.h, _dp(finv), _dp(fvisc), _dp(dev), _dp(dt)))
+        return finv, fvisc, dev, dt
+
+    def turbo(self) -> np.ndarray:
+        out = np.zeros(10)
+        self._check(lib().tbfm_gpu_get_turbo(self.h, _dp(out)))
+        return out
+
+    def geometry(self, which: int) -> np.ndarray:
+        ni, nj, nk = self.ni, self.nj, self.nk
+        shapes = {0: (ni + 1, nj, nk, 3), 1: (ni, nj + 1, nk, 3), 2: (ni, nj, nk + 1, 3),
+                  3: (ni, nj, nk), 4: (ni, nj, nk, 3)}
+        out = np.zeros(shapes[which])
+        self._check(lib().tbfm_gpu_get_geometry(self.h, which, _dp(out)))
+        return out
+
+    # -- profiling ---------------------------------------------------------------
+    def profile(self, enable: bool = True, reset: bool = True) -> None:
+        self._check(lib().tbfm_gpu_profile(self.h, int(enable), int(reset)))
+
+    def profile_get(self):
+        ms, n, b = C.c_double(), C.c_long(), C.c_double()
+        self._check(lib().tbfm_gpu_profile_get(self.h, C.byref(ms), C.byref(n), C.byref(b)))
+        return ms.value, n.value, b.value
+
+    def sync(self) -> None:
+        self._check(lib().tbfm_gpu_sync(self.h))

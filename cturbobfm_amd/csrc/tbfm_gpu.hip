@@ -1,0 +1,986 @@
+// C-ABI of the MI355X explicit residual path: case set-up on the host, device buffers
+// in the i-fastest layout, and the per-iteration launch sequence (include/tbfm_gpu.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/tbfm_gpu.h"
+#include "tbfm_fused.hpp"
+#include "tbfm_host.hpp"
+
+using namespace tbfm;
+using tbfm::dev::CaseDev;
+using tbfm::dev::StageDev;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct HipError : std::runtime_error {
+    hipError_t code;
+    HipError(hipError_t c, const char* what) : std::runtime_error(std::string(what) + ": " + hipGetErrorString(c)), code(c) {}
+};
+
+#define HIPCHK(x)                                  \
+    do {                                           \
+        hipError_t _e = (x);                       \
+        if (_e != hipSuccess) throw HipError(_e, #x); \
+    } while (0)
+
+template <typename T>
+T* dalloc(size_t n, std::vector<void*>& owned) {
+    void* p = nullptr;
+    if (n == 0) n = 1;
+    HIPCHK(hipMalloc(&p, n * sizeof(T)));
+    HIPCHK(hipMemset(p, 0, n * sizeof(T)));
+    owned.push_back(p);
+    return (T*)p;
+}
+
+template <typename T>
+T* upload(const std::vector<T>& h, std::vector<void*>& owned) {
+    T* d = dalloc<T>(h.size(), owned);
+    if (!h.empty()) HIPCHK(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return d;
+}
+
+}  // namespace
+
+struct tbfm_case {
+    long ni = 0, nj = 0, nk = 0, N = 0;
+    Config cfg;
+    Topology topo = Topology::TwoD;
+    Scheme scheme = Scheme::JST;
+    Precision precision = Precision::FP64;
+    std::vector<double> rk;
+    Geometry geo;
+    std::vector<Patch> patches;
+    hipStream_t stream = nullptr;
+    std::vector<void*> owned;
+    CaseDev c{};
+    double* U[3] = {nullptr, nullptr, nullptr};
+    int u0 = 0;                       // index of the buffer holding U^(0)
+    double* rper = nullptr;
+    double* partial = nullptr;
+    double* partial_min = nullptr;
+    long nparts_stage = 0, nparts_per = 0, nblk_dt = 0;
+    dev::ConvDev conv{};
+    long hist_cap = 0;
+    long iter_host = 1;               // next iteration number
+    int kc = 8;
+    dim3 grid_stage;
+    // profiling
+    bool prof = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double prof_ms = 0.0;
+    long prof_launches = 0;
+    double bytes_per_node = 0.0;
+    // graph of three iterations (the buffer rotation has period 3)
+    hipGraphExec_t graph = nullptr;
+    int graph_u0 = -1;
+    std::string err;
+    // host copies of static data for get_geometry
+};
+
+namespace {
+
+long nodeDev(const tbfm_case* c, long i, long j, long k) { return (k * c->nj + j) * c->ni + i; }
+
+// reference order (k fastest) <-> device order (i fastest)
+void toDevice(const tbfm_case* c, const double* src, std::vector<double>& dst) {
+    dst.resize(c->N);
+    for (long i = 0; i < c->ni; ++i)
+        for (long j = 0; j < c->nj; ++j)
+            for (long k = 0; k < c->nk; ++k) dst[nodeDev(c, i, j, k)] = src[(i * c->nj + j) * c->nk + k];
+}
+void toHost(const tbfm_case* c, const double* src, double* dst) {
+    for (long i = 0; i < c->ni; ++i)
+        for (long j = 0; j < c->nj; ++j)
+            for (long k = 0; k < c->nk; ++k) dst[(i * c->nj + j) * c->nk + k] = src[nodeDev(c, i, j, k)];
+}
+
+// face arrays (reference xyz triples) -> three SoA device arrays, i fastest
+void faceSoA(const std::vector<double>& src, long ai, long aj, long ak, std::vector<double>& x,
+             std::vector<double>& y, std::vector<double>& z) {
+    const long n = ai * aj * ak;
+    x.resize(n); y.resize(n); z.resize(n);
+    for (long i = 0; i < ai; ++i)
+        for (long j = 0; j < aj; ++j)
+            for (long k = 0; k < ak; ++k) {
+                const long s = ((i * aj + j) * ak + k) * 3;
+                const long d = (k * aj + j) * ai + i;
+                x[d] = src[s]; y[d] = src[s + 1]; z[d] = src[s + 2];
+            }
+}
+
+double atan2Positive(double y, double x) {
+    double a = std::atan2(y, x);
+    if (a < 0) a += 2 * M_PI;
+    return a;
+}
+
+void setup(tbfm_case* cs, const GridInput& grid) {
+    Config& cfg = cs->cfg;
+    cs->ni = grid.ni; cs->nj = grid.nj; cs->nk = grid.nk;
+    cs->N = grid.ni * grid.nj * grid.nk;
+    const long NI = cs->ni, NJ = cs->nj, NK = cs->nk, N = cs->N;
+    cs->topo = cfg.getTopology();
+    cs->scheme = cfg.getAdvectionScheme();
+    cs->precision = cfg.getGpuPrecision();
+    cs->rk = cfg.getTimeIntegrationCoeffs();
+    if (cs->topo == Topology::ThreeD && NK < 4) throw std::runtime_error("3D topology needs NK >= 4");
+    if (cs->topo != Topology::ThreeD && NK != 1) throw std::runtime_error("2D/axisymmetric topology needs NK = 1");
+    if (NI < 3 || (cs->topo != Topology::OneD && NJ < 3))
+        throw std::runtime_error("the residual stencil needs at least 3 points per active direction");
+    const bool bfm = cfg.isBFMActive();
+    cs->geo = buildGeometry(grid, cs->topo, bfm);
+    const Geometry& g = cs->geo;
+
+    HIPCHK(hipStreamCreateWithFlags(&cs->stream, hipStreamNonBlocking));
+    auto& own = cs->owned;
+    CaseDev& c = cs->c;
+    c.ni = (int)NI; c.nj = (int)NJ; c.nk = (int)NK; c.N = N;
+    c.topo = (int)cs->topo;
+    c.scheme = (int)cs->scheme;
+    c.muscl = cfg.getMUSCLreconstruction() ? 1 : 0;
+    c.limiter = (int)cfg.getFluxLimiter();
+    c.gamma = cfg.getFluidGamma();
+    c.gm1 = c.gamma - 1;
+    c.rgas = cfg.getFluidGasConstant();
+    c.cp = c.gamma * c.rgas / (c.gamma - 1.0);
+    c.cfl = cfg.getCFL();
+    c.dt_mode = (int)cfg.getTimeStepMethod();
+    c.fixed_dt = cfg.getFixedTimeStep();
+    c.ramp_iter = cfg.getOutletPressureRampMaxIterations();
+    c.init_p = cfg.has("INIT_PRESSURE") ? cfg.getInitPressure() : 101325.0;
+    c.bfm_active = bfm ? 1 : 0;
+    c.bfm_model = (int)cfg.getBFMModel();
+    c.blockage_active = cfg.isBlockageActive() ? 1 : 0;
+    c.dtheta = (NK > 1) ? (2 * M_PI / (double)(NK - 1)) : 0.0;
+
+    // ---- patches, periodicity, boundary-face maps (solver.cpp:296-367, 491-588)
+    double perAngleDeg = 0.0, perTranslation = 0.0;
+    bool periodic = false;
+    for (auto& p : cs->patches) {
+        if (p.type == PatchType::Periodic) {
+            if (p.values.size() < 3) throw std::runtime_error("periodic patch " + p.name + " needs 3 values");
+            if (p.values[0] == 0) {
+                periodic = true;
+                perTranslation = p.values[1];
+                perAngleDeg = p.values[2];
+                if (perAngleDeg == 360.0) perAngleDeg = 0.0;
+            }
+            if (p.k0 != p.k1) throw std::runtime_error("Periodic boundary only supported on i-j patches");
+            if (p.i0 != 0 || p.j0 != 0 || p.i1 != NI || p.j1 != NJ)
+                throw std::runtime_error("Periodic boundary only supported on full i-j patches");
+        }
+    }
+    const double perAngleRad = perAngleDeg * M_PI / 180.0;
+    if (periodic) {
+        // Mesh::checkPeriodicity (src/mesh.cpp:584-619)
+        for (long i = 0; i < NI; ++i)
+            for (long j = 0; j < NJ; ++j) {
+                long a = (i * NJ + j) * NK, b = (i * NJ + j) * NK + NK - 1;
+                double ta = atan2Positive(grid.z[a], grid.y[a]);
+                double tb = atan2Positive(grid.z[b] - perTranslation, grid.y[b]);
+                double d = std::fmod(tb - ta, 2.0 * M_PI);
+                if (std::abs(d - perAngleRad) > 1E-4)
+                    throw std::invalid_argument("Periodicity check failed at (i,j)=(" + std::to_string(i) + "," +
+                                                std::to_string(j) + ")");
+            }
+    }
+    c.periodic = periodic ? 1 : 0;
+    c.per_c = std::cos(perAngleRad); c.per_s = std::sin(perAngleRad);
+    c.per_cm = std::cos(-perAngleRad); c.per_sm = std::sin(-perAngleRad);
+    if ((int)cs->patches.size() > dev::MAX_PATCH) throw std::runtime_error("too many boundary patches");
+
+    std::vector<int> pI(2 * NJ * NK, -1), pJ(2 * NI * NK, -1), pK(2 * NI * NJ, -1);
+    std::vector<int> profOfPatch(cs->patches.size(), -1);
+    int nprof = 0;
+    for (size_t q = 0; q < cs->patches.size(); ++q) {
+        const Patch& p = cs->patches[q];
+        dev::PatchDev pd{};
+        switch (p.type) {
+            case PatchType::Inlet:
+                if (p.values.size() < 5) throw std::runtime_error("inlet patch " + p.name + " needs p0, T0, direction");
+                pd.type = dev::PT_INLET; pd.v0 = p.values[0]; pd.v1 = p.values[1];
+                break;
+            case PatchType::Outlet:
+                if (p.values.empty()) throw std::runtime_error("outlet patch " + p.name + " needs a pressure");
+                pd.type = dev::PT_OUTLET; pd.v0 = p.values[0];
+                break;
+            case PatchType::RadialEquilibrium:
+                if (p.i0 != p.i1) throw std::runtime_error("Radial equilibrium only supported on j-k patches");
+                if (p.i0 == 0) throw std::runtime_error("Radial equilibrium only supported to the last i-position");
+                if (p.j1 <= p.j0) throw std::runtime_error("Invalid boundary indices for radial equilibrium patch: " + p.name);
+                if (nprof >= dev::MAX_PROF) throw std::runtime_error("too many radial-equilibrium patches");
+                pd.type = dev::PT_RADIAL; pd.profile = nprof;
+                profOfPatch[q] = nprof++;
+                break;
+            case PatchType::Wall: pd.type = dev::PT_WALL; break;
+            default: pd.type = dev::PT_FAKE; break;
+        }
+        c.patch[q] = pd;
+        if (p.i0 == p.i1) {
+            long t = (p.i0 == 0) ? 0 : 1;
+            for (long j = p.j0; j < p.j1; ++j)
+                for (long k = p.k0; k < p.k1; ++k) pI[t * NJ * NK + k * NJ + j] = (int)q;
+        } else if (p.j0 == p.j1) {
+            long t = (p.j0 == 0) ? 0 : 1;
+            for (long i = p.i0; i < p.i1; ++i)
+                for (long k = p.k0; k < p.k1; ++k) pJ[t * NI * NK + k * NI + i] = (int)q;
+        } else if (p.k0 == p.k1) {
+            long t = (p.k0 == 0) ? 0 : 1;
+            for (long i = p.i0; i < p.i1; ++i)
+                for (long j = p.j0; j < p.j1; ++j) pK[t * NI * NJ + j * NI + i] = (int)q;
+        } else {
+            throw std::runtime_error("Invalid boundary definition: " + p.name);
+        }
+    }
+    auto missing = [&](const std::vector<int>& m, const char* d) {
+        for (size_t q = 0; q < m.size(); ++q)
+            if (m[q] < 0) throw std::runtime_error(std::string("Missing boundary condition on a ") + d + " boundary face");
+    };
+    missing(pI, "I");
+    if (cs->topo != Topology::OneD) missing(pJ, "J");
+    if (cs->topo == Topology::ThreeD) missing(pK, "K");
+    // inlet directions per boundary face (boundary_inlet.cpp:14-35), static geometry
+    const bool cyl = cfg.isInletCylindrical();
+    auto inletDirs = [&](int dir, const std::vector<int>& pm, long n1, long n2) {
+        std::vector<double> d(2 * 3 * n1 * n2, 0.0);
+        const long nsz = n1 * n2;
+        for (int side = 0; side < 2; ++side)
+            for (long q = 0; q < nsz; ++q) {
+                int pidx = pm[side * nsz + q];
+                if (pidx < 0 || cs->patches[pidx].type != PatchType::Inlet) continue;
+                const auto& vals = cs->patches[pidx].values;
+                long a = q % n1, b = q / n1;   // (fast, slow) index of the face
+                long fi, fj, fk;
+                double S[3];
+                if (dir == 0) {
+                    fi = side ? NI : 0; fj = a; fk = b;
+                    const double* s = &g.surfI[((fi * NJ + fj) * NK + fk) * 3];
+                    S[0] = s[0]; S[1] = s[1]; S[2] = s[2];
+                } else if (dir == 1) {
+                    fi = a; fj = side ? NJ : 0; fk = b;
+                    const double* s = &g.surfJ[((fi * (NJ + 1) + fj) * NK + fk) * 3];
+                    S[0] = s[0]; S[1] = s[1]; S[2] = s[2];
+                } else {
+                    fi = a; fj = b; fk = side ? NK : 0;
+                    const double* s = &g.surfK[((fi * NJ + fj) * (NK + 1) + fk) * 3];
+                    S[0] = s[0]; S[1] = s[1]; S[2] = s[2];
+                }
+                if (!side) { S[0] = -S[0]; S[1] = -S[1]; S[2] = -S[2]; }
+                double dx = vals[2], dy = vals[3], dz = vals[4];
+                if (cyl) {
+                    long vi = std::min(fi, NI - 1), vj = std::min(fj, NJ - 1), vk = std::min(fk, NK - 1);
+                    double th = g.theta[(vi * NJ + vj) * NK + vk];
+                    double cth = std::cos(th), sth = std::sin(th);
+                    double ny = dy * cth - dz * sth, nz = dy * sth + dz * cth;
+                    dy = ny; dz = nz;
+                }
+                if (dx == 1.0 && dy == 1.0 && dz == 1.0) { dx = -S[0]; dy = -S[1]; dz = -S[2]; }
+                double m = std::sqrt(dx * dx + dy * dy + dz * dz);
+                dx /= m; dy /= m; dz /= m;
+                d[(side * 3 + 0) * nsz + q] = dx;
+                d[(side * 3 + 1) * nsz + q] = dy;
+                d[(side * 3 + 2) * nsz + q] = dz;
+            }
+        return d;
+    };
+    c.pI = upload(pI, own); c.pJ = upload(pJ, own); c.pK = upload(pK, own);
+    c.dirI = upload(inletDirs(0, pI, NJ, NK), own);
+    c.dirJ = upload(inletDirs(1, pJ, NI, NK), own);
+    c.dirK = upload(inletDirs(2, pK, NI, NJ), own);
+    c.bI = dalloc<double>(2 * 5 * NJ * NK, own);
+    c.bJ = dalloc<double>(2 * 5 * NI * NK, own);
+    c.bK = dalloc<double>(2 * 5 * NI * NJ, own);
+
+    // ---- radial equilibrium profiles (solver.cpp:296-331)
+    c.nprof = nprof;
+    std::vector<double> prof_r(dev::MAX_PROF * NJ, 0.0), prof_cs(dev::MAX_PROF * NJ, 1.0), prof_sn(dev::MAX_PROF * NJ, 0.0);
+    for (size_t q = 0; q < cs->patches.size(); ++q) {
+        int pr = profOfPatch[q];
+        if (pr < 0) continue;
+        const Patch& p = cs->patches[q];
+        c.prof_i[pr] = (int)(p.i0 > 0 ? p.i0 - 1 : 0);
+        c.prof_j0[pr] = (int)p.j0;
+        c.prof_n[pr] = (int)(p.j1 - p.j0);
+        for (long j = p.j0; j < p.j1; ++j) {
+            prof_r[pr * NJ + (j - p.j0)] = g.radius[((p.i1 - 1) * NJ + j) * NK + 0];
+            double th = g.theta[(c.prof_i[pr] * NJ + j) * NK + 0];
+            prof_cs[pr * NJ + (j - p.j0)] = std::cos(th);
+            prof_sn[pr * NJ + (j - p.j0)] = std::sin(th);
+        }
+    }
+    c.prof_p = dalloc<double>(dev::MAX_PROF * NJ, own);
+    c.prof_r = upload(prof_r, own);
+    c.prof_cs = upload(prof_cs, own);
+    c.prof_sn = upload(prof_sn, own);
+
+    // ---- geometry in device layout
+    std::vector<double> x, y, z;
+    faceSoA(g.surfI, NI + 1, NJ, NK, x, y, z);
+    c.six = upload(x, own); c.siy = upload(y, own); c.siz = upload(z, own);
+    faceSoA(g.surfJ, NI, NJ + 1, NK, x, y, z);
+    c.sjx = upload(x, own); c.sjy = upload(y, own); c.sjz = upload(z, own);
+    faceSoA(g.surfK, NI, NJ, NK + 1, x, y, z);
+    c.skx = upload(x, own); c.sky = upload(y, own); c.skz = upload(z, own);
+    faceSoA(g.dual, NI + 1, NJ + 1, NK + 1, x, y, z);
+    c.dux = upload(x, own); c.duy = upload(y, own); c.duz = upload(z, own);
+    std::vector<double> tmp;
+    toDevice(cs, g.volume.data(), tmp);
+    c.vol = upload(tmp, own);
+
+    // ---- body-force slab and source data (solver.cpp:2790-2897, source_bfm_base.cpp)
+    auto field = [&](const char* name, bool required) -> const std::vector<double>* {
+        const std::vector<double>* f = grid.field(name);
+        if (!f && required) throw std::runtime_error(std::string(name) + " not found in the grid file");
+        return f;
+    };
+    std::vector<unsigned char> blade2d(NI * NJ, 0);
+    long bi0 = NI, bi1 = 0;
+    const bool axi = (cs->topo == Topology::Axisymmetric);
+    const bool hall = bfm && (c.bfm_model == 1 || c.bfm_model == 2);
+    const std::vector<double>* present = bfm ? field("bladePresent", cs->topo == Topology::ThreeD) : nullptr;
+    const std::vector<double>* nblades = bfm ? field("numberBlades", true) : nullptr;
+    const std::vector<double>* blockage = bfm ? field("blockage", true) : nullptr;
+    if (bfm && cs->topo == Topology::ThreeD)
+        for (long i = 0; i < NI; ++i)
+            for (long j = 0; j < NJ; ++j) blade2d[j * NI + i] = ((*present)[(i * NJ + j) * NK] > 0.0) ? 1 : 0;
+    auto gateOf = [&](long n) {
+        const double* gb = &g.gradBlockage[3 * n];
+        return std::sqrt(gb[0] * gb[0] + gb[1] * gb[1] + gb[2] * gb[2]) > 1E-10;
+    };
+    if (axi) { bi0 = 0; bi1 = NI; }
+    if (bfm)
+        for (long i = 0; i < NI; ++i)
+            for (long j = 0; j < NJ; ++j)
+                for (long k = 0; k < NK; ++k) {
+                    long n = (i * NJ + j) * NK + k;
+                    bool pres = present && (*present)[n] > 0.0 && cs->topo == Topology::ThreeD;
+                    if (pres || gateOf(n)) { bi0 = std::min(bi0, i); bi1 = std::max(bi1, i + 1); }
+                }
+    if (bi1 <= bi0) { bi0 = 0; bi1 = 0; }
+    c.bi0 = (int)bi0; c.bi1 = (int)bi1;
+    const long SW = bi1 - bi0, NS = std::max<long>(SW * NJ * NK, 1);
+    std::vector<unsigned char> sflag(NS, 0);
+    std::vector<double> s_cs(NS, 1.0), s_sn(NS, 0.0), s_r(NS, 0.0), s_om(NS, 0.0), s_b(NS, 1.0), s_gbx(NS, 0.0),
+        s_gby(NS, 0.0), s_gbz(NS, 0.0), s_nax(NS, 0.0), s_nrad(NS, 0.0), s_ntan(NS, 1.0), s_pitch(NS, 1.0),
+        s_spitch(NS, 1.0), s_sol(NS, 0.0), s_beta0(NS, 0.0), s_kp(NS, 0.0);
+    const std::vector<double>* rpm = (bfm && (hall || cs->topo == Topology::ThreeD)) ? field("rpm", hall) : nullptr;
+    const std::vector<double>* nax = hall ? field("normalAxial", true) : nullptr;
+    const std::vector<double>* nrad = hall ? field("normalRadial", true) : nullptr;
+    const std::vector<double>* ntan = hall ? field("normalTangential", true) : nullptr;
+    const std::vector<double>* metal = hall ? field("bladeMetalAngle", false) : nullptr;
+    const std::vector<double>* slen = (c.bfm_model == 2) ? field("streamwiseLength", true) : nullptr;
+    const std::vector<double>* beta0 = (c.bfm_model == 2) ? field("liftDragBeta0", true) : nullptr;
+    const std::vector<double>* kp = (c.bfm_model == 2) ? field("liftDragKpEtamax", true) : nullptr;
+    const long le = cfg.getLeadingEdgeIndex(), te = cfg.getTrailingEdgeIndex();
+    const double scaling = cfg.getRotationalSpeedScalingFactor();
+    for (long i = bi0; i < bi1; ++i)
+        for (long j = 0; j < NJ; ++j)
+            for (long k = 0; k < NK; ++k) {
+                const long n = (i * NJ + j) * NK + k;
+                const long s = (k * NJ + j) * SW + (i - bi0);
+                unsigned char fl = 0;
+                if (present && (*present)[n] > 0.0) fl |= 1;
+                if (bfm && gateOf(n)) fl |= 2;
+                if (nblades && (*nblades)[n] != 0) fl |= 4;
+                sflag[s] = fl;
+                s_cs[s] = std::cos(g.theta[n]);
+                s_sn[s] = std::sin(g.theta[n]);
+                s_r[s] = g.radius[n];
+                if (rpm) {
+                    double om = (*rpm)[n] * 2 * M_PI / 60;
+                    om *= scaling;
+                    s_om[s] = om;
+                }
+                if (bfm) {
+                    s_b[s] = (*blockage)[n];
+                    s_gbx[s] = g.gradBlockage[3 * n];
+                    s_gby[s] = g.gradBlockage[3 * n + 1];
+                    s_gbz[s] = g.gradBlockage[3 * n + 2];
+                }
+                if (hall && (fl & 4)) {
+                    s_nax[s] = (*nax)[n]; s_nrad[s] = (*nrad)[n]; s_ntan[s] = (*ntan)[n];
+                    double pitch = 2.0 * M_PI * g.radius[n] / (*nblades)[n];
+                    s_pitch[s] = pitch;
+                    s_spitch[s] = pitch * std::cos(metal ? (*metal)[n] : 0.0);
+                    if (c.bfm_model == 2) {
+                        if (le < 0 || le >= NI || te < 0 || te >= NI)
+                            throw std::runtime_error("LEADING/TRAILING_EDGE_INDEX outside the grid");
+                        double chord = (*slen)[(te * NJ + j) * NK + k] - (*slen)[(le * NJ + j) * NK + k];
+                        s_sol[s] = chord / pitch;
+                        s_beta0[s] = (*beta0)[n];
+                        s_kp[s] = (*kp)[n];
+                    }
+                }
+            }
+    std::vector<double> axi_gf(NI * NJ, 0.0);
+    if (axi)
+        for (long i = 0; i < NI; ++i)
+            for (long j = 0; j < NJ; ++j) {
+                const double* s = &g.surfK[((i * NJ + j) * (NK + 1) + 0) * 3];
+                axi_gf[j * NI + i] = std::sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]) * g.wedgeAngle;
+            }
+    c.blade2d = upload(blade2d, own);
+    c.sflag = upload(sflag, own);
+    c.s_cs = upload(s_cs, own); c.s_sn = upload(s_sn, own); c.s_r = upload(s_r, own); c.s_om = upload(s_om, own);
+    c.s_b = upload(s_b, own); c.s_gbx = upload(s_gbx, own); c.s_gby = upload(s_gby, own); c.s_gbz = upload(s_gbz, own);
+    c.s_nax = upload(s_nax, own); c.s_nrad = upload(s_nrad, own); c.s_ntan = upload(s_ntan, own);
+    c.s_pitch = upload(s_pitch, own); c.s_spitch = upload(s_spitch, own); c.s_sol = upload(s_sol, own);
+    c.s_beta0 = upload(s_beta0, own); c.s_kp = upload(s_kp, own);
+    c.axi_gf = upload(axi_gf, own);
+    c.finv = dalloc<double>(3 * N, own);
+    c.fvisc = dalloc<double>(3 * N, own);
+    c.dev = dalloc<double>(N, own);
+    c.dt = dalloc<double>(N, own);
+    c.dtmin = dalloc<double>(1, own);
+    c.stop = dalloc<int>(1, own);
+    c.iter = dalloc<long>(1, own);
+
+    // ---- state buffers, scratch, reductions
+    for (int b = 0; b < 3; ++b) cs->U[b] = dalloc<double>(5 * N, own);
+    cs->rper = dalloc<double>(10 * NI * NJ, own);
+    cs->kc = (cs->topo == Topology::ThreeD) ? 8 : 1;
+    cs->grid_stage = dim3((unsigned)((NI + dev::TI - 1) / dev::TI), (unsigned)((NJ + dev::TJ - 1) / dev::TJ),
+                          (unsigned)((NK + cs->kc - 1) / cs->kc));
+    cs->nparts_stage = (long)cs->grid_stage.x * cs->grid_stage.y * cs->grid_stage.z;
+    cs->nparts_per = periodic ? (NI * NJ + 255) / 256 : 0;
+    cs->partial = dalloc<double>(5 * (cs->nparts_stage + cs->nparts_per), own);
+    cs->nblk_dt = (N + 255) / 256;
+    cs->partial_min = dalloc<double>(cs->nblk_dt, own);
+    cs->hist_cap = std::max<long>(cfg.has("N_ITERATIONS") ? cfg.getMaxIterations() : 0, 1);
+    cs->conv.hist = dalloc<double>(5 * cs->hist_cap, own);
+    cs->conv.init = dalloc<double>(5, own);
+    cs->conv.conv_at = dalloc<long>(1, own);
+    cs->conv.diverged = dalloc<int>(1, own);
+    cs->conv.hist_len = cs->hist_cap;
+    cs->conv.drop = (double)cfg.getResidualsDropConvergence();
+    cs->conv.steady = cfg.isSimulationSteady() ? 1 : 0;
+    long one = 1;
+    HIPCHK(hipMemcpy(c.iter, &one, sizeof(long), hipMemcpyHostToDevice));
+    // algorithmic bytes per node and stage: U^(k-1), U^(0), U^(k) (5 each), dt, V and one
+    // face vector per active direction
+    const int dirs = (cs->topo == Topology::ThreeD) ? 3 : (cs->topo == Topology::OneD ? 1 : 2);
+    cs->bytes_per_node = 8.0 * (15 + 2 + 3 * dirs);
+
+    // ---- initial solution (solver.cpp:853-888, fluid_ideal.cpp:155-170)
+    std::vector<double> U0(5 * N);
+    {
+        std::vector<double> dir = cfg.getInitDirection();
+        const double gm = c.gamma, Rg = c.rgas, T = cfg.getInitTemperature(), p = cfg.getInitPressure();
+        double dm = std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+        double a = std::sqrt(gm * Rg * T);
+        double M = cfg.getInitMachNumber();
+        double vx = dir[0] / dm * a * M, vy = dir[1] / dm * a * M, vz = dir[2] / dm * a * M;
+        double rho = p / (Rg * T);
+        double e = p / ((gm - 1) * rho);
+        double vm = std::sqrt(vx * vx + vy * vy + vz * vz);
+        double et = e + 0.5 * (vm * vm);
+        for (long n = 0; n < N; ++n) {
+            U0[n] = rho; U0[N + n] = rho * vx; U0[2 * N + n] = rho * vy; U0[3 * N + n] = rho * vz; U0[4 * N + n] = rho * et;
+        }
+    }
+    HIPCHK(hipEventCreate(&cs->ev0));
+    HIPCHK(hipEventCreate(&cs->ev1));
+    int rc = tbfm_gpu_set_state(cs, U0.data());
+    if (rc != TBFM_OK) throw std::runtime_error(cs->err);
+}
+
+int fail(tbfm_case* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+template <typename F>
+int guarded(tbfm_case* c, F&& f) {
+    try {
+        return f();
+    } catch (const HipError& e) {
+        return fail(c, e.code == hipErrorOutOfMemory ? TBFM_ERR_OUT_OF_MEMORY : TBFM_ERR_HIP, e.what());
+    } catch (const std::bad_alloc&) {
+        return fail(c, TBFM_ERR_OUT_OF_MEMORY, "host out of memory");
+    } catch (const std::exception& e) {
+        return fail(c, TBFM_ERR_ARGUMENT, e.what());
+    }
+}
+
+// ------------------------------------------------------------------- launches
+template <int SCH, bool THREED>
+void launchStage(tbfm_case* cs, const StageDev& st) {
+    if (cs->precision == Precision::Mixed)
+        hipLaunchKernelGGL((dev::k_stage<SCH, THREED, float>), cs->grid_stage, dim3(dev::NT), 0, cs->stream, cs->c, st);
+    else
+        hipLaunchKernelGGL((dev::k_stage<SCH, THREED, double>), cs->grid_stage, dim3(dev::NT), 0, cs->stream, cs->c, st);
+}
+
+void stage(tbfm_case* cs, const double* Uin, double* Uout, double alpha, bool last) {
+    CaseDev& c = cs->c;
+    const long nb = (long)(2 * cs->nj * cs->nk + 2 * cs->ni * cs->nk + 2 * cs->ni * cs->nj);
+    hipLaunchKernelGGL(dev::k_boundary, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, cs->stream, c, Uin);
+    StageDev st{};
+    st.Uin = Uin;
+    st.U0 = cs->U[cs->u0];
+    st.Uout = Uout;
+    st.alpha = alpha;
+    st.last = last ? 1 : 0;
+    st.rper = cs->rper;
+    st.partial = cs->partial;
+    st.kc = cs->kc;
+    if (cs->prof) HIPCHK(hipEventRecord(cs->ev0, cs->stream));
+    const bool three = (cs->topo == Topology::ThreeD);
+    if (cs->scheme == Scheme::JST) {
+        if (three) launchStage<0, true>(cs, st); else launchStage<0, false>(cs, st);
+    } else {
+        if (three) launchStage<1, true>(cs, st); else launchStage<1, false>(cs, st);
+    }
+    if (cs->prof) {
+        HIPCHK(hipEventRecord(cs->ev1, cs->stream));
+        HIPCHK(hipEventSynchronize(cs->ev1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, cs->ev0, cs->ev1));
+        cs->prof_ms += ms;
+        cs->prof_launches += 1;
+    }
+    if (c.periodic && cs->topo == Topology::ThreeD) {
+        hipLaunchKernelGGL(dev::k_periodic, dim3((unsigned)cs->nparts_per), dim3(256), 0, cs->stream, c, st,
+                           cs->nparts_stage);
+    }
+    if (last)
+        hipLaunchKernelGGL(dev::k_norms, dim3(1), dim3(256), 0, cs->stream, c, (const double*)cs->partial,
+                           (int)(cs->nparts_stage + cs->nparts_per), cs->conv);
+    HIPCHK(hipGetLastError());
+}
+
+void timestep(tbfm_case* cs, const double* U) {
+    hipLaunchKernelGGL(dev::k_timestep, dim3((unsigned)cs->nblk_dt), dim3(256), 0, cs->stream, cs->c, U, cs->partial_min);
+    if (cs->c.dt_mode == 1)
+        hipLaunchKernelGGL(dev::k_min_reduce, dim3(1), dim3(256), 0, cs->stream, cs->c, (const double*)cs->partial_min,
+                           (int)cs->nblk_dt);
+}
+
+// One explicit iteration (solver.cpp:1337-1381); rotates the three state buffers.
+void iteration(tbfm_case* cs) {
+    const double* U0 = cs->U[cs->u0];
+    timestep(cs, U0);
+    if (cs->c.nprof > 0)
+        hipLaunchKernelGGL(dev::k_radial_profile, dim3(1), dim3(64), 0, cs->stream, cs->c, U0);
+    const int x = (cs->u0 + 1) % 3, y = (cs->u0 + 2) % 3;
+    const double* in = U0;
+    int out = x;
+    const size_t ns = cs->rk.size();
+    for (size_t s = 0; s < ns; ++s) {
+        stage(cs, in, cs->U[out], cs->rk[s], s + 1 == ns);
+        in = cs->U[out];
+        out = (out == x) ? y : x;
+    }
+    cs->u0 = (out == x) ? y : x;   // buffer written last
+    HIPCHK(hipGetLastError());
+}
+
+}  // namespace
+
+// =================================================================== C-ABI
+extern "C" {
+
+const char* tbfm_gpu_last_error(const tbfm_case* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+void tbfm_gpu_dims(const tbfm_case* c, long* ni, long* nj, long* nk) {
+    *ni = c->ni; *nj = c->nj; *nk = c->nk;
+}
+
+int tbfm_gpu_case_create(const tbfm_case_desc* d, tbfm_case** out) {
+    if (!d || !out) return fail(nullptr, TBFM_ERR_ARGUMENT, "null argument");
+    *out = nullptr;
+    auto cs = std::make_unique<tbfm_case>();
+    int rc = guarded(nullptr, [&]() {
+        GridInput g;
+        g.ni = d->ni; g.nj = d->nj; g.nk = d->nk;
+        const long N = d->ni * d->nj * d->nk;
+        if (N <= 0 || !d->x || !d->y || !d->z) throw std::invalid_argument("bad grid dimensions or coordinates");
+        g.x.assign(d->x, d->x + N); g.y.assign(d->y, d->y + N); g.z.assign(d->z, d->z + N);
+        for (int f = 0; f < d->nfields; ++f) g.fields[d->field_names[f]].assign(d->field_data[f], d->field_data[f] + N);
+        for (int q = 0; q < d->nconfig; ++q) cs->cfg.set(d->config_keys[q], d->config_values[q]);
+        for (int q = 0; q < d->npatches; ++q) {
+            Patch p;
+            p.name = d->patch_names[q];
+            const long* r = d->patch_ranges + 6 * q;
+            p.i0 = r[0]; p.i1 = r[1]; p.j0 = r[2]; p.j1 = r[3]; p.k0 = r[4]; p.k1 = r[5];
+            auto info = cs->cfg.getList(p.name);
+            if (info.empty()) throw std::runtime_error("Boundary info for " + p.name + " is missing in configuration.");
+            p.type = patchTypeFromString(info[0]);
+            for (size_t m = 1; m < info.size(); ++m) p.values.push_back(std::stod(info[m]));
+            cs->patches.push_back(p);
+        }
+        setup(cs.get(), g);
+        return TBFM_OK;
+    });
+    if (rc != TBFM_OK) {
+        tbfm_gpu_case_destroy(cs.release());
+        return rc;
+    }
+    *out = cs.release();
+    return TBFM_OK;
+}
+
+int tbfm_gpu_case_create_from_ini(const char* ini, tbfm_case** out) {
+    if (!ini || !out) return fail(nullptr, TBFM_ERR_ARGUMENT, "null argument");
+    *out = nullptr;
+    auto cs = std::make_unique<tbfm_case>();
+    int rc = guarded(nullptr, [&]() {
+        cs->cfg = Config(ini);
+        auto resolve = [&](const std::string& p) { return (!p.empty() && p[0] == '/') ? p : cs->cfg.baseDir + "/" + p; };
+        GridInput g = readGridCsv(resolve(cs->cfg.getGridFile()));
+        cs->patches = readBoundaryCsv(resolve(cs->cfg.getBoundaryFile()), cs->cfg);
+        setup(cs.get(), g);
+        return TBFM_OK;
+    });
+    if (rc != TBFM_OK) {
+        tbfm_gpu_case_destroy(cs.release());
+        return rc;
+    }
+    *out = cs.release();
+    return TBFM_OK;
+}
+
+int tbfm_gpu_case_destroy(tbfm_case* c) {
+    if (!c) return TBFM_OK;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->graph) hipGraphExecDestroy(c->graph);
+    for (void* p : c->owned) hipFree(p);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return TBFM_OK;
+}
+
+int tbfm_gpu_set_state(tbfm_case* c, const double* U) {
+    if (!c || !U) return fail(c, TBFM_ERR_ARGUMENT, "null argument");
+    return guarded(c, [&]() {
+        std::vector<double> dU(5 * c->N), tmp;
+        for (int v = 0; v < 5; ++v) {
+            toDevice(c, U + v * c->N, tmp);
+            std::copy(tmp.begin(), tmp.end(), dU.begin() + v * c->N);
+        }
+        HIPCHK(hipMemcpyAsync(c->U[c->u0], dU.data(), dU.size() * 8, hipMemcpyHostToDevice, c->stream));
+        // radial-equilibrium profiles from the new state (solver.cpp:818-834)
+        std::vector<double> prof(dev::MAX_PROF * c->nj, 0.0);
+        const double g = c->c.gamma;
+        for (int q = 0; q < c->c.nprof; ++q)
+            for (int m = 0; m < c->c.prof_n[q]; ++m) {
+                long n = ((long)c->c.prof_i[q] * c->nj + c->c.prof_j0[q] + m) * c->nk + 0;
+                double rho = U[n], W1 = U[c->N + n] / rho, W2 = U[2 * c->N + n] / rho, W3 = U[3 * c->N + n] / rho,
+                       W4 = U[4 * c->N + n] / rho;
+                double vm = std::sqrt(W1 * W1 + W2 * W2 + W3 * W3);
+                double e = W4 - 0.5 * (vm * vm);
+                prof[q * c->nj + m] = (g - 1) * rho * e;
+            }
+        HIPCHK(hipMemcpyAsync(c->c.prof_p, prof.data(), prof.size() * 8, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return TBFM_OK;
+    });
+}
+
+int tbfm_gpu_get_state(tbfm_case* c, double* U) {
+    if (!c || !U) return fail(c, TBFM_ERR_ARGUMENT, "null argument");
+    return guarded(c, [&]() {
+        std::vector<double> dU(5 * c->N);
+        HIPCHK(hipMemcpyAsync(dU.data(), c->U[c->u0], dU.size() * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int v = 0; v < 5; ++v) toHost(c, dU.data() + v * c->N, U + v * c->N);
+        return TBFM_OK;
+    });
+}
+
+int tbfm_gpu_advance(tbfm_case* c, long n, double* logres, long* converged_at) {
+    if (!c || n < 0) return fail(c, TBFM_ERR_ARGUMENT, "bad argument");
+    return guarded(c, [&]() {
+        const long first = c->iter_host;
+        if (first + n - 1 > c->hist_cap) {
+            // grow the residual history on the device
+            long cap = std::max(first + n - 1, 2 * c->hist_cap);
+            double* h = dalloc<double>(5 * cap, c->owned);
+            HIPCHK(hipMemcpyAsync(h, c->conv.hist, 5 * c->hist_cap * 8, hipMemcpyDeviceToDevice, c->stream));
+            c->conv.hist = h;
+            c->conv.hist_len = cap;
+            c->hist_cap = cap;
+        }
+        for (long q = 0; q < n; ++q) iteration(c);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->iter_host += n;
+        long conv = 0;
+        int div = 0;
+        HIPCHK(hipMemcpy(&conv, c->conv.conv_at, sizeof(long), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(&div, c->conv.diverged, sizeof(int), hipMemcpyDeviceToHost));
+        if (logres) {
+            long last = first + n - 1;
+            if (conv > 0 && conv < last) last = conv;
+            long rows = std::max<long>(last - first + 1, 0);
+            std::vector<double> h(5 * c->hist_cap);
+            HIPCHK(hipMemcpy(h.data(), c->conv.hist, h.size() * 8, hipMemcpyDeviceToHost));
+            for (long r = 0; r < rows; ++r)
+                for (int v = 0; v < 5; ++v) logres[r * 5 + v] = h[(first - 1 + r) * 5 + v];
+        }
+        if (converged_at) *converged_at = conv;
+        if (div) {
+            c->err = "Solver diverged: NaN or Inf detected in residual variable " + std::to_string(div - 1);
+            return TBFM_ERR_DIVERGED;
+        }
+        return TBFM_OK;
+    });
+}
+
+int tbfm_gpu_compute_timestep(tbfm_case* c, const double* U, double* dt) {
+    if (!c || !U || !dt) return fail(c, TBFM_ERR_ARGUMENT, "null argument");
+    return guarded(c, [&]() {
+        std::vector<double> dU(5 * c->N), tmp;
+        for (int v = 0; v < 5; ++v) {
+            toDevice(c, U + v * c->N, tmp);
+            std::copy(tmp.begin(), tmp.end(), dU.begin() + v * c->N);
+        }
+        int scratch = (c->u0 + 1) % 3;
+        HIPCHK(hipMemcpy(c->U[scratch], dU.data(), dU.size() * 8, hipMemcpyHostToDevice));
+        timestep(c, c->U[scratch]);
+        std::vector<double> d(c->N);
+        if (c->c.dt_mode == 1) {
+            double m = 0;
+            HIPCHK(hipMemcpyAsync(&m, c->c.dtmin, 8, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            std::fill(d.begin(), d.end(), m);
+        } else {
+            HIPCHK(hipMemcpyAsync(d.data(), c->c.dt, d.size() * 8, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
+        toHost(c, d.data(), dt);
+        return TBFM_OK;
+    });
+}
+
+int tbfm_gpu_compute_residual(tbfm_case* c, const double* U, long it, int update_profiles, double* R) {
+    if (!c || !U || !R) return fail(c, TBFM_ERR_ARGUMENT, "null argument");
+    return guarded(c, [&]() {
+        // R = (U0 - Uout) / (alpha dt / V) is ill-conditioned; instead run the stage with
+        // U0 = 0, alpha = -1 and dt = V so that Uout = R exactly:
+        //   factor = (-1 * V) / V = -1  ->  Uout = 0 - R * (-1) = R
+        std::vector<double> dU(5 * c->N), tmp;
+        for (int v = 0; v < 5; ++v) {
+            toDevice(c, U + v * c->N, tmp);
+            std::copy(tmp.begin(), tmp.end(), dU.begin() + v * c->N);
+        }
+        const int a = (c->u0 + 1) % 3, b = (c->u0 + 2) % 3;
+        HIPCHK(hipMemcpy(c->U[a], dU.data(), dU.size() * 8, hipMemcpyHostToDevice));
+        long itv = it;
+        int zero = 0;
+        HIPCHK(hipMemcpy(c->c.iter, &itv, sizeof(long), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->c.stop, &zero, sizeof(int), hipMemcpyHostToDevice));
+        if (update_profiles && c->c.nprof > 0)
+            hipLaunchKernelGGL(dev::k_radial_profile, dim3(1), dim3(64), 0, c->stream, c->c, (const double*)c->U[a]);
+        // dt := V (device layout), U0 := 0
+        CaseDev saved = c->c;
+        double* dtv = dalloc<double>(c->N, c->owned);
+        HIPCHK(hipMemcpy(dtv, c->c.vol, c->N * 8, hipMemcpyDeviceToDevice));
+        double* zeros = dalloc<double>(5 * c->N, c->owned);
+        c->c.dt = dtv;
+        c->c.dt_mode = 0;
+        int u0save = c->u0;
+        double* u0buf = c->U[c->u0];
+        c->U[c->u0] = zeros;
+        stage(c, c->U[a], c->U[b], -1.0, false);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->U[c->u0] = u0buf;
+        c->u0 = u0save;
+        c->c = saved;
+        std::vector<double> d(5 * c->N);
+        HIPCHK(hipMemcpy(d.data(), c->U[b], d.size() * 8, hipMemcpyDeviceToHost));
+        for (int v = 0; v < 5; ++v) toHost(c, d.data() + v * c->N, R + v * c->N);
+        // restore the iteration counter
+        itv = c->iter_host;
+        HIPCHK(hipMemcpy(c->c.iter, &itv, sizeof(long), hipMemcpyHostToDevice));
+        // free the temporaries
+        hipFree(dtv);
+        hipFree(zeros);
+        c->owned.pop_back();
+        c->owned.pop_back();
+        return TBFM_OK;
+    });
+}
+
+int tbfm_gpu_get_side_outputs(tbfm_case* c, double* finv, double* fvisc, double* devang, double* dt) {
+    if (!c) return fail(c, TBFM_ERR_ARGUMENT, "null argument");
+    return guarded(c, [&]() {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        std::vector<double> h3(3 * c->N), h(c->N), comp(c->N), out(c->N);
+        auto vec3 = [&](const double* d, double* dst) {
+            HIPCHK(hipMemcpy(h3.data(), d, h3.size() * 8, hipMemcpyDeviceToHost));
+            for (int q = 0; q < 3; ++q) {
+                for (long n = 0; n < c->N; ++n) comp[n] = h3[3 * n + q];
+                toHost(c, comp.data(), out.data());
+                for (long n = 0; n < c->N; ++n) dst[3 * n + q] = out[n];
+            }
+        };
+        if (finv) vec3(c->c.finv, finv);
+        if (fvisc) vec3(c->c.fvisc, fvisc);
+        if (devang) {
+            HIPCHK(hipMemcpy(h.data(), c->c.dev, h.size() * 8, hipMemcpyDeviceToHost));
+            toHost(c, h.data(), devang);
+        }
+        if (dt) {
+            if (c->c.dt_mode == 1) {
+                double m = 0;
+                HIPCHK(hipMemcpy(&m, c->c.dtmin, 8, hipMemcpyDeviceToHost));
+                std::fill(dt, dt + c->N, m);
+            } else {
+                HIPCHK(hipMemcpy(h.data(), c->c.dt, h.size() * 8, hipMemcpyDeviceToHost));
+                toHost(c, h.data(), dt);
+            }
+        }
+        return TBFM_OK;
+    });
+}
+
+int tbfm_gpu_get_turbo(tbfm_case* c, double* out) {
+    if (!c || !out) return fail(c, TBFM_ERR_ARGUMENT, "null argument");
+    return guarded(c, [&]() {
+        // boundary slices are small: evaluate on the host from the device state
+        std::vector<double> U(5 * c->N);
+        int rc = tbfm_gpu_get_state(c, U.data());
+        if (rc != TBFM_OK) return rc;
+        const long NI = c->ni, NJ = c->nj, NK = c->nk, N = c->N;
+        const Geometry& g = c->geo;
+        const double gm = c->c.gamma, Rg = c->c.rgas;
+        auto at = [&](long i, long j, long k) { return (i * NJ + j) * NK + k; };
+        auto surf = [&](int b, long p, long q) -> const double* {
+            if (b == 0) return &g.surfI[((0 * NJ + p) * NK + q) * 3];
+            if (b == 1) return &g.surfI[((NI * NJ + p) * NK + q) * 3];
+            if (b == 2) return &g.surfJ[((p * (NJ + 1) + 0) * NK + q) * 3];
+            if (b == 3) return &g.surfJ[((p * (NJ + 1) + NJ) * NK + q) * 3];
+            if (b == 4) return &g.surfK[((p * NJ + q) * (NK + 1) + 0) * 3];
+            return &g.surfK[((p * NJ + q) * (NK + 1) + NK) * 3];
+        };
+        auto node = [&](int b, long p, long q) {
+            if (b == 0) return at(0, p, q);
+            if (b == 1) return at(NI - 1, p, q);
+            if (b == 2) return at(p, 0, q);
+            if (b == 3) return at(p, NJ - 1, q);
+            if (b == 4) return at(p, q, 0);
+            return at(p, q, NK - 1);
+        };
+        auto extent = [&](int b, long& n1, long& n2) {
+            if (b < 2) { n1 = NJ; n2 = NK; } else if (b < 4) { n1 = NI; n2 = NK; } else { n1 = NI; n2 = NJ; }
+        };
+        double mf[6];
+        for (int b = 0; b < 6; ++b) {
+            long n1, n2;
+            extent(b, n1, n2);
+            double s = 0.0;
+            for (long p = 0; p < n1; ++p)
+                for (long q = 0; q < n2; ++q) {
+                    const double* S = surf(b, p, q);
+                    long n = node(b, p, q);
+                    s += S[0] * U[N + n] + S[1] * U[2 * N + n] + S[2] * U[3 * N + n];
+                }
+            mf[b] = s;
+            out[b] = s;
+        }
+        int td = c->cfg.getTurboDirection();
+        int bi[2] = {0, 1};
+        if (td == 1) { bi[0] = 1; bi[1] = 0; } else if (td == 2) { bi[0] = 2; bi[1] = 3; } else if (td == 3) { bi[0] = 3; bi[1] = 2; }
+        double m = 0.5 * (std::abs(mf[bi[0]]) + std::abs(mf[bi[1]]));
+        double perDeg = 0.0;
+        for (auto& p : c->patches)
+            if (p.type == PatchType::Periodic && p.values[0] == 0) perDeg = (p.values[2] == 360.0) ? 0.0 : p.values[2];
+        if (c->topo == Topology::Axisymmetric) m *= 2.0 * M_PI / g.wedgeAngle;
+        else if (perDeg != 0.0) m *= 360.0 / perDeg;
+        out[6] = m;
+        double ptm[2], ttm[2];
+        for (int s = 0; s < 2; ++s) {
+            int b = bi[s];
+            long n1, n2;
+            extent(b, n1, n2);
+            double sp = 0.0, st = 0.0;
+            for (long p = 0; p < n1; ++p)
+                for (long q = 0; q < n2; ++q) {
+                    const double* S = surf(b, p, q);
+                    long n = node(b, p, q);
+                    double rho = U[n], ux = U[N + n] / rho, uy = U[2 * N + n] / rho, uz = U[3 * N + n] / rho,
+                           et = U[4 * N + n] / rho;
+                    double um = std::sqrt(ux * ux + uy * uy + uz * uz);
+                    double e = et - 0.5 * (um * um);
+                    double pr = (gm - 1) * rho * e;
+                    double a = std::sqrt(gm * pr / rho);
+                    double M = um / a;
+                    double pt = pr * std::pow(1.0 + (gm - 1.0) / 2.0 * M * M, gm / (gm - 1));
+                    double T = pr / (rho * Rg);
+                    double Tt = T * (1.0 + (gm - 1.0) / 2.0 * M * M);
+                    sp += S[0] * (rho * ux * pt) + S[1] * (rho * uy * pt) + S[2] * (rho * uz * pt);
+                    st += S[0] * (rho * ux * Tt) + S[1] * (rho * uy * Tt) + S[2] * (rho * uz * Tt);
+                }
+            double mfl = mf[b];
+            if (std::abs(mfl) < 1e-12) mfl = (mfl >= 0.0 ? 1e-12 : -1e-12);
+            ptm[s] = sp / mfl;
+            ttm[s] = st / mfl;
+        }
+        double PR = (std::abs(ptm[0]) > 1e-12) ? ptm[1] / ptm[0] : 1.0;
+        double TR = (std::abs(ttm[0]) > 1e-12) ? ttm[1] / ttm[0] : 1.0;
+        out[7] = PR;
+        out[8] = TR;
+        double dT = TR - 1.0, dTs = std::pow(PR, (gm - 1.0) / gm) - 1.0, eta;
+        if (std::abs(dT) < 1e-12) eta = 1.0;
+        else if (dT >= 0.0) eta = dTs / dT;
+        else eta = (std::abs(dTs) < 1e-12) ? 1.0 : dT / dTs;
+        out[9] = eta;
+        return TBFM_OK;
+    });
+}
+
+int tbfm_gpu_get_geometry(tbfm_case* c, int which, double* out) {
+    if (!c || !out) return fail(c, TBFM_ERR_ARGUMENT, "null argument");
+    const Geometry& g = c->geo;
+    const std::vector<double>* src = nullptr;
+    switch (which) {
+        case 0: src = &g.surfI; break;
+        case 1: src = &g.surfJ; break;
+        case 2: src = &g.surfK; break;
+        case 3: src = &g.volume; break;
+        case 4: src = &g.gradBlockage; break;
+        default: return fail(c, TBFM_ERR_ARGUMENT, "unknown geometry array");
+    }
+    if (src->empty()) return fail(c, TBFM_ERR_ARGUMENT, "geometry array not available");
+    std::memcpy(out, src->data(), src->size() * 8);
+    return TBFM_OK;
+}
+
+int tbfm_gpu_profile(tbfm_case* c, int enable, int reset) {
+    if (!c) return fail(c, TBFM_ERR_ARGUMENT, "null argument");
+    c->prof = enable != 0;
+    if (reset) { c->prof_ms = 0.0; c->prof_launches = 0; }
+    return TBFM_OK;
+}
+
+int tbfm_gpu_profile_get(tbfm_case* c, double* ms, long* launches, double* bytes) {
+    if (!c) return fail(c, TBFM_ERR_ARGUMENT, "null argument");
+    if (ms) *ms = c->prof_ms;
+    if (launches) *launches = c->prof_launches;
+    if (bytes) *bytes = c->bytes_per_node * (double)c->N;
+    return TBFM_OK;
+}
+
+int tbfm_gpu_sync(tbfm_case* c) {
+    if (!c) return fail(c, TBFM_ERR_ARGUMENT, "null argument");
+    return guarded(c, [&]() {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return TBFM_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,410 @@
+// Point physics on the device: ideal-gas relations, face fluxes, boundary fluxes and
+// body-force sources.  Each function names the reference lines it reproduces; the
+// operation order is the reference's so that FP64 results are bit-identical.
+#pragma once
+#include "tbfm_kernels.hpp"
+
+namespace tbfm {
+namespace dev {
+
+// ---------------------------------------------------------------- ideal gas
+// static energy et - 0.5*pow(|u|,2) with |u| = sqrt(u.u)   (src/fluid_ideal.cpp:70-77)
+template <typename T>
+__device__ __forceinline__ T static_energy(T u, T v, T w, T et) {
+    T m = sqrt(u * u + v * v + w * w);
+    return et - T(0.5) * (m * m);
+}
+
+// W = (rho, rhoU/rho, rhoV/rho, rhoW/rho, rhoE/rho) (src/math_utils.cpp:49-57), then
+// p = (g-1) rho e (fluid_ideal.cpp:22-24) and the JST spectral factor |u| + a with
+// a = sqrt(g p / rho) (advection_jst.cpp:40-45, fluid_ideal.cpp:83-87)
+template <typename T>
+__device__ __forceinline__ void derive(const T U[5], T W[5], T& p, T& rf, T g, T gm1) {
+    W[0] = U[0];
+    W[1] = U[1] / U[0];
+    W[2] = U[2] / U[0];
+    W[3] = U[3] / U[0];
+    W[4] = U[4] / U[0];
+    T m = sqrt(W[1] * W[1] + W[2] * W[2] + W[3] * W[3]);
+    T e = W[4] - T(0.5) * (m * m);
+    p = gm1 * W[0] * e;
+    T a = sqrt(g * p / W[0]);
+    rf = m + a;
+}
+
+// Physical flux through S of primitive state W (src/math_utils.cpp:71-86).  Returns
+// |S| so the caller can scale by it, as the residual assembly does.
+template <typename T>
+__device__ __forceinline__ T phys_flux(const T W[5], T sx, T sy, T sz, T gm1, T F[5]) {
+    T sm = sqrt(sx * sx + sy * sy + sz * sz);
+    T nx = sx / sm, ny = sy / sm, nz = sz / sm;
+    T un = W[1] * nx + W[2] * ny + W[3] * nz;
+    T e = static_energy(W[1], W[2], W[3], W[4]);
+    T p = gm1 * W[0] * e;
+    T ht = W[4] + p / W[0];
+    F[0] = W[0] * un;
+    F[1] = W[0] * un * W[1] + p * nx;
+    F[2] = W[0] * un * W[2] + p * ny;
+    F[3] = W[0] * un * W[3] + p * nz;
+    F[4] = W[0] * un * ht;
+    return sm;
+}
+
+// JST pressure sensor |p1 - 2 p2 + p3| / (p1 + 2 p2 + p3)   (advection_jst.cpp:47-57)
+template <typename T>
+__device__ __forceinline__ T jst_sensor(T p1, T p2, T p3) {
+    return fabs((p1 - 2 * p2 + p3) / (p1 + 2 * p2 + p3));
+}
+
+// JST face flux times |S| (advection_jst.cpp:4-38; k2 = 0.5, k4 = 0.02, c4 = 2,
+// advection_jst.hpp:23).  Conserved differences are formed in double; the flux type FT
+// carries the rest.
+template <typename FT>
+__device__ __forceinline__ void jst_face(const double Ull[5], const double Ul[5], const double Ur[5],
+                                         const double Urr[5], const double Wl[5], const double Wr[5],
+                                         double rfl, double rfr, double sl, double sr, double sx, double sy,
+                                         double sz, double gm1, double out[5]) {
+    FT Wavg[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) Wavg[v] = (FT(Wl[v]) + FT(Wr[v])) * FT(0.5);
+    FT r = FT(dmaxd(rfl, rfr));
+    FT s = FT(dmaxd(sl, sr));
+    FT psi2 = FT(0.5) * s * r;
+    FT t4 = FT(0.02) * r - FT(2.0) * psi2;
+    FT psi4 = (FT(0.0) < t4) ? t4 : FT(0.0);
+    FT F[5];
+    FT sm = phys_flux<FT>(Wavg, FT(sx), FT(sy), FT(sz), FT(gm1), F);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        double d1 = Ur[v] - Ul[v];
+        double d3 = ((Urr[v] - Ur[v]) - d1 * 2) + (Ul[v] - Ull[v]);
+        FT dis = FT(d1) * psi2 - FT(d3) * psi4;
+        F[v] -= dis;
+        out[v] = double(F[v] * sm);
+    }
+}
+
+// MUSCL reconstruction of primitives (advection_base.cpp:10-65)
+template <typename T>
+__device__ __forceinline__ T muscl_limiter(int lim, T s) {
+    switch (lim) {
+        case 1: return (s <= T(0)) ? T(0) : (s * s + s) / (T(1) + s * s);
+        case 2: return (s + fabs(s)) / (T(1) + fabs(s));
+        case 3: {
+            T m = (T(1) < s) ? T(1) : s;   // std::min(1, s)
+            return (T(0) < m) ? m : T(0);  // std::max(0, m)
+        }
+        default: return T(0);
+    }
+}
+template <typename T>
+__device__ __forceinline__ T muscl_ratio(T l, T c, T r) {
+    T dR = r - c;
+    T dL = c - l;
+    return (fabs(dR) < T(1E-12)) ? T(0) : dL / dR;
+}
+
+template <typename T>
+__device__ __forceinline__ T roe_avg(T rl, T rr, T pl, T pr) {
+    return (sqrt(rl) * pl + sqrt(rr) * pr) / (sqrt(rl) + sqrt(rr));
+}
+
+// Roe flux with Vinokur-Montagne averaging and entropy fix 0.01, times |S|
+// (src/advection_roe.cpp:6-221, include/advection_roe.hpp:63)
+template <typename FT>
+__device__ __forceinline__ void roe_face(const double Wll_[5], const double Wl_[5], const double Wr_[5],
+                                         const double Wrr_[5], int muscl, int lim, double sx_, double sy_,
+                                         double sz_, double g_, double out[5]) {
+    const FT g = FT(g_), gm1 = FT(g_ - 1.0);
+    FT Wl[5], Wr[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) { Wl[v] = FT(Wl_[v]); Wr[v] = FT(Wr_[v]); }
+    if (muscl) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+            FT sL = muscl_ratio<FT>(FT(Wll_[v]), FT(Wl_[v]), FT(Wr_[v]));
+            FT sR = muscl_ratio<FT>(FT(Wl_[v]), FT(Wr_[v]), FT(Wrr_[v]));
+            FT lL = muscl_limiter<FT>(lim, sL), lR = muscl_limiter<FT>(lim, sR);
+            FT wl = FT(Wl_[v]), wr = FT(Wr_[v]), wrr = FT(Wrr_[v]);
+            Wl[v] = wl + lL * (wr - wl) * FT(0.5);
+            Wr[v] = wr - lR * (wrr - wr) * FT(0.5);
+        }
+    }
+    const FT sx = FT(sx_), sy = FT(sy_), sz = FT(sz_);
+    // normal triad (:41-58)
+    FT sm = sqrt(sx * sx + sy * sy + sz * sz);
+    FT n1x, n1y, n1z;
+    if (sm == FT(0)) { n1x = n1y = n1z = FT(0); }
+    else { n1x = sx / sm; n1y = sy / sm; n1z = sz / sm; }
+    FT t2x, t2y, t2z;
+    if (n1x == FT(0) && n1y == FT(0)) { t2x = FT(1); t2y = FT(0); t2z = FT(0); }
+    else { t2x = -n1y; t2y = n1x; t2z = FT(0); }
+    FT m2 = sqrt(t2x * t2x + t2y * t2y + t2z * t2z);
+    FT n2x, n2y, n2z;
+    if (m2 == FT(0)) { n2x = n2y = n2z = FT(0); }
+    else { n2x = t2x / m2; n2y = t2y / m2; n2z = t2z / m2; }
+    FT n3x = n1y * n2z - n1z * n2y, n3y = n1z * n2x - n1x * n2z, n3z = n1x * n2y - n1y * n2x;
+    // rotated primitives (:60-64)
+    FT L[5], R[5];
+    L[0] = Wl[0]; L[4] = Wl[4];
+    L[1] = Wl[1] * n1x + Wl[2] * n1y + Wl[3] * n1z;
+    L[2] = Wl[1] * n2x + Wl[2] * n2y + Wl[3] * n2z;
+    L[3] = Wl[1] * n3x + Wl[2] * n3y + Wl[3] * n3z;
+    R[0] = Wr[0]; R[4] = Wr[4];
+    R[1] = Wr[1] * n1x + Wr[2] * n1y + Wr[3] * n1z;
+    R[2] = Wr[1] * n2x + Wr[2] * n2y + Wr[3] * n2z;
+    R[3] = Wr[1] * n3x + Wr[2] * n3y + Wr[3] * n3z;
+    // Roe averages (:66-144)
+    const FT rhoL = L[0], rhoR = R[0];
+    FT pLt = gm1 * rhoL * static_energy(L[1], L[2], L[3], L[4]);
+    FT pRt = gm1 * rhoR * static_energy(R[1], R[2], R[3], R[4]);
+    FT htL = L[4] + pLt / rhoL, htR = R[4] + pRt / rhoR;
+    FT rhoA = sqrt(rhoL * rhoR);
+    FT u1A = roe_avg(rhoL, rhoR, L[1], R[1]);
+    FT u2A = roe_avg(rhoL, rhoR, L[2], R[2]);
+    FT u3A = roe_avg(rhoL, rhoR, L[3], R[3]);
+    FT htA = roe_avg(rhoL, rhoR, htL, htR);
+    FT eL = L[4] - FT(0.5) * (L[1] * L[1] + L[2] * L[2] + L[3] * L[3]);
+    FT eR = R[4] - FT(0.5) * (R[1] * R[1] + R[2] * R[2] + R[3] * R[3]);
+    FT pL = gm1 * rhoL * eL, pR = gm1 * rhoR * eR;
+    FT chiL = (g - FT(1.0)) * eL, kapL = (g - FT(1.0)) * rhoL;
+    FT chiR = (g - FT(1.0)) * eR, kapR = (g - FT(1.0)) * rhoR;
+    FT barChi = FT(0.5) * (chiL + chiR), barKap = FT(0.5) * (kapL + kapR);
+    FT barRho = FT(0.5) * (rhoL + rhoR), barE = FT(0.5) * (eL + eR);
+    FT dP = pR - pL, dRho = rhoR - rhoL, dE = eR - eL;
+    FT dRhoHat = (barRho > FT(1e-12)) ? dRho / barRho : FT(0.0);
+    FT dEHat = (fabs(barE) > FT(1e-12)) ? dE / fabs(barE) : FT(0.0);
+    FT dNorm2 = dRhoHat * dRhoHat + dEHat * dEHat;
+    FT chiT = barChi, kapT = barKap;
+    if (dNorm2 > FT(1e-12)) {
+        FT res = dP - (barChi * dRho + barKap * dE);
+        chiT += (res * dRhoHat) / (dNorm2 * barRho);
+        kapT += (res * dEHat) / (dNorm2 * fabs(barE));
+    }
+    if (barKap > FT(0.0) && kapT <= FT(0.0)) kapT = FT(0.01) * barKap;
+    FT kapP = (rhoA > FT(1e-12)) ? kapT / rhoA : FT(0.4);
+    FT eA = roe_avg(rhoL, rhoR, eL, eR);
+    FT chiP = chiT - kapP * eA;
+    FT q2 = u1A * u1A + u2A * u2A + u3A * u3A;
+    FT hA = htA - FT(0.5) * q2;
+    FT a2 = chiP + kapP * hA;
+    if (a2 <= FT(0.0)) {
+        FT aL = sqrt(g * (g - FT(1.0)) * eL), aR = sqrt(g * (g - FT(1.0)) * eR);
+        a2 = FT(0.5) * (aL * aL + aR * aR);
+    }
+    FT aA = sqrt(a2);
+    // eigen-system (:151-167)
+    FT lam[5] = {u1A - aA, u1A, u1A, u1A, u1A + aA};
+    FT e5 = htA - (aA * aA) / kapP;
+    FT rv[5][5] = {{FT(1.0), u1A - aA, u2A, u3A, htA - aA * u1A},
+                   {FT(1.0), u1A, u2A, u3A, e5},
+                   {FT(0.0), FT(0.0), FT(1.0), FT(0.0), u2A},
+                   {FT(0.0), FT(0.0), FT(0.0), FT(1.0), u3A},
+                   {FT(1.0), u1A + aA, u2A, u3A, htA + aA * u1A}};
+    // wave strengths (:169-182)
+    FT dRho2 = R[0] - L[0], dU1 = R[1] - L[1], dU2 = R[2] - L[2], dU3 = R[3] - L[3];
+    FT dP2 = pRt - pLt;
+    FT w[5];
+    w[0] = FT(1.0) / (FT(2.0) * aA * aA) * (dP2 - rhoA * aA * dU1);
+    w[1] = dRho2 - dP2 / (aA * aA);
+    w[2] = rhoA * dU2;
+    w[3] = rhoA * dU3;
+    w[4] = FT(1.0) / (FT(2.0) * aA * aA) * (dP2 + rhoA * aA * dU1);
+    // assembly (:184-221): physical fluxes along (1,0,0) in the rotated frame
+    FT FL[5], FR[5];
+    FT one = FT(1.0), zero = FT(0.0);
+    phys_flux<FT>(L, one, zero, zero, gm1, FL);
+    phys_flux<FT>(R, one, zero, zero, gm1, FR);
+    FT Fr[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) Fr[i] = (FL[i] + FR[i]) * FT(0.5);
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+            FT fe = (fabs(lam[j]) < FT(0.01)) ? FT(0.01) : fabs(lam[j]);
+            Fr[i] -= FT(0.5) * w[j] * fe * rv[j][i];
+        }
+    FT F[5];
+    F[0] = Fr[0];
+    F[1] = Fr[1] * (n1x * one + n1y * zero + n1z * zero) + Fr[2] * (n2x * one + n2y * zero + n2z * zero) +
+           Fr[3] * (n3x * one + n3y * zero + n3z * zero);
+    F[2] = Fr[1] * (n1x * zero + n1y * one + n1z * zero) + Fr[2] * (n2x * zero + n2y * one + n2z * zero) +
+           Fr[3] * (n3x * zero + n3y * one + n3z * zero);
+    F[3] = Fr[1] * (n1x * zero + n1y * zero + n1z * one) + Fr[2] * (n2x * zero + n2y * zero + n2z * one) +
+           Fr[3] * (n3x * zero + n3y * zero + n3z * one);
+    F[4] = Fr[4];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) out[v] = double(F[v] * sm);
+}
+
+// -------------------------------------------------------------- boundary fluxes
+// Subsonic inlet, ideal gas (src/boundary_base.cpp:7-124)
+__device__ inline void inlet_flux(const CaseDev& c, const double U[5], double sx, double sy, double sz,
+                                  double pt, double Tt, double dx, double dy, double dz, double F[5]) {
+    const double g = c.gamma, Rg = c.rgas;
+    double W[5], p, rf;
+    derive<double>(U, W, p, rf, c.gamma, c.gm1);
+    double aInt = rf - sqrt(W[1] * W[1] + W[2] * W[2] + W[3] * W[3]);
+    // sound speed recomputed exactly as computeSoundSpeed_rho_u_et
+    aInt = sqrt(g * p / W[0]);
+    double sm = sqrt(sx * sx + sy * sy + sz * sz);
+    double nx = sx / sm, ny = sy / sm, nz = sz / sm;
+    double unInt = W[1] * nx + W[2] * ny + W[3] * nz;
+    double Z = W[0] * aInt;
+    double Cg = dx * nx + dy * ny + dz * nz;
+    if (Cg >= 0.0) Cg = -1.0;
+    double Rout = p + Z * unInt;
+    double cp = g * Rg / (g - 1.0);
+    double htIn = cp * Tt;
+    double pLow = 0.05 * pt, pHigh = 0.999999 * pt;
+    double pB = 0.5 * (pLow + pHigh);
+    const double ex = (g - 1.0) / g;
+    for (int it = 0; it < 30; ++it) {
+        double pMid = 0.5 * (pLow + pHigh);
+        double TMid = Tt * pow(pMid / pt, ex);
+        double hMid = cp * TMid;
+        double VMid = sqrt(2.0 * dmaxd(0.0, htIn - hMid));
+        double res = pMid + Z * Cg * VMid - Rout;
+        if (fabs(res) < 1e-4 * pt) { pB = pMid; break; }
+        if (res < 0.0) pLow = pMid; else pHigh = pMid;
+        pB = pMid;
+    }
+    double TB = Tt * pow(pB / pt, ex);
+    double hB = cp * TB;
+    double VB = sqrt(2.0 * dmaxd(0.0, htIn - hB));
+    double vx = dx * VB, vy = dy * VB, vz = dz * VB;
+    double rhoB = pB / (Rg * TB);
+    double eB = pB / ((g - 1) * rhoB);
+    double etB = eB + 0.5 * (vx * vx + vy * vy + vz * vz);
+    double Wb[5] = {rhoB, vx, vy, vz, etB};
+    phys_flux<double>(Wb, sx, sy, sz, c.gm1, F);
+}
+
+// Outlet with static pressure and optional ramp (boundary_base.cpp:181-224,
+// config.cpp:430-439)
+__device__ inline void outlet_flux(const CaseDev& c, const double U[5], double sx, double sy, double sz,
+                                   double pout, long it, double F[5]) {
+    double W[5], p, rf;
+    derive<double>(U, W, p, rf, c.gamma, c.gm1);
+    double a = sqrt(c.gamma * p / W[0]);
+    double m = sqrt(W[1] * W[1] + W[2] * W[2] + W[3] * W[3]);
+    if (m >= a) {
+        phys_flux<double>(W, sx, sy, sz, c.gm1, F);
+        return;
+    }
+    double ratio = double(it) / c.ramp_iter;
+    double pB = (ratio >= 1.0) ? pout : c.init_p + (pout - c.init_p) * ratio;
+    double rhoB = pB * W[0] / p;
+    double eB = pB / ((c.gamma - 1) * rhoB);
+    double etB = eB + 0.5 * (W[1] * W[1] + W[2] * W[2] + W[3] * W[3]);
+    double Wb[5] = {rhoB, W[1], W[2], W[3], etB};
+    phys_flux<double>(Wb, sx, sy, sz, c.gm1, F);
+}
+
+// Inviscid wall (src/boundary_inviscid_wall.cpp:5-30)
+__device__ inline void wall_flux(const CaseDev& c, const double U[5], double sx, double sy, double sz, double F[5]) {
+    double W[5], p, rf;
+    derive<double>(U, W, p, rf, c.gamma, c.gm1);
+    double sm = sqrt(sx * sx + sy * sy + sz * sz);
+    double nx = sx / sm, ny = sy / sm, nz = sz / sm;
+    double un = W[1] * nx + W[2] * ny + W[3] * nz;
+    double a = sqrt(c.gamma * p / W[0]);
+    double pw = p + W[0] * un * a;
+    if (pw < 0.01 * p) pw = 0.01 * p;
+    F[0] = 0.0; F[1] = pw * nx; F[2] = pw * ny; F[3] = pw * nz; F[4] = 0.0;
+}
+
+// ------------------------------------------------------------ body-force sources
+struct V3 {
+    double x, y, z;
+};
+__device__ __forceinline__ V3 v3(double x, double y, double z) { return V3{x, y, z}; }
+__device__ __forceinline__ double vdot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ double vmag(V3 a) { return sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+__device__ __forceinline__ V3 vsub(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ V3 vmul(V3 a, double s) { return v3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ V3 vdiv(V3 a, double s) { return v3(a.x / s, a.y / s, a.z / s); }
+__device__ __forceinline__ V3 vnrm(V3 a) { double m = vmag(a); return (m == 0) ? v3(0, 0, 0) : vdiv(a, m); }
+__device__ __forceinline__ V3 cyl_of(V3 v, double cs, double sn) { return v3(v.x, v.y * cs + v.z * sn, -v.y * sn + v.z * cs); }
+__device__ __forceinline__ V3 cart_of(V3 v, double cs, double sn) { return v3(v.x, v.y * cs - v.z * sn, v.y * sn + v.z * cs); }
+
+// Blockage + Hall / lift-drag source of one node (source_bfm_base.cpp:25-199,
+// source_bfm_hall.cpp:3-34, source_bfm_lift_drag.cpp:3-88).  s = slab index.
+__device__ inline void bfm_source(const CaseDev& c, long s, const double W[5], double V, double S[5],
+                                  double fi[3], double fv[3], double& dev) {
+    double blk[5] = {0, 0, 0, 0, 0};
+    fi[0] = fi[1] = fi[2] = 0.0;
+    fv[0] = fv[1] = fv[2] = 0.0;
+    dev = 0.0;
+    if (c.blockage_active) {
+        double e = static_energy<double>(W[1], W[2], W[3], W[4]);
+        double p = c.gm1 * W[0] * e;
+        double ht = W[4] + p / W[0];
+        double b = c.s_b[s];
+        double dd = W[1] * c.s_gbx[s] + W[2] * c.s_gby[s] + W[3] * c.s_gbz[s];
+        double common = -1.0 / b * W[0] * dd;
+        blk[0] = common * V;
+        blk[1] = common * W[1] * V;
+        blk[2] = common * W[2] * V;
+        blk[3] = common * W[3] * V;
+        blk[4] = common * ht * V;
+    }
+    double body[5] = {0, 0, 0, 0, 0};
+    if ((c.sflag[s] & 4) && (c.bfm_model == 1 || c.bfm_model == 2)) {
+        // computeFlowState (source_bfm_base.cpp:104-149)
+        const double cs = c.s_cs[s], sn = c.s_sn[s], r = c.s_r[s], om = c.s_om[s];
+        V3 vcy = cyl_of(v3(W[1], W[2], W[3]), cs, sn);
+        V3 rel = vsub(vcy, v3(0, 0, om * r));
+        V3 nc = v3(c.s_nax[s], c.s_nrad[s], c.s_ntan[s]);
+        // deviation angle (:152-166)
+        V3 n = vdiv(nc, vmag(nc));
+        V3 sp = v3(-rel.y, rel.x, 0.0);
+        sp = vdiv(sp, vmag(sp));
+        V3 vin = vsub(rel, vmul(sp, vdot(rel, sp)));
+        double dv = -asin(vdot(vin, n) / vmag(vin));
+        // inviscid force direction (:168-199, stall model off)
+        V3 rd = vnrm(rel), nd = vnrm(nc);
+        V3 ver = vnrm(vsub(nd, vmul(rd, vdot(nd, rd))));
+        V3 sd = vnrm(v3(rel.y, -rel.x, 0.0));
+        V3 idir = vnrm(vsub(ver, vmul(sd, vdot(ver, sd))));
+        dev = dv;
+        if (c.bfm_model == 1) {
+            double fm = vdot(rel, rel) * M_PI * dv / c.s_pitch[s] / fabs(c.s_ntan[s]);
+            V3 fcy = vmul(idir, fm);
+            V3 fca = cart_of(fcy, cs, sn);
+            fi[0] = fca.x; fi[1] = fca.y; fi[2] = fca.z;
+            double sv[5] = {0, fca.x, fca.y, fca.z, fcy.z * om * r};
+#pragma unroll
+            for (int v = 0; v < 5; ++v) body[v] = sv[v] * V * W[0];
+        } else {
+            double vmer = sqrt(vcy.x * vcy.x + vcy.y * vcy.y);
+            double flow_angle = atan2(rel.z, vmer);
+            V3 relc = cart_of(rel, cs, sn);
+            double w = vmag(relc);
+            double beta0 = c.s_beta0[s];
+            double sign = (c.s_ntan[s] >= 0.0) ? 1.0 : -1.0;
+            double fm = 0.5 * w * w * 2.0 * M_PI / c.s_spitch[s] * sign * (-flow_angle + beta0);
+            V3 fcy = vmul(idir, fm);
+            V3 fca = cart_of(fcy, cs, sn);
+            fi[0] = fca.x; fi[1] = fca.y; fi[2] = fca.z;
+            double si[5] = {0, fca.x, fca.y, fca.z, fcy.z * om * r};
+            double dd = flow_angle - beta0;
+            double Kp = c.s_kp[s] + 2.0 * M_PI * c.s_sol[s] * (dd * dd);
+            double fm2 = Kp * w * w / c.s_spitch[s];
+            V3 vdir = vmul(vnrm(rel), -1.0);
+            vdir = v3(-vnrm(rel).x, -vnrm(rel).y, -vnrm(rel).z);
+            V3 vcyl = vmul(vdir, fm2);
+            V3 vcar = cart_of(vcyl, cs, sn);
+            fv[0] = vcar.x; fv[1] = vcar.y; fv[2] = vcar.z;
+            double sv[5] = {0, vcar.x, vcar.y, vcar.z, vcyl.z * om * r};
+#pragma unroll
+            for (int v = 0; v < 5; ++v) body[v] = si[v] * V * W[0] + sv[v] * V * W[0];
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 5; ++v) S[v] = blk[v] + body[v] * (1.0 + 0.0);
+}
+
+}  // namespace dev
+}  // namespace tbfm

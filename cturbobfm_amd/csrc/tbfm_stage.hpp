@@ -1,0 +1,223 @@
+// Kernels of one explicit iteration: time step, radial-equilibrium profiles, boundary
+// fluxes, the fused residual + RK stage kernel, periodic closure and norm reductions.
+#pragma once
+#include "tbfm_physics.hpp"
+
+namespace tbfm {
+namespace dev {
+
+__device__ __forceinline__ long nid(const CaseDev& c, int i, int j, int k) {
+    return ((long)k * c.nj + j) * c.ni + i;
+}
+
+__device__ __forceinline__ void load5(const double* U, long N, long n, double out[5]) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) out[v] = U[v * N + n];
+}
+
+// rotateStateVectorAlongXAxis (src/math_utils.cpp:109-122)
+__device__ __forceinline__ void rotx(double s[5], double cs, double sn) {
+    double y = s[2] * cs - s[3] * sn;
+    double z = s[2] * sn + s[3] * cs;
+    s[2] = y;
+    s[3] = z;
+}
+
+// Node gk of the (i,j) line along k, including the two virtual planes used by the face
+// loop: -1 and NK (periodic wrap with rotation, solver.cpp:1964-1979, or linear
+// extrapolation, solver.cpp:1980-1993).
+__device__ inline void load_k(const CaseDev& c, const double* U, int i, int j, int gk, double out[5]) {
+    const int NK = c.nk;
+    if (gk >= 0 && gk < NK) {
+        load5(U, c.N, nid(c, i, j, gk), out);
+    } else if (c.periodic) {
+        if (gk < 0) { load5(U, c.N, nid(c, i, j, NK - 2), out); rotx(out, c.per_cm, c.per_sm); }
+        else { load5(U, c.N, nid(c, i, j, 1), out); rotx(out, c.per_c, c.per_s); }
+    } else {
+        double a[5], b[5];
+        if (gk < 0) {
+            load5(U, c.N, nid(c, i, j, 0), a);
+            load5(U, c.N, nid(c, i, j, 1), b);
+#pragma unroll
+            for (int v = 0; v < 5; ++v) out[v] = a[v] - (b[v] - a[v]);
+        } else {
+            load5(U, c.N, nid(c, i, j, NK - 1), a);
+            load5(U, c.N, nid(c, i, j, NK - 2), b);
+#pragma unroll
+            for (int v = 0; v < 5; ++v) out[v] = a[v] + (a[v] - b[v]);
+        }
+    }
+}
+
+// --------------------------------------------------------------------- time step
+// Solver::computeTimestepArray, inviscid (src/solver.cpp:1531-1630)
+__global__ void __launch_bounds__(256) k_timestep(CaseDev c, const double* U, double* partial_min) {
+    __shared__ double smin[256];
+    if (*c.stop) return;
+    const long n = (long)blockIdx.x * 256 + threadIdx.x;
+    double dtv = 1.0e300;
+    if (n < c.N) {
+        if (c.dt_mode == 2) {
+            dtv = c.fixed_dt;
+        } else {
+            const int i = (int)(n % c.ni);
+            const int j = (int)((n / c.ni) % c.nj);
+            const int k = (int)(n / ((long)c.ni * c.nj));
+            const long DI = c.ni + 1, DJ = c.nj + 1;
+            const long d0 = ((long)k * DJ + j) * DI + i;
+            const long dI = d0 + 1, dJ = d0 + DI, dK = d0 + DI * DJ;
+            double eIx = c.dux[dI] - c.dux[d0], eIy = c.duy[dI] - c.duy[d0], eIz = c.duz[dI] - c.duz[d0];
+            double eJx = c.dux[dJ] - c.dux[d0], eJy = c.duy[dJ] - c.duy[d0], eJz = c.duz[dJ] - c.duz[d0];
+            double eKx = c.dux[dK] - c.dux[d0], eKy = c.duy[dK] - c.duy[d0], eKz = c.duz[dK] - c.duz[d0];
+            double dsI = sqrt(eIx * eIx + eIy * eIy + eIz * eIz);
+            double dsJ = sqrt(eJx * eJx + eJy * eJy + eJz * eJz);
+            double dsK = sqrt(eKx * eKx + eKy * eKy + eKz * eKz);
+            double Un[5], W[5], p, rf;
+            load5(U, c.N, n, Un);
+            derive<double>(Un, W, p, rf, c.gamma, c.gm1);
+            double a = sqrt(c.gamma * p / W[0]);
+            double ui = fabs(W[1] * (eIx / dsI) + W[2] * (eIy / dsI) + W[3] * (eIz / dsI));
+            double lam = (ui + a) / dsI;
+            if (c.topo != TOPO_1D) {
+                double uj = fabs(W[1] * (eJx / dsJ) + W[2] * (eJy / dsJ) + W[3] * (eJz / dsJ));
+                lam += (uj + a) / dsJ;
+            }
+            if (c.topo == TOPO_3D) {
+                double uk = fabs(W[1] * (eKx / dsK) + W[2] * (eKy / dsK) + W[3] * (eKz / dsK));
+                lam += (uk + a) / dsK;
+            }
+            double invH2 = 1.0 / (dsI * dsI);
+            double lv = 4.0 * (0.0 / W[0] + 0.0 / W[0]) * invH2;
+            dtv = c.cfl / (lam + lv);
+        }
+        c.dt[n] = dtv;
+    }
+    if (c.dt_mode == 1) {
+        smin[threadIdx.x] = dtv;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if (threadIdx.x < s) smin[threadIdx.x] = dmind(smin[threadIdx.x], smin[threadIdx.x + s]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) partial_min[blockIdx.x] = smin[0];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_min_reduce(CaseDev c, const double* partial, int n) {
+    __shared__ double smin[256];
+    if (*c.stop) return;
+    double m = 1.0e300;
+    for (int q = threadIdx.x; q < n; q += 256) m = dmind(m, partial[q]);
+    smin[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) smin[threadIdx.x] = dmind(smin[threadIdx.x], smin[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) c.dtmin[0] = smin[0];
+}
+
+// ------------------------------------------------------- radial equilibrium profile
+// Solver::updateRadialProfiles + integrateRadialEquilibrium, radial_equilibrium branch
+// (src/solver.cpp:2688-2787, src/math_utils.cpp:141-170).  One thread per profile: the
+// trapezoid scan over the outlet column is sequential in j.
+__global__ void k_radial_profile(CaseDev c, const double* U) {
+    if (*c.stop) return;
+    const int q = threadIdx.x;
+    if (q >= c.nprof) return;
+    const int n = c.prof_n[q];
+    const int i = c.prof_i[q];
+    double* P = c.prof_p + (long)q * c.nj;
+    const double* r = c.prof_r + (long)q * c.nj;
+    const double* cs = c.prof_cs + (long)q * c.nj;
+    const double* sn = c.prof_sn + (long)q * c.nj;
+    if (n <= 1) return;   // single point: profile keeps the hub value
+    double rho_prev = 0, ut_prev = 0;
+    for (int m = 0; m < n; ++m) {
+        const int j = c.prof_j0[q] + m;
+        double Un[5], W[5];
+        load5(U, c.N, nid(c, i, j, 0), Un);
+        W[0] = Un[0];
+        W[1] = Un[1] / Un[0];
+        W[2] = Un[2] / Un[0];
+        W[3] = Un[3] / Un[0];
+        double ut = fabs(-W[2] * sn[m] + W[3] * cs[m]);
+        double rho = W[0];
+        if (m > 0) {
+            double dr = r[m] - r[m - 1];
+            double d1 = rho_prev * ut_prev * ut_prev / r[m - 1];
+            double d2 = rho * ut * ut / r[m];
+            P[m] = P[m - 1] + 0.5 * (d1 + d2) * dr;
+        }
+        rho_prev = rho;
+        ut_prev = ut;
+    }
+}
+
+// ------------------------------------------------------------- boundary fluxes
+// All boundary faces of all sides, one thread per face.  Start faces use the
+// outward vector -S and the node at the face; end faces +S and the node before it
+// (solver.cpp:1911-1957).  Result already multiplied by |S|.
+__global__ void __launch_bounds__(256) k_boundary(CaseDev c, const double* U) {
+    if (*c.stop) return;
+    const long NI = c.ni, NJ = c.nj, NK = c.nk;
+    long t = (long)blockIdx.x * 256 + threadIdx.x;
+    const long nI = NJ * NK, nJ = NI * NK, nK = NI * NJ;
+    const bool doJ = (c.topo != TOPO_1D), doK = (c.topo == TOPO_3D);
+    int dir, side;
+    long q;
+    if (t < 2 * nI) { dir = 0; side = (int)(t / nI); q = t % nI; }
+    else if (doJ && t < 2 * nI + 2 * nJ) { t -= 2 * nI; dir = 1; side = (int)(t / nJ); q = t % nJ; }
+    else if (doK && t < 2 * nI + 2 * nJ + 2 * nK) { t -= 2 * nI + 2 * nJ; dir = 2; side = (int)(t / nK); q = t % nK; }
+    else return;
+    int i, j, k, fi, fj, fk;
+    long sidx, nsz;
+    const double *sx, *sy, *sz;
+    const int* pid;
+    const double* dirv;
+    double* out;
+    if (dir == 0) {
+        j = (int)(q % NJ); k = (int)(q / NJ);
+        fi = side ? (int)NI : 0; fj = j; fk = k;
+        i = side ? (int)NI - 1 : 0;
+        sidx = ((long)k * NJ + j) * (NI + 1) + fi;
+        sx = c.six; sy = c.siy; sz = c.siz; pid = c.pI; dirv = c.dirI; out = c.bI; nsz = nI;
+    } else if (dir == 1) {
+        i = (int)(q % NI); k = (int)(q / NI);
+        fi = i; fj = side ? (int)NJ : 0; fk = k;
+        j = side ? (int)NJ - 1 : 0;
+        sidx = ((long)k * (NJ + 1) + fj) * NI + i;
+        sx = c.sjx; sy = c.sjy; sz = c.sjz; pid = c.pJ; dirv = c.dirJ; out = c.bJ; nsz = nJ;
+    } else {
+        i = (int)(q % NI); j = (int)(q / NI);
+        fi = i; fj = j; fk = side ? (int)NK : 0;
+        k = side ? (int)NK - 1 : 0;
+        sidx = ((long)fk * NJ + j) * NI + i;
+        sx = c.skx; sy = c.sky; sz = c.skz; pid = c.pK; dirv = c.dirK; out = c.bK; nsz = nK;
+    }
+    double Sx = sx[sidx], Sy = sy[sidx], Sz = sz[sidx];
+    if (!side) { Sx = -Sx; Sy = -Sy; Sz = -Sz; }
+    double Un[5];
+    load5(U, c.N, nid(c, i, j, k), Un);
+    const int p = pid[side * nsz + q];
+    const PatchDev pa = c.patch[p];
+    double F[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    const long it = *c.iter;
+    if (pa.type == PT_INLET) {
+        const double* d = dirv + (long)side * 3 * nsz;
+        inlet_flux(c, Un, Sx, Sy, Sz, pa.v0, pa.v1, d[q], d[nsz + q], d[2 * nsz + q], F);
+    } else if (pa.type == PT_OUTLET) {
+        outlet_flux(c, Un, Sx, Sy, Sz, pa.v0, it, F);
+    } else if (pa.type == PT_RADIAL) {
+        outlet_flux(c, Un, Sx, Sy, Sz, c.prof_p[(long)pa.profile * NJ + fj], it, F);
+    } else if (pa.type == PT_WALL) {
+        wall_flux(c, Un, Sx, Sy, Sz, F);
+    }
+    const double sm = sqrt(Sx * Sx + Sy * Sy + Sz * Sz);
+    double* o = out + (long)side * 5 * nsz;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) o[(long)v * nsz + q] = F[v] * sm;
+}
+
+}  // namespace dev
+}  // namespace tbfm

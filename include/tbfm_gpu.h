@@ -1,0 +1,132 @@
+/* C-ABI of the MI355X explicit residual path (JST / Roe-MUSCL fluxes, Hall / lift-drag
+ * body forces, local / global / fixed time step, RK41A / RK31B stage update).
+ *
+ * The reference has no FFI; its operator seams are per-element C++ virtuals
+ * (AdvectionBase::computeFlux, include/advection_base.hpp:21-25;
+ * BoundaryBase::computeBoundaryFlux, include/boundary_base.hpp:26-32;
+ * SourceBFMBase::computeTotalSource, include/source_bfm_base.hpp:20-31) called from
+ * Solver::computeResiduals (src/solver.cpp:1783-1833) inside Solver::solve
+ * (src/solver.cpp:1298-1424).  A GPU cannot sit behind per-face virtual calls, so the
+ * boundary is whole-array at Solver level: a case is created once from the same inputs
+ * the reference reads (grid CSV columns, input.ini keys, boundaries CSV rows) and is
+ * then advanced N explicit iterations on the device.
+ *
+ * Conventions
+ *   - Host arrays use the reference's layout: node index (i*NJ + j)*NK + k (k fastest,
+ *     include/types.hpp:411,428); a conserved state is 5 such arrays back to back
+ *     (rho, rhoU, rhoV, rhoW, rhoE).  The library transposes to its device layout.
+ *   - Host pointers are only read or written during the call; none is retained.
+ *   - Every entry point returns TBFM_OK (0) or a negative code and never throws; the
+ *     message of the last failure is tbfm_gpu_last_error(case) (or NULL case for
+ *     creation failures).
+ *   - One host thread may use a case at a time; each case owns its HIP stream, and
+ *     different cases may be used concurrently.
+ */
+#ifndef TBFM_GPU_H
+#define TBFM_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TBFM_OK 0
+#define TBFM_ERR_ARGUMENT (-1)
+#define TBFM_ERR_HIP (-2)
+#define TBFM_ERR_OUT_OF_MEMORY (-3)
+#define TBFM_ERR_DIVERGED (-4)
+#define TBFM_ERR_UNSUPPORTED (-5)
+
+typedef struct tbfm_case tbfm_case;
+
+/* Everything the reference reads from disk, passed as arrays.
+ *   x, y, z, field_data[f]: N = ni*nj*nk doubles each, reference order (k fastest);
+ *     field names are the grid-CSV column names (types.hpp:1084-1126), e.g.
+ *     "blockage", "normalAxial", "rpm", "bladePresent", "numberBlades".
+ *   config_keys/values: input.ini KEY = value pairs (config.cpp:15-44), including the
+ *     patch lines NAME = type, v1, v2, ...
+ *   patch_names / patch_ranges: the boundaries-CSV rows (solver.cpp:232-294), six
+ *     longs per patch: I_MIN, I_MAX, J_MIN, J_MAX, K_MIN, K_MAX. */
+typedef struct {
+    long ni, nj, nk;
+    const double* x;
+    const double* y;
+    const double* z;
+    int nfields;
+    const char* const* field_names;
+    const double* const* field_data;
+    int nconfig;
+    const char* const* config_keys;
+    const char* const* config_values;
+    int npatches;
+    const char* const* patch_names;
+    const long* patch_ranges;
+} tbfm_case_desc;
+
+/* Replaces Solver::Solver(config, mesh) (src/solver.cpp:18-31) for the GPU path:
+ * geometry (Mesh, src/mesh.cpp:11-40), boundary maps (solver.cpp:491-561), BFM set-up
+ * (solver.cpp:701-749) and the initial solution (solver.cpp:777-888). */
+int tbfm_gpu_case_create(const tbfm_case_desc* desc, tbfm_case** out);
+
+/* Same, reading input.ini, GRID_FILE and BOUNDARY_CONDITIONS_FILEPATH from disk like
+ * `turbobfm input.ini` (src/turbobfm.cpp:7-39). */
+int tbfm_gpu_case_create_from_ini(const char* ini_path, tbfm_case** out);
+
+int tbfm_gpu_case_destroy(tbfm_case* c);
+
+/* Last error message of a case, or of the last failed creation when c is NULL. */
+const char* tbfm_gpu_last_error(const tbfm_case* c);
+
+void tbfm_gpu_dims(const tbfm_case* c, long* ni, long* nj, long* nk);
+
+/* Conserved state (5*N doubles, reference layout).  set_state also re-derives the
+ * radial-equilibrium outlet profiles from the new state, as the reference does after
+ * initialisation (solver.cpp:818-834). */
+int tbfm_gpu_set_state(tbfm_case* c, const double* U);
+int tbfm_gpu_get_state(tbfm_case* c, double* U);
+
+/* Replaces the explicit branch of Solver::solve (solver.cpp:1337-1424): runs up to
+ * n_iter iterations (time step, radial profiles, RK stages with periodic closure,
+ * residual norms, convergence test).  logres (may be NULL) receives n_iter*5 values of
+ * log10(||R_v||_2 / N) (solver.cpp:1488-1508); rows after convergence are left
+ * untouched.  *converged_at is the iteration (counted from the case's start) at which
+ * the reference's convergence test (solver.cpp:2985-3015) passed, 0 if it has not.
+ * Returns TBFM_ERR_DIVERGED when a norm is NaN/Inf, like the reference's
+ * "Solver diverged" exception. */
+int tbfm_gpu_advance(tbfm_case* c, long n_iter, double* logres, long* converged_at);
+
+/* One Solver::computeResiduals (solver.cpp:1783-1833) of the given state at iteration
+ * `it`, with the time step of that state; R is 5*N (reference layout).  When
+ * update_profiles is non-zero the radial-equilibrium profiles are refreshed from U first
+ * (solver.cpp:2688-2787).  Used for per-stage parity. */
+int tbfm_gpu_compute_residual(tbfm_case* c, const double* U, long it, int update_profiles, double* R);
+
+/* Solver::computeTimestepArray (solver.cpp:1531-1630) of the given state; dt is N. */
+int tbfm_gpu_compute_timestep(tbfm_case* c, const double* U, double* dt);
+
+/* Side outputs the reference keeps (solver.hpp:370-371): inviscid and viscous body
+ * force (3*N, xyz per node), deviation angle (N) and time step (N); any may be NULL. */
+int tbfm_gpu_get_side_outputs(tbfm_case* c, double* finv, double* fvisc, double* dev, double* dt);
+
+/* Solver::updateMassFlows + updateTurboPerformance (solver.cpp:1633-1781) of the
+ * current state: out[0..5] boundary mass flows (I/J/K start/end), out[6] mass flow,
+ * out[7] total-pressure ratio, out[8] total-temperature ratio, out[9] efficiency. */
+int tbfm_gpu_get_turbo(tbfm_case* c, double* out);
+
+/* Static geometry as uploaded (reference layout), for parity checks of the mesh:
+ * which 0..2 surfaces I/J/K, 3 volumes, 4 blockage gradient. */
+int tbfm_gpu_get_geometry(tbfm_case* c, int which, double* out);
+
+/* Timing of the fused residual+RK stage kernel: when enabled, every stage-kernel launch
+ * is bracketed by HIP events on the case stream.  get returns the accumulated kernel
+ * milliseconds and launch count since the last reset, plus the algorithmic bytes of one
+ * launch (per-node contract times nodes). */
+int tbfm_gpu_profile(tbfm_case* c, int enable, int reset);
+int tbfm_gpu_profile_get(tbfm_case* c, double* stage_ms, long* launches, double* bytes_per_launch);
+
+/* Synchronises the case stream. */
+int tbfm_gpu_sync(tbfm_case* c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
