@@ -1,0 +1,145 @@
+"""Benchmark of the explicit residual path on one MI355X.
+
+Workload (default): the annular rotor with Hall body force, 193 x 49 x 129 nodes, JST,
+RK41A, CFL 2, local time step, plain FP64 (synthetic grid and fields, see
+cturbobfm_amd/cases.py).  A step is one explicit iteration: time step, radial-equilibrium
+profile, and four fused residual + RK stages over all nodes.
+
+Prints one JSON line:
+  value       cell-stage updates/s = nodes * stages * steps / wall time of the timed steps
+  roofline    the fused stage kernel: algorithmic bytes per launch (208 B per node for 3-D
+              FP64, +96 B per body-force node) / its average launch time from HIP events on
+              the case stream, against 8 TB/s; traffic from the committed rocprofv3
+              FETCH_SIZE/WRITE_SIZE summary when present
+  cpu_baseline the CPU restatement (oracle/liboracle.so, OpenMP) timed on a bounded sample
+              of the same workload on this host
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def body_force_nodes(case) -> int:
+    """Blade-row nodes (bladePresent > 0) of the workload, reported in the config."""
+    f = case["fields"]
+    if case["config"].get("BFM_ACTIVE", "no") != "yes" or "bladePresent" not in f:
+        return 0
+    return int(np.count_nonzero(f["bladePresent"] > 0))
+
+
+def traffic_from_profiles(kernel_prefix: str):
+    """HBM bytes per launch of the stage kernel from the committed counter summary."""
+    path = os.path.join(ROOT, "profiles", "stage_kernel_rotor3d_193x49x129_counters.md")
+    if not os.path.exists(path):
+        return None
+    txt = open(path).read()
+    m = re.search(r"hbm_bytes_per_launch:\s*([0-9.eE+]+)", txt)
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(case, seconds: float = 15.0):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # the checker; timed here only as the reported CPU baseline
+    threads = int(os.environ.get("OMP_NUM_THREADS", str(os.cpu_count() or 1)))
+    oc = O.OracleCase(case)
+    stages = oc.params.n_stages
+    oc.run(1, threads=threads)        # warm-up (page-in, thread start)
+    t0 = time.perf_counter()
+    n = 0
+    while time.perf_counter() - t0 < seconds:
+        oc.run(1, threads=threads)
+        n += 1
+    dt = time.perf_counter() - t0
+    nodes = case["ni"] * case["nj"] * case["nk"]
+    return {"value": nodes * stages * n / dt, "unit": "cell-stage updates/s", "cores": threads,
+            "kind": "port", "sample": "%d iterations (%d stages each) of the same %dx%dx%d case, %.1f s"
+            % (n, stages, case["ni"], case["nj"], case["nk"], dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--size", default="193x49x129")
+    ap.add_argument("--scheme", default="JST")
+    ap.add_argument("--precision", default="fp64")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="only run the plain copy kernel of this many bytes (counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (requested --gpus %d)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    from cturbobfm_amd import Solver, cases, copy_calibration
+
+    if args.copy_calibration:
+        ms = copy_calibration(args.copy_calibration, 20)
+        print(json.dumps({"copy_bytes": args.copy_calibration, "ms_per_copy": ms,
+                          "GBps": 2 * args.copy_calibration / ms / 1e6}))
+        return
+
+    ni, nj, nk = (int(v) for v in args.size.split("x"))
+    case = cases.rotor3d(ni, nj, nk, n_iterations=args.warmup + 2 * args.steps + 10, scheme=args.scheme)
+    overrides = {"GPU_PRECISION": args.precision}
+    s = Solver.from_case(case, overrides)
+    nodes = ni * nj * nk
+    stages = 4
+    s.advance(args.warmup)
+    s.sync()
+    s.profile(True, True)
+    t0 = time.perf_counter()
+    s.advance(args.steps)
+    s.sync()
+    t1 = time.perf_counter()
+    stage_ms, launches, alg_bytes = s.profile_get()
+    s.profile(False, True)
+    wall = t1 - t0
+    value = nodes * stages * args.steps / wall
+    nbf = body_force_nodes(case)
+    avg_ms = stage_ms / max(launches, 1)
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    traffic = traffic_from_profiles("k_stage")
+    out = {
+        "metric": "cell-stage updates/s + HBM roofline %; p50/p99 time-to-converge vs plain FP64",
+        "value": value,
+        "unit": "cell-stage updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f64-state/f32-flux",
+        "data": "synthetic annular rotor grid and blade fields (cturbobfm_amd/cases.py), free-stream start",
+        "config": {"workload": "annular rotor %dx%dx%d nodes, Hall body force, %s, RK41A, CFL 2, local dt, %s"
+                               % (ni, nj, nk, args.scheme, args.precision),
+                   "nodes": nodes, "stages_per_step": stages, "body_force_nodes": nbf},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_stage (fused residual + RK stage)", "avg_launch_ms": avg_ms,
+                     "algorithmic_bytes_per_launch": alg_bytes, "launches": launches},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

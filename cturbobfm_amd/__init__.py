@@ -42,7 +42,7 @@ EXPORTS = ["tbfm_gpu_case_create", "tbfm_gpu_case_create_from_ini", "tbfm_gpu_ca
            "tbfm_gpu_last_error", "tbfm_gpu_dims", "tbfm_gpu_set_state", "tbfm_gpu_get_state",
            "tbfm_gpu_advance", "tbfm_gpu_compute_residual", "tbfm_gpu_compute_timestep",
            "tbfm_gpu_get_side_outputs", "tbfm_gpu_get_turbo", "tbfm_gpu_get_geometry",
-           "tbfm_gpu_profile", "tbfm_gpu_profile_get", "tbfm_gpu_sync"]
+           "tbfm_gpu_profile", "tbfm_gpu_profile_get", "tbfm_gpu_sync", "tbfm_gpu_copy_calibration"]
 
 _lib = None
 
@@ -77,12 +77,22 @@ def lib():
         L.tbfm_gpu_profile.argtypes = [vp, C.c_int, C.c_int]
         L.tbfm_gpu_profile_get.argtypes = [vp, dp, C.POINTER(C.c_long), dp]
         L.tbfm_gpu_sync.argtypes = [vp]
+        L.tbfm_gpu_copy_calibration.argtypes = [C.c_long, C.c_int, dp]
         _lib = L
     return _lib
 
 
 def _dp(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def copy_calibration(nbytes: int, repeats: int = 10) -> float:
+    """Milliseconds per plain device copy of nbytes (k_copy_calibration)."""
+    ms = C.c_double()
+    rc = lib().tbfm_gpu_copy_calibration(nbytes, repeats, C.byref(ms))
+    if rc != TBFM_OK:
+        raise SolverError(lib().tbfm_gpu_last_error(None).decode())
+    return ms.value
 
 
 class Solver:

@@ -78,9 +78,12 @@ struct tbfm_case {
     // profiling
     bool prof = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> pool;     // event pairs recorded around stage-kernel launches
+    size_t pool_used = 0;             // pairs recorded since the last read-out
     double prof_ms = 0.0;
     long prof_launches = 0;
     double bytes_per_node = 0.0;
+    long body_force_nodes = 0;        // nodes whose stage also runs the blade sources
     // graph of three iterations (the buffer rotation has period 3)
     hipGraphExec_t graph = nullptr;
     int graph_u0 = -1;
@@ -89,6 +92,8 @@ struct tbfm_case {
 };
 
 namespace {
+
+void drainProfile(tbfm_case* c);
 
 long nodeDev(const tbfm_case* c, long i, long j, long k) { return (k * c->nj + j) * c->ni + i; }
 
@@ -394,6 +399,7 @@ void setup(tbfm_case* cs, const GridInput& grid) {
                 if (bfm && gateOf(n)) fl |= 2;
                 if (nblades && (*nblades)[n] != 0) fl |= 4;
                 sflag[s] = fl;
+                if ((fl & 3) && bfm) cs->body_force_nodes += 1;
                 s_cs[s] = std::cos(g.theta[n]);
                 s_sn[s] = std::sin(g.theta[n]);
                 s_r[s] = g.radius[n];
@@ -536,7 +542,16 @@ void stage(tbfm_case* cs, const double* Uin, double* Uout, double alpha, bool la
     st.rper = cs->rper;
     st.partial = cs->partial;
     st.kc = cs->kc;
-    if (cs->prof) HIPCHK(hipEventRecord(cs->ev0, cs->stream));
+    if (cs->prof) {
+        if (cs->pool_used * 2 + 2 > cs->pool.size()) {
+            hipEvent_t a, b;
+            HIPCHK(hipEventCreate(&a));
+            HIPCHK(hipEventCreate(&b));
+            cs->pool.push_back(a);
+            cs->pool.push_back(b);
+        }
+        HIPCHK(hipEventRecord(cs->pool[2 * cs->pool_used], cs->stream));
+    }
     const bool three = (cs->topo == Topology::ThreeD);
     if (cs->scheme == Scheme::JST) {
         if (three) launchStage<0, true>(cs, st); else launchStage<0, false>(cs, st);
@@ -544,12 +559,8 @@ void stage(tbfm_case* cs, const double* Uin, double* Uout, double alpha, bool la
         if (three) launchStage<1, true>(cs, st); else launchStage<1, false>(cs, st);
     }
     if (cs->prof) {
-        HIPCHK(hipEventRecord(cs->ev1, cs->stream));
-        HIPCHK(hipEventSynchronize(cs->ev1));
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, cs->ev0, cs->ev1));
-        cs->prof_ms += ms;
-        cs->prof_launches += 1;
+        HIPCHK(hipEventRecord(cs->pool[2 * cs->pool_used + 1], cs->stream));
+        cs->pool_used += 1;
     }
     if (c.periodic && cs->topo == Topology::ThreeD) {
         hipLaunchKernelGGL(dev::k_periodic, dim3((unsigned)cs->nparts_per), dim3(256), 0, cs->stream, c, st,
@@ -657,6 +668,7 @@ int tbfm_gpu_case_destroy(tbfm_case* c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     if (c->graph) hipGraphExecDestroy(c->graph);
     for (void* p : c->owned) hipFree(p);
+    for (hipEvent_t e : c->pool) hipEventDestroy(e);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -717,6 +729,7 @@ int tbfm_gpu_advance(tbfm_case* c, long n, double* logres, long* converged_at) {
         }
         for (long q = 0; q < n; ++q) iteration(c);
         HIPCHK(hipStreamSynchronize(c->stream));
+        if (c->prof) drainProfile(c);
         c->iter_host += n;
         long conv = 0;
         int div = 0;
@@ -960,19 +973,75 @@ int tbfm_gpu_get_geometry(tbfm_case* c, int which, double* out) {
     return TBFM_OK;
 }
 
+namespace {
+// Folds the recorded event pairs into the running totals (synchronises the stream).
+void drainProfile(tbfm_case* c) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (size_t q = 0; q < c->pool_used; ++q) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, c->pool[2 * q], c->pool[2 * q + 1]));
+        c->prof_ms += ms;
+        c->prof_launches += 1;
+    }
+    c->pool_used = 0;
+}
+}  // namespace
+
 int tbfm_gpu_profile(tbfm_case* c, int enable, int reset) {
     if (!c) return fail(c, TBFM_ERR_ARGUMENT, "null argument");
-    c->prof = enable != 0;
-    if (reset) { c->prof_ms = 0.0; c->prof_launches = 0; }
-    return TBFM_OK;
+    return guarded(c, [&]() {
+        drainProfile(c);
+        c->prof = enable != 0;
+        if (reset) { c->prof_ms = 0.0; c->prof_launches = 0; }
+        return TBFM_OK;
+    });
 }
 
 int tbfm_gpu_profile_get(tbfm_case* c, double* ms, long* launches, double* bytes) {
     if (!c) return fail(c, TBFM_ERR_ARGUMENT, "null argument");
-    if (ms) *ms = c->prof_ms;
-    if (launches) *launches = c->prof_launches;
-    if (bytes) *bytes = c->bytes_per_node * (double)c->N;
-    return TBFM_OK;
+    return guarded(c, [&]() {
+        drainProfile(c);
+        if (ms) *ms = c->prof_ms;
+        if (launches) *launches = c->prof_launches;
+        // 208 B (3-D FP64) per node plus 96 B per body-force node: n_ax, n_r, n_theta,
+        // N_b, rpm, bladePresent, b, grad b (3), r, theta
+        if (bytes) *bytes = c->bytes_per_node * (double)c->N + 96.0 * (double)c->body_force_nodes;
+        return TBFM_OK;
+    });
+}
+
+namespace {
+// Plain streaming copy used to calibrate the HBM byte counters of rocprofv3.
+__global__ void __launch_bounds__(256) k_copy_calibration(const double* __restrict__ a, double* __restrict__ b, long n) {
+    for (long q = (long)blockIdx.x * 256 + threadIdx.x; q < n; q += (long)gridDim.x * 256) b[q] = a[q];
+}
+}  // namespace
+
+int tbfm_gpu_copy_calibration(long nbytes, int repeats, double* ms_per_copy) {
+    if (nbytes <= 0 || repeats <= 0) return fail(nullptr, TBFM_ERR_ARGUMENT, "bad argument");
+    return guarded(nullptr, [&]() {
+        std::vector<void*> own;
+        const long n = nbytes / 8;
+        double* a = dalloc<double>(n, own);
+        double* b = dalloc<double>(n, own);
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        const unsigned blocks = (unsigned)std::min<long>((n + 255) / 256, 256L * 64);
+        hipLaunchKernelGGL(k_copy_calibration, dim3(blocks), dim3(256), 0, 0, (const double*)a, b, n);
+        HIPCHK(hipEventRecord(e0, 0));
+        for (int r = 0; r < repeats; ++r)
+            hipLaunchKernelGGL(k_copy_calibration, dim3(blocks), dim3(256), 0, 0, (const double*)a, b, n);
+        HIPCHK(hipEventRecord(e1, 0));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms_per_copy) *ms_per_copy = ms / repeats;
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        for (void* p : own) hipFree(p);
+        return TBFM_OK;
+    });
 }
 
 int tbfm_gpu_sync(tbfm_case* c) {

@@ -123,6 +123,13 @@ int tbfm_gpu_get_geometry(tbfm_case* c, int which, double* out);
 int tbfm_gpu_profile(tbfm_case* c, int enable, int reset);
 int tbfm_gpu_profile_get(tbfm_case* c, double* stage_ms, long* launches, double* bytes_per_launch);
 
+/* Measurement helper with no counterpart in the reference: runs `repeats` plain
+ * device-to-device copies of nbytes (kernel k_copy_calibration) on the default stream
+ * and returns the average milliseconds per copy.  rocprofv3's FETCH_SIZE/WRITE_SIZE of
+ * this kernel, against its known nbytes read and written, scales the counters of the
+ * residual kernels to HBM bytes. */
+int tbfm_gpu_copy_calibration(long nbytes, int repeats, double* ms_per_copy);
+
 /* Synchronises the case stream. */
 int tbfm_gpu_sync(tbfm_case* c);
 

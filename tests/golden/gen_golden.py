@@ -1,0 +1,160 @@
+"""Writes the golden vectors of tests/golden/*.npz from the reference's own translation
+units (oracle/_ref/libtbfm_ref.so, built by `make -C oracle ref` from /root/reference/src).
+
+Each fixture holds seeded inputs and the reference's outputs:
+  geometry_<case>.npz     Mesh arrays (surfaces, volumes, blockage gradient) of small grids
+  flux_<scheme>.npz       AdvectionJst / AdvectionRoe(+MUSCL, 4 limiters) face fluxes
+  boundary.npz            inlet (cartesian, face-normal, cylindrical), outlet (+ramp),
+                          radial-equilibrium outlet and inviscid-wall fluxes
+  bfm_<model>.npz         SourceBFMBase::computeTotalSource for Hall and lift-drag
+                          (+ blockage), with inviscid/viscous force and deviation angle
+  fluid.npz               FluidIdeal p, a, h_t, T, p_t, T_t, s
+  radial_eq.npz           integrateRadialEquilibrium
+
+Run from the repository root:  python tests/golden/gen_golden.py
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+from cturbobfm_amd import cases  # noqa: E402
+import oracle as O  # noqa: E402
+
+
+def dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def random_states(rng, n):
+    rho = rng.uniform(0.6, 1.6, n)
+    u = rng.uniform(-220, 220, (n, 3))
+    p = rng.uniform(5e4, 1.6e5, n)
+    e = p / (0.4 * rho)
+    et = e + 0.5 * np.sum(u * u, axis=1)
+    U = np.stack([rho, rho * u[:, 0], rho * u[:, 1], rho * u[:, 2], rho * et], axis=1)
+    W = np.stack([rho, u[:, 0], u[:, 1], u[:, 2], et], axis=1)
+    return U, W
+
+
+def fixture_cases():
+    """Small grids covering 2-D, 3-D periodic (360 and 45 degrees) and BFM fields."""
+    out = {
+        "bump2d": cases.bump2d(9, 6),
+        "rotor3d_hall": cases.rotor3d(11, 5, 7, bfm_model="Hall"),
+        "rotor3d_liftdrag": cases.rotor3d(11, 5, 7, bfm_model="Lift-Drag", scheme="ROE", muscl=True,
+                                          limiter="van albada"),
+    }
+    return out
+
+
+def main():
+    R = O.ref_lib()
+    if R is None:
+        raise SystemExit("build the reference checker first: make -C oracle ref")
+    rng = np.random.default_rng(20261203)
+    tmp = os.path.join(ROOT, "cases_tmp", "golden")
+    for name, case in fixture_cases().items():
+        ini = cases.write_case(case, os.path.join(tmp, name))
+        assert R.ref_case_load(ini.encode()) == 0
+        ni, nj, nk = case["ni"], case["nj"], case["nk"]
+        shapes = {0: (ni + 1, nj, nk, 3), 1: (ni, nj + 1, nk, 3), 2: (ni, nj, nk + 1, 3), 6: (ni, nj, nk)}
+        arrays = {}
+        for w, shp in shapes.items():
+            a = np.zeros(shp)
+            R.ref_get_geometry(w, dp(a))
+            arrays["geom%d" % w] = a
+        if case["config"].get("BFM_ACTIVE") == "yes":
+            a = np.zeros((ni, nj, nk, 3))
+            R.ref_get_geometry(9, dp(a))
+            arrays["gradb"] = a
+        np.savez_compressed(os.path.join(HERE, "geometry_%s.npz" % name), **arrays)
+
+        n = 256
+        # face fluxes of the case's scheme
+        U4 = [random_states(rng, n)[0] for _ in range(4)]
+        S = rng.normal(size=(n, 3))
+        S[:8, :2] = 0.0          # normals along z exercise the Roe triad fallback
+        F = np.zeros((n, 5))
+        for q in range(n):
+            R.ref_face_flux(*[dp(np.ascontiguousarray(u[q])) for u in U4], dp(np.ascontiguousarray(S[q])),
+                            dp(F[q]))
+        np.savez_compressed(os.path.join(HERE, "flux_%s.npz" % name), Ull=U4[0], Ul=U4[1], Ur=U4[2],
+                            Urr=U4[3], S=S, F=F)
+        if name == "bump2d":
+            # the MUSCL limiters need their own configurations
+            for lim in ("van albada", "van leer", "min mod"):
+                c2 = cases.bump2d(9, 6, scheme="ROE", muscl=True, limiter=lim)
+                ini2 = cases.write_case(c2, os.path.join(tmp, "bump2d_" + lim.replace(" ", "_")))
+                assert R.ref_case_load(ini2.encode()) == 0
+                U4 = [random_states(rng, n)[0] for _ in range(4)]
+                # nearly smooth stencils so that the limiter ratios span (-inf, inf)
+                U4[3] = U4[2] + (U4[2] - U4[1]) * rng.uniform(-1.5, 2.5, (n, 1))
+                S = rng.normal(size=(n, 3))
+                F = np.zeros((n, 5))
+                for q in range(n):
+                    R.ref_face_flux(*[dp(np.ascontiguousarray(u[q])) for u in U4],
+                                    dp(np.ascontiguousarray(S[q])), dp(F[q]))
+                np.savez_compressed(os.path.join(HERE, "flux_roe_muscl_%s.npz" % lim.replace(" ", "_")),
+                                    Ull=U4[0], Ul=U4[1], Ur=U4[2], Urr=U4[3], S=S, F=F)
+            assert R.ref_case_load(ini.encode()) == 0
+        if name == "rotor3d_hall":
+            # boundary fluxes: kind, values, state, surface, face, iteration, profile
+            recs = []
+            prof = rng.uniform(9.0e4, 1.0e5, nj)
+            for q in range(n):
+                Ui, _ = random_states(rng, 1)
+                Ui = Ui[0]
+                Sv = rng.normal(size=3)
+                face = np.array([0, rng.integers(0, nj), rng.integers(0, nk)], dtype=np.int64)
+                for kind, vals in ((0, [101325.0, 288.15, 1.0, 0.0, 0.0]), (0, [101325.0, 288.15, 1.0, 1.0, 1.0]),
+                                   (0, [1.3e5, 310.0, 1.0, 0.3, -0.2]), (1, [95000.0]), (2, [0.0]), (3, [])):
+                    out = np.zeros(5)
+                    v = np.array(vals + [0.0] * (8 - len(vals)))
+                    R.ref_boundary_flux(kind, dp(v), len(vals), dp(Ui), dp(Sv),
+                                        face.ctypes.data_as(C.POINTER(C.c_long)), 3, dp(prof), nj, dp(out))
+                    recs.append((kind, v, Ui, Sv, face.astype(np.float64), out))
+            np.savez_compressed(os.path.join(HERE, "boundary.npz"),
+                                kind=np.array([r[0] for r in recs]), values=np.array([r[1] for r in recs]),
+                                U=np.array([r[2] for r in recs]), S=np.array([r[3] for r in recs]),
+                                face=np.array([r[4] for r in recs]), F=np.array([r[5] for r in recs]),
+                                profile=prof, iteration=3)
+            # fluid relations
+            _, W = random_states(rng, n)
+            fs = np.zeros((n, 7))
+            for q in range(n):
+                R.ref_fluid_state(dp(np.ascontiguousarray(W[q])), dp(fs[q]))
+            np.savez_compressed(os.path.join(HERE, "fluid.npz"), W=W, out=fs)
+            # radial equilibrium
+            m = 17
+            rr = np.linspace(0.2, 0.3, m)
+            rho = rng.uniform(1.0, 1.3, m)
+            ut = rng.uniform(20.0, 150.0, m)
+            p = np.zeros(m)
+            R.ref_radial_equilibrium(dp(rho), dp(ut), dp(rr), m, 95000.0, dp(p))
+            np.savez_compressed(os.path.join(HERE, "radial_eq.npz"), rho=rho, ut=ut, r=rr, hub=95000.0, p=p)
+        if case["config"].get("BFM_ACTIVE") == "yes":
+            idx, Ws, outs, fis, fvs, devs = [], [], [], [], [], []
+            present = case["fields"]["bladePresent"]
+            cand = np.argwhere(present > 0)
+            for q in range(n):
+                i, j, k = cand[rng.integers(0, len(cand))]
+                _, W = random_states(rng, 1)
+                W = W[0]
+                out, fi, fv, dv = np.zeros(5), np.zeros(3), np.zeros(3), C.c_double()
+                R.ref_bfm_source(int(i), int(j), int(k), dp(W), 0.0, dp(out), dp(fi), dp(fv), C.byref(dv))
+                idx.append((i, j, k)); Ws.append(W); outs.append(out); fis.append(fi); fvs.append(fv)
+                devs.append(dv.value)
+            np.savez_compressed(os.path.join(HERE, "bfm_%s.npz" % name), idx=np.array(idx), W=np.array(Ws),
+                                S=np.array(outs), finv=np.array(fis), fvisc=np.array(fvs), dev=np.array(devs))
+    print("golden fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,89 @@
+"""The reference's regression cases (test/regression_tests/<case>/) as case dicts.
+
+Grid and reference-solution files are data copied from the reference's test folders
+into tests/golden/regression/; the boundaries CSV the reference expects is not shipped
+there (its .gitignore drops *.csv), so the patch ranges are synthesized from the patch
+names and grid sizes.  The configuration values are those of each case's input.ini."""
+import os
+
+import numpy as np
+
+from cturbobfm_amd import cases
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "regression")
+
+COMMON = {"KIND_SOLVER": "Euler", "FLUID_GAMMA": "1.4", "FLUID_R_CONSTANT": "287.05", "FLUID_MODEL": "Ideal",
+          "TIME_INTEGRATION_TYPE": "0", "TIME_STEP_METHOD": "local", "N_ITERATIONS": "99999",
+          "SAVE_ITERATIONS_INTERVAL": "250", "LOWER_WALL": "inviscid_wall", "UPPER_WALL": "inviscid_wall"}
+
+CONFIGS = {
+    "CircularBump2D": dict(COMMON, TOPOLOGY="2D", INFLOW="inlet, 140513.23, 316.224, 1, 0, 0",
+                           OUTFLOW="outlet, 101300", INIT_MACH_NUMBER="0.5", INIT_TEMPERATURE="285",
+                           INIT_PRESSURE="101335", INIT_DIRECTION="1, 0, 0", CFL="0.6",
+                           RESIDUALS_DROP_CONVERGENCE="4", CONVECTION_SCHEME="Roe", MUSCL_RECONSTRUCTION="no",
+                           FLUX_LIMITER="van leer"),
+    "CircularBump3D_PeriodicSwirl": dict(COMMON, TOPOLOGY="3D", INFLOW="inlet, 140513.23, 316.224, 1, 0, -0.5",
+                                         OUTFLOW="radial_equilibrium, 101300", INLET_REFERENCE_FRAME="cylindrical",
+                                         PER1="periodic, 0, 0, 45", PER2="periodic, 1, 0, 45",
+                                         INIT_MACH_NUMBER="0.7", INIT_TEMPERATURE="285", INIT_PRESSURE="101335",
+                                         INIT_DIRECTION="1, 0, 0", CFL="1.3", RESIDUALS_DROP_CONVERGENCE="4",
+                                         CONVECTION_SCHEME="JST", MUSCL_RECONSTRUCTION="no",
+                                         FLUX_LIMITER="van leer"),
+    "AxisymmetricSwirlingChannel": dict(COMMON, TOPOLOGY="axisymmetric", INFLOW="inlet, 130000, 300, 1, 0, 0.5",
+                                        OUTFLOW="radial_equilibrium, 101300", INIT_MACH_NUMBER="0.5",
+                                        INIT_TEMPERATURE="300", INIT_PRESSURE="101300",
+                                        INIT_DIRECTION="1, 0, 0.3", CFL="1.2", RESIDUALS_DROP_CONVERGENCE="3",
+                                        CONVECTION_SCHEME="JST"),
+}
+
+
+def load_case(name):
+    folder = os.path.join(HERE, name)
+    cfg = dict(CONFIGS[name])
+    grid = os.path.join(folder, "grid.csv")
+    ni = nj = nk = 0
+    skip = 0
+    with open(grid) as f:
+        for line in f:
+            skip += 1
+            if line.startswith("NI="):
+                ni = int(line[3:])
+            elif line.startswith("NJ="):
+                nj = int(line[3:])
+            elif line.startswith("NK="):
+                nk = int(line[3:])
+            elif line.startswith("x,y,z"):
+                break
+    data = np.loadtxt(grid, delimiter=",", skiprows=skip, ndmin=2)
+    X, Y, Z = (data[:, c].reshape(ni, nj, nk).copy() for c in range(3))
+    patches = [("INFLOW", 0, 0, 0, nj, 0, nk), ("OUTFLOW", ni, ni, 0, nj, 0, nk),
+               ("LOWER_WALL", 0, ni, 0, 0, 0, nk), ("UPPER_WALL", 0, ni, nj, nj, 0, nk)]
+    if nk > 1:
+        patches += [("PER1", 0, ni, 0, nj, 0, 0), ("PER2", 0, ni, 0, nj, nk, nk)]
+    return {"ni": ni, "nj": nj, "nk": nk, "x": X, "y": Y, "z": Z, "fields": {}, "config": cfg,
+            "patches": patches, "name": name}
+
+
+def load_reference_solution(name):
+    path = os.path.join(HERE, name, "reference.csv")
+    with open(path) as f:
+        dims = [int(f.readline().split("=")[1]) for _ in range(3)]
+        header = f.readline().strip().split(",")
+    data = np.loadtxt(path, delimiter=",", skiprows=4, ndmin=2)
+    return {h: data[:, c].reshape(dims) for c, h in enumerate(header)}
+
+
+def derived_fields(U, gamma=1.4, R=287.05):
+    rho = U[0]
+    u2 = (U[1] ** 2 + U[2] ** 2 + U[3] ** 2) / rho ** 2
+    p = (gamma - 1) * rho * (U[4] / rho - 0.5 * u2)
+    return {"Density": rho, "Pressure": p, "Temperature": p / (rho * R)}
+
+
+def rel_l2(ref, res):
+    return float(np.linalg.norm(ref - res) / np.linalg.norm(ref))
+
+
+def case_files(name, folder):
+    """input.ini + grid + boundaries for `turbobfm input.ini` runs."""
+    return cases.write_case(load_case(name), folder)

@@ -1,0 +1,24 @@
+"""The reference's regression cases (CircularBump2D: Roe 2-D; CircularBump3D_PeriodicSwirl:
+JST 3-D, 45-degree sector, cylindrical inlet, radial-equilibrium outlet;
+AxisymmetricSwirlingChannel: JST axisymmetric wedge) run to the reference's convergence
+test with the CPU restatement and compared with the reference's reference.csv on
+Density, Pressure and Temperature by relative L2, with the reference's own tolerance
+3e-2 (test/regression_tests/test_cases.py:82-83)."""
+import pytest
+
+import oracle as O
+import regression_cases as RC
+
+
+@pytest.mark.parametrize("name", list(RC.CONFIGS))
+def test_oracle_matches_reference_solution(name):
+    case = RC.load_case(name)
+    oc = O.OracleCase(case)
+    lr, conv, _ = oc.run(40000, threads=4)
+    assert conv > 0, "did not converge"
+    ref = RC.load_reference_solution(name)
+    got = RC.derived_fields(oc.state())
+    for key in ("Density", "Pressure", "Temperature"):
+        err = RC.rel_l2(ref[key], got[key])
+        print(name, key, "%.3e" % err)
+        assert err < 3e-2
