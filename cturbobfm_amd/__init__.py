@@ -42,7 +42,8 @@ EXPORTS = ["tbfm_gpu_case_create", "tbfm_gpu_case_create_from_ini", "tbfm_gpu_ca
            "tbfm_gpu_last_error", "tbfm_gpu_dims", "tbfm_gpu_set_state", "tbfm_gpu_get_state",
            "tbfm_gpu_advance", "tbfm_gpu_compute_residual", "tbfm_gpu_compute_timestep",
            "tbfm_gpu_get_side_outputs", "tbfm_gpu_get_turbo", "tbfm_gpu_get_geometry",
-           "tbfm_gpu_profile", "tbfm_gpu_profile_get", "tbfm_gpu_sync", "tbfm_gpu_copy_calibration"]
+           "tbfm_gpu_profile", "tbfm_gpu_profile_get", "tbfm_gpu_sync", "tbfm_gpu_copy_calibration",
+           "tbfm_gpu_tuning"]
 
 _lib = None
 
@@ -78,6 +79,8 @@ def lib():
         L.tbfm_gpu_profile_get.argtypes = [vp, dp, C.POINTER(C.c_long), dp]
         L.tbfm_gpu_sync.argtypes = [vp]
         L.tbfm_gpu_copy_calibration.argtypes = [C.c_long, C.c_int, dp]
+        L.tbfm_gpu_tuning.restype = C.c_char_p
+        L.tbfm_gpu_tuning.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -233,6 +236,9 @@ class Solver:
         ms, n, b = C.c_double(), C.c_long(), C.c_double()
         self._check(lib().tbfm_gpu_profile_get(self.h, C.byref(ms), C.byref(n), C.byref(b)))
         return ms.value, n.value, b.value
+
+    def tuning(self) -> str:
+        return lib().tbfm_gpu_tuning(self.h).decode()
 
     def sync(self) -> None:
         self._check(lib().tbfm_gpu_sync(self.h))

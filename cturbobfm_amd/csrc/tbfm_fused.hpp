@@ -1,115 +1,150 @@
 // The fused residual + RK stage kernel.
 //
-// One block owns a TI x TJ tile of (i, j) columns and marches along k over `kc` planes.
-// Per plane it stages the conserved state, primitives, pressure and spectral factor of
-// the tile plus a 2-node halo in LDS (ghost nodes at the domain ends are built exactly
-// as the reference extrapolates them), evaluates every I and J face of the tile once,
-// and takes the K faces from a per-thread register queue of the column (planes k-1..k+2).
-// The residual of each node is then assembled in the reference's order
+// One 256-thread block owns a TI x TJ tile of (i, j) columns and marches along k over
+// `kc` planes.  Per plane it stages the conserved state, primitives, pressure and the
+// JST spectral factor |u| + a of the tile plus a 2-node halo in LDS (ghost nodes at the
+// domain ends are built exactly as the reference extrapolates them), evaluates every I
+// and J face of the tile once (JST pressure sensors inline from the staged pressures),
+// and takes the K faces from a per-thread register queue of the column (planes
+// k-1..k+2).  Face results go straight to their own LDS arrays (no register staging
+// across a barrier) and the residual norms accumulate in LDS, which keeps the register
+// budget at two waves per SIMD.
+// The residual of each node is assembled in the reference's order
 //   (I left, I right, J left, J right, K left, K right, geometric, Gong, BFM)
 // (solver.cpp:1794-1821, 1997-2006, 2853-2891) and the RK update
-// U = U0 - R * (alpha dt / V) (solver.cpp:2230-2249) is applied in place.  Nodes on the
-// periodic planes k = 0 and k = NK-1 park their residual for the periodic-closure kernel.
+// U = U0 - R * (alpha dt / V) (solver.cpp:2230-2249) is applied.  Nodes on the periodic
+// planes k = 0 and k = NK-1 park their residual for the periodic-closure kernel.
 #pragma once
 #include "tbfm_stage.hpp"
 
 namespace tbfm {
 namespace dev {
 
-template <int SCH>
-struct PlaneLds {
-    double U[5][NP];
-    double W[5][NP];
-    double P[NP];
-    double RF[NP];
-    double SI[TJ][TI + 2];      // JST I-sensor of nodes i0-1 .. i0+TI
-    double SJ[TJ + 2][TI];      // JST J-sensor of rows j0-1 .. j0+TJ
-    double FI[5][TJ][TI + 1];   // I-face flux*|S|: face i0+m of row j0+tj
-    double FJ[5][TJ + 1][TI];   // J-face flux*|S|
-    double red[NT];
+template <int TI_, int TJ_>
+struct Tile {
+    static constexpr int TI = TI_, TJ = TJ_;
+    static constexpr int HI = TI + 4, HJ = TJ + 4, NP = HI * HJ;
+    static constexpr int NFI = 5 * TJ * (TI + 1), NFJ = 5 * (TJ + 1) * TI;
+    static_assert(TI * TJ == NT, "tiles have 256 columns");
+    static_assert(7 * NP >= NFI + NFJ, "face buffers must fit in the primitive region");
 };
+
+template <class TL>
+struct PlaneLds {
+    double U[5][TL::NP];
+    struct {
+        double W[5][TL::NP];
+        double P[TL::NP];
+        double RF[TL::NP];
+    } d;
+    struct {
+        double FI[TL::NFI];   // [v][tj][m]  m = 0..TI, I face of node i0+m (its left face)
+        double FJ[TL::NFJ];   // [v][r][ti] r = 0..TJ
+    } f;
+    double red[5][NT];        // per-thread residual-norm accumulators
+};
+
+template <class TL>
+__device__ __forceinline__ int fiI(int v, int tj, int m) { return (v * TL::TJ + tj) * (TL::TI + 1) + m; }
+template <class TL>
+__device__ __forceinline__ int fiJ(int v, int r, int ti) { return (v * (TL::TJ + 1) + r) * TL::TI + ti; }
 
 // Fills LDS position `pos` of plane k: real node, ghost (one step outside the domain in
 // i or j) or unused.
-template <int SCH>
-__device__ inline void fill_position(const CaseDev& c, const double* U, PlaneLds<SCH>& L, int pos, int i0, int j0, int k) {
-    const int hi = pos % HI, hj = pos / HI;
+template <class TL>
+__device__ inline void fill_position(const CaseDev& c, const double* __restrict__ U, PlaneLds<TL>& L, int pos,
+                                     int i0, int j0, int k) {
+    const int hi = pos % TL::HI, hj = pos / TL::HI;
     const int gi = i0 - 2 + hi, gj = j0 - 2 + hj;
     const int NI = c.ni, NJ = c.nj;
     const bool iin = (gi >= 0 && gi < NI), jin = (gj >= 0 && gj < NJ);
     double Un[5];
-    bool ok = true;
     if (iin && jin) {
         load5(U, c.N, nid(c, gi, gj, k), Un);
     } else if (jin && (gi == -1 || gi == NI)) {
         double a[5], b[5];
-        if (gi < 0) {
-            load5(U, c.N, nid(c, 0, gj, k), a);
-            load5(U, c.N, nid(c, 1, gj, k), b);
+        const int ia = (gi < 0) ? 0 : NI - 1, ib = (gi < 0) ? 1 : NI - 2;
+        load5(U, c.N, nid(c, ia, gj, k), a);
+        load5(U, c.N, nid(c, ib, gj, k), b);
 #pragma unroll
-            for (int v = 0; v < 5; ++v) Un[v] = a[v] - (b[v] - a[v]);
-        } else {
-            load5(U, c.N, nid(c, NI - 1, gj, k), a);
-            load5(U, c.N, nid(c, NI - 2, gj, k), b);
-#pragma unroll
-            for (int v = 0; v < 5; ++v) Un[v] = a[v] + (a[v] - b[v]);
-        }
+        for (int v = 0; v < 5; ++v) Un[v] = (gi < 0) ? a[v] - (b[v] - a[v]) : a[v] + (a[v] - b[v]);
     } else if (iin && (gj == -1 || gj == NJ) && c.topo != TOPO_1D) {
         double a[5], b[5];
-        if (gj < 0) {
-            load5(U, c.N, nid(c, gi, 0, k), a);
-            load5(U, c.N, nid(c, gi, 1, k), b);
+        const int ja = (gj < 0) ? 0 : NJ - 1, jb = (gj < 0) ? 1 : NJ - 2;
+        load5(U, c.N, nid(c, gi, ja, k), a);
+        load5(U, c.N, nid(c, gi, jb, k), b);
 #pragma unroll
-            for (int v = 0; v < 5; ++v) Un[v] = a[v] - (b[v] - a[v]);
-        } else {
-            load5(U, c.N, nid(c, gi, NJ - 1, k), a);
-            load5(U, c.N, nid(c, gi, NJ - 2, k), b);
-#pragma unroll
-            for (int v = 0; v < 5; ++v) Un[v] = a[v] + (a[v] - b[v]);
-        }
+        for (int v = 0; v < 5; ++v) Un[v] = (gj < 0) ? a[v] - (b[v] - a[v]) : a[v] + (a[v] - b[v]);
     } else {
-        ok = false;
-    }
-    if (!ok) {
 #pragma unroll
-        for (int v = 0; v < 5; ++v) { L.U[v][pos] = 1.0; L.W[v][pos] = 1.0; }
-        L.P[pos] = 1.0;
-        L.RF[pos] = 0.0;
+        for (int v = 0; v < 5; ++v) { L.U[v][pos] = 1.0; L.d.W[v][pos] = 1.0; }
+        L.d.P[pos] = 1.0;
+        L.d.RF[pos] = 0.0;
         return;
     }
     double W[5], p, rf;
     derive<double>(Un, W, p, rf, c.gamma, c.gm1);
 #pragma unroll
-    for (int v = 0; v < 5; ++v) { L.U[v][pos] = Un[v]; L.W[v][pos] = W[v]; }
-    L.P[pos] = p;
-    L.RF[pos] = rf;
+    for (int v = 0; v < 5; ++v) { L.U[v][pos] = Un[v]; L.d.W[v][pos] = W[v]; }
+    L.d.P[pos] = p;
+    L.d.RF[pos] = rf;
 }
 
 // Face flux*|S| between LDS positions l and r (with ll, rr) along i or j.
-template <int SCH, typename FT>
-__device__ __forceinline__ void lds_face(const CaseDev& c, const PlaneLds<SCH>& L, int ll, int l, int r, int rr,
-                                         double sl, double sr, double sx, double sy, double sz, double out[5]) {
+template <class TL, int SCH, typename FT>
+__device__ __forceinline__ void lds_face(const CaseDev& c, const PlaneLds<TL>& L, int ll, int l, int r, int rr,
+                                         double sx, double sy, double sz, double out[5]) {
     if (SCH == 0) {
-        double Ull[5], Ul[5], Ur[5], Urr[5], Wl[5], Wr[5];
+        // JST sensors of the left and right nodes from the staged pressures
+        const double pll = L.d.P[ll], pl = L.d.P[l], pr = L.d.P[r], prr = L.d.P[rr];
+        const double sl = jst_sensor<double>(pll, pl, pr);
+        const double sr = jst_sensor<double>(pl, pr, prr);
+        FT Wavg[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) Wavg[v] = (FT(L.d.W[v][l]) + FT(L.d.W[v][r])) * FT(0.5);
+        FT rr_ = FT(dmaxd(L.d.RF[l], L.d.RF[r]));
+        FT s = FT(dmaxd(sl, sr));
+        FT psi2 = FT(0.5) * s * rr_;
+        FT t4 = FT(0.02) * rr_ - FT(2.0) * psi2;
+        FT psi4 = (FT(0.0) < t4) ? t4 : FT(0.0);
+        FT F[5];
+        FT sm = phys_flux<FT>(Wavg, FT(sx), FT(sy), FT(sz), FT(c.gm1), F);
 #pragma unroll
         for (int v = 0; v < 5; ++v) {
-            Ull[v] = L.U[v][ll]; Ul[v] = L.U[v][l]; Ur[v] = L.U[v][r]; Urr[v] = L.U[v][rr];
-            Wl[v] = L.W[v][l]; Wr[v] = L.W[v][r];
+            const double ul = L.U[v][l], ur = L.U[v][r];
+            double d1 = ur - ul;
+            double d3 = ((L.U[v][rr] - ur) - d1 * 2) + (ul - L.U[v][ll]);
+            FT dis = FT(d1) * psi2 - FT(d3) * psi4;
+            out[v] = double((F[v] - dis) * sm);
         }
-        jst_face<FT>(Ull, Ul, Ur, Urr, Wl, Wr, L.RF[l], L.RF[r], sl, sr, sx, sy, sz, c.gm1, out);
     } else {
         double Wll[5], Wl[5], Wr[5], Wrr[5];
 #pragma unroll
         for (int v = 0; v < 5; ++v) {
-            Wll[v] = L.W[v][ll]; Wl[v] = L.W[v][l]; Wr[v] = L.W[v][r]; Wrr[v] = L.W[v][rr];
+            Wll[v] = L.d.W[v][ll]; Wl[v] = L.d.W[v][l]; Wr[v] = L.d.W[v][r]; Wrr[v] = L.d.W[v][rr];
         }
         roe_face<FT>(Wll, Wl, Wr, Wrr, c.muscl, c.limiter, sx, sy, sz, c.gamma, out);
     }
 }
 
-template <int SCH, bool THREED, typename FT>
+// K face from the column queue (slots: 0 = left-left, 1 = left, 2 = right, 3 = right-right)
+template <int SCH, typename FT>
+__device__ __forceinline__ void queue_face(const CaseDev& c, const double qU[4][5], const double qW[4][5],
+                                           const double qP[4], const double qRF[4], double sx, double sy,
+                                           double sz, double out[5]) {
+    if (SCH == 0) {
+        double sl = jst_sensor<double>(qP[0], qP[1], qP[2]);
+        double sr = jst_sensor<double>(qP[1], qP[2], qP[3]);
+        jst_face<FT>(qU[0], qU[1], qU[2], qU[3], qW[1], qW[2], qRF[1], qRF[2], sl, sr, sx, sy, sz, c.gm1, out);
+    } else {
+        roe_face<FT>(qW[0], qW[1], qW[2], qW[3], c.muscl, c.limiter, sx, sy, sz, c.gamma, out);
+    }
+}
+
+template <class TL, int SCH, bool THREED, typename FT>
 __global__ void __launch_bounds__(NT) k_stage(CaseDev c, StageDev st) {
-    __shared__ PlaneLds<SCH> L;
+    constexpr int TI = TL::TI, TJ = TL::TJ, HI = TL::HI, NP = TL::NP;
+    __shared__ PlaneLds<TL> L;
     if (*c.stop) return;
     const int tid = threadIdx.x;
     const int ti = tid % TI, tj = tid / TI;
@@ -123,16 +158,18 @@ __global__ void __launch_bounds__(NT) k_stage(CaseDev c, StageDev st) {
     const long N = c.N;
     const int own = (tj + 2) * HI + (ti + 2);
     const bool bladecol = THREED && c.bfm_active && act && c.blade2d[(long)j * NI + i];
-    const long it = *c.iter;
+    const bool srcOn = (c.topo == TOPO_AXI) || c.bfm_active;
+    const double* __restrict__ Uin = st.Uin;
 
     // K queue: slots 0..3 hold planes k-1, k, k+1, k+2 at the top of step k
     double qU[4][5], qW[4][5], qP[4], qRF[4];
     double FKl[5] = {0, 0, 0, 0, 0};
     if (THREED && act) {
+#pragma unroll
         for (int s = 0; s < 4; ++s) {
             const int gk = kc0 - 2 + s;
             if (gk >= -1 && gk <= NK) {
-                load_k(c, st.Uin, i, j, gk, qU[s]);
+                load_k(c, Uin, i, j, gk, qU[s]);
                 derive<double>(qU[s], qW[s], qP[s], qRF[s], c.gamma, c.gm1);
             } else {
 #pragma unroll
@@ -140,33 +177,26 @@ __global__ void __launch_bounds__(NT) k_stage(CaseDev c, StageDev st) {
                 qP[s] = 1.0; qRF[s] = 0.0;
             }
         }
-        // K face on the left of node kc0
-        if (kc0 == 0) {
-            if (!bladecol) {
+        if (!bladecol) {
+            if (kc0 == 0) {
                 const long q = (long)j * NI + i;
 #pragma unroll
                 for (int v = 0; v < 5; ++v) FKl[v] = c.bK[(long)v * NI * NJ + q];
-            }
-        } else if (!bladecol) {
-            const long sidx = ((long)kc0 * NJ + j) * NI + i;
-            if (SCH == 0) {
-                double sl = jst_sensor<double>(qP[0], qP[1], qP[2]);
-                double sr = jst_sensor<double>(qP[1], qP[2], qP[3]);
-                jst_face<FT>(qU[0], qU[1], qU[2], qU[3], qW[1], qW[2], qRF[1], qRF[2], sl, sr,
-                             c.skx[sidx], c.sky[sidx], c.skz[sidx], c.gm1, FKl);
             } else {
-                roe_face<FT>(qW[0], qW[1], qW[2], qW[3], c.muscl, c.limiter, c.skx[sidx], c.sky[sidx],
-                             c.skz[sidx], c.gamma, FKl);
+                const long sidx = ((long)kc0 * NJ + j) * NI + i;
+                queue_face<SCH, FT>(c, qU, qW, qP, qRF, c.skx[sidx], c.sky[sidx], c.skz[sidx], FKl);
             }
         }
     }
-    double nrm[5] = {0, 0, 0, 0, 0};
+    if (st.last)
+#pragma unroll
+        for (int v = 0; v < 5; ++v) L.red[v][tid] = 0.0;
 
+#pragma unroll 1
     for (int k = kc0; k < kc1; ++k) {
         double Uc[5], Wc[5], Pc = 0.0, RFc = 0.0;
         if (THREED) {
             if (act) {
-                // shift the queue and bring in plane k+2
 #pragma unroll
                 for (int s = 0; s < 3; ++s) {
 #pragma unroll
@@ -175,7 +205,7 @@ __global__ void __launch_bounds__(NT) k_stage(CaseDev c, StageDev st) {
                     qRF[s] = qRF[s + 1];
                 }
                 if (k + 2 <= NK) {
-                    load_k(c, st.Uin, i, j, k + 2, qU[3]);
+                    load_k(c, Uin, i, j, k + 2, qU[3]);
                     derive<double>(qU[3], qW[3], qP[3], qRF[3], c.gamma, c.gm1);
                 }
 #pragma unroll
@@ -184,53 +214,27 @@ __global__ void __launch_bounds__(NT) k_stage(CaseDev c, StageDev st) {
                 RFc = qRF[1];
             }
         } else if (act) {
-            load5(st.Uin, N, nid(c, i, j, k), Uc);
+            load5(Uin, N, nid(c, i, j, k), Uc);
             derive<double>(Uc, Wc, Pc, RFc, c.gamma, c.gm1);
         }
-        // ---- stage plane k (own node from registers, halo by all threads)
+        // ---- stage plane k: own node from registers, halo by all threads
         if (act) {
 #pragma unroll
-            for (int v = 0; v < 5; ++v) { L.U[v][own] = Uc[v]; L.W[v][own] = Wc[v]; }
-            L.P[own] = Pc;
-            L.RF[own] = RFc;
+            for (int v = 0; v < 5; ++v) { L.U[v][own] = Uc[v]; L.d.W[v][own] = Wc[v]; }
+            L.d.P[own] = Pc;
+            L.d.RF[own] = RFc;
         }
+#pragma unroll 1
         for (int pos = tid; pos < NP; pos += NT) {
             const int hi = pos % HI, hj = pos / HI;
             const bool inTile = (hi >= 2 && hi < TI + 2 && hj >= 2 && hj < TJ + 2);
             if (inTile && (i0 + hi - 2) < NI && (j0 + hj - 2) < NJ) continue;  // owner wrote it
-            fill_position<SCH>(c, st.Uin, L, pos, i0, j0, k);
+            fill_position<TL>(c, Uin, L, pos, i0, j0, k);
         }
         __syncthreads();
-        // ---- JST sensors of the nodes the tile's faces touch
-        if (SCH == 0) {
-            for (int q = tid; q < TJ * (TI + 2); q += NT) {
-                const int m = q % (TI + 2), r = q / (TI + 2);
-                const int gi = i0 - 1 + m;
-                double s = 0.0;
-                if (gi >= 0 && gi < NI) {
-                    const int pos = (r + 2) * HI + (m + 1);
-                    s = jst_sensor<double>(L.P[pos - 1], L.P[pos], L.P[pos + 1]);
-                }
-                L.SI[r][m] = s;
-            }
-            if (doJ) {
-                for (int q = tid; q < (TJ + 2) * TI; q += NT) {
-                    const int m = q % TI, r = q / TI;
-                    const int gj = j0 - 1 + r;
-                    double s = 0.0;
-                    if (gj >= 0 && gj < NJ) {
-                        const int pos = (r + 1) * HI + (m + 2);
-                        s = jst_sensor<double>(L.P[pos - HI], L.P[pos], L.P[pos + HI]);
-                    }
-                    L.SJ[r][m] = s;
-                }
-            }
-            __syncthreads();
-        }
-        // ---- faces
+        // ---- faces, written straight to LDS
         double FKr[5] = {0, 0, 0, 0, 0};
         if (act) {
-            // I face on the left of the node (face index i)
             double F[5];
             if (i == 0) {
                 const long q = (long)k * NJ + j;
@@ -238,11 +242,10 @@ __global__ void __launch_bounds__(NT) k_stage(CaseDev c, StageDev st) {
                 for (int v = 0; v < 5; ++v) F[v] = c.bI[(long)v * NJ * NK + q];
             } else {
                 const long sidx = ((long)k * NJ + j) * (NI + 1) + i;
-                lds_face<SCH, FT>(c, L, own - 2, own - 1, own, own + 1, (SCH == 0) ? L.SI[tj][ti] : 0.0,
-                                  (SCH == 0) ? L.SI[tj][ti + 1] : 0.0, c.six[sidx], c.siy[sidx], c.siz[sidx], F);
+                lds_face<TL, SCH, FT>(c, L, own - 2, own - 1, own, own + 1, c.six[sidx], c.siy[sidx], c.siz[sidx], F);
             }
 #pragma unroll
-            for (int v = 0; v < 5; ++v) L.FI[v][tj][ti] = F[v];
+            for (int v = 0; v < 5; ++v) L.f.FI[fiI<TL>(v, tj, ti)] = F[v];
             if (ti == TI - 1 || i == NI - 1) {
                 if (i + 1 == NI) {
                     const long q = (long)k * NJ + j;
@@ -250,11 +253,10 @@ __global__ void __launch_bounds__(NT) k_stage(CaseDev c, StageDev st) {
                     for (int v = 0; v < 5; ++v) F[v] = c.bI[(long)(5 + v) * NJ * NK + q];
                 } else {
                     const long sidx = ((long)k * NJ + j) * (NI + 1) + i + 1;
-                    lds_face<SCH, FT>(c, L, own - 1, own, own + 1, own + 2, (SCH == 0) ? L.SI[tj][ti + 1] : 0.0,
-                                      (SCH == 0) ? L.SI[tj][ti + 2] : 0.0, c.six[sidx], c.siy[sidx], c.siz[sidx], F);
+                    lds_face<TL, SCH, FT>(c, L, own - 1, own, own + 1, own + 2, c.six[sidx], c.siy[sidx], c.siz[sidx], F);
                 }
 #pragma unroll
-                for (int v = 0; v < 5; ++v) L.FI[v][tj][ti + 1] = F[v];
+                for (int v = 0; v < 5; ++v) L.f.FI[fiI<TL>(v, tj, ti + 1)] = F[v];
             }
             if (doJ) {
                 if (j == 0) {
@@ -263,11 +265,11 @@ __global__ void __launch_bounds__(NT) k_stage(CaseDev c, StageDev st) {
                     for (int v = 0; v < 5; ++v) F[v] = c.bJ[(long)v * NI * NK + q];
                 } else {
                     const long sidx = ((long)k * (NJ + 1) + j) * NI + i;
-                    lds_face<SCH, FT>(c, L, own - 2 * HI, own - HI, own, own + HI, (SCH == 0) ? L.SJ[tj][ti] : 0.0,
-                                      (SCH == 0) ? L.SJ[tj + 1][ti] : 0.0, c.sjx[sidx], c.sjy[sidx], c.sjz[sidx], F);
+                    lds_face<TL, SCH, FT>(c, L, own - 2 * HI, own - HI, own, own + HI, c.sjx[sidx], c.sjy[sidx],
+                                          c.sjz[sidx], F);
                 }
 #pragma unroll
-                for (int v = 0; v < 5; ++v) L.FJ[v][tj][ti] = F[v];
+                for (int v = 0; v < 5; ++v) L.f.FJ[fiJ<TL>(v, tj, ti)] = F[v];
                 if (tj == TJ - 1 || j == NJ - 1) {
                     if (j + 1 == NJ) {
                         const long q = (long)k * NI + i;
@@ -275,14 +277,13 @@ __global__ void __launch_bounds__(NT) k_stage(CaseDev c, StageDev st) {
                         for (int v = 0; v < 5; ++v) F[v] = c.bJ[(long)(5 + v) * NI * NK + q];
                     } else {
                         const long sidx = ((long)k * (NJ + 1) + j + 1) * NI + i;
-                        lds_face<SCH, FT>(c, L, own - HI, own, own + HI, own + 2 * HI, (SCH == 0) ? L.SJ[tj + 1][ti] : 0.0,
-                                          (SCH == 0) ? L.SJ[tj + 2][ti] : 0.0, c.sjx[sidx], c.sjy[sidx], c.sjz[sidx], F);
+                        lds_face<TL, SCH, FT>(c, L, own - HI, own, own + HI, own + 2 * HI, c.sjx[sidx], c.sjy[sidx],
+                                              c.sjz[sidx], F);
                     }
 #pragma unroll
-                    for (int v = 0; v < 5; ++v) L.FJ[v][tj + 1][ti] = F[v];
+                    for (int v = 0; v < 5; ++v) L.f.FJ[fiJ<TL>(v, tj + 1, ti)] = F[v];
                 }
             }
-            // K face on the right of the node (face index k+1)
             if (THREED && !bladecol) {
                 if (k + 1 == NK) {
                     const long q = (long)j * NI + i;
@@ -290,15 +291,7 @@ __global__ void __launch_bounds__(NT) k_stage(CaseDev c, StageDev st) {
                     for (int v = 0; v < 5; ++v) FKr[v] = c.bK[(long)(5 + v) * NI * NJ + q];
                 } else {
                     const long sidx = ((long)(k + 1) * NJ + j) * NI + i;
-                    if (SCH == 0) {
-                        double sl = jst_sensor<double>(qP[0], qP[1], qP[2]);
-                        double sr = jst_sensor<double>(qP[1], qP[2], qP[3]);
-                        jst_face<FT>(qU[0], qU[1], qU[2], qU[3], qW[1], qW[2], qRF[1], qRF[2], sl, sr,
-                                     c.skx[sidx], c.sky[sidx], c.skz[sidx], c.gm1, FKr);
-                    } else {
-                        roe_face<FT>(qW[0], qW[1], qW[2], qW[3], c.muscl, c.limiter, c.skx[sidx], c.sky[sidx],
-                                     c.skz[sidx], c.gamma, FKr);
-                    }
+                    queue_face<SCH, FT>(c, qU, qW, qP, qRF, c.skx[sidx], c.sky[sidx], c.skz[sidx], FKr);
                 }
             }
         }
@@ -310,11 +303,13 @@ __global__ void __launch_bounds__(NT) k_stage(CaseDev c, StageDev st) {
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
                 double r = 0.0;
-                r = (i == 0) ? r + L.FI[v][tj][ti] : r - L.FI[v][tj][ti];
-                r = r + L.FI[v][tj][ti + 1];
+                const double fl = L.f.FI[fiI<TL>(v, tj, ti)], fr = L.f.FI[fiI<TL>(v, tj, ti + 1)];
+                r = (i == 0) ? r + fl : r - fl;
+                r = r + fr;
                 if (doJ) {
-                    r = (j == 0) ? r + L.FJ[v][tj][ti] : r - L.FJ[v][tj][ti];
-                    r = r + L.FJ[v][tj + 1][ti];
+                    const double gl = L.f.FJ[fiJ<TL>(v, tj, ti)], gr = L.f.FJ[fiJ<TL>(v, tj + 1, ti)];
+                    r = (j == 0) ? r + gl : r - gl;
+                    r = r + gr;
                 }
                 if (THREED && !bladecol) {
                     r = (k == 0) ? r + FKl[v] : r - FKl[v];
@@ -322,112 +317,45 @@ __global__ void __launch_bounds__(NT) k_stage(CaseDev c, StageDev st) {
                 }
                 R[v] = r;
             }
-            // sources (solver.cpp:2790-2897)
-            const bool srcOn = (c.topo == TOPO_AXI) || (THREED && c.bfm_active) || (c.topo == TOPO_2D && c.bfm_active);
-            if (srcOn && i >= c.bi0 && i < c.bi1) {
-                const long s = ((long)k * NJ + j) * (c.bi1 - c.bi0) + (i - c.bi0);
-                const unsigned char fl = c.sflag[s];
-                const double V = c.vol[n];
-                const bool present = (fl & 1) != 0;
-                const bool geo = (c.topo == TOPO_AXI) || (THREED && present);
-                if (geo) {
-                    double e = static_energy<double>(Wc[1], Wc[2], Wc[3], Wc[4]);
-                    double p = c.gm1 * Wc[0] * e;
-                    const double cs = c.s_cs[s], sn = c.s_sn[s];
-                    V3 vcy = cyl_of(v3(Wc[1], Wc[2], Wc[3]), cs, sn);
-                    V3 scy;
-                    double g[5];
-                    if (c.topo == TOPO_AXI) {
-                        const double gf = c.axi_gf[(long)j * NI + i];
-                        scy = v3(0.0, (+Wc[0] * vcy.z * vcy.z + p) * gf, -Wc[0] * vcy.y * vcy.z * gf);
-                        V3 sca = cart_of(scy, cs, sn);
-                        g[0] = 0.0; g[1] = sca.x; g[2] = sca.y; g[3] = sca.z; g[4] = 0.0;
-                    } else {
-                        const double r = c.s_r[s];
-                        scy = v3(0.0, (+Wc[0] * vcy.z * vcy.z + p) / r, -Wc[0] * vcy.y * vcy.z / r);
-                        V3 sca = cart_of(scy, cs, sn);
-                        g[0] = 0.0 * V; g[1] = sca.x * V; g[2] = sca.y * V; g[3] = sca.z * V; g[4] = 0.0 * V;
-                    }
-#pragma unroll
-                    for (int v = 0; v < 5; ++v) R[v] -= g[v];
-                }
-                if (THREED && present) {
-                    // Gong source -Omega dU/dtheta, one-sided 2nd order (solver.cpp:3074-3145)
-                    const double om = c.s_om[s];
-                    if (!(fabs(om) < 1E-3)) {
-                        double U0g[5], U1g[5], U2g[5];
-                        if (om > 0) {
-                            const int k1 = (k == 0) ? NK - 2 : (k == 1 ? 0 : k - 1);
-                            const int k2 = (k == 0) ? NK - 3 : (k == 1 ? NK - 2 : k - 2);
-                            load5(st.Uin, N, n, U0g);
-                            load5(st.Uin, N, nid(c, i, j, k1), U1g);
-                            load5(st.Uin, N, nid(c, i, j, k2), U2g);
-                        } else {
-                            int k0g = k, k1 = k + 1, k2 = k + 2;
-                            if (k == NK - 1) { k0g = 0; k1 = 1; k2 = 2; }
-                            else if (k == NK - 2) { k1 = 0; k2 = 1; }
-                            load5(st.Uin, N, nid(c, i, j, k0g), U0g);
-                            load5(st.Uin, N, nid(c, i, j, k1), U1g);
-                            load5(st.Uin, N, nid(c, i, j, k2), U2g);
-                        }
-#pragma unroll
-                        for (int v = 0; v < 5; ++v) {
-                            double d;
-                            if (om > 0.0) d = (U0g[v] * 3.0 - U1g[v] * 4.0 + U2g[v]) / (2 * c.dtheta);
-                            else d = (U0g[v] * (-3.0) + U1g[v] * 4.0 - U2g[v]) / (2 * c.dtheta);
-                            R[v] -= (-d * om) * V;
-                        }
-                    }
-                }
-                if (c.bfm_active && (fl & 2)) {
-                    double S[5], fi[3], fv[3], dv;
-                    bfm_source(c, s, Wc, V, S, fi, fv, dv);
-#pragma unroll
-                    for (int v = 0; v < 5; ++v) R[v] -= S[v];
-                    if (st.last && (fl & 4)) {
-                        c.finv[3 * n] = fi[0]; c.finv[3 * n + 1] = fi[1]; c.finv[3 * n + 2] = fi[2];
-                        c.fvisc[3 * n] = fv[0]; c.fvisc[3 * n + 1] = fv[1]; c.fvisc[3 * n + 2] = fv[2];
-                        c.dev[n] = dv;
-                    }
-                }
-            }
+            if (srcOn && i >= c.bi0 && i < c.bi1) node_sources<THREED>(st.cdev, Uin, st.last, i, j, k, n, Wc, R);
             if (THREED && c.periodic && (k == 0 || k == NK - 1)) {
                 double* rp = st.rper + (long)(k == 0 ? 0 : 5) * NI * NJ;
 #pragma unroll
                 for (int v = 0; v < 5; ++v) rp[(long)v * NI * NJ + (long)j * NI + i] = R[v];
+            } else if (st.rmode) {
+#pragma unroll
+                for (int v = 0; v < 5; ++v) st.Uout[v * N + n] = R[v];
             } else {
                 const double dtv = (c.dt_mode == 1) ? c.dtmin[0] : c.dt[n];
                 const double factor = st.alpha * dtv / c.vol[n];
 #pragma unroll
-                for (int v = 0; v < 5; ++v) {
-                    st.Uout[v * N + n] = st.U0[v * N + n] - R[v] * factor;
-                    nrm[v] += R[v] * R[v];
-                }
+                for (int v = 0; v < 5; ++v) st.Uout[v * N + n] = st.U0[v * N + n] - R[v] * factor;
+                if (st.last)
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) L.red[v][tid] += R[v] * R[v];
             }
         }
 #pragma unroll
         for (int v = 0; v < 5; ++v) FKl[v] = FKr[v];
         __syncthreads();
     }
-    // ---- per-block norm partial sums (deterministic tree)
+    // ---- per-block norm partial sums (deterministic tree, through the U region)
     if (st.last) {
         const long b = ((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-        for (int v = 0; v < 5; ++v) {
-            L.red[tid] = nrm[v];
-            __syncthreads();
-            for (int s = NT / 2; s > 0; s >>= 1) {
-                if (tid < s) L.red[tid] += L.red[tid + s];
-                __syncthreads();
-            }
-            if (tid == 0) st.partial[b * 5 + v] = L.red[0];
+        for (int s = NT / 2; s > 0; s >>= 1) {
+            if (tid < s)
+#pragma unroll
+                for (int v = 0; v < 5; ++v) L.red[v][tid] += L.red[v][tid + s];
             __syncthreads();
         }
+        if (tid < 5) st.partial[b * 5 + tid] = L.red[tid][0];
     }
 }
 
 // Periodic closure of the k = 0 / k = NK-1 node-mates: residual averaging
 // (solver.cpp:1836-1853), RK update of both, then solution averaging
 // (solver.cpp:2308-2327).  Writes its own norm partials after the stage kernel's.
+// With st.rmode the averaged residuals themselves are written (residual entry point).
 __global__ void __launch_bounds__(256) k_periodic(CaseDev c, StageDev st, long partial_offset) {
     __shared__ double red[256];
     if (*c.stop) return;
@@ -450,29 +378,37 @@ __global__ void __launch_bounds__(256) k_periodic(CaseDev c, StageDev st, long p
 #pragma unroll
         for (int v = 0; v < 5; ++v) back[v] = avg[v];
         rotx(back, c.per_c, c.per_s);
-        const double dt0 = (c.dt_mode == 1) ? c.dtmin[0] : c.dt[n0];
-        const double dt1 = (c.dt_mode == 1) ? c.dtmin[0] : c.dt[n1];
-        const double f0 = st.alpha * dt0 / c.vol[n0];
-        const double f1 = st.alpha * dt1 / c.vol[n1];
-        double Ua[5], Ub[5];
+        if (st.rmode) {
 #pragma unroll
-        for (int v = 0; v < 5; ++v) {
-            Ua[v] = st.U0[v * N + n0] - avg[v] * f0;
-            Ub[v] = st.U0[v * N + n1] - back[v] * f1;
-            nrm[v] = avg[v] * avg[v] + back[v] * back[v];
-        }
-        rotx(Ub, c.per_cm, c.per_sm);
-        double Uavg[5];
+            for (int v = 0; v < 5; ++v) {
+                st.Uout[v * N + n0] = avg[v];
+                st.Uout[v * N + n1] = back[v];
+            }
+        } else {
+            const double dt0 = (c.dt_mode == 1) ? c.dtmin[0] : c.dt[n0];
+            const double dt1 = (c.dt_mode == 1) ? c.dtmin[0] : c.dt[n1];
+            const double f0 = st.alpha * dt0 / c.vol[n0];
+            const double f1 = st.alpha * dt1 / c.vol[n1];
+            double Ua[5], Ub[5];
 #pragma unroll
-        for (int v = 0; v < 5; ++v) Uavg[v] = (Ua[v] + Ub[v]) * 0.5;
-        double Uback[5];
+            for (int v = 0; v < 5; ++v) {
+                Ua[v] = st.U0[v * N + n0] - avg[v] * f0;
+                Ub[v] = st.U0[v * N + n1] - back[v] * f1;
+                nrm[v] = avg[v] * avg[v] + back[v] * back[v];
+            }
+            rotx(Ub, c.per_cm, c.per_sm);
+            double Uavg[5];
 #pragma unroll
-        for (int v = 0; v < 5; ++v) Uback[v] = Uavg[v];
-        rotx(Uback, c.per_c, c.per_s);
+            for (int v = 0; v < 5; ++v) Uavg[v] = (Ua[v] + Ub[v]) * 0.5;
+            double Uback[5];
 #pragma unroll
-        for (int v = 0; v < 5; ++v) {
-            st.Uout[v * N + n0] = Uavg[v];
-            st.Uout[v * N + n1] = Uback[v];
+            for (int v = 0; v < 5; ++v) Uback[v] = Uavg[v];
+            rotx(Uback, c.per_c, c.per_s);
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+                st.Uout[v * N + n0] = Uavg[v];
+                st.Uout[v * N + n1] = Uback[v];
+            }
         }
     }
     if (st.last) {
@@ -559,9 +495,6 @@ __global__ void __launch_bounds__(256) k_norms(CaseDev c, const double* partial,
         *c.iter = it + 1;
     }
 }
-
-// Per-iteration bookkeeping before the stages: keep the iteration counter.
-__global__ void k_noop(CaseDev c) {}
 
 }  // namespace dev
 }  // namespace tbfm

@@ -64,6 +64,7 @@ struct tbfm_case {
     hipStream_t stream = nullptr;
     std::vector<void*> owned;
     CaseDev c{};
+    CaseDev* cdev = nullptr;          // device-resident copy of c
     double* U[3] = {nullptr, nullptr, nullptr};
     int u0 = 0;                       // index of the buffer holding U^(0)
     double* rper = nullptr;
@@ -73,7 +74,10 @@ struct tbfm_case {
     dev::ConvDev conv{};
     long hist_cap = 0;
     long iter_host = 1;               // next iteration number
-    int kc = 8;
+    int kc = 8;                       // k planes per stage-kernel block
+    int tile = 0;                     // 0: 32x8, 1: 16x16, 2: 64x4 (i x j columns)
+    bool tuned = false;
+    std::string tuning;               // autotuner report
     dim3 grid_stage;
     // profiling
     bool prof = false;
@@ -122,6 +126,17 @@ void faceSoA(const std::vector<double>& src, long ai, long aj, long ak, std::vec
                 const long d = (k * aj + j) * ai + i;
                 x[d] = src[s]; y[d] = src[s + 1]; z[d] = src[s + 2];
             }
+}
+
+int tileI(int t) { return t == 0 ? 32 : (t == 1 ? 16 : 64); }
+
+void setTiling(tbfm_case* cs, int tile, int kc) {
+    const int ti = tileI(tile), tj = 256 / ti;
+    cs->tile = tile;
+    cs->kc = kc;
+    cs->grid_stage = dim3((unsigned)((cs->ni + ti - 1) / ti), (unsigned)((cs->nj + tj - 1) / tj),
+                          (unsigned)((cs->nk + kc - 1) / kc));
+    cs->nparts_stage = (long)cs->grid_stage.x * cs->grid_stage.y * cs->grid_stage.z;
 }
 
 double atan2Positive(double y, double x) {
@@ -456,11 +471,16 @@ void setup(tbfm_case* cs, const GridInput& grid) {
     for (int b = 0; b < 3; ++b) cs->U[b] = dalloc<double>(5 * N, own);
     cs->rper = dalloc<double>(10 * NI * NJ, own);
     cs->kc = (cs->topo == Topology::ThreeD) ? 8 : 1;
-    cs->grid_stage = dim3((unsigned)((NI + dev::TI - 1) / dev::TI), (unsigned)((NJ + dev::TJ - 1) / dev::TJ),
-                          (unsigned)((NK + cs->kc - 1) / cs->kc));
-    cs->nparts_stage = (long)cs->grid_stage.x * cs->grid_stage.y * cs->grid_stage.z;
+    cs->tile = 0;
+    setTiling(cs, cs->tile, cs->kc);
     cs->nparts_per = periodic ? (NI * NJ + 255) / 256 : 0;
-    cs->partial = dalloc<double>(5 * (cs->nparts_stage + cs->nparts_per), own);
+    // norm partials sized for the finest tiling the autotuner may pick
+    long maxParts = 0;
+    for (int t = 0; t < 3; ++t) {
+        const int ti = tileI(t), tj = 256 / ti;
+        maxParts = std::max(maxParts, ((NI + ti - 1) / ti) * ((NJ + tj - 1) / tj) * NK);
+    }
+    cs->partial = dalloc<double>(5 * (maxParts + cs->nparts_per), own);
     cs->nblk_dt = (N + 255) / 256;
     cs->partial_min = dalloc<double>(cs->nblk_dt, own);
     cs->hist_cap = std::max<long>(cfg.has("N_ITERATIONS") ? cfg.getMaxIterations() : 0, 1);
@@ -473,6 +493,8 @@ void setup(tbfm_case* cs, const GridInput& grid) {
     cs->conv.steady = cfg.isSimulationSteady() ? 1 : 0;
     long one = 1;
     HIPCHK(hipMemcpy(c.iter, &one, sizeof(long), hipMemcpyHostToDevice));
+    cs->cdev = dalloc<CaseDev>(1, own);
+    HIPCHK(hipMemcpy(cs->cdev, &c, sizeof(CaseDev), hipMemcpyHostToDevice));
     // algorithmic bytes per node and stage: U^(k-1), U^(0), U^(k) (5 each), dt, V and one
     // face vector per active direction
     const int dirs = (cs->topo == Topology::ThreeD) ? 3 : (cs->topo == Topology::OneD ? 1 : 2);
@@ -521,15 +543,33 @@ int guarded(tbfm_case* c, F&& f) {
 }
 
 // ------------------------------------------------------------------- launches
-template <int SCH, bool THREED>
-void launchStage(tbfm_case* cs, const StageDev& st) {
+template <class TL, int SCH, bool THREED>
+void launchStagePrec(tbfm_case* cs, const StageDev& st) {
     if (cs->precision == Precision::Mixed)
-        hipLaunchKernelGGL((dev::k_stage<SCH, THREED, float>), cs->grid_stage, dim3(dev::NT), 0, cs->stream, cs->c, st);
+        hipLaunchKernelGGL((dev::k_stage<TL, SCH, THREED, float>), cs->grid_stage, dim3(dev::NT), 0, cs->stream, cs->c, st);
     else
-        hipLaunchKernelGGL((dev::k_stage<SCH, THREED, double>), cs->grid_stage, dim3(dev::NT), 0, cs->stream, cs->c, st);
+        hipLaunchKernelGGL((dev::k_stage<TL, SCH, THREED, double>), cs->grid_stage, dim3(dev::NT), 0, cs->stream, cs->c, st);
 }
 
-void stage(tbfm_case* cs, const double* Uin, double* Uout, double alpha, bool last) {
+template <int SCH, bool THREED>
+void launchStage(tbfm_case* cs, const StageDev& st) {
+    switch (cs->tile) {
+        case 1: launchStagePrec<dev::Tile<16, 16>, SCH, THREED>(cs, st); break;
+        case 2: launchStagePrec<dev::Tile<64, 4>, SCH, THREED>(cs, st); break;
+        default: launchStagePrec<dev::Tile<32, 8>, SCH, THREED>(cs, st); break;
+    }
+}
+
+void launchStageKernel(tbfm_case* cs, const StageDev& st) {
+    const bool three = (cs->topo == Topology::ThreeD);
+    if (cs->scheme == Scheme::JST) {
+        if (three) launchStage<0, true>(cs, st); else launchStage<0, false>(cs, st);
+    } else {
+        if (three) launchStage<1, true>(cs, st); else launchStage<1, false>(cs, st);
+    }
+}
+
+void stage(tbfm_case* cs, const double* Uin, double* Uout, double alpha, bool last, bool rmode = false) {
     CaseDev& c = cs->c;
     const long nb = (long)(2 * cs->nj * cs->nk + 2 * cs->ni * cs->nk + 2 * cs->ni * cs->nj);
     hipLaunchKernelGGL(dev::k_boundary, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, cs->stream, c, Uin);
@@ -542,6 +582,8 @@ void stage(tbfm_case* cs, const double* Uin, double* Uout, double alpha, bool la
     st.rper = cs->rper;
     st.partial = cs->partial;
     st.kc = cs->kc;
+    st.cdev = cs->cdev;
+    st.rmode = rmode ? 1 : 0;
     if (cs->prof) {
         if (cs->pool_used * 2 + 2 > cs->pool.size()) {
             hipEvent_t a, b;
@@ -552,12 +594,7 @@ void stage(tbfm_case* cs, const double* Uin, double* Uout, double alpha, bool la
         }
         HIPCHK(hipEventRecord(cs->pool[2 * cs->pool_used], cs->stream));
     }
-    const bool three = (cs->topo == Topology::ThreeD);
-    if (cs->scheme == Scheme::JST) {
-        if (three) launchStage<0, true>(cs, st); else launchStage<0, false>(cs, st);
-    } else {
-        if (three) launchStage<1, true>(cs, st); else launchStage<1, false>(cs, st);
-    }
+    launchStageKernel(cs, st);
     if (cs->prof) {
         HIPCHK(hipEventRecord(cs->pool[2 * cs->pool_used + 1], cs->stream));
         cs->pool_used += 1;
@@ -577,6 +614,50 @@ void timestep(tbfm_case* cs, const double* U) {
     if (cs->c.dt_mode == 1)
         hipLaunchKernelGGL(dev::k_min_reduce, dim3(1), dim3(256), 0, cs->stream, cs->c, (const double*)cs->partial_min,
                            (int)cs->nblk_dt);
+}
+
+// Autotuner of the stage kernel's tile shape and k-planes per block: times a few
+// launches of every candidate on the case's current state (the residual mode writes to
+// scratch, the state is untouched) and keeps the fastest.  Results are identical for
+// every tiling; only the time differs.
+void autotune(tbfm_case* cs) {
+    if (cs->tuned) return;
+    cs->tuned = true;
+    if (cs->nk <= 1 && cs->ni * cs->nj < 4096) return;   // tiny 2-D cases: default shape
+    const double* U0 = cs->U[cs->u0];
+    double* scratch = cs->U[(cs->u0 + 1) % 3];
+    timestep(cs, U0);
+    const long nb = (long)(2 * cs->nj * cs->nk + 2 * cs->ni * cs->nk + 2 * cs->ni * cs->nj);
+    hipLaunchKernelGGL(dev::k_boundary, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, cs->stream, cs->c, U0);
+    std::vector<int> kcs = (cs->topo == Topology::ThreeD) ? std::vector<int>{2, 4, 8, 16} : std::vector<int>{1};
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    float best = 1e30f;
+    int bt = cs->tile, bk = cs->kc;
+    char buf[96];
+    for (int t = 0; t < 3; ++t)
+        for (int kc : kcs) {
+            setTiling(cs, t, kc);
+            StageDev st{};
+            st.Uin = U0; st.U0 = U0; st.Uout = scratch; st.alpha = 1.0; st.last = 0; st.rper = cs->rper;
+            st.partial = cs->partial; st.kc = kc; st.cdev = cs->cdev; st.rmode = 1;
+            launchStageKernel(cs, st);   // warm-up
+            HIPCHK(hipEventRecord(e0, cs->stream));
+            for (int r = 0; r < 3; ++r) launchStageKernel(cs, st);
+            HIPCHK(hipEventRecord(e1, cs->stream));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            std::snprintf(buf, sizeof buf, "%dx%d/kc%d:%.4fms ", tileI(t), 256 / tileI(t), kc, ms / 3);
+            cs->tuning += buf;
+            if (ms < best) { best = ms; bt = t; bk = kc; }
+        }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    setTiling(cs, bt, bk);
+    std::snprintf(buf, sizeof buf, "-> %dx%d/kc%d", tileI(bt), 256 / tileI(bt), bk);
+    cs->tuning += buf;
 }
 
 // One explicit iteration (solver.cpp:1337-1381); rotates the three state buffers.
@@ -727,17 +808,28 @@ int tbfm_gpu_advance(tbfm_case* c, long n, double* logres, long* converged_at) {
             c->conv.hist_len = cap;
             c->hist_cap = cap;
         }
-        for (long q = 0; q < n; ++q) iteration(c);
+        autotune(c);
+        const int u0Start = c->u0;
+        int stopped = 0;
+        HIPCHK(hipMemcpy(&stopped, c->c.stop, sizeof(int), hipMemcpyDeviceToHost));
+        if (!stopped)
+            for (long q = 0; q < n; ++q) iteration(c);
         HIPCHK(hipStreamSynchronize(c->stream));
         if (c->prof) drainProfile(c);
-        c->iter_host += n;
-        long conv = 0;
+        long conv = 0, devIter = 0;
         int div = 0;
         HIPCHK(hipMemcpy(&conv, c->conv.conv_at, sizeof(long), hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(&div, c->conv.diverged, sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(&devIter, c->c.iter, sizeof(long), hipMemcpyDeviceToHost));
+        // Iterations launched after the device stopped (convergence or divergence) were
+        // no-ops: the state is the one after the last executed iteration, whose buffer
+        // follows from the rotation (RK41A advances U^(0) by 2 buffers, RK31B by 1).
+        const long executed = stopped ? 0 : (devIter - first);
+        const int step = (c->rk.size() % 2 == 0) ? 2 : 1;
+        c->u0 = (int)((u0Start + step * executed) % 3);
+        c->iter_host = first + executed;
         if (logres) {
-            long last = first + n - 1;
-            if (conv > 0 && conv < last) last = conv;
+            long last = first + executed - 1;
             long rows = std::max<long>(last - first + 1, 0);
             std::vector<double> h(5 * c->hist_cap);
             HIPCHK(hipMemcpy(h.data(), c->conv.hist, h.size() * 8, hipMemcpyDeviceToHost));
@@ -782,9 +874,6 @@ int tbfm_gpu_compute_timestep(tbfm_case* c, const double* U, double* dt) {
 int tbfm_gpu_compute_residual(tbfm_case* c, const double* U, long it, int update_profiles, double* R) {
     if (!c || !U || !R) return fail(c, TBFM_ERR_ARGUMENT, "null argument");
     return guarded(c, [&]() {
-        // R = (U0 - Uout) / (alpha dt / V) is ill-conditioned; instead run the stage with
-        // U0 = 0, alpha = -1 and dt = V so that Uout = R exactly:
-        //   factor = (-1 * V) / V = -1  ->  Uout = 0 - R * (-1) = R
         std::vector<double> dU(5 * c->N), tmp;
         for (int v = 0; v < 5; ++v) {
             toDevice(c, U + v * c->N, tmp);
@@ -793,37 +882,20 @@ int tbfm_gpu_compute_residual(tbfm_case* c, const double* U, long it, int update
         const int a = (c->u0 + 1) % 3, b = (c->u0 + 2) % 3;
         HIPCHK(hipMemcpy(c->U[a], dU.data(), dU.size() * 8, hipMemcpyHostToDevice));
         long itv = it;
-        int zero = 0;
+        int stopSaved = 0, zero = 0;
+        HIPCHK(hipMemcpy(&stopSaved, c->c.stop, sizeof(int), hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(c->c.iter, &itv, sizeof(long), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(c->c.stop, &zero, sizeof(int), hipMemcpyHostToDevice));
         if (update_profiles && c->c.nprof > 0)
             hipLaunchKernelGGL(dev::k_radial_profile, dim3(1), dim3(64), 0, c->stream, c->c, (const double*)c->U[a]);
-        // dt := V (device layout), U0 := 0
-        CaseDev saved = c->c;
-        double* dtv = dalloc<double>(c->N, c->owned);
-        HIPCHK(hipMemcpy(dtv, c->c.vol, c->N * 8, hipMemcpyDeviceToDevice));
-        double* zeros = dalloc<double>(5 * c->N, c->owned);
-        c->c.dt = dtv;
-        c->c.dt_mode = 0;
-        int u0save = c->u0;
-        double* u0buf = c->U[c->u0];
-        c->U[c->u0] = zeros;
-        stage(c, c->U[a], c->U[b], -1.0, false);
+        stage(c, c->U[a], c->U[b], 0.0, false, true);
         HIPCHK(hipStreamSynchronize(c->stream));
-        c->U[c->u0] = u0buf;
-        c->u0 = u0save;
-        c->c = saved;
         std::vector<double> d(5 * c->N);
         HIPCHK(hipMemcpy(d.data(), c->U[b], d.size() * 8, hipMemcpyDeviceToHost));
         for (int v = 0; v < 5; ++v) toHost(c, d.data() + v * c->N, R + v * c->N);
-        // restore the iteration counter
         itv = c->iter_host;
         HIPCHK(hipMemcpy(c->c.iter, &itv, sizeof(long), hipMemcpyHostToDevice));
-        // free the temporaries
-        hipFree(dtv);
-        hipFree(zeros);
-        c->owned.pop_back();
-        c->owned.pop_back();
+        HIPCHK(hipMemcpy(c->c.stop, &stopSaved, sizeof(int), hipMemcpyHostToDevice));
         return TBFM_OK;
     });
 }
@@ -1043,6 +1115,8 @@ int tbfm_gpu_copy_calibration(long nbytes, int repeats, double* ms_per_copy) {
         return TBFM_OK;
     });
 }
+
+const char* tbfm_gpu_tuning(const tbfm_case* c) { return c ? c->tuning.c_str() : ""; }
 
 int tbfm_gpu_sync(tbfm_case* c) {
     if (!c) return fail(c, TBFM_ERR_ARGUMENT, "null argument");

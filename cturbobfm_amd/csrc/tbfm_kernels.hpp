@@ -18,12 +18,7 @@
 namespace tbfm {
 namespace dev {
 
-constexpr int TI = 32;          // tile width along i (one half-wavefront row)
-constexpr int TJ = 8;           // tile rows along j
-constexpr int NT = TI * TJ;     // threads per block
-constexpr int HI = TI + 4;      // LDS plane width with a 2-node halo on each side
-constexpr int HJ = TJ + 4;
-constexpr int NP = HI * HJ;
+constexpr int NT = 256;         // threads per stage-kernel block (tile columns)
 
 enum : int { PT_INLET = 0, PT_OUTLET = 1, PT_RADIAL = 2, PT_WALL = 3, PT_FAKE = 4 };
 enum : int { TOPO_2D = 0, TOPO_3D = 1, TOPO_AXI = 2, TOPO_1D = 3 };
@@ -93,6 +88,8 @@ struct StageDev {
     double* rper;          // [2][5][NI*NJ] residuals of the periodic planes
     double* partial;       // [blocks][5] norm partial sums
     int kc;                // k planes per block
+    const struct CaseDev* cdev;   // device-resident copy of the case description
+    int rmode;             // write the residual instead of the update (residual entry point)
 };
 
 __device__ __forceinline__ double dmaxd(double a, double b) { return (a < b) ? b : a; }

@@ -49,6 +49,82 @@ __device__ inline void load_k(const CaseDev& c, const double* U, int i, int j, i
     }
 }
 
+// ------------------------------------------------------------------ node sources
+// Geometric (3-D blade rows: solver.cpp:2855-2867; axisymmetric: :2833-2852), Gong
+// (:2870-2876, :3074-3145) and body-force (:2878-2893) sources of one node, subtracted
+// from R in that order.  Out of line and reading the case from its device-resident
+// copy: only the body-force slab runs it, and keeping it out of the fused kernel's
+// register allocation doubles that kernel's occupancy.
+template <bool THREED>
+__device__ __noinline__ void node_sources(const CaseDev* __restrict__ cd, const double* __restrict__ Uin, int last,
+                                          int i, int j, int k, long n, const double* Wc, double* R) {
+    const CaseDev& c = *cd;
+    const int NI = c.ni, NJ = c.nj, NK = c.nk;
+    const long N = c.N;
+    const long s = ((long)k * NJ + j) * (c.bi1 - c.bi0) + (i - c.bi0);
+    const unsigned char fl = c.sflag[s];
+    const double V = c.vol[n];
+    const bool present = (fl & 1) != 0;
+    const bool geo = (c.topo == TOPO_AXI) || (THREED && present);
+    if (geo) {
+        double e = static_energy<double>(Wc[1], Wc[2], Wc[3], Wc[4]);
+        double p = c.gm1 * Wc[0] * e;
+        const double cs = c.s_cs[s], sn = c.s_sn[s];
+        V3 vcy = cyl_of(v3(Wc[1], Wc[2], Wc[3]), cs, sn);
+        V3 scy;
+        double g[5];
+        if (c.topo == TOPO_AXI) {
+            const double gf = c.axi_gf[(long)j * NI + i];
+            scy = v3(0.0, (+Wc[0] * vcy.z * vcy.z + p) * gf, -Wc[0] * vcy.y * vcy.z * gf);
+            V3 sca = cart_of(scy, cs, sn);
+            g[0] = 0.0; g[1] = sca.x; g[2] = sca.y; g[3] = sca.z; g[4] = 0.0;
+        } else {
+            const double r = c.s_r[s];
+            scy = v3(0.0, (+Wc[0] * vcy.z * vcy.z + p) / r, -Wc[0] * vcy.y * vcy.z / r);
+            V3 sca = cart_of(scy, cs, sn);
+            g[0] = 0.0 * V; g[1] = sca.x * V; g[2] = sca.y * V; g[3] = sca.z * V; g[4] = 0.0 * V;
+        }
+#pragma unroll
+        for (int v = 0; v < 5; ++v) R[v] -= g[v];
+    }
+    if (THREED && present) {
+        const double om = c.s_om[s];
+        if (!(fabs(om) < 1E-3)) {
+            double U0g[5], U1g[5], U2g[5];
+            int k0g = k, k1, k2;
+            if (om > 0) {
+                k1 = (k == 0) ? NK - 2 : (k == 1 ? 0 : k - 1);
+                k2 = (k == 0) ? NK - 3 : (k == 1 ? NK - 2 : k - 2);
+            } else {
+                k1 = k + 1; k2 = k + 2;
+                if (k == NK - 1) { k0g = 0; k1 = 1; k2 = 2; }
+                else if (k == NK - 2) { k1 = 0; k2 = 1; }
+            }
+            load5(Uin, N, nid(c, i, j, k0g), U0g);
+            load5(Uin, N, nid(c, i, j, k1), U1g);
+            load5(Uin, N, nid(c, i, j, k2), U2g);
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+                double d;
+                if (om > 0.0) d = (U0g[v] * 3.0 - U1g[v] * 4.0 + U2g[v]) / (2 * c.dtheta);
+                else d = (U0g[v] * (-3.0) + U1g[v] * 4.0 - U2g[v]) / (2 * c.dtheta);
+                R[v] -= (-d * om) * V;
+            }
+        }
+    }
+    if (c.bfm_active && (fl & 2)) {
+        double S[5], fi[3], fv[3], dv;
+        bfm_source(c, s, Wc, V, S, fi, fv, dv);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) R[v] -= S[v];
+        if (last && (fl & 4)) {
+            c.finv[3 * n] = fi[0]; c.finv[3 * n + 1] = fi[1]; c.finv[3 * n + 2] = fi[2];
+            c.fvisc[3 * n] = fv[0]; c.fvisc[3 * n + 1] = fv[1]; c.fvisc[3 * n + 2] = fv[2];
+            c.dev[n] = dv;
+        }
+    }
+}
+
 // --------------------------------------------------------------------- time step
 // Solver::computeTimestepArray, inviscid (src/solver.cpp:1531-1630)
 __global__ void __launch_bounds__(256) k_timestep(CaseDev c, const double* U, double* partial_min) {

@@ -130,6 +130,10 @@ int tbfm_gpu_profile_get(tbfm_case* c, double* stage_ms, long* launches, double*
  * residual kernels to HBM bytes. */
 int tbfm_gpu_copy_calibration(long nbytes, int repeats, double* ms_per_copy);
 
+/* Report of the stage-kernel autotuner (tile shape and k planes per block, timed on
+ * the case's own state before its first iteration); empty before the first advance. */
+const char* tbfm_gpu_tuning(const tbfm_case* c);
+
 /* Synchronises the case stream. */
 int tbfm_gpu_sync(tbfm_case* c);
 
