@@ -1,14 +1,16 @@
 // The fused residual + RK stage kernel.
 //
 // One 256-thread block owns a TI x TJ tile of (i, j) columns and marches along k over
-// `kc` planes.  Per plane it stages the conserved state, primitives, pressure and the
-// JST spectral factor |u| + a of the tile plus a 2-node halo in LDS (ghost nodes at the
-// domain ends are built exactly as the reference extrapolates them), evaluates every I
-// and J face of the tile once (JST pressure sensors inline from the staged pressures),
-// and takes the K faces from a per-thread register queue of the column (planes
-// k-1..k+2).  Face results go straight to their own LDS arrays (no register staging
-// across a barrier) and the residual norms accumulate in LDS, which keeps the register
-// budget at two waves per SIMD.
+// `kc` planes.  Per plane it stages the tile plus a 2-node halo in LDS (JST: conserved
+// state, primitives, pressure and spectral factor |u| + a; Roe: primitives only).  The
+// own node of each column comes from a per-thread register queue along k (planes
+// k-1..k+2, which also feeds the K faces); the halo is filled by consecutive lanes, and
+// ghost nodes at the domain ends are built exactly as the reference extrapolates them.
+// Every I and J face of the tile is then evaluated once in a single divergence-free
+// pass over a flat face index, with the face's unit normal and area read from
+// host-precomputed arrays (the same IEEE operations the reference applies per call).
+// Face results go straight to LDS arrays and the residual norms accumulate in LDS,
+// which keeps the register budget at two waves per SIMD.
 // The residual of each node is assembled in the reference's order
 //   (I left, I right, J left, J right, K left, K right, geometric, Gong, BFM)
 // (solver.cpp:1794-1821, 1997-2006, 2853-2891) and the RK update
@@ -51,7 +53,7 @@ __device__ __forceinline__ int fiJ(int v, int r, int ti) { return (v * (TL::TJ +
 
 // Fills LDS position `pos` of plane k: real node, ghost (one step outside the domain in
 // i or j) or unused.
-template <class TL>
+template <class TL, int SCH>
 __device__ inline void fill_position(const CaseDev& c, const double* __restrict__ U, PlaneLds<TL>& L, int pos,
                                      int i0, int j0, int k) {
     const int hi = pos % TL::HI, hj = pos / TL::HI;
@@ -77,23 +79,33 @@ __device__ inline void fill_position(const CaseDev& c, const double* __restrict_
         for (int v = 0; v < 5; ++v) Un[v] = (gj < 0) ? a[v] - (b[v] - a[v]) : a[v] + (a[v] - b[v]);
     } else {
 #pragma unroll
-        for (int v = 0; v < 5; ++v) { L.U[v][pos] = 1.0; L.d.W[v][pos] = 1.0; }
-        L.d.P[pos] = 1.0;
-        L.d.RF[pos] = 0.0;
+        for (int v = 0; v < 5; ++v) L.d.W[v][pos] = 1.0;
+        if constexpr (SCH == 0) {
+#pragma unroll
+            for (int v = 0; v < 5; ++v) L.U[v][pos] = 1.0;
+            L.d.P[pos] = 1.0;
+            L.d.RF[pos] = 0.0;
+        }
         return;
     }
     double W[5], p, rf;
     derive<double>(Un, W, p, rf, c.gamma, c.gm1);
 #pragma unroll
-    for (int v = 0; v < 5; ++v) { L.U[v][pos] = Un[v]; L.d.W[v][pos] = W[v]; }
-    L.d.P[pos] = p;
-    L.d.RF[pos] = rf;
+    for (int v = 0; v < 5; ++v) L.d.W[v][pos] = W[v];
+    if constexpr (SCH == 0) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) L.U[v][pos] = Un[v];
+        L.d.P[pos] = p;
+        L.d.RF[pos] = rf;
+    }
 }
 
-// Face flux*|S| between LDS positions l and r (with ll, rr) along i or j.
+// Face flux*|S| between LDS positions l and r (with ll = l - st, rr = r + st) along i
+// or j; n and area are the face's unit normal and |S|.
 template <class TL, int SCH, typename FT>
-__device__ __forceinline__ void lds_face(const CaseDev& c, const PlaneLds<TL>& L, int ll, int l, int r, int rr,
-                                         double sx, double sy, double sz, double out[5]) {
+__device__ __forceinline__ void lds_face(const CaseDev& c, const PlaneLds<TL>& L, int l, int st, double nx, double ny,
+                                         double nz, double area, double out[5]) {
+    const int r = l + st, ll = l - st, rr = r + st;
     if (SCH == 0) {
         // JST sensors of the left and right nodes from the staged pressures
         const double pll = L.d.P[ll], pl = L.d.P[l], pr = L.d.P[r], prr = L.d.P[rr];
@@ -108,7 +120,8 @@ __device__ __forceinline__ void lds_face(const CaseDev& c, const PlaneLds<TL>& L
         FT t4 = FT(0.02) * rr_ - FT(2.0) * psi2;
         FT psi4 = (FT(0.0) < t4) ? t4 : FT(0.0);
         FT F[5];
-        FT sm = phys_flux<FT>(Wavg, FT(sx), FT(sy), FT(sz), FT(c.gm1), F);
+        phys_flux_n<FT>(Wavg, FT(nx), FT(ny), FT(nz), FT(c.gm1), F);
+        const FT sm = FT(area);
 #pragma unroll
         for (int v = 0; v < 5; ++v) {
             const double ul = L.U[v][l], ur = L.U[v][r];
@@ -123,27 +136,101 @@ __device__ __forceinline__ void lds_face(const CaseDev& c, const PlaneLds<TL>& L
         for (int v = 0; v < 5; ++v) {
             Wll[v] = L.d.W[v][ll]; Wl[v] = L.d.W[v][l]; Wr[v] = L.d.W[v][r]; Wrr[v] = L.d.W[v][rr];
         }
-        roe_face<FT>(Wll, Wl, Wr, Wrr, c.muscl, c.limiter, sx, sy, sz, c.gamma, out);
+        roe_face<FT>(Wll, Wl, Wr, Wrr, c.muscl, c.limiter, nx, ny, nz, area, c.gamma, out);
     }
 }
 
-// K face from the column queue (slots: 0 = left-left, 1 = left, 2 = right, 3 = right-right)
-template <int SCH, typename FT>
-__device__ __forceinline__ void queue_face(const CaseDev& c, const double qU[4][5], const double qW[4][5],
-                                           const double qP[4], const double qRF[4], double sx, double sy,
-                                           double sz, double out[5]) {
-    if (SCH == 0) {
-        double sl = jst_sensor<double>(qP[0], qP[1], qP[2]);
-        double sr = jst_sensor<double>(qP[1], qP[2], qP[3]);
-        jst_face<FT>(qU[0], qU[1], qU[2], qU[3], qW[1], qW[2], qRF[1], qRF[2], sl, sr, sx, sy, sz, c.gm1, out);
-    } else {
-        roe_face<FT>(qW[0], qW[1], qW[2], qW[3], c.muscl, c.limiter, sx, sy, sz, c.gamma, out);
+// Column queue along k: at step k the slots 0..3 hold planes k-1, k, k+1, k+2.  JST
+// keeps the conserved state and pressure of all four slots and the primitives and
+// spectral factor of slots 1 and 2 only (those of a new plane are derived when it
+// reaches slot 2); Roe keeps the primitives of all four.  Unused slot values (planes
+// beyond -1 .. NK) are never read by a face.
+template <int SCH>
+struct KQueue;
+
+template <>
+struct KQueue<0> {
+    double U[4][5], P[4], W[2][5], RF[2];
+    __device__ __forceinline__ void put(const CaseDev& c, int s, const double* u) {
+        double w[5], rf;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) U[s][v] = u[v];
+        derive<double>(U[s], w, P[s], rf, c.gamma, c.gm1);
     }
+    __device__ __forceinline__ void prim2(const CaseDev& c) {   // primitives of slot 2
+        double p;
+        derive<double>(U[2], W[1], p, RF[1], c.gamma, c.gm1);
+    }
+    __device__ __forceinline__ void init_end(const CaseDev& c) {
+        double p;
+        derive<double>(U[1], W[0], p, RF[0], c.gamma, c.gm1);
+        prim2(c);
+    }
+    __device__ __forceinline__ void shift() {
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+#pragma unroll
+            for (int v = 0; v < 5; ++v) U[s][v] = U[s + 1][v];
+            P[s] = P[s + 1];
+        }
+#pragma unroll
+        for (int v = 0; v < 5; ++v) W[0][v] = W[1][v];
+        RF[0] = RF[1];
+    }
+    __device__ __forceinline__ void after_shift(const CaseDev& c) { prim2(c); }
+    __device__ __forceinline__ const double* Wc() const { return W[0]; }
+    template <typename FT>
+    __device__ __forceinline__ void face(const CaseDev& c, int f, double out[5]) const {
+        const double sl = jst_sensor<double>(P[0], P[1], P[2]);
+        const double sr = jst_sensor<double>(P[1], P[2], P[3]);
+        jst_face<FT>(U[0], U[1], U[2], U[3], W[0], W[1], RF[0], RF[1], sl, sr, c.nkx[f], c.nky[f], c.nkz[f],
+                     c.nka[f], c.gm1, out);
+    }
+};
+
+template <>
+struct KQueue<1> {
+    double W[4][5];
+    __device__ __forceinline__ void put(const CaseDev& c, int s, const double* u) {
+        double uu[5], p, rf;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) uu[v] = u[v];
+        derive<double>(uu, W[s], p, rf, c.gamma, c.gm1);
+    }
+    __device__ __forceinline__ void init_end(const CaseDev&) {}
+    __device__ __forceinline__ void shift() {
+#pragma unroll
+        for (int s = 0; s < 3; ++s)
+#pragma unroll
+            for (int v = 0; v < 5; ++v) W[s][v] = W[s + 1][v];
+    }
+    __device__ __forceinline__ void after_shift(const CaseDev&) {}
+    __device__ __forceinline__ const double* Wc() const { return W[1]; }
+    template <typename FT>
+    __device__ __forceinline__ void face(const CaseDev& c, int f, double out[5]) const {
+        roe_face<FT>(W[0], W[1], W[2], W[3], c.muscl, c.limiter, c.nkx[f], c.nky[f], c.nkz[f], c.nka[f], c.gamma,
+                     out);
+    }
+};
+
+// LDS position of halo slot h (0 .. NP-NT-1): two full rows above, two below, and the two
+// columns on each side of the tile rows.  Consecutive lanes take consecutive slots so the
+// halo work occupies as few wavefronts as possible.
+template <class TL>
+__device__ __forceinline__ int halo_pos(int h) {
+    constexpr int HI = TL::HI, TJ = TL::TJ;
+    if (h < 2 * HI) return h;                                   // rows 0, 1
+    h -= 2 * HI;
+    if (h < 2 * HI) return (TJ + 2) * HI + h;                  // rows TJ+2, TJ+3
+    h -= 2 * HI;
+    const int row = 2 + h / 4, col = h % 4;                     // side columns of tile rows
+    return row * HI + ((col < 2) ? col : TL::TI + col);
 }
 
 template <class TL, int SCH, bool THREED, typename FT>
 __global__ void __launch_bounds__(NT) k_stage(CaseDev c, StageDev st) {
     constexpr int TI = TL::TI, TJ = TL::TJ, HI = TL::HI, NP = TL::NP;
+    constexpr int NFI = TJ * (TI + 1), NFJ = (TJ + 1) * TI;
     __shared__ PlaneLds<TL> L;
     if (*c.stop) return;
     const int tid = threadIdx.x;
@@ -155,36 +242,31 @@ __global__ void __launch_bounds__(NT) k_stage(CaseDev c, StageDev st) {
     const int kc0 = blockIdx.z * st.kc;
     const int kc1 = min(NK, kc0 + st.kc);
     const bool doJ = (c.topo != TOPO_1D);
-    const long N = c.N;
+    const int N = (int)c.N;
+    const int NIJ = NI * NJ;
     const int own = (tj + 2) * HI + (ti + 2);
-    const bool bladecol = THREED && c.bfm_active && act && c.blade2d[(long)j * NI + i];
+    const bool bladecol = THREED && c.bfm_active && act && c.blade2d[j * NI + i];
     const bool srcOn = (c.topo == TOPO_AXI) || c.bfm_active;
     const double* __restrict__ Uin = st.Uin;
 
-    // K queue: slots 0..3 hold planes k-1, k, k+1, k+2 at the top of step k
-    double qU[4][5], qW[4][5], qP[4], qRF[4];
+    // K queue of the column (planes k-1 .. k+2 at the top of step k)
+    KQueue<SCH> q;
     double FKl[5] = {0, 0, 0, 0, 0};
     if (THREED && act) {
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const int gk = kc0 - 2 + s;
-            if (gk >= -1 && gk <= NK) {
-                load_k(c, Uin, i, j, gk, qU[s]);
-                derive<double>(qU[s], qW[s], qP[s], qRF[s], c.gamma, c.gm1);
-            } else {
-#pragma unroll
-                for (int v = 0; v < 5; ++v) { qU[s][v] = 1.0; qW[s][v] = 1.0; }
-                qP[s] = 1.0; qRF[s] = 0.0;
-            }
+            double u[5] = {1.0, 1.0, 1.0, 1.0, 1.0};
+            if (gk >= -1 && gk <= NK) load_k(c, Uin, i, j, gk, u);
+            q.put(c, s, u);
         }
+        q.init_end(c);
         if (!bladecol) {
             if (kc0 == 0) {
-                const long q = (long)j * NI + i;
 #pragma unroll
-                for (int v = 0; v < 5; ++v) FKl[v] = c.bK[(long)v * NI * NJ + q];
+                for (int v = 0; v < 5; ++v) FKl[v] = c.bK[v * NIJ + j * NI + i];
             } else {
-                const long sidx = ((long)kc0 * NJ + j) * NI + i;
-                queue_face<SCH, FT>(c, qU, qW, qP, qRF, c.skx[sidx], c.sky[sidx], c.skz[sidx], FKl);
+                q.template face<FT>(c, (kc0 * NJ + j) * NI + i, FKl);
             }
         }
     }
@@ -194,111 +276,95 @@ __global__ void __launch_bounds__(NT) k_stage(CaseDev c, StageDev st) {
 
 #pragma unroll 1
     for (int k = kc0; k < kc1; ++k) {
-        double Uc[5], Wc[5], Pc = 0.0, RFc = 0.0;
+        double Wc[5];
         if (THREED) {
             if (act) {
-#pragma unroll
-                for (int s = 0; s < 3; ++s) {
-#pragma unroll
-                    for (int v = 0; v < 5; ++v) { qU[s][v] = qU[s + 1][v]; qW[s][v] = qW[s + 1][v]; }
-                    qP[s] = qP[s + 1];
-                    qRF[s] = qRF[s + 1];
-                }
+                q.shift();
                 if (k + 2 <= NK) {
-                    load_k(c, Uin, i, j, k + 2, qU[3]);
-                    derive<double>(qU[3], qW[3], qP[3], qRF[3], c.gamma, c.gm1);
+                    double u[5];
+                    load_k(c, Uin, i, j, k + 2, u);
+                    q.put(c, 3, u);
                 }
+                q.after_shift(c);
+                const double* w = q.Wc();
 #pragma unroll
-                for (int v = 0; v < 5; ++v) { Uc[v] = qU[1][v]; Wc[v] = qW[1][v]; }
-                Pc = qP[1];
-                RFc = qRF[1];
+                for (int v = 0; v < 5; ++v) Wc[v] = w[v];
+                // stage the own node (JST: state, primitives, pressure, spectral factor)
+                if constexpr (SCH == 0) {
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) { L.U[v][own] = q.U[1][v]; L.d.W[v][own] = Wc[v]; }
+                    L.d.P[own] = q.P[1];
+                    L.d.RF[own] = q.RF[0];
+                } else {
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) L.d.W[v][own] = Wc[v];
+                }
             }
         } else if (act) {
+            double Uc[5], Pc, RFc;
             load5(Uin, N, nid(c, i, j, k), Uc);
             derive<double>(Uc, Wc, Pc, RFc, c.gamma, c.gm1);
-        }
-        // ---- stage plane k: own node from registers, halo by all threads
-        if (act) {
 #pragma unroll
-            for (int v = 0; v < 5; ++v) { L.U[v][own] = Uc[v]; L.d.W[v][own] = Wc[v]; }
-            L.d.P[own] = Pc;
-            L.d.RF[own] = RFc;
+            for (int v = 0; v < 5; ++v) L.d.W[v][own] = Wc[v];
+            if constexpr (SCH == 0) {
+#pragma unroll
+                for (int v = 0; v < 5; ++v) L.U[v][own] = Uc[v];
+                L.d.P[own] = Pc;
+                L.d.RF[own] = RFc;
+            }
         }
-#pragma unroll 1
-        for (int pos = tid; pos < NP; pos += NT) {
-            const int hi = pos % HI, hj = pos / HI;
-            const bool inTile = (hi >= 2 && hi < TI + 2 && hj >= 2 && hj < TJ + 2);
-            if (inTile && (i0 + hi - 2) < NI && (j0 + hj - 2) < NJ) continue;  // owner wrote it
-            fill_position<TL>(c, Uin, L, pos, i0, j0, k);
-        }
+        // tile positions outside the domain (possibly ghosts) and the halo by contiguous lanes
+        if (!act) fill_position<TL, SCH>(c, Uin, L, own, i0, j0, k);
+        for (int h = tid; h < NP - NT; h += NT) fill_position<TL, SCH>(c, Uin, L, halo_pos<TL>(h), i0, j0, k);
         __syncthreads();
-        // ---- faces, written straight to LDS
-        double FKr[5] = {0, 0, 0, 0, 0};
-        if (act) {
+        // ---- every I and J face of the tile once: q < NFI are I faces (row r, face m),
+        // then J faces (row r of faces, column m); one code path, no divergence
+        for (int fq = tid; fq < NFI + (doJ ? NFJ : 0); fq += NT) {
+            const bool isI = fq < NFI;
+            const int qq = isI ? fq : fq - NFI;
+            const int r = isI ? qq / (TI + 1) : qq / TI;
+            const int m = isI ? qq % (TI + 1) : qq % TI;
+            const int gi = i0 + m, gj = j0 + r;          // I: face i-index, row; J: column, face j-index
+            const bool valid = isI ? (gj < NJ && gi <= NI) : (gi < NI && gj <= NJ);
+            if (!valid) continue;
+            const int gf = isI ? gi : gj, nf = isI ? NI : NJ;
             double F[5];
-            if (i == 0) {
-                const long q = (long)k * NJ + j;
+            if (gf == 0 || gf == nf) {
+                const double* b = isI ? c.bI : c.bJ;
+                const int nsz = isI ? NJ * NK : NI * NK;
+                const int bq = isI ? k * NJ + gj : k * NI + gi;
+                const int off = (gf == 0) ? 0 : 5 * nsz;
 #pragma unroll
-                for (int v = 0; v < 5; ++v) F[v] = c.bI[(long)v * NJ * NK + q];
+                for (int v = 0; v < 5; ++v) F[v] = b[off + v * nsz + bq];
             } else {
-                const long sidx = ((long)k * NJ + j) * (NI + 1) + i;
-                lds_face<TL, SCH, FT>(c, L, own - 2, own - 1, own, own + 1, c.six[sidx], c.siy[sidx], c.siz[sidx], F);
+                const int fidx = isI ? (k * NJ + gj) * (NI + 1) + gi : (k * (NJ + 1) + gj) * NI + gi;
+                const double* nxp = isI ? c.nix : c.njx;
+                const double* nyp = isI ? c.niy : c.njy;
+                const double* nzp = isI ? c.niz : c.njz;
+                const double* nap = isI ? c.nia : c.nja;
+                const int l = isI ? (r + 2) * HI + (m + 1) : (r + 1) * HI + (m + 2);
+                lds_face<TL, SCH, FT>(c, L, l, isI ? 1 : HI, nxp[fidx], nyp[fidx], nzp[fidx], nap[fidx], F);
             }
+            double* out = isI ? L.f.FI : L.f.FJ;
+            const int base = isI ? r * (TI + 1) + m : r * TI + m;
+            const int vstride = isI ? NFI : NFJ;
 #pragma unroll
-            for (int v = 0; v < 5; ++v) L.f.FI[fiI<TL>(v, tj, ti)] = F[v];
-            if (ti == TI - 1 || i == NI - 1) {
-                if (i + 1 == NI) {
-                    const long q = (long)k * NJ + j;
+            for (int v = 0; v < 5; ++v) out[v * vstride + base] = F[v];
+        }
+        // ---- K face on the right of the node (face index k+1) from the column queue
+        double FKr[5] = {0, 0, 0, 0, 0};
+        if (THREED && act && !bladecol) {
+            if (k + 1 == NK) {
 #pragma unroll
-                    for (int v = 0; v < 5; ++v) F[v] = c.bI[(long)(5 + v) * NJ * NK + q];
-                } else {
-                    const long sidx = ((long)k * NJ + j) * (NI + 1) + i + 1;
-                    lds_face<TL, SCH, FT>(c, L, own - 1, own, own + 1, own + 2, c.six[sidx], c.siy[sidx], c.siz[sidx], F);
-                }
-#pragma unroll
-                for (int v = 0; v < 5; ++v) L.f.FI[fiI<TL>(v, tj, ti + 1)] = F[v];
-            }
-            if (doJ) {
-                if (j == 0) {
-                    const long q = (long)k * NI + i;
-#pragma unroll
-                    for (int v = 0; v < 5; ++v) F[v] = c.bJ[(long)v * NI * NK + q];
-                } else {
-                    const long sidx = ((long)k * (NJ + 1) + j) * NI + i;
-                    lds_face<TL, SCH, FT>(c, L, own - 2 * HI, own - HI, own, own + HI, c.sjx[sidx], c.sjy[sidx],
-                                          c.sjz[sidx], F);
-                }
-#pragma unroll
-                for (int v = 0; v < 5; ++v) L.f.FJ[fiJ<TL>(v, tj, ti)] = F[v];
-                if (tj == TJ - 1 || j == NJ - 1) {
-                    if (j + 1 == NJ) {
-                        const long q = (long)k * NI + i;
-#pragma unroll
-                        for (int v = 0; v < 5; ++v) F[v] = c.bJ[(long)(5 + v) * NI * NK + q];
-                    } else {
-                        const long sidx = ((long)k * (NJ + 1) + j + 1) * NI + i;
-                        lds_face<TL, SCH, FT>(c, L, own - HI, own, own + HI, own + 2 * HI, c.sjx[sidx], c.sjy[sidx],
-                                              c.sjz[sidx], F);
-                    }
-#pragma unroll
-                    for (int v = 0; v < 5; ++v) L.f.FJ[fiJ<TL>(v, tj + 1, ti)] = F[v];
-                }
-            }
-            if (THREED && !bladecol) {
-                if (k + 1 == NK) {
-                    const long q = (long)j * NI + i;
-#pragma unroll
-                    for (int v = 0; v < 5; ++v) FKr[v] = c.bK[(long)(5 + v) * NI * NJ + q];
-                } else {
-                    const long sidx = ((long)(k + 1) * NJ + j) * NI + i;
-                    queue_face<SCH, FT>(c, qU, qW, qP, qRF, c.skx[sidx], c.sky[sidx], c.skz[sidx], FKr);
-                }
+                for (int v = 0; v < 5; ++v) FKr[v] = c.bK[(5 + v) * NIJ + j * NI + i];
+            } else {
+                q.template face<FT>(c, ((k + 1) * NJ + j) * NI + i, FKr);
             }
         }
         __syncthreads();
         // ---- residual assembly, sources, RK update
         if (act) {
-            const long n = nid(c, i, j, k);
+            const int n = (k * NJ + j) * NI + i;
             double R[5];
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
@@ -319,9 +385,9 @@ __global__ void __launch_bounds__(NT) k_stage(CaseDev c, StageDev st) {
             }
             if (srcOn && i >= c.bi0 && i < c.bi1) node_sources<THREED>(st.cdev, Uin, st.last, i, j, k, n, Wc, R);
             if (THREED && c.periodic && (k == 0 || k == NK - 1)) {
-                double* rp = st.rper + (long)(k == 0 ? 0 : 5) * NI * NJ;
+                double* rp = st.rper + (k == 0 ? 0 : 5) * NIJ;
 #pragma unroll
-                for (int v = 0; v < 5; ++v) rp[(long)v * NI * NJ + (long)j * NI + i] = R[v];
+                for (int v = 0; v < 5; ++v) rp[v * NIJ + j * NI + i] = R[v];
             } else if (st.rmode) {
 #pragma unroll
                 for (int v = 0; v < 5; ++v) st.Uout[v * N + n] = R[v];
@@ -339,7 +405,7 @@ __global__ void __launch_bounds__(NT) k_stage(CaseDev c, StageDev st) {
         for (int v = 0; v < 5; ++v) FKl[v] = FKr[v];
         __syncthreads();
     }
-    // ---- per-block norm partial sums (deterministic tree, through the U region)
+    // ---- per-block norm partial sums (deterministic tree)
     if (st.last) {
         const long b = ((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
         for (int s = NT / 2; s > 0; s >>= 1) {

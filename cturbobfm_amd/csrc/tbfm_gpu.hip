@@ -154,6 +154,9 @@ void setup(tbfm_case* cs, const GridInput& grid) {
     cs->scheme = cfg.getAdvectionScheme();
     cs->precision = cfg.getGpuPrecision();
     cs->rk = cfg.getTimeIntegrationCoeffs();
+    // kernels index state (5N), face and dual-node arrays with 32-bit offsets
+    if (5 * N >= 2147483647L || (NI + 1) * (NJ + 1) * (NK + 1) >= 2147483647L)
+        throw std::runtime_error("grid too large for one case (5*N must stay below 2^31)");
     if (cs->topo == Topology::ThreeD && NK < 4) throw std::runtime_error("3D topology needs NK >= 4");
     if (cs->topo != Topology::ThreeD && NK != 1) throw std::runtime_error("2D/axisymmetric topology needs NK = 1");
     if (NI < 3 || (cs->topo != Topology::OneD && NJ < 3))
@@ -352,6 +355,22 @@ void setup(tbfm_case* cs, const GridInput& grid) {
     c.sjx = upload(x, own); c.sjy = upload(y, own); c.sjz = upload(z, own);
     faceSoA(g.surfK, NI, NJ, NK + 1, x, y, z);
     c.skx = upload(x, own); c.sky = upload(y, own); c.skz = upload(z, own);
+    auto normals = [&](const std::vector<double>& S, long ai, long aj, long ak, const double** nx, const double** ny,
+                       const double** nz, const double** na) {
+        std::vector<double> a, bx, by, bz;
+        faceSoA(S, ai, aj, ak, bx, by, bz);
+        a.resize(bx.size());
+        for (size_t q = 0; q < bx.size(); ++q) {
+            // computeAdvectionFluxFromPrimitive: normal = surface / surface.magnitude()
+            const double m = std::sqrt(bx[q] * bx[q] + by[q] * by[q] + bz[q] * bz[q]);
+            a[q] = m;
+            bx[q] = bx[q] / m; by[q] = by[q] / m; bz[q] = bz[q] / m;
+        }
+        *nx = upload(bx, own); *ny = upload(by, own); *nz = upload(bz, own); *na = upload(a, own);
+    };
+    normals(g.surfI, NI + 1, NJ, NK, &c.nix, &c.niy, &c.niz, &c.nia);
+    normals(g.surfJ, NI, NJ + 1, NK, &c.njx, &c.njy, &c.njz, &c.nja);
+    normals(g.surfK, NI, NJ, NK + 1, &c.nkx, &c.nky, &c.nkz, &c.nka);
     faceSoA(g.dual, NI + 1, NJ + 1, NK + 1, x, y, z);
     c.dux = upload(x, own); c.duy = upload(y, own); c.duz = upload(z, own);
     std::vector<double> tmp;

@@ -48,6 +48,9 @@ struct CaseDev {
     double dtheta;                // Gong: 2*pi/(NK-1)
     // geometry
     const double *six, *siy, *siz, *sjx, *sjy, *sjz, *skx, *sky, *skz;
+    // unit normals S/|S| and areas |S| of every face (same layout), computed on the host
+    // with the reference's own operations (Vector3D::magnitude, operator/)
+    const double *nix, *niy, *niz, *nia, *njx, *njy, *njz, *nja, *nkx, *nky, *nkz, *nka;
     const double *vol;
     const double *dux, *duy, *duz;     // dual nodes (NI+1)(NJ+1)(NK+1)
     // boundary faces: patch id and inlet direction per face; flux buffers

@@ -50,6 +50,20 @@ __device__ __forceinline__ T phys_flux(const T W[5], T sx, T sy, T sz, T gm1, T 
     return sm;
 }
 
+// Same with the unit normal n = S/|S| supplied (precomputed with identical operations).
+template <typename T>
+__device__ __forceinline__ void phys_flux_n(const T W[5], T nx, T ny, T nz, T gm1, T F[5]) {
+    T un = W[1] * nx + W[2] * ny + W[3] * nz;
+    T e = static_energy(W[1], W[2], W[3], W[4]);
+    T p = gm1 * W[0] * e;
+    T ht = W[4] + p / W[0];
+    F[0] = W[0] * un;
+    F[1] = W[0] * un * W[1] + p * nx;
+    F[2] = W[0] * un * W[2] + p * ny;
+    F[3] = W[0] * un * W[3] + p * nz;
+    F[4] = W[0] * un * ht;
+}
+
 // JST pressure sensor |p1 - 2 p2 + p3| / (p1 + 2 p2 + p3)   (advection_jst.cpp:47-57)
 template <typename T>
 __device__ __forceinline__ T jst_sensor(T p1, T p2, T p3) {
@@ -62,8 +76,8 @@ __device__ __forceinline__ T jst_sensor(T p1, T p2, T p3) {
 template <typename FT>
 __device__ __forceinline__ void jst_face(const double Ull[5], const double Ul[5], const double Ur[5],
                                          const double Urr[5], const double Wl[5], const double Wr[5],
-                                         double rfl, double rfr, double sl, double sr, double sx, double sy,
-                                         double sz, double gm1, double out[5]) {
+                                         double rfl, double rfr, double sl, double sr, double nx, double ny,
+                                         double nz, double area, double gm1, double out[5]) {
     FT Wavg[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) Wavg[v] = (FT(Wl[v]) + FT(Wr[v])) * FT(0.5);
@@ -73,7 +87,8 @@ __device__ __forceinline__ void jst_face(const double Ull[5], const double Ul[5]
     FT t4 = FT(0.02) * r - FT(2.0) * psi2;
     FT psi4 = (FT(0.0) < t4) ? t4 : FT(0.0);
     FT F[5];
-    FT sm = phys_flux<FT>(Wavg, FT(sx), FT(sy), FT(sz), FT(gm1), F);
+    phys_flux_n<FT>(Wavg, FT(nx), FT(ny), FT(nz), FT(gm1), F);
+    const FT sm = FT(area);
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
         double d1 = Ur[v] - Ul[v];
@@ -113,8 +128,8 @@ __device__ __forceinline__ T roe_avg(T rl, T rr, T pl, T pr) {
 // (src/advection_roe.cpp:6-221, include/advection_roe.hpp:63)
 template <typename FT>
 __device__ __forceinline__ void roe_face(const double Wll_[5], const double Wl_[5], const double Wr_[5],
-                                         const double Wrr_[5], int muscl, int lim, double sx_, double sy_,
-                                         double sz_, double g_, double out[5]) {
+                                         const double Wrr_[5], int muscl, int lim, double nx_, double ny_,
+                                         double nz_, double area_, double g_, double out[5]) {
     const FT g = FT(g_), gm1 = FT(g_ - 1.0);
     FT Wl[5], Wr[5];
 #pragma unroll
@@ -130,12 +145,9 @@ __device__ __forceinline__ void roe_face(const double Wll_[5], const double Wl_[
             Wr[v] = wr - lR * (wrr - wr) * FT(0.5);
         }
     }
-    const FT sx = FT(sx_), sy = FT(sy_), sz = FT(sz_);
-    // normal triad (:41-58)
-    FT sm = sqrt(sx * sx + sy * sy + sz * sz);
-    FT n1x, n1y, n1z;
-    if (sm == FT(0)) { n1x = n1y = n1z = FT(0); }
-    else { n1x = sx / sm; n1y = sy / sm; n1z = sz / sm; }
+    // normal triad (:41-58); n1 = S.normalized() is supplied
+    const FT sm = FT(area_);
+    const FT n1x = FT(nx_), n1y = FT(ny_), n1z = FT(nz_);
     FT t2x, t2y, t2z;
     if (n1x == FT(0) && n1y == FT(0)) { t2x = FT(1); t2y = FT(0); t2z = FT(0); }
     else { t2x = -n1y; t2y = n1x; t2z = FT(0); }

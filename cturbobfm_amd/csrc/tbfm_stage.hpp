@@ -6,11 +6,12 @@
 namespace tbfm {
 namespace dev {
 
-__device__ __forceinline__ long nid(const CaseDev& c, int i, int j, int k) {
-    return ((long)k * c.nj + j) * c.ni + i;
+// Node and state offsets are 32-bit: the library rejects grids with 5*N >= 2^31.
+__device__ __forceinline__ int nid(const CaseDev& c, int i, int j, int k) {
+    return (k * c.nj + j) * c.ni + i;
 }
 
-__device__ __forceinline__ void load5(const double* U, long N, long n, double out[5]) {
+__device__ __forceinline__ void load5(const double* U, int N, int n, double out[5]) {
 #pragma unroll
     for (int v = 0; v < 5; ++v) out[v] = U[v * N + n];
 }
@@ -26,26 +27,27 @@ __device__ __forceinline__ void rotx(double s[5], double cs, double sn) {
 // Node gk of the (i,j) line along k, including the two virtual planes used by the face
 // loop: -1 and NK (periodic wrap with rotation, solver.cpp:1964-1979, or linear
 // extrapolation, solver.cpp:1980-1993).
+// The source planes of a virtual plane are written relative to gk (not as the constants
+// 0, 1, NK-2, NK-1) so that the compiler does not hoist their addresses out of the k loop
+// into registers that the fused kernel needs.
 __device__ inline void load_k(const CaseDev& c, const double* U, int i, int j, int gk, double out[5]) {
     const int NK = c.nk;
+    const int N = (int)c.N;
+    const int col = j * c.ni + i, NIJ = c.ni * c.nj;
     if (gk >= 0 && gk < NK) {
-        load5(U, c.N, nid(c, i, j, gk), out);
+        load5(U, N, gk * NIJ + col, out);
     } else if (c.periodic) {
-        if (gk < 0) { load5(U, c.N, nid(c, i, j, NK - 2), out); rotx(out, c.per_cm, c.per_sm); }
-        else { load5(U, c.N, nid(c, i, j, 1), out); rotx(out, c.per_c, c.per_s); }
+        const bool lo = gk < 0;
+        load5(U, N, (lo ? gk + NK - 1 : gk - NK + 1) * NIJ + col, out);
+        rotx(out, lo ? c.per_cm : c.per_c, lo ? c.per_sm : c.per_s);
     } else {
+        // gk = -1: a = plane 0, b = plane 1; gk = NK: a = plane NK-1, b = plane NK-2
+        const int d = (gk < 0) ? 1 : -1;
         double a[5], b[5];
-        if (gk < 0) {
-            load5(U, c.N, nid(c, i, j, 0), a);
-            load5(U, c.N, nid(c, i, j, 1), b);
+        load5(U, N, (gk + d) * NIJ + col, a);
+        load5(U, N, (gk + 2 * d) * NIJ + col, b);
 #pragma unroll
-            for (int v = 0; v < 5; ++v) out[v] = a[v] - (b[v] - a[v]);
-        } else {
-            load5(U, c.N, nid(c, i, j, NK - 1), a);
-            load5(U, c.N, nid(c, i, j, NK - 2), b);
-#pragma unroll
-            for (int v = 0; v < 5; ++v) out[v] = a[v] + (a[v] - b[v]);
-        }
+        for (int v = 0; v < 5; ++v) out[v] = (gk < 0) ? a[v] - (b[v] - a[v]) : a[v] + (a[v] - b[v]);
     }
 }
 
