@@ -51,33 +51,50 @@ __device__ __forceinline__ int fiI(int v, int tj, int m) { return (v * TL::TJ + 
 template <class TL>
 __device__ __forceinline__ int fiJ(int v, int r, int ti) { return (v * (TL::TJ + 1) + r) * TL::TI + ti; }
 
-// Fills LDS position `pos` of plane k: real node, ghost (one step outside the domain in
-// i or j) or unused.
-template <class TL, int SCH>
-__device__ inline void fill_position(const CaseDev& c, const double* __restrict__ U, PlaneLds<TL>& L, int pos,
-                                     int i0, int j0, int k) {
+// Reads the conserved state of LDS position `pos` of plane k: a real node or a ghost one
+// step outside the domain in i or j (linear extrapolation, solver.cpp:1980-1993); false
+// for a position no face reads.
+template <class TL>
+__device__ __forceinline__ bool fetch_position(const CaseDev& c, const double* __restrict__ U, int pos, int i0,
+                                               int j0, int k, double Un[5]) {
     const int hi = pos % TL::HI, hj = pos / TL::HI;
     const int gi = i0 - 2 + hi, gj = j0 - 2 + hj;
     const int NI = c.ni, NJ = c.nj;
+    const int N = (int)c.N;
     const bool iin = (gi >= 0 && gi < NI), jin = (gj >= 0 && gj < NJ);
-    double Un[5];
     if (iin && jin) {
-        load5(U, c.N, nid(c, gi, gj, k), Un);
-    } else if (jin && (gi == -1 || gi == NI)) {
-        double a[5], b[5];
-        const int ia = (gi < 0) ? 0 : NI - 1, ib = (gi < 0) ? 1 : NI - 2;
-        load5(U, c.N, nid(c, ia, gj, k), a);
-        load5(U, c.N, nid(c, ib, gj, k), b);
-#pragma unroll
-        for (int v = 0; v < 5; ++v) Un[v] = (gi < 0) ? a[v] - (b[v] - a[v]) : a[v] + (a[v] - b[v]);
-    } else if (iin && (gj == -1 || gj == NJ) && c.topo != TOPO_1D) {
-        double a[5], b[5];
-        const int ja = (gj < 0) ? 0 : NJ - 1, jb = (gj < 0) ? 1 : NJ - 2;
-        load5(U, c.N, nid(c, gi, ja, k), a);
-        load5(U, c.N, nid(c, gi, jb, k), b);
-#pragma unroll
-        for (int v = 0; v < 5; ++v) Un[v] = (gj < 0) ? a[v] - (b[v] - a[v]) : a[v] + (a[v] - b[v]);
+        load5(U, N, nid(c, gi, gj, k), Un);
+        return true;
+    }
+    const bool ighost = jin && (gi == -1 || gi == NI);
+    const bool jghost = iin && (gj == -1 || gj == NJ) && c.topo != TOPO_1D;
+    if (!ighost && !jghost) return false;
+    // a = boundary node, b = its inner neighbour
+    int na, nb;
+    bool lo;
+    if (ighost) {
+        lo = gi < 0;
+        na = nid(c, lo ? gi + 1 : gi - 1, gj, k);
+        nb = nid(c, lo ? gi + 2 : gi - 2, gj, k);
     } else {
+        lo = gj < 0;
+        na = nid(c, gi, lo ? gj + 1 : gj - 1, k);
+        nb = nid(c, gi, lo ? gj + 2 : gj - 2, k);
+    }
+    double A[5], B[5];
+    load5(U, N, na, A);
+    load5(U, N, nb, B);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) Un[v] = lo ? A[v] - (B[v] - A[v]) : A[v] + (A[v] - B[v]);
+    return true;
+}
+
+// Stores what the faces read at LDS position `pos` (JST: state, primitives, pressure,
+// spectral factor; Roe: primitives); unused positions get harmless constants.
+template <class TL, int SCH>
+__device__ __forceinline__ void commit_position(const CaseDev& c, PlaneLds<TL>& L, int pos, bool ok,
+                                                const double Un[5]) {
+    if (!ok) {
 #pragma unroll
         for (int v = 0; v < 5; ++v) L.d.W[v][pos] = 1.0;
         if constexpr (SCH == 0) {
@@ -98,6 +115,14 @@ __device__ inline void fill_position(const CaseDev& c, const double* __restrict_
         L.d.P[pos] = p;
         L.d.RF[pos] = rf;
     }
+}
+
+template <class TL, int SCH>
+__device__ inline void fill_position(const CaseDev& c, const double* __restrict__ U, PlaneLds<TL>& L, int pos,
+                                     int i0, int j0, int k) {
+    double Un[5];
+    const bool ok = fetch_position<TL>(c, U, pos, i0, j0, k, Un);
+    commit_position<TL, SCH>(c, L, pos, ok, Un);
 }
 
 // Face flux*|S| between LDS positions l and r (with ll = l - st, rr = r + st) along i
@@ -273,19 +298,34 @@ __global__ void __launch_bounds__(NT) k_stage(CaseDev c, StageDev st) {
     if (st.last)
 #pragma unroll
         for (int v = 0; v < 5; ++v) L.red[v][tid] = 0.0;
+    // Software pipeline: the next step's column plane and halo slot are read while the
+    // current plane's faces are evaluated.
+    double pre[5];
+    if (THREED && act && kc0 + 2 <= NK) load_k(c, Uin, i, j, kc0 + 2, pre);
+    const bool hown = tid < NP - NT;
+    const int hpos = hown ? halo_pos<TL>(tid) : 0;
+    double hU[5];
+    bool hok = false;
+    if (hown) hok = fetch_position<TL>(c, Uin, hpos, i0, j0, kc0, hU);
 
 #pragma unroll 1
     for (int k = kc0; k < kc1; ++k) {
+        const int n = (k * NJ + j) * NI + i;
+        const bool parked = THREED && c.periodic && (k == 0 || k == NK - 1);
+        // update operands, read early
+        double u0v[5], dtv = 0.0, volv = 1.0;
+        if (act && !parked && !st.rmode) {
+            load5(st.U0, N, n, u0v);
+            dtv = (c.dt_mode == 1) ? c.dtmin[0] : c.dt[n];
+            volv = c.vol[n];
+        }
         double Wc[5];
         if (THREED) {
             if (act) {
                 q.shift();
-                if (k + 2 <= NK) {
-                    double u[5];
-                    load_k(c, Uin, i, j, k + 2, u);
-                    q.put(c, 3, u);
-                }
+                if (k + 2 <= NK) q.put(c, 3, pre);
                 q.after_shift(c);
+                if (k + 1 < kc1 && k + 3 <= NK) load_k(c, Uin, i, j, k + 3, pre);
                 const double* w = q.Wc();
 #pragma unroll
                 for (int v = 0; v < 5; ++v) Wc[v] = w[v];
@@ -315,7 +355,11 @@ __global__ void __launch_bounds__(NT) k_stage(CaseDev c, StageDev st) {
         }
         // tile positions outside the domain (possibly ghosts) and the halo by contiguous lanes
         if (!act) fill_position<TL, SCH>(c, Uin, L, own, i0, j0, k);
-        for (int h = tid; h < NP - NT; h += NT) fill_position<TL, SCH>(c, Uin, L, halo_pos<TL>(h), i0, j0, k);
+        if (hown) {
+            commit_position<TL, SCH>(c, L, hpos, hok, hU);
+            if (k + 1 < kc1) hok = fetch_position<TL>(c, Uin, hpos, i0, j0, k + 1, hU);
+        }
+        for (int h = tid + NT; h < NP - NT; h += NT) fill_position<TL, SCH>(c, Uin, L, halo_pos<TL>(h), i0, j0, k);
         __syncthreads();
         // ---- every I and J face of the tile once: q < NFI are I faces (row r, face m),
         // then J faces (row r of faces, column m); one code path, no divergence
@@ -364,7 +408,6 @@ __global__ void __launch_bounds__(NT) k_stage(CaseDev c, StageDev st) {
         __syncthreads();
         // ---- residual assembly, sources, RK update
         if (act) {
-            const int n = (k * NJ + j) * NI + i;
             double R[5];
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
@@ -384,7 +427,7 @@ __global__ void __launch_bounds__(NT) k_stage(CaseDev c, StageDev st) {
                 R[v] = r;
             }
             if (srcOn && i >= c.bi0 && i < c.bi1) node_sources<THREED>(st.cdev, Uin, st.last, i, j, k, n, Wc, R);
-            if (THREED && c.periodic && (k == 0 || k == NK - 1)) {
+            if (parked) {
                 double* rp = st.rper + (k == 0 ? 0 : 5) * NIJ;
 #pragma unroll
                 for (int v = 0; v < 5; ++v) rp[v * NIJ + j * NI + i] = R[v];
@@ -392,10 +435,9 @@ __global__ void __launch_bounds__(NT) k_stage(CaseDev c, StageDev st) {
 #pragma unroll
                 for (int v = 0; v < 5; ++v) st.Uout[v * N + n] = R[v];
             } else {
-                const double dtv = (c.dt_mode == 1) ? c.dtmin[0] : c.dt[n];
-                const double factor = st.alpha * dtv / c.vol[n];
+                const double factor = st.alpha * dtv / volv;
 #pragma unroll
-                for (int v = 0; v < 5; ++v) st.Uout[v * N + n] = st.U0[v * N + n] - R[v] * factor;
+                for (int v = 0; v < 5; ++v) st.Uout[v * N + n] = u0v[v] - R[v] * factor;
                 if (st.last)
 #pragma unroll
                     for (int v = 0; v < 5; ++v) L.red[v][tid] += R[v] * R[v];
