@@ -95,7 +95,7 @@ def main():
         return
 
     ni, nj, nk = (int(v) for v in args.size.split("x"))
-    case = cases.rotor3d(ni, nj, nk, n_iterations=args.warmup + 2 * args.steps + 10, scheme=args.scheme)
+    case = cases.rotor3d(ni, nj, nk, n_iterations=args.warmup + 3 * args.steps + 10, scheme=args.scheme)
     overrides = {"GPU_PRECISION": args.precision}
     s = Solver.from_case(case, overrides)
     nodes = ni * nj * nk
@@ -109,6 +109,11 @@ def main():
     t1 = time.perf_counter()
     stage_ms, launches, alg_bytes = s.profile_get()
     s.profile(False, True)
+    # the same steps through the captured-graph path (no per-kernel events), for reference
+    g0 = time.perf_counter()
+    s.advance(args.steps)
+    s.sync()
+    g1 = time.perf_counter()
     wall = t1 - t0
     value = nodes * stages * args.steps / wall
     nbf = body_force_nodes(case)
@@ -130,7 +135,9 @@ def main():
         "data": "synthetic annular rotor grid and blade fields (cturbobfm_amd/cases.py), free-stream start",
         "config": {"workload": "annular rotor %dx%dx%d nodes, Hall body force, %s, RK41A, CFL 2, local dt, %s"
                                % (ni, nj, nk, args.scheme, args.precision),
-                   "nodes": nodes, "stages_per_step": stages, "body_force_nodes": nbf},
+                   "nodes": nodes, "stages_per_step": stages, "body_force_nodes": nbf,
+                   "stage_tiling": s.tuning().split("->")[-1].strip(),
+                   "ms_per_step_graph": (g1 - g0) / args.steps * 1e3},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_stage (fused residual + RK stage)", "avg_launch_ms": avg_ms,

@@ -43,7 +43,10 @@ EXPORTS = ["tbfm_gpu_case_create", "tbfm_gpu_case_create_from_ini", "tbfm_gpu_ca
            "tbfm_gpu_advance", "tbfm_gpu_compute_residual", "tbfm_gpu_compute_timestep",
            "tbfm_gpu_get_side_outputs", "tbfm_gpu_get_turbo", "tbfm_gpu_get_geometry",
            "tbfm_gpu_profile", "tbfm_gpu_profile_get", "tbfm_gpu_sync", "tbfm_gpu_copy_calibration",
-           "tbfm_gpu_tuning"]
+           "tbfm_gpu_tuning", "tbfm_gpu_case_device_bytes", "tbfm_gpu_case_set_convergence",
+           "tbfm_gpu_case_status", "tbfm_gpu_batch_create", "tbfm_gpu_batch_admit", "tbfm_gpu_batch_retire",
+           "tbfm_gpu_batch_size", "tbfm_gpu_batch_advance", "tbfm_gpu_batch_destroy",
+           "tbfm_gpu_batch_last_error"]
 
 _lib = None
 
@@ -81,6 +84,17 @@ def lib():
         L.tbfm_gpu_copy_calibration.argtypes = [C.c_long, C.c_int, dp]
         L.tbfm_gpu_tuning.restype = C.c_char_p
         L.tbfm_gpu_tuning.argtypes = [vp]
+        L.tbfm_gpu_case_device_bytes.argtypes = [vp, C.POINTER(C.c_long)]
+        L.tbfm_gpu_case_set_convergence.argtypes = [vp, C.c_int, C.c_double]
+        L.tbfm_gpu_case_status.argtypes = [vp, C.POINTER(C.c_long), C.POINTER(C.c_long), C.POINTER(C.c_int), dp]
+        L.tbfm_gpu_batch_create.argtypes = [C.POINTER(vp)]
+        L.tbfm_gpu_batch_admit.argtypes = [vp, vp]
+        L.tbfm_gpu_batch_retire.argtypes = [vp, vp]
+        L.tbfm_gpu_batch_size.argtypes = [vp]
+        L.tbfm_gpu_batch_advance.argtypes = [vp, C.c_long]
+        L.tbfm_gpu_batch_destroy.argtypes = [vp]
+        L.tbfm_gpu_batch_last_error.restype = C.c_char_p
+        L.tbfm_gpu_batch_last_error.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -242,3 +256,66 @@ class Solver:
 
     def sync(self) -> None:
         self._check(lib().tbfm_gpu_sync(self.h))
+
+    # -- batching / serving support ----------------------------------------------
+    def device_bytes(self) -> int:
+        b = C.c_long()
+        self._check(lib().tbfm_gpu_case_device_bytes(self.h, C.byref(b)))
+        return b.value
+
+    def set_convergence(self, density_only: bool, drop: float) -> None:
+        self._check(lib().tbfm_gpu_case_set_convergence(self.h, int(density_only), float(drop)))
+
+    def status(self):
+        """(iterations executed, convergence iteration or 0, stopped, last log residuals);
+        raises SolverError if the case diverged."""
+        it, conv, st = C.c_long(), C.c_long(), C.c_int()
+        lr = np.zeros(5)
+        rc = lib().tbfm_gpu_case_status(self.h, C.byref(it), C.byref(conv), C.byref(st), _dp(lr))
+        self.iterations = it.value
+        self.converged_at = conv.value
+        self._check(rc)
+        return it.value, conv.value, bool(st.value), lr
+
+
+class Batch:
+    """Cases advancing together: one launch per kernel over all members, replayed as one
+    captured HIP graph per iteration (``tbfm_gpu_batch_*``).  Members keep their own
+    state, counters and convergence test, and compute the same numbers alone and in any
+    mix."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        rc = lib().tbfm_gpu_batch_create(C.byref(h))
+        if rc != TBFM_OK:
+            raise SolverError(lib().tbfm_gpu_batch_last_error(None).decode())
+        self.h = h
+        self.members = []
+
+    def _check(self, rc):
+        if rc != TBFM_OK:
+            raise SolverError(lib().tbfm_gpu_batch_last_error(self.h).decode())
+
+    def admit(self, s: Solver) -> None:
+        self._check(lib().tbfm_gpu_batch_admit(self.h, s.h))
+        self.members.append(s)
+
+    def retire(self, s: Solver) -> None:
+        self._check(lib().tbfm_gpu_batch_retire(self.h, s.h))
+        self.members.remove(s)
+
+    def __len__(self) -> int:
+        return lib().tbfm_gpu_batch_size(self.h)
+
+    def advance(self, n: int) -> None:
+        self._check(lib().tbfm_gpu_batch_advance(self.h, n))
+
+    def close(self) -> None:
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.tbfm_gpu_batch_destroy(self.h)
+            self.h = None
+            self.members = []
+
+    def __del__(self):
+        self.close()
+

@@ -175,21 +175,23 @@ struct KQueue;
 
 template <>
 struct KQueue<0> {
-    double U[4][5], P[4], W[2][5], RF[2];
+    // primitives of slots 1, 2 without the density, which is U[slot][0]
+    double U[4][5], P[4], Wv[2][4], RF[2];
     __device__ __forceinline__ void put(const CaseDev& c, int s, const double* u) {
         double w[5], rf;
 #pragma unroll
         for (int v = 0; v < 5; ++v) U[s][v] = u[v];
         derive<double>(U[s], w, P[s], rf, c.gamma, c.gm1);
     }
-    __device__ __forceinline__ void prim2(const CaseDev& c) {   // primitives of slot 2
-        double p;
-        derive<double>(U[2], W[1], p, RF[1], c.gamma, c.gm1);
+    __device__ __forceinline__ void prim(const CaseDev& c, int s, int d) {   // primitives of slot s into d
+        double w[5], p;
+        derive<double>(U[s], w, p, RF[d], c.gamma, c.gm1);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Wv[d][v] = w[v + 1];
     }
     __device__ __forceinline__ void init_end(const CaseDev& c) {
-        double p;
-        derive<double>(U[1], W[0], p, RF[0], c.gamma, c.gm1);
-        prim2(c);
+        prim(c, 1, 0);
+        prim(c, 2, 1);
     }
     __device__ __forceinline__ void shift() {
 #pragma unroll
@@ -199,17 +201,23 @@ struct KQueue<0> {
             P[s] = P[s + 1];
         }
 #pragma unroll
-        for (int v = 0; v < 5; ++v) W[0][v] = W[1][v];
+        for (int v = 0; v < 4; ++v) Wv[0][v] = Wv[1][v];
         RF[0] = RF[1];
     }
-    __device__ __forceinline__ void after_shift(const CaseDev& c) { prim2(c); }
-    __device__ __forceinline__ const double* Wc() const { return W[0]; }
+    __device__ __forceinline__ void after_shift(const CaseDev& c) { prim(c, 2, 1); }
+    __device__ __forceinline__ void Wc(double w[5]) const {
+        w[0] = U[1][0];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) w[v + 1] = Wv[0][v];
+    }
     template <typename FT>
     __device__ __forceinline__ void face(const CaseDev& c, int f, double out[5]) const {
         const double sl = jst_sensor<double>(P[0], P[1], P[2]);
         const double sr = jst_sensor<double>(P[1], P[2], P[3]);
-        jst_face<FT>(U[0], U[1], U[2], U[3], W[0], W[1], RF[0], RF[1], sl, sr, c.nkx[f], c.nky[f], c.nkz[f],
-                     c.nka[f], c.gm1, out);
+        const double Wl[5] = {U[1][0], Wv[0][0], Wv[0][1], Wv[0][2], Wv[0][3]};
+        const double Wr[5] = {U[2][0], Wv[1][0], Wv[1][1], Wv[1][2], Wv[1][3]};
+        jst_face<FT>(U[0], U[1], U[2], U[3], Wl, Wr, RF[0], RF[1], sl, sr, c.nrm[c.noffK + f], c.nrm[c.noffK + c.nfK + f],
+                     c.nrm[c.noffK + 2 * c.nfK + f], c.nrm[c.noffK + 3 * c.nfK + f], c.gm1, out);
     }
 };
 
@@ -230,11 +238,14 @@ struct KQueue<1> {
             for (int v = 0; v < 5; ++v) W[s][v] = W[s + 1][v];
     }
     __device__ __forceinline__ void after_shift(const CaseDev&) {}
-    __device__ __forceinline__ const double* Wc() const { return W[1]; }
+    __device__ __forceinline__ void Wc(double w[5]) const {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) w[v] = W[1][v];
+    }
     template <typename FT>
     __device__ __forceinline__ void face(const CaseDev& c, int f, double out[5]) const {
-        roe_face<FT>(W[0], W[1], W[2], W[3], c.muscl, c.limiter, c.nkx[f], c.nky[f], c.nkz[f], c.nka[f], c.gamma,
-                     out);
+        roe_face<FT>(W[0], W[1], W[2], W[3], c.muscl, c.limiter, c.nrm[c.noffK + f], c.nrm[c.noffK + c.nfK + f],
+                     c.nrm[c.noffK + 2 * c.nfK + f], c.nrm[c.noffK + 3 * c.nfK + f], c.gamma, out);
     }
 };
 
@@ -252,20 +263,29 @@ __device__ __forceinline__ int halo_pos(int h) {
     return row * HI + ((col < 2) ? col : TL::TI + col);
 }
 
+// The stage operands of one block: its case's buffers for this launch's roles.
+struct StageOps {
+    const double* Uin;     // U^(k-1)
+    const double* U0;      // U^(0)
+    double* Uout;          // U^(k)
+    double alpha;
+    int last, rmode;
+};
+
 template <class TL, int SCH, bool THREED, typename FT>
-__global__ void __launch_bounds__(NT) k_stage(CaseDev c, StageDev st) {
+__device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, const Work w, const StageOps st) {
     constexpr int TI = TL::TI, TJ = TL::TJ, HI = TL::HI, NP = TL::NP;
     constexpr int NFI = TJ * (TI + 1), NFJ = (TJ + 1) * TI;
     __shared__ PlaneLds<TL> L;
-    if (*c.stop) return;
+    const CaseDev& c = *cp;
     const int tid = threadIdx.x;
     const int ti = tid % TI, tj = tid / TI;
-    const int i0 = blockIdx.x * TI, j0 = blockIdx.y * TJ;
+    const int i0 = w.x * TI, j0 = w.y * TJ;
     const int i = i0 + ti, j = j0 + tj;
     const int NI = c.ni, NJ = c.nj, NK = c.nk;
     const bool act = (i < NI) && (j < NJ);
-    const int kc0 = blockIdx.z * st.kc;
-    const int kc1 = min(NK, kc0 + st.kc);
+    const int kc0 = w.z * c.kc;
+    const int kc1 = min(NK, kc0 + c.kc);
     const bool doJ = (c.topo != TOPO_1D);
     const int N = (int)c.N;
     const int NIJ = NI * NJ;
@@ -289,7 +309,7 @@ __global__ void __launch_bounds__(NT) k_stage(CaseDev c, StageDev st) {
         if (!bladecol) {
             if (kc0 == 0) {
 #pragma unroll
-                for (int v = 0; v < 5; ++v) FKl[v] = c.bK[v * NIJ + j * NI + i];
+                for (int v = 0; v < 5; ++v) FKl[v] = c.bnd[c.boffK + v * NIJ + j * NI + i];
             } else {
                 q.template face<FT>(c, (kc0 * NJ + j) * NI + i, FKl);
             }
@@ -312,7 +332,7 @@ __global__ void __launch_bounds__(NT) k_stage(CaseDev c, StageDev st) {
     for (int k = kc0; k < kc1; ++k) {
         const int n = (k * NJ + j) * NI + i;
         const bool parked = THREED && c.periodic && (k == 0 || k == NK - 1);
-        // update operands, read early
+        // update operands, read early and consumed at the update
         double u0v[5], dtv = 0.0, volv = 1.0;
         if (act && !parked && !st.rmode) {
             load5(st.U0, N, n, u0v);
@@ -326,9 +346,7 @@ __global__ void __launch_bounds__(NT) k_stage(CaseDev c, StageDev st) {
                 if (k + 2 <= NK) q.put(c, 3, pre);
                 q.after_shift(c);
                 if (k + 1 < kc1 && k + 3 <= NK) load_k(c, Uin, i, j, k + 3, pre);
-                const double* w = q.Wc();
-#pragma unroll
-                for (int v = 0; v < 5; ++v) Wc[v] = w[v];
+                q.Wc(Wc);
                 // stage the own node (JST: state, primitives, pressure, spectral factor)
                 if constexpr (SCH == 0) {
 #pragma unroll
@@ -374,20 +392,17 @@ __global__ void __launch_bounds__(NT) k_stage(CaseDev c, StageDev st) {
             const int gf = isI ? gi : gj, nf = isI ? NI : NJ;
             double F[5];
             if (gf == 0 || gf == nf) {
-                const double* b = isI ? c.bI : c.bJ;
                 const int nsz = isI ? NJ * NK : NI * NK;
                 const int bq = isI ? k * NJ + gj : k * NI + gi;
-                const int off = (gf == 0) ? 0 : 5 * nsz;
+                const int off = (isI ? 0 : c.boffJ) + ((gf == 0) ? 0 : 5 * nsz) + bq;
 #pragma unroll
-                for (int v = 0; v < 5; ++v) F[v] = b[off + v * nsz + bq];
+                for (int v = 0; v < 5; ++v) F[v] = c.bnd[off + v * nsz];
             } else {
-                const int fidx = isI ? (k * NJ + gj) * (NI + 1) + gi : (k * (NJ + 1) + gj) * NI + gi;
-                const double* nxp = isI ? c.nix : c.njx;
-                const double* nyp = isI ? c.niy : c.njy;
-                const double* nzp = isI ? c.niz : c.njz;
-                const double* nap = isI ? c.nia : c.nja;
+                const int fst = isI ? c.nfI : c.nfJ;
+                const int fo = isI ? (k * NJ + gj) * (NI + 1) + gi : c.noffJ + (k * (NJ + 1) + gj) * NI + gi;
                 const int l = isI ? (r + 2) * HI + (m + 1) : (r + 1) * HI + (m + 2);
-                lds_face<TL, SCH, FT>(c, L, l, isI ? 1 : HI, nxp[fidx], nyp[fidx], nzp[fidx], nap[fidx], F);
+                lds_face<TL, SCH, FT>(c, L, l, isI ? 1 : HI, c.nrm[fo], c.nrm[fo + fst], c.nrm[fo + 2 * fst],
+                                      c.nrm[fo + 3 * fst], F);
             }
             double* out = isI ? L.f.FI : L.f.FJ;
             const int base = isI ? r * (TI + 1) + m : r * TI + m;
@@ -400,7 +415,7 @@ __global__ void __launch_bounds__(NT) k_stage(CaseDev c, StageDev st) {
         if (THREED && act && !bladecol) {
             if (k + 1 == NK) {
 #pragma unroll
-                for (int v = 0; v < 5; ++v) FKr[v] = c.bK[(5 + v) * NIJ + j * NI + i];
+                for (int v = 0; v < 5; ++v) FKr[v] = c.bnd[c.boffK + (5 + v) * NIJ + j * NI + i];
             } else {
                 q.template face<FT>(c, ((k + 1) * NJ + j) * NI + i, FKr);
             }
@@ -426,9 +441,9 @@ __global__ void __launch_bounds__(NT) k_stage(CaseDev c, StageDev st) {
                 }
                 R[v] = r;
             }
-            if (srcOn && i >= c.bi0 && i < c.bi1) node_sources<THREED>(st.cdev, Uin, st.last, i, j, k, n, Wc, R);
+            if (srcOn && i >= c.bi0 && i < c.bi1) node_sources<THREED>(cp, Uin, st.last, i, j, k, n, Wc, R);
             if (parked) {
-                double* rp = st.rper + (k == 0 ? 0 : 5) * NIJ;
+                double* rp = c.rper + (k == 0 ? 0 : 5) * NIJ;
 #pragma unroll
                 for (int v = 0; v < 5; ++v) rp[v * NIJ + j * NI + i] = R[v];
             } else if (st.rmode) {
@@ -449,34 +464,62 @@ __global__ void __launch_bounds__(NT) k_stage(CaseDev c, StageDev st) {
     }
     // ---- per-block norm partial sums (deterministic tree)
     if (st.last) {
-        const long b = ((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        const long b = ((long)w.z * c.sgy + w.y) * c.sgx + w.x;
         for (int s = NT / 2; s > 0; s >>= 1) {
             if (tid < s)
 #pragma unroll
                 for (int v = 0; v < 5; ++v) L.red[v][tid] += L.red[v][tid + s];
             __syncthreads();
         }
-        if (tid < 5) st.partial[b * 5 + tid] = L.red[tid][0];
+        if (tid < 5) c.partial[b * 5 + tid] = L.red[tid][0];
     }
+}
+
+// Work: one entry per (i, j) tile and k chunk of a case.  The case and work tables are
+// also passed as restrict kernel parameters (read-only for the launch), which lets the
+// compiler keep case fields in scalar registers or reload them freely.
+template <class TL, int SCH, bool THREED, typename FT>
+__global__ void __launch_bounds__(NT) k_stage(const CaseDev* __restrict__ cases, const Work* __restrict__ work,
+                                              Launch La) {
+    const Work w = work[blockIdx.x];
+    const CaseDev* __restrict__ cp = cases + w.cs;
+    if (*cp->stop) return;
+    StageOps st;
+    st.Uin = cp->U[La.in];
+    st.U0 = cp->U[0];
+    st.Uout = cp->U[La.out];
+    st.alpha = cp->alpha[La.stage];
+    st.last = La.last;
+    st.rmode = La.rmode;
+    stage_block<TL, SCH, THREED, FT>(cp, w, st);
 }
 
 // Periodic closure of the k = 0 / k = NK-1 node-mates: residual averaging
 // (solver.cpp:1836-1853), RK update of both, then solution averaging
 // (solver.cpp:2308-2327).  Writes its own norm partials after the stage kernel's.
 // With st.rmode the averaged residuals themselves are written (residual entry point).
-__global__ void __launch_bounds__(256) k_periodic(CaseDev c, StageDev st, long partial_offset) {
+// Work: one entry per 256 node-mate pairs of a periodic 3-D case.
+__global__ void __launch_bounds__(256) k_periodic(Launch La) {
     __shared__ double red[256];
+    const Work w = La.work[blockIdx.x];
+    const CaseDev& c = La.cases[w.cs];
     if (*c.stop) return;
+    StageOps st;
+    st.U0 = c.U[0];
+    st.Uout = c.U[La.out];
+    st.alpha = c.alpha[La.stage];
+    st.last = La.last;
+    st.rmode = La.rmode;
     const long NI = c.ni, NJ = c.nj, NK = c.nk, N = c.N;
-    const long q = (long)blockIdx.x * 256 + threadIdx.x;
+    const long q = (long)w.x * 256 + threadIdx.x;
     double nrm[5] = {0, 0, 0, 0, 0};
     if (q < NI * NJ) {
         const long n0 = q, n1 = (NK - 1) * NI * NJ + q;
         double R1[5], R2[5];
 #pragma unroll
         for (int v = 0; v < 5; ++v) {
-            R1[v] = st.rper[(long)v * NI * NJ + q];
-            R2[v] = st.rper[(long)(5 + v) * NI * NJ + q];
+            R1[v] = c.rper[(long)v * NI * NJ + q];
+            R2[v] = c.rper[(long)(5 + v) * NI * NJ + q];
         }
         rotx(R2, c.per_cm, c.per_sm);
         double avg[5];
@@ -527,7 +570,7 @@ __global__ void __launch_bounds__(256) k_periodic(CaseDev c, StageDev st, long p
                 if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
                 __syncthreads();
             }
-            if (threadIdx.x == 0) st.partial[(partial_offset + blockIdx.x) * 5 + v] = red[0];
+            if (threadIdx.x == 0) c.partial[((long)c.nparts_stage + w.x) * 5 + v] = red[0];
             __syncthreads();
         }
     }
@@ -535,21 +578,17 @@ __global__ void __launch_bounds__(256) k_periodic(CaseDev c, StageDev st, long p
 
 // Residual norms of the last stage (solver.cpp:1488-1508), the baseline of
 // iterations 1-5 (solver.cpp:1426-1439) and the convergence test
-// (solver.cpp:2985-3015), all on the device.  Sets *stop once converged or diverged.
-struct ConvDev {
-    double* hist;        // [max_iter][5] log residuals
-    double* init;        // [5] baseline
-    long* conv_at;       // iteration of convergence, 0 if none
-    int* diverged;       // 1 + variable index
-    long hist_len;
-    double drop;
-    int steady;
-};
-
-__global__ void __launch_bounds__(256) k_norms(CaseDev c, const double* partial, int nparts, ConvDev cv) {
+// (solver.cpp:2985-3015), all on the device.  Sets *stop once converged or diverged and
+// advances the iteration counter.  Work: one entry per case.
+__global__ void __launch_bounds__(256) k_norms(Launch La) {
     __shared__ double red[256];
     __shared__ double lr[5];
+    const Work w = La.work[blockIdx.x];
+    const CaseDev& c = La.cases[w.cs];
     if (*c.stop) return;
+    const ConvDev& cv = c.conv;
+    const double* partial = c.partial;
+    const int nparts = c.nparts_stage + c.nparts_per;
     for (int v = 0; v < 5; ++v) {
         double s = 0.0;
         for (int q = threadIdx.x; q < nparts; q += 256) s += partial[(long)q * 5 + v];
@@ -589,6 +628,7 @@ __global__ void __launch_bounds__(256) k_norms(CaseDev c, const double* partial,
         } else if (cv.steady) {
             bool ok = true;
             for (int v = 0; v < 5 && ok; ++v) {
+                if (cv.density_only && v > 0) break;
                 if (v == 3 && c.topo != TOPO_3D && c.topo != TOPO_AXI) continue;
                 if (v == 2 && c.topo == TOPO_1D) continue;
                 if (cv.init[v] <= -15.99) continue;

@@ -6,7 +6,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -16,7 +18,6 @@
 
 using namespace tbfm;
 using tbfm::dev::CaseDev;
-using tbfm::dev::StageDev;
 
 namespace {
 
@@ -33,6 +34,8 @@ struct HipError : std::runtime_error {
         if (_e != hipSuccess) throw HipError(_e, #x); \
     } while (0)
 
+thread_local size_t* g_bytes = nullptr;   // device-byte counter of the case being set up
+
 template <typename T>
 T* dalloc(size_t n, std::vector<void*>& owned) {
     void* p = nullptr;
@@ -40,6 +43,7 @@ T* dalloc(size_t n, std::vector<void*>& owned) {
     HIPCHK(hipMalloc(&p, n * sizeof(T)));
     HIPCHK(hipMemset(p, 0, n * sizeof(T)));
     owned.push_back(p);
+    if (g_bytes) *g_bytes += n * sizeof(T);
     return (T*)p;
 }
 
@@ -52,6 +56,41 @@ T* upload(const std::vector<T>& h, std::vector<void*>& owned) {
 
 }  // namespace
 
+namespace {
+
+// Device work table of one kernel of a launch plan.
+struct Table {
+    dev::Work* d = nullptr;
+    size_t cap = 0;
+    int n = 0;
+};
+
+// Stage-kernel instantiation shared by the cases of one group.
+struct StageGroup {
+    int tile = 0, sch = 0;
+    bool three = false, mixed = false;
+    Table t;
+};
+
+// Launch plan of a set of cases that advance together (one case, or the members of a
+// batch): the device case table, the work tables of every kernel and the captured
+// graph of one iteration.
+struct Plan {
+    hipStream_t stream = nullptr;
+    std::vector<tbfm_case*> members;
+    CaseDev* dcases = nullptr;        // device case table
+    size_t dcap = 0;
+    bool ownsCases = false;
+    Table dt, mn, prof, bnd, per, norm;
+    std::vector<StageGroup> groups;
+    int nstages = 0;
+    hipGraphExec_t graph = nullptr;
+    bool graphValid = false;
+    bool tablesValid = false;
+};
+
+}  // namespace
+
 struct tbfm_case {
     long ni = 0, nj = 0, nk = 0, N = 0;
     Config cfg;
@@ -61,38 +100,31 @@ struct tbfm_case {
     std::vector<double> rk;
     Geometry geo;
     std::vector<Patch> patches;
-    hipStream_t stream = nullptr;
     std::vector<void*> owned;
+    size_t dev_bytes = 0;             // device memory held by the case
     CaseDev c{};
-    CaseDev* cdev = nullptr;          // device-resident copy of c
-    double* U[3] = {nullptr, nullptr, nullptr};
-    int u0 = 0;                       // index of the buffer holding U^(0)
-    double* rper = nullptr;
-    double* partial = nullptr;
-    double* partial_min = nullptr;
-    long nparts_stage = 0, nparts_per = 0, nblk_dt = 0;
-    dev::ConvDev conv{};
+    CaseDev* cdev = nullptr;          // device copy: the case table of the case's own plan
     long hist_cap = 0;
     long iter_host = 1;               // next iteration number
-    int kc = 8;                       // k planes per stage-kernel block
     int tile = 0;                     // 0: 32x8, 1: 16x16, 2: 64x4 (i x j columns)
     bool tuned = false;
     std::string tuning;               // autotuner report
-    dim3 grid_stage;
-    // profiling
+    Plan solo;                        // the case advancing alone
+    tbfm_batch* batch = nullptr;      // batch the case is admitted to
+    // profiling of the stage-kernel launches (eager launches only)
     bool prof = false;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> pool;     // event pairs recorded around stage-kernel launches
     size_t pool_used = 0;             // pairs recorded since the last read-out
     double prof_ms = 0.0;
     long prof_launches = 0;
     double bytes_per_node = 0.0;
     long body_force_nodes = 0;        // nodes whose stage also runs the blade sources
-    // graph of three iterations (the buffer rotation has period 3)
-    hipGraphExec_t graph = nullptr;
-    int graph_u0 = -1;
     std::string err;
-    // host copies of static data for get_geometry
+};
+
+struct tbfm_batch {
+    Plan plan;
+    std::string err;
 };
 
 namespace {
@@ -132,11 +164,13 @@ int tileI(int t) { return t == 0 ? 32 : (t == 1 ? 16 : 64); }
 
 void setTiling(tbfm_case* cs, int tile, int kc) {
     const int ti = tileI(tile), tj = 256 / ti;
+    CaseDev& c = cs->c;
     cs->tile = tile;
-    cs->kc = kc;
-    cs->grid_stage = dim3((unsigned)((cs->ni + ti - 1) / ti), (unsigned)((cs->nj + tj - 1) / tj),
-                          (unsigned)((cs->nk + kc - 1) / kc));
-    cs->nparts_stage = (long)cs->grid_stage.x * cs->grid_stage.y * cs->grid_stage.z;
+    c.kc = kc;
+    c.sgx = (int)((cs->ni + ti - 1) / ti);
+    c.sgy = (int)((cs->nj + tj - 1) / tj);
+    c.sgz = (int)((cs->nk + kc - 1) / kc);
+    c.nparts_stage = c.sgx * c.sgy * c.sgz;
 }
 
 double atan2Positive(double y, double x) {
@@ -154,9 +188,10 @@ void setup(tbfm_case* cs, const GridInput& grid) {
     cs->scheme = cfg.getAdvectionScheme();
     cs->precision = cfg.getGpuPrecision();
     cs->rk = cfg.getTimeIntegrationCoeffs();
-    // kernels index state (5N), face and dual-node arrays with 32-bit offsets
-    if (5 * N >= 2147483647L || (NI + 1) * (NJ + 1) * (NK + 1) >= 2147483647L)
-        throw std::runtime_error("grid too large for one case (5*N must stay below 2^31)");
+    // kernels index the state (5N values) and the face-normal array (12 values per node)
+    // with 32-bit offsets
+    if (5 * N >= 2147483647L || 4 * ((NI + 1) * NJ * NK + NI * (NJ + 1) * NK + NI * NJ * (NK + 1)) >= 2147483647L)
+        throw std::runtime_error("grid too large for one case (32-bit offsets: about 170 million nodes at most)");
     if (cs->topo == Topology::ThreeD && NK < 4) throw std::runtime_error("3D topology needs NK >= 4");
     if (cs->topo != Topology::ThreeD && NK != 1) throw std::runtime_error("2D/axisymmetric topology needs NK = 1");
     if (NI < 3 || (cs->topo != Topology::OneD && NJ < 3))
@@ -165,7 +200,8 @@ void setup(tbfm_case* cs, const GridInput& grid) {
     cs->geo = buildGeometry(grid, cs->topo, bfm);
     const Geometry& g = cs->geo;
 
-    HIPCHK(hipStreamCreateWithFlags(&cs->stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&cs->solo.stream, hipStreamNonBlocking));
+    g_bytes = &cs->dev_bytes;
     auto& own = cs->owned;
     CaseDev& c = cs->c;
     c.ni = (int)NI; c.nj = (int)NJ; c.nk = (int)NK; c.N = N;
@@ -321,9 +357,9 @@ void setup(tbfm_case* cs, const GridInput& grid) {
     c.dirI = upload(inletDirs(0, pI, NJ, NK), own);
     c.dirJ = upload(inletDirs(1, pJ, NI, NK), own);
     c.dirK = upload(inletDirs(2, pK, NI, NJ), own);
-    c.bI = dalloc<double>(2 * 5 * NJ * NK, own);
-    c.bJ = dalloc<double>(2 * 5 * NI * NK, own);
-    c.bK = dalloc<double>(2 * 5 * NI * NJ, own);
+    c.boffJ = (int)(10 * NJ * NK);
+    c.boffK = (int)(10 * NJ * NK + 10 * NI * NK);
+    c.bnd = dalloc<double>(10 * (NJ * NK + NI * NK + NI * NJ), own);
 
     // ---- radial equilibrium profiles (solver.cpp:296-331)
     c.nprof = nprof;
@@ -355,22 +391,32 @@ void setup(tbfm_case* cs, const GridInput& grid) {
     c.sjx = upload(x, own); c.sjy = upload(y, own); c.sjz = upload(z, own);
     faceSoA(g.surfK, NI, NJ, NK + 1, x, y, z);
     c.skx = upload(x, own); c.sky = upload(y, own); c.skz = upload(z, own);
-    auto normals = [&](const std::vector<double>& S, long ai, long aj, long ak, const double** nx, const double** ny,
-                       const double** nz, const double** na) {
-        std::vector<double> a, bx, by, bz;
-        faceSoA(S, ai, aj, ak, bx, by, bz);
-        a.resize(bx.size());
-        for (size_t q = 0; q < bx.size(); ++q) {
-            // computeAdvectionFluxFromPrimitive: normal = surface / surface.magnitude()
-            const double m = std::sqrt(bx[q] * bx[q] + by[q] * by[q] + bz[q] * bz[q]);
-            a[q] = m;
-            bx[q] = bx[q] / m; by[q] = by[q] / m; bz[q] = bz[q] / m;
-        }
-        *nx = upload(bx, own); *ny = upload(by, own); *nz = upload(bz, own); *na = upload(a, own);
-    };
-    normals(g.surfI, NI + 1, NJ, NK, &c.nix, &c.niy, &c.niz, &c.nia);
-    normals(g.surfJ, NI, NJ + 1, NK, &c.njx, &c.njy, &c.njz, &c.nja);
-    normals(g.surfK, NI, NJ, NK + 1, &c.nkx, &c.nky, &c.nkz, &c.nka);
+    // unit normals and areas: [I: nx ny nz |S|][J: ...][K: ...] in one array
+    c.nfI = (int)((NI + 1) * NJ * NK);
+    c.nfJ = (int)(NI * (NJ + 1) * NK);
+    c.nfK = (int)(NI * NJ * (NK + 1));
+    c.noffJ = 4 * c.nfI;
+    c.noffK = 4 * (c.nfI + c.nfJ);
+    {
+        std::vector<double> nrm(4L * (c.nfI + c.nfJ + c.nfK));
+        auto normals = [&](const std::vector<double>& S, long ai, long aj, long ak, long off) {
+            std::vector<double> bx, by, bz;
+            faceSoA(S, ai, aj, ak, bx, by, bz);
+            const long nf = (long)bx.size();
+            for (long q = 0; q < nf; ++q) {
+                // computeAdvectionFluxFromPrimitive: normal = surface / surface.magnitude()
+                const double m = std::sqrt(bx[q] * bx[q] + by[q] * by[q] + bz[q] * bz[q]);
+                nrm[off + q] = bx[q] / m;
+                nrm[off + nf + q] = by[q] / m;
+                nrm[off + 2 * nf + q] = bz[q] / m;
+                nrm[off + 3 * nf + q] = m;
+            }
+        };
+        normals(g.surfI, NI + 1, NJ, NK, 0);
+        normals(g.surfJ, NI, NJ + 1, NK, c.noffJ);
+        normals(g.surfK, NI, NJ, NK + 1, c.noffK);
+        c.nrm = upload(nrm, own);
+    }
     faceSoA(g.dual, NI + 1, NJ + 1, NK + 1, x, y, z);
     c.dux = upload(x, own); c.duy = upload(y, own); c.duz = upload(z, own);
     std::vector<double> tmp;
@@ -486,34 +532,40 @@ void setup(tbfm_case* cs, const GridInput& grid) {
     c.stop = dalloc<int>(1, own);
     c.iter = dalloc<long>(1, own);
 
-    // ---- state buffers, scratch, reductions
-    for (int b = 0; b < 3; ++b) cs->U[b] = dalloc<double>(5 * N, own);
-    cs->rper = dalloc<double>(10 * NI * NJ, own);
-    cs->kc = (cs->topo == Topology::ThreeD) ? 8 : 1;
-    cs->tile = 0;
-    setTiling(cs, cs->tile, cs->kc);
-    cs->nparts_per = periodic ? (NI * NJ + 255) / 256 : 0;
+    // ---- state buffers (roles A, B, C), reductions, convergence
+    for (int b = 0; b < 3; ++b) c.U[b] = dalloc<double>(5 * N, own);
+    c.nstages = (int)cs->rk.size();
+    for (int q = 0; q < 4; ++q) c.alpha[q] = (q < c.nstages) ? cs->rk[q] : 0.0;
+    c.rper = dalloc<double>(10 * NI * NJ, own);
+    setTiling(cs, 0, (cs->topo == Topology::ThreeD) ? 8 : 1);
+    c.nparts_per = (periodic && cs->topo == Topology::ThreeD) ? (int)((NI * NJ + 255) / 256) : 0;
     // norm partials sized for the finest tiling the autotuner may pick
     long maxParts = 0;
     for (int t = 0; t < 3; ++t) {
         const int ti = tileI(t), tj = 256 / ti;
         maxParts = std::max(maxParts, ((NI + ti - 1) / ti) * ((NJ + tj - 1) / tj) * NK);
     }
-    cs->partial = dalloc<double>(5 * (maxParts + cs->nparts_per), own);
-    cs->nblk_dt = (N + 255) / 256;
-    cs->partial_min = dalloc<double>(cs->nblk_dt, own);
+    c.partial = dalloc<double>(5 * (maxParts + c.nparts_per), own);
+    c.nblk_dt = (int)((N + 255) / 256);
+    c.partial_min = dalloc<double>(c.nblk_dt, own);
     cs->hist_cap = std::max<long>(cfg.has("N_ITERATIONS") ? cfg.getMaxIterations() : 0, 1);
-    cs->conv.hist = dalloc<double>(5 * cs->hist_cap, own);
-    cs->conv.init = dalloc<double>(5, own);
-    cs->conv.conv_at = dalloc<long>(1, own);
-    cs->conv.diverged = dalloc<int>(1, own);
-    cs->conv.hist_len = cs->hist_cap;
-    cs->conv.drop = (double)cfg.getResidualsDropConvergence();
-    cs->conv.steady = cfg.isSimulationSteady() ? 1 : 0;
+    c.conv.hist = dalloc<double>(5 * cs->hist_cap, own);
+    c.conv.init = dalloc<double>(5, own);
+    c.conv.conv_at = dalloc<long>(1, own);
+    c.conv.diverged = dalloc<int>(1, own);
+    c.conv.hist_len = cs->hist_cap;
+    c.conv.drop = (double)cfg.getResidualsDropConvergence();
+    c.conv.steady = cfg.isSimulationSteady() ? 1 : 0;
+    c.conv.density_only = 0;
     long one = 1;
     HIPCHK(hipMemcpy(c.iter, &one, sizeof(long), hipMemcpyHostToDevice));
     cs->cdev = dalloc<CaseDev>(1, own);
     HIPCHK(hipMemcpy(cs->cdev, &c, sizeof(CaseDev), hipMemcpyHostToDevice));
+    g_bytes = nullptr;
+    cs->solo.members = {cs};
+    cs->solo.dcases = cs->cdev;
+    cs->solo.dcap = 1;
+    cs->solo.ownsCases = false;
     // algorithmic bytes per node and stage: U^(k-1), U^(0), U^(k) (5 each), dt, V and one
     // face vector per active direction
     const int dirs = (cs->topo == Topology::ThreeD) ? 3 : (cs->topo == Topology::OneD ? 1 : 2);
@@ -536,8 +588,6 @@ void setup(tbfm_case* cs, const GridInput& grid) {
             U0[n] = rho; U0[N + n] = rho * vx; U0[2 * N + n] = rho * vy; U0[3 * N + n] = rho * vz; U0[4 * N + n] = rho * et;
         }
     }
-    HIPCHK(hipEventCreate(&cs->ev0));
-    HIPCHK(hipEventCreate(&cs->ev1));
     int rc = tbfm_gpu_set_state(cs, U0.data());
     if (rc != TBFM_OK) throw std::runtime_error(cs->err);
 }
@@ -548,154 +598,345 @@ int fail(tbfm_case* c, int code, const std::string& msg) {
     return code;
 }
 
+int failb(tbfm_batch* b, int code, const std::string& msg) {
+    if (b) b->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
 template <typename F>
-int guarded(tbfm_case* c, F&& f) {
+int guarded(std::string* err, F&& f) {
+    auto set = [&](const std::string& m) {
+        if (err) *err = m;
+        else g_create_error = m;
+    };
     try {
         return f();
     } catch (const HipError& e) {
-        return fail(c, e.code == hipErrorOutOfMemory ? TBFM_ERR_OUT_OF_MEMORY : TBFM_ERR_HIP, e.what());
+        set(e.what());
+        return e.code == hipErrorOutOfMemory ? TBFM_ERR_OUT_OF_MEMORY : TBFM_ERR_HIP;
     } catch (const std::bad_alloc&) {
-        return fail(c, TBFM_ERR_OUT_OF_MEMORY, "host out of memory");
+        set("host out of memory");
+        return TBFM_ERR_OUT_OF_MEMORY;
     } catch (const std::exception& e) {
-        return fail(c, TBFM_ERR_ARGUMENT, e.what());
+        set(e.what());
+        return TBFM_ERR_ARGUMENT;
+    }
+}
+template <typename F>
+int guarded(tbfm_case* c, F&& f) { return guarded(c ? &c->err : nullptr, f); }
+
+// ------------------------------------------------------------------ plans
+void freeTable(Table& t) {
+    if (t.d) hipFree(t.d);
+    t = Table{};
+}
+
+void uploadTable(Table& t, const std::vector<dev::Work>& w) {
+    if (w.size() > t.cap) {
+        freeTable(t);
+        t.cap = std::max<size_t>(w.size(), 16);
+        HIPCHK(hipMalloc(&t.d, t.cap * sizeof(dev::Work)));
+    }
+    t.n = (int)w.size();
+    if (!w.empty()) HIPCHK(hipMemcpy(t.d, w.data(), w.size() * sizeof(dev::Work), hipMemcpyHostToDevice));
+}
+
+void freePlan(Plan& P) {
+    if (P.stream) hipStreamSynchronize(P.stream);
+    if (P.graph) hipGraphExecDestroy(P.graph);
+    P.graph = nullptr;
+    for (Table* t : {&P.dt, &P.mn, &P.prof, &P.bnd, &P.per, &P.norm}) freeTable(*t);
+    for (auto& g : P.groups) freeTable(g.t);
+    P.groups.clear();
+    if (P.ownsCases && P.dcases) hipFree(P.dcases);
+    P.dcases = nullptr;
+}
+
+// Copies the host case descriptions of the plan's members into its device case table.
+void uploadCases(Plan& P) {
+    if (P.ownsCases) {
+        if (P.members.size() > P.dcap) {
+            if (P.dcases) hipFree(P.dcases);
+            P.dcap = std::max<size_t>(P.members.size(), 8);
+            HIPCHK(hipMalloc(&P.dcases, P.dcap * sizeof(CaseDev)));
+        }
+        std::vector<CaseDev> h;
+        for (tbfm_case* m : P.members) h.push_back(m->c);
+        if (!h.empty()) HIPCHK(hipMemcpy(P.dcases, h.data(), h.size() * sizeof(CaseDev), hipMemcpyHostToDevice));
+    } else {
+        HIPCHK(hipMemcpy(P.dcases, &P.members[0]->c, sizeof(CaseDev), hipMemcpyHostToDevice));
     }
 }
 
-// ------------------------------------------------------------------- launches
+// Re-publishes a case's host description to every device case table that holds it
+// (its own plan's and, once the batch's tables are built, its batch's).
+void pushCase(tbfm_case* cs) {
+    HIPCHK(hipStreamSynchronize(cs->solo.stream));
+    HIPCHK(hipMemcpy(cs->cdev, &cs->c, sizeof(CaseDev), hipMemcpyHostToDevice));
+    if (cs->batch && cs->batch->plan.tablesValid) {
+        Plan& P = cs->batch->plan;
+        HIPCHK(hipStreamSynchronize(P.stream));
+        for (size_t m = 0; m < P.members.size(); ++m)
+            if (P.members[m] == cs) HIPCHK(hipMemcpy(P.dcases + m, &cs->c, sizeof(CaseDev), hipMemcpyHostToDevice));
+    }
+}
+
+// Builds the work tables of every kernel from the members' current tiling.
+void buildTables(Plan& P) {
+    std::vector<dev::Work> dt, mn, prof, bnd, per, norm;
+    for (auto& g : P.groups) freeTable(g.t);
+    P.groups.clear();
+    std::vector<std::vector<dev::Work>> gw;
+    P.nstages = P.members.empty() ? 0 : P.members[0]->c.nstages;
+    for (int m = 0; m < (int)P.members.size(); ++m) {
+        tbfm_case* cs = P.members[m];
+        const CaseDev& c = cs->c;
+        if (c.nstages != P.nstages) throw std::runtime_error("cases of one batch must use the same RK scheme");
+        for (int b = 0; b < c.nblk_dt; ++b) dt.push_back({m, b, 0, 0});
+        if (c.dt_mode == 1) mn.push_back({m, 0, 0, 0});
+        if (c.nprof > 0) prof.push_back({m, 0, 0, 0});
+        const long nb = 2 * cs->nj * cs->nk + ((cs->topo != Topology::OneD) ? 2 * cs->ni * cs->nk : 0) +
+                        ((cs->topo == Topology::ThreeD) ? 2 * cs->ni * cs->nj : 0);
+        for (long b = 0; b < (nb + 255) / 256; ++b) bnd.push_back({m, (int)b, 0, 0});
+        for (int b = 0; b < c.nparts_per; ++b) per.push_back({m, b, 0, 0});
+        norm.push_back({m, 0, 0, 0});
+        const int sch = (cs->scheme == Scheme::JST) ? 0 : 1;
+        const bool three = (cs->topo == Topology::ThreeD), mixed = (cs->precision == Precision::Mixed);
+        size_t gi = 0;
+        for (; gi < P.groups.size(); ++gi) {
+            const StageGroup& g = P.groups[gi];
+            if (g.tile == cs->tile && g.sch == sch && g.three == three && g.mixed == mixed) break;
+        }
+        if (gi == P.groups.size()) {
+            StageGroup g;
+            g.tile = cs->tile; g.sch = sch; g.three = three; g.mixed = mixed;
+            P.groups.push_back(g);
+            gw.emplace_back();
+        }
+        // k chunks outermost so that the blocks of one plane range run together
+        for (int z = 0; z < c.sgz; ++z)
+            for (int y = 0; y < c.sgy; ++y)
+                for (int x = 0; x < c.sgx; ++x) gw[gi].push_back({m, x, y, z});
+    }
+    uploadTable(P.dt, dt);
+    uploadTable(P.mn, mn);
+    uploadTable(P.prof, prof);
+    uploadTable(P.bnd, bnd);
+    uploadTable(P.per, per);
+    uploadTable(P.norm, norm);
+    for (size_t g = 0; g < P.groups.size(); ++g) uploadTable(P.groups[g].t, gw[g]);
+    P.tablesValid = true;
+    P.graphValid = false;
+}
+
+dev::Launch launchArg(const Plan& P, const Table& t, int in, int out, int stage, int last, int rmode) {
+    dev::Launch L;
+    L.cases = P.dcases;
+    L.work = t.d;
+    L.in = in; L.out = out; L.stage = stage; L.last = last; L.rmode = rmode;
+    return L;
+}
+
 template <class TL, int SCH, bool THREED>
-void launchStagePrec(tbfm_case* cs, const StageDev& st) {
-    if (cs->precision == Precision::Mixed)
-        hipLaunchKernelGGL((dev::k_stage<TL, SCH, THREED, float>), cs->grid_stage, dim3(dev::NT), 0, cs->stream, cs->c, st);
+void launchStageT(const Plan& P, const StageGroup& g, const dev::Launch& L) {
+    const CaseDev* cs = L.cases;
+    const dev::Work* wk = L.work;
+    if (g.mixed)
+        hipLaunchKernelGGL((dev::k_stage<TL, SCH, THREED, float>), dim3((unsigned)g.t.n), dim3(dev::NT), 0, P.stream, cs, wk, L);
     else
-        hipLaunchKernelGGL((dev::k_stage<TL, SCH, THREED, double>), cs->grid_stage, dim3(dev::NT), 0, cs->stream, cs->c, st);
+        hipLaunchKernelGGL((dev::k_stage<TL, SCH, THREED, double>), dim3((unsigned)g.t.n), dim3(dev::NT), 0, P.stream, cs, wk, L);
 }
 
 template <int SCH, bool THREED>
-void launchStage(tbfm_case* cs, const StageDev& st) {
-    switch (cs->tile) {
-        case 1: launchStagePrec<dev::Tile<16, 16>, SCH, THREED>(cs, st); break;
-        case 2: launchStagePrec<dev::Tile<64, 4>, SCH, THREED>(cs, st); break;
-        default: launchStagePrec<dev::Tile<32, 8>, SCH, THREED>(cs, st); break;
+void launchStageS(const Plan& P, const StageGroup& g, const dev::Launch& L) {
+    switch (g.tile) {
+        case 1: launchStageT<dev::Tile<16, 16>, SCH, THREED>(P, g, L); break;
+        case 2: launchStageT<dev::Tile<64, 4>, SCH, THREED>(P, g, L); break;
+        default: launchStageT<dev::Tile<32, 8>, SCH, THREED>(P, g, L); break;
     }
 }
 
-void launchStageKernel(tbfm_case* cs, const StageDev& st) {
-    const bool three = (cs->topo == Topology::ThreeD);
-    if (cs->scheme == Scheme::JST) {
-        if (three) launchStage<0, true>(cs, st); else launchStage<0, false>(cs, st);
-    } else {
-        if (three) launchStage<1, true>(cs, st); else launchStage<1, false>(cs, st);
+void launchStageGroups(const Plan& P, int in, int out, int stage, int last, int rmode) {
+    for (const StageGroup& g : P.groups) {
+        if (g.t.n == 0) continue;
+        const dev::Launch L = launchArg(P, g.t, in, out, stage, last, rmode);
+        if (g.sch == 0) {
+            if (g.three) launchStageS<0, true>(P, g, L); else launchStageS<0, false>(P, g, L);
+        } else {
+            if (g.three) launchStageS<1, true>(P, g, L); else launchStageS<1, false>(P, g, L);
+        }
     }
 }
 
-void stage(tbfm_case* cs, const double* Uin, double* Uout, double alpha, bool last, bool rmode = false) {
-    CaseDev& c = cs->c;
-    const long nb = (long)(2 * cs->nj * cs->nk + 2 * cs->ni * cs->nk + 2 * cs->ni * cs->nj);
-    hipLaunchKernelGGL(dev::k_boundary, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, cs->stream, c, Uin);
-    StageDev st{};
-    st.Uin = Uin;
-    st.U0 = cs->U[cs->u0];
-    st.Uout = Uout;
-    st.alpha = alpha;
-    st.last = last ? 1 : 0;
-    st.rper = cs->rper;
-    st.partial = cs->partial;
-    st.kc = cs->kc;
-    st.cdev = cs->cdev;
-    st.rmode = rmode ? 1 : 0;
-    if (cs->prof) {
-        if (cs->pool_used * 2 + 2 > cs->pool.size()) {
+void launchTimestep(const Plan& P, int in) {
+    if (P.dt.n) hipLaunchKernelGGL(dev::k_timestep, dim3((unsigned)P.dt.n), dim3(256), 0, P.stream, launchArg(P, P.dt, in, 0, 0, 0, 0));
+    if (P.mn.n) hipLaunchKernelGGL(dev::k_min_reduce, dim3((unsigned)P.mn.n), dim3(256), 0, P.stream, launchArg(P, P.mn, in, 0, 0, 0, 0));
+}
+
+void launchProfiles(const Plan& P, int in) {
+    if (P.prof.n) hipLaunchKernelGGL(dev::k_radial_profile, dim3((unsigned)P.prof.n), dim3(64), 0, P.stream, launchArg(P, P.prof, in, 0, 0, 0, 0));
+}
+
+void launchBoundary(const Plan& P, int in) {
+    if (P.bnd.n) hipLaunchKernelGGL(dev::k_boundary, dim3((unsigned)P.bnd.n), dim3(256), 0, P.stream, launchArg(P, P.bnd, in, 0, 0, 0, 0));
+}
+
+// One RK stage of every member: boundary fluxes, fused residual + update, periodic
+// closure, and after the last stage the norms and convergence test.
+void launchStage(Plan& P, int s, int in, int out, bool last, bool rmode, tbfm_case* prof) {
+    launchBoundary(P, in);
+    if (prof) {
+        if (prof->pool_used * 2 + 2 > prof->pool.size()) {
             hipEvent_t a, b;
             HIPCHK(hipEventCreate(&a));
             HIPCHK(hipEventCreate(&b));
-            cs->pool.push_back(a);
-            cs->pool.push_back(b);
+            prof->pool.push_back(a);
+            prof->pool.push_back(b);
         }
-        HIPCHK(hipEventRecord(cs->pool[2 * cs->pool_used], cs->stream));
+        HIPCHK(hipEventRecord(prof->pool[2 * prof->pool_used], P.stream));
     }
-    launchStageKernel(cs, st);
-    if (cs->prof) {
-        HIPCHK(hipEventRecord(cs->pool[2 * cs->pool_used + 1], cs->stream));
-        cs->pool_used += 1;
+    launchStageGroups(P, in, out, s, last ? 1 : 0, rmode ? 1 : 0);
+    if (prof) {
+        HIPCHK(hipEventRecord(prof->pool[2 * prof->pool_used + 1], P.stream));
+        prof->pool_used += 1;
     }
-    if (c.periodic && cs->topo == Topology::ThreeD) {
-        hipLaunchKernelGGL(dev::k_periodic, dim3((unsigned)cs->nparts_per), dim3(256), 0, cs->stream, c, st,
-                           cs->nparts_stage);
-    }
-    if (last)
-        hipLaunchKernelGGL(dev::k_norms, dim3(1), dim3(256), 0, cs->stream, c, (const double*)cs->partial,
-                           (int)(cs->nparts_stage + cs->nparts_per), cs->conv);
+    if (P.per.n)
+        hipLaunchKernelGGL(dev::k_periodic, dim3((unsigned)P.per.n), dim3(256), 0, P.stream,
+                           launchArg(P, P.per, in, out, s, last ? 1 : 0, rmode ? 1 : 0));
+    if (last) hipLaunchKernelGGL(dev::k_norms, dim3((unsigned)P.norm.n), dim3(256), 0, P.stream, launchArg(P, P.norm, 0, 0, 0, 0, 0));
+}
+
+// One explicit iteration of every member (solver.cpp:1337-1381): time step, radial
+// profiles, then the RK stages A -> B -> C -> ... -> A with the state back in A.
+void launchIteration(Plan& P, tbfm_case* prof) {
+    launchTimestep(P, 0);
+    launchProfiles(P, 0);
+    for (int s = 0; s < P.nstages; ++s)
+        launchStage(P, s, dev::stage_in(s), dev::stage_out(s, P.nstages), s + 1 == P.nstages, false, prof);
     HIPCHK(hipGetLastError());
 }
 
-void timestep(tbfm_case* cs, const double* U) {
-    hipLaunchKernelGGL(dev::k_timestep, dim3((unsigned)cs->nblk_dt), dim3(256), 0, cs->stream, cs->c, U, cs->partial_min);
-    if (cs->c.dt_mode == 1)
-        hipLaunchKernelGGL(dev::k_min_reduce, dim3(1), dim3(256), 0, cs->stream, cs->c, (const double*)cs->partial_min,
-                           (int)cs->nblk_dt);
+// Captures one iteration of the plan into a graph (no memory operations inside).
+void capture(Plan& P) {
+    if (P.graphValid) return;
+    if (P.graph) {
+        HIPCHK(hipGraphExecDestroy(P.graph));
+        P.graph = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(P.stream, hipStreamCaptureModeThreadLocal));
+    try {
+        launchIteration(P, nullptr);
+    } catch (...) {
+        hipStreamEndCapture(P.stream, &g);
+        if (g) hipGraphDestroy(g);
+        throw;
+    }
+    HIPCHK(hipStreamEndCapture(P.stream, &g));
+    hipError_t e = hipGraphInstantiate(&P.graph, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    HIPCHK(e);
+    P.graphValid = true;
 }
 
-// Autotuner of the stage kernel's tile shape and k-planes per block: times a few
-// launches of every candidate on the case's current state (the residual mode writes to
-// scratch, the state is untouched) and keeps the fastest.  Results are identical for
-// every tiling; only the time differs.
+// Autotuner of the stage kernel's tile shape and k planes per block: times launches of
+// the first RK stage of every candidate on the case's current state (reading buffer A,
+// writing only buffer B, which holds nothing between iterations), two rounds of five,
+// and keeps the fastest minimum.  Results are identical for every tiling, only the time
+// differs; the choice is cached per grid size, scheme, topology and precision.
+std::map<std::string, std::pair<int, int>> g_tuned;
+std::mutex g_tuned_mu;
+
 void autotune(tbfm_case* cs) {
     if (cs->tuned) return;
     cs->tuned = true;
-    if (cs->nk <= 1 && cs->ni * cs->nj < 4096) return;   // tiny 2-D cases: default shape
-    const double* U0 = cs->U[cs->u0];
-    double* scratch = cs->U[(cs->u0 + 1) % 3];
-    timestep(cs, U0);
-    const long nb = (long)(2 * cs->nj * cs->nk + 2 * cs->ni * cs->nk + 2 * cs->ni * cs->nj);
-    hipLaunchKernelGGL(dev::k_boundary, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, cs->stream, cs->c, U0);
+    Plan& P = cs->solo;
+    const std::string key = std::to_string(cs->ni) + "x" + std::to_string(cs->nj) + "x" + std::to_string(cs->nk) + "/" +
+                            std::to_string((int)cs->scheme) + std::to_string(cs->c.muscl) + "/" +
+                            std::to_string((int)cs->topo) + "/" + std::to_string((int)cs->precision);
+    {
+        std::lock_guard<std::mutex> lk(g_tuned_mu);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) {
+            setTiling(cs, it->second.first, it->second.second);
+            cs->tuning = "cached -> " + std::to_string(tileI(cs->tile)) + "x" + std::to_string(256 / tileI(cs->tile)) +
+                         "/kc" + std::to_string(cs->c.kc);
+            pushCase(cs);
+            return;
+        }
+    }
+    if (cs->nk <= 1 && cs->ni * cs->nj < 4096) {   // tiny 2-D cases: default shape
+        pushCase(cs);
+        return;
+    }
     std::vector<int> kcs = (cs->topo == Topology::ThreeD) ? std::vector<int>{2, 4, 8, 16} : std::vector<int>{1};
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     float best = 1e30f;
-    int bt = cs->tile, bk = cs->kc;
+    int bt = cs->tile, bk = cs->c.kc;
     char buf[96];
+    std::vector<std::pair<int, int>> cand;
     for (int t = 0; t < 3; ++t)
-        for (int kc : kcs) {
-            setTiling(cs, t, kc);
-            StageDev st{};
-            st.Uin = U0; st.U0 = U0; st.Uout = scratch; st.alpha = 1.0; st.last = 0; st.rper = cs->rper;
-            st.partial = cs->partial; st.kc = kc; st.cdev = cs->cdev; st.rmode = 1;
-            launchStageKernel(cs, st);   // warm-up
-            HIPCHK(hipEventRecord(e0, cs->stream));
-            for (int r = 0; r < 3; ++r) launchStageKernel(cs, st);
-            HIPCHK(hipEventRecord(e1, cs->stream));
-            HIPCHK(hipEventSynchronize(e1));
-            float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-            std::snprintf(buf, sizeof buf, "%dx%d/kc%d:%.4fms ", tileI(t), 256 / tileI(t), kc, ms / 3);
-            cs->tuning += buf;
-            if (ms < best) { best = ms; bt = t; bk = kc; }
+        for (int kc : kcs) cand.push_back({t, kc});
+    std::vector<float> tmin(cand.size(), 1e30f);
+    for (int round = 0; round < 2; ++round)
+        for (size_t q = 0; q < cand.size(); ++q) {
+            setTiling(cs, cand[q].first, cand[q].second);
+            pushCase(cs);
+            buildTables(P);
+            launchTimestep(P, 0);
+            launchBoundary(P, 0);
+            launchStageGroups(P, 0, 1, 0, 0, 0);   // warm-up
+            for (int r = 0; r < 5; ++r) {
+                HIPCHK(hipEventRecord(e0, P.stream));
+                launchStageGroups(P, 0, 1, 0, 0, 0);
+                HIPCHK(hipEventRecord(e1, P.stream));
+                HIPCHK(hipEventSynchronize(e1));
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                tmin[q] = std::min(tmin[q], ms);
+            }
         }
+    for (size_t q = 0; q < cand.size(); ++q) {
+        std::snprintf(buf, sizeof buf, "%dx%d/kc%d:%.4fms ", tileI(cand[q].first), 256 / tileI(cand[q].first),
+                      cand[q].second, tmin[q]);
+        cs->tuning += buf;
+        if (tmin[q] < best) { best = tmin[q]; bt = cand[q].first; bk = cand[q].second; }
+    }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     setTiling(cs, bt, bk);
+    pushCase(cs);
+    buildTables(P);
     std::snprintf(buf, sizeof buf, "-> %dx%d/kc%d", tileI(bt), 256 / tileI(bt), bk);
     cs->tuning += buf;
+    std::lock_guard<std::mutex> lk(g_tuned_mu);
+    g_tuned[key] = {bt, bk};
 }
 
-// One explicit iteration (solver.cpp:1337-1381); rotates the three state buffers.
-void iteration(tbfm_case* cs) {
-    const double* U0 = cs->U[cs->u0];
-    timestep(cs, U0);
-    if (cs->c.nprof > 0)
-        hipLaunchKernelGGL(dev::k_radial_profile, dim3(1), dim3(64), 0, cs->stream, cs->c, U0);
-    const int x = (cs->u0 + 1) % 3, y = (cs->u0 + 2) % 3;
-    const double* in = U0;
-    int out = x;
-    const size_t ns = cs->rk.size();
-    for (size_t s = 0; s < ns; ++s) {
-        stage(cs, in, cs->U[out], cs->rk[s], s + 1 == ns);
-        in = cs->U[out];
-        out = (out == x) ? y : x;
-    }
-    cs->u0 = (out == x) ? y : x;   // buffer written last
-    HIPCHK(hipGetLastError());
+// Grows a case's device residual history to hold iteration `last`.
+void growHistory(tbfm_case* c, long last) {
+    if (last <= c->hist_cap) return;
+    const long cap = std::max(last, 2 * c->hist_cap);
+    size_t* saved = g_bytes;
+    g_bytes = &c->dev_bytes;
+    double* h = dalloc<double>(5 * cap, c->owned);
+    g_bytes = saved;
+    HIPCHK(hipStreamSynchronize(c->solo.stream));
+    HIPCHK(hipMemcpy(h, c->c.conv.hist, 5 * c->hist_cap * 8, hipMemcpyDeviceToDevice));
+    c->c.conv.hist = h;
+    c->c.conv.hist_len = cap;
+    c->hist_cap = cap;
+    pushCase(c);
+}
+
+// Host bookkeeping after a plan advanced: which iterations each member executed.
+void settle(tbfm_case* c) {
+    long devIter = 0;
+    HIPCHK(hipMemcpy(&devIter, c->c.iter, sizeof(long), hipMemcpyDeviceToHost));
+    c->iter_host = devIter;
 }
 
 }  // namespace
@@ -713,7 +954,7 @@ int tbfm_gpu_case_create(const tbfm_case_desc* d, tbfm_case** out) {
     if (!d || !out) return fail(nullptr, TBFM_ERR_ARGUMENT, "null argument");
     *out = nullptr;
     auto cs = std::make_unique<tbfm_case>();
-    int rc = guarded(nullptr, [&]() {
+    int rc = guarded((std::string*)nullptr, [&]() {
         GridInput g;
         g.ni = d->ni; g.nj = d->nj; g.nk = d->nk;
         const long N = d->ni * d->nj * d->nk;
@@ -735,6 +976,7 @@ int tbfm_gpu_case_create(const tbfm_case_desc* d, tbfm_case** out) {
         setup(cs.get(), g);
         return TBFM_OK;
     });
+    g_bytes = nullptr;
     if (rc != TBFM_OK) {
         tbfm_gpu_case_destroy(cs.release());
         return rc;
@@ -747,7 +989,7 @@ int tbfm_gpu_case_create_from_ini(const char* ini, tbfm_case** out) {
     if (!ini || !out) return fail(nullptr, TBFM_ERR_ARGUMENT, "null argument");
     *out = nullptr;
     auto cs = std::make_unique<tbfm_case>();
-    int rc = guarded(nullptr, [&]() {
+    int rc = guarded((std::string*)nullptr, [&]() {
         cs->cfg = Config(ini);
         auto resolve = [&](const std::string& p) { return (!p.empty() && p[0] == '/') ? p : cs->cfg.baseDir + "/" + p; };
         GridInput g = readGridCsv(resolve(cs->cfg.getGridFile()));
@@ -755,6 +997,7 @@ int tbfm_gpu_case_create_from_ini(const char* ini, tbfm_case** out) {
         setup(cs.get(), g);
         return TBFM_OK;
     });
+    g_bytes = nullptr;
     if (rc != TBFM_OK) {
         tbfm_gpu_case_destroy(cs.release());
         return rc;
@@ -763,15 +1006,15 @@ int tbfm_gpu_case_create_from_ini(const char* ini, tbfm_case** out) {
     return TBFM_OK;
 }
 
+int tbfm_gpu_batch_retire(tbfm_batch* b, tbfm_case* c);
+
 int tbfm_gpu_case_destroy(tbfm_case* c) {
     if (!c) return TBFM_OK;
-    if (c->stream) hipStreamSynchronize(c->stream);
-    if (c->graph) hipGraphExecDestroy(c->graph);
+    if (c->batch) tbfm_gpu_batch_retire(c->batch, c);
+    freePlan(c->solo);
     for (void* p : c->owned) hipFree(p);
     for (hipEvent_t e : c->pool) hipEventDestroy(e);
-    if (c->ev0) hipEventDestroy(c->ev0);
-    if (c->ev1) hipEventDestroy(c->ev1);
-    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->solo.stream) hipStreamDestroy(c->solo.stream);
     delete c;
     return TBFM_OK;
 }
@@ -784,7 +1027,8 @@ int tbfm_gpu_set_state(tbfm_case* c, const double* U) {
             toDevice(c, U + v * c->N, tmp);
             std::copy(tmp.begin(), tmp.end(), dU.begin() + v * c->N);
         }
-        HIPCHK(hipMemcpyAsync(c->U[c->u0], dU.data(), dU.size() * 8, hipMemcpyHostToDevice, c->stream));
+        hipStream_t st = c->solo.stream;
+        HIPCHK(hipMemcpyAsync(c->c.U[0], dU.data(), dU.size() * 8, hipMemcpyHostToDevice, st));
         // radial-equilibrium profiles from the new state (solver.cpp:818-834)
         std::vector<double> prof(dev::MAX_PROF * c->nj, 0.0);
         const double g = c->c.gamma;
@@ -797,8 +1041,8 @@ int tbfm_gpu_set_state(tbfm_case* c, const double* U) {
                 double e = W4 - 0.5 * (vm * vm);
                 prof[q * c->nj + m] = (g - 1) * rho * e;
             }
-        HIPCHK(hipMemcpyAsync(c->c.prof_p, prof.data(), prof.size() * 8, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipMemcpyAsync(c->c.prof_p, prof.data(), prof.size() * 8, hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
         return TBFM_OK;
     });
 }
@@ -807,60 +1051,62 @@ int tbfm_gpu_get_state(tbfm_case* c, double* U) {
     if (!c || !U) return fail(c, TBFM_ERR_ARGUMENT, "null argument");
     return guarded(c, [&]() {
         std::vector<double> dU(5 * c->N);
-        HIPCHK(hipMemcpyAsync(dU.data(), c->U[c->u0], dU.size() * 8, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipMemcpyAsync(dU.data(), c->c.U[0], dU.size() * 8, hipMemcpyDeviceToHost, c->solo.stream));
+        HIPCHK(hipStreamSynchronize(c->solo.stream));
         for (int v = 0; v < 5; ++v) toHost(c, dU.data() + v * c->N, U + v * c->N);
         return TBFM_OK;
     });
 }
 
+namespace {
+// Residual history rows [first, last] of a case and its convergence / divergence status.
+int report(tbfm_case* c, long first, double* logres, long* converged_at) {
+    long conv = 0;
+    int div = 0;
+    HIPCHK(hipMemcpy(&conv, c->c.conv.conv_at, sizeof(long), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&div, c->c.conv.diverged, sizeof(int), hipMemcpyDeviceToHost));
+    if (logres) {
+        const long rows = std::max<long>(c->iter_host - first, 0);
+        if (rows > 0) {
+            std::vector<double> h(5 * rows);
+            HIPCHK(hipMemcpy(h.data(), c->c.conv.hist + (first - 1) * 5, h.size() * 8, hipMemcpyDeviceToHost));
+            std::copy(h.begin(), h.end(), logres);
+        }
+    }
+    if (converged_at) *converged_at = conv;
+    if (div) {
+        c->err = "Solver diverged: NaN or Inf detected in residual variable " + std::to_string(div - 1);
+        return TBFM_ERR_DIVERGED;
+    }
+    return TBFM_OK;
+}
+}  // namespace
+
 int tbfm_gpu_advance(tbfm_case* c, long n, double* logres, long* converged_at) {
     if (!c || n < 0) return fail(c, TBFM_ERR_ARGUMENT, "bad argument");
+    if (c->batch) return fail(c, TBFM_ERR_ARGUMENT, "case is admitted to a batch: advance the batch");
     return guarded(c, [&]() {
         const long first = c->iter_host;
-        if (first + n - 1 > c->hist_cap) {
-            // grow the residual history on the device
-            long cap = std::max(first + n - 1, 2 * c->hist_cap);
-            double* h = dalloc<double>(5 * cap, c->owned);
-            HIPCHK(hipMemcpyAsync(h, c->conv.hist, 5 * c->hist_cap * 8, hipMemcpyDeviceToDevice, c->stream));
-            c->conv.hist = h;
-            c->conv.hist_len = cap;
-            c->hist_cap = cap;
-        }
+        growHistory(c, first + n - 1);
         autotune(c);
-        const int u0Start = c->u0;
+        Plan& P = c->solo;
+        if (!P.tablesValid) buildTables(P);
         int stopped = 0;
         HIPCHK(hipMemcpy(&stopped, c->c.stop, sizeof(int), hipMemcpyDeviceToHost));
-        if (!stopped)
-            for (long q = 0; q < n; ++q) iteration(c);
-        HIPCHK(hipStreamSynchronize(c->stream));
+        if (!stopped) {
+            if (c->prof) {
+                for (long q = 0; q < n; ++q) launchIteration(P, c);
+            } else {
+                capture(P);
+                for (long q = 0; q < n; ++q) HIPCHK(hipGraphLaunch(P.graph, P.stream));
+            }
+        }
+        HIPCHK(hipStreamSynchronize(P.stream));
         if (c->prof) drainProfile(c);
-        long conv = 0, devIter = 0;
-        int div = 0;
-        HIPCHK(hipMemcpy(&conv, c->conv.conv_at, sizeof(long), hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(&div, c->conv.diverged, sizeof(int), hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(&devIter, c->c.iter, sizeof(long), hipMemcpyDeviceToHost));
-        // Iterations launched after the device stopped (convergence or divergence) were
-        // no-ops: the state is the one after the last executed iteration, whose buffer
-        // follows from the rotation (RK41A advances U^(0) by 2 buffers, RK31B by 1).
-        const long executed = stopped ? 0 : (devIter - first);
-        const int step = (c->rk.size() % 2 == 0) ? 2 : 1;
-        c->u0 = (int)((u0Start + step * executed) % 3);
-        c->iter_host = first + executed;
-        if (logres) {
-            long last = first + executed - 1;
-            long rows = std::max<long>(last - first + 1, 0);
-            std::vector<double> h(5 * c->hist_cap);
-            HIPCHK(hipMemcpy(h.data(), c->conv.hist, h.size() * 8, hipMemcpyDeviceToHost));
-            for (long r = 0; r < rows; ++r)
-                for (int v = 0; v < 5; ++v) logres[r * 5 + v] = h[(first - 1 + r) * 5 + v];
-        }
-        if (converged_at) *converged_at = conv;
-        if (div) {
-            c->err = "Solver diverged: NaN or Inf detected in residual variable " + std::to_string(div - 1);
-            return TBFM_ERR_DIVERGED;
-        }
-        return TBFM_OK;
+        // iterations launched after the device stopped (convergence or divergence) were
+        // no-ops; the state in buffer A is the one after the last executed iteration
+        settle(c);
+        return report(c, first, logres, converged_at);
     });
 }
 
@@ -872,18 +1118,20 @@ int tbfm_gpu_compute_timestep(tbfm_case* c, const double* U, double* dt) {
             toDevice(c, U + v * c->N, tmp);
             std::copy(tmp.begin(), tmp.end(), dU.begin() + v * c->N);
         }
-        int scratch = (c->u0 + 1) % 3;
-        HIPCHK(hipMemcpy(c->U[scratch], dU.data(), dU.size() * 8, hipMemcpyHostToDevice));
-        timestep(c, c->U[scratch]);
+        Plan& P = c->solo;
+        if (!P.tablesValid) buildTables(P);
+        HIPCHK(hipMemcpy(c->c.U[1], dU.data(), dU.size() * 8, hipMemcpyHostToDevice));
+        launchTimestep(P, 1);
+        HIPCHK(hipGetLastError());
         std::vector<double> d(c->N);
         if (c->c.dt_mode == 1) {
             double m = 0;
-            HIPCHK(hipMemcpyAsync(&m, c->c.dtmin, 8, hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(hipStreamSynchronize(c->stream));
+            HIPCHK(hipMemcpyAsync(&m, c->c.dtmin, 8, hipMemcpyDeviceToHost, P.stream));
+            HIPCHK(hipStreamSynchronize(P.stream));
             std::fill(d.begin(), d.end(), m);
         } else {
-            HIPCHK(hipMemcpyAsync(d.data(), c->c.dt, d.size() * 8, hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(hipStreamSynchronize(c->stream));
+            HIPCHK(hipMemcpyAsync(d.data(), c->c.dt, d.size() * 8, hipMemcpyDeviceToHost, P.stream));
+            HIPCHK(hipStreamSynchronize(P.stream));
         }
         toHost(c, d.data(), dt);
         return TBFM_OK;
@@ -898,19 +1146,20 @@ int tbfm_gpu_compute_residual(tbfm_case* c, const double* U, long it, int update
             toDevice(c, U + v * c->N, tmp);
             std::copy(tmp.begin(), tmp.end(), dU.begin() + v * c->N);
         }
-        const int a = (c->u0 + 1) % 3, b = (c->u0 + 2) % 3;
-        HIPCHK(hipMemcpy(c->U[a], dU.data(), dU.size() * 8, hipMemcpyHostToDevice));
+        Plan& P = c->solo;
+        if (!P.tablesValid) buildTables(P);
+        HIPCHK(hipMemcpy(c->c.U[1], dU.data(), dU.size() * 8, hipMemcpyHostToDevice));
         long itv = it;
         int stopSaved = 0, zero = 0;
         HIPCHK(hipMemcpy(&stopSaved, c->c.stop, sizeof(int), hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(c->c.iter, &itv, sizeof(long), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(c->c.stop, &zero, sizeof(int), hipMemcpyHostToDevice));
-        if (update_profiles && c->c.nprof > 0)
-            hipLaunchKernelGGL(dev::k_radial_profile, dim3(1), dim3(64), 0, c->stream, c->c, (const double*)c->U[a]);
-        stage(c, c->U[a], c->U[b], 0.0, false, true);
-        HIPCHK(hipStreamSynchronize(c->stream));
+        if (update_profiles) launchProfiles(P, 1);
+        launchStage(P, 0, 1, 2, false, true, nullptr);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(P.stream));
         std::vector<double> d(5 * c->N);
-        HIPCHK(hipMemcpy(d.data(), c->U[b], d.size() * 8, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(d.data(), c->c.U[2], d.size() * 8, hipMemcpyDeviceToHost));
         for (int v = 0; v < 5; ++v) toHost(c, d.data() + v * c->N, R + v * c->N);
         itv = c->iter_host;
         HIPCHK(hipMemcpy(c->c.iter, &itv, sizeof(long), hipMemcpyHostToDevice));
@@ -922,7 +1171,7 @@ int tbfm_gpu_compute_residual(tbfm_case* c, const double* U, long it, int update
 int tbfm_gpu_get_side_outputs(tbfm_case* c, double* finv, double* fvisc, double* devang, double* dt) {
     if (!c) return fail(c, TBFM_ERR_ARGUMENT, "null argument");
     return guarded(c, [&]() {
-        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipStreamSynchronize(c->solo.stream));
         std::vector<double> h3(3 * c->N), h(c->N), comp(c->N), out(c->N);
         auto vec3 = [&](const double* d, double* dst) {
             HIPCHK(hipMemcpy(h3.data(), d, h3.size() * 8, hipMemcpyDeviceToHost));
@@ -1067,7 +1316,7 @@ int tbfm_gpu_get_geometry(tbfm_case* c, int which, double* out) {
 namespace {
 // Folds the recorded event pairs into the running totals (synchronises the stream).
 void drainProfile(tbfm_case* c) {
-    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipStreamSynchronize(c->solo.stream));
     for (size_t q = 0; q < c->pool_used; ++q) {
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, c->pool[2 * q], c->pool[2 * q + 1]));
@@ -1110,7 +1359,7 @@ __global__ void __launch_bounds__(256) k_copy_calibration(const double* __restri
 
 int tbfm_gpu_copy_calibration(long nbytes, int repeats, double* ms_per_copy) {
     if (nbytes <= 0 || repeats <= 0) return fail(nullptr, TBFM_ERR_ARGUMENT, "bad argument");
-    return guarded(nullptr, [&]() {
+    return guarded((std::string*)nullptr, [&]() {
         std::vector<void*> own;
         const long n = nbytes / 8;
         double* a = dalloc<double>(n, own);
@@ -1140,9 +1389,131 @@ const char* tbfm_gpu_tuning(const tbfm_case* c) { return c ? c->tuning.c_str() :
 int tbfm_gpu_sync(tbfm_case* c) {
     if (!c) return fail(c, TBFM_ERR_ARGUMENT, "null argument");
     return guarded(c, [&]() {
-        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipStreamSynchronize(c->solo.stream));
         return TBFM_OK;
     });
+}
+
+
+// ------------------------------------------------------------------ batches
+int tbfm_gpu_case_device_bytes(const tbfm_case* c, long* bytes) {
+    if (!c || !bytes) return fail(nullptr, TBFM_ERR_ARGUMENT, "null argument");
+    *bytes = (long)c->dev_bytes;
+    return TBFM_OK;
+}
+
+int tbfm_gpu_case_set_convergence(tbfm_case* c, int density_only, double drop) {
+    if (!c || !(drop > 0.0)) return fail(c, TBFM_ERR_ARGUMENT, "bad argument");
+    return guarded(c, [&]() {
+        c->c.conv.density_only = density_only ? 1 : 0;
+        c->c.conv.drop = drop;
+        c->c.conv.steady = 1;
+        pushCase(c);
+        return TBFM_OK;
+    });
+}
+
+int tbfm_gpu_case_status(tbfm_case* c, long* iterations, long* converged_at, int* stopped, double* last_logres) {
+    if (!c) return fail(c, TBFM_ERR_ARGUMENT, "null argument");
+    return guarded(c, [&]() {
+        long conv = 0;
+        int st = 0;
+        HIPCHK(hipMemcpy(&conv, c->c.conv.conv_at, sizeof(long), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(&st, c->c.stop, sizeof(int), hipMemcpyDeviceToHost));
+        if (iterations) *iterations = c->iter_host - 1;
+        if (converged_at) *converged_at = conv;
+        if (stopped) *stopped = st;
+        if (last_logres) {
+            if (c->iter_host > 1)
+                HIPCHK(hipMemcpy(last_logres, c->c.conv.hist + (c->iter_host - 2) * 5, 5 * 8, hipMemcpyDeviceToHost));
+            else
+                for (int v = 0; v < 5; ++v) last_logres[v] = 0.0;
+        }
+        int div = 0;
+        HIPCHK(hipMemcpy(&div, c->c.conv.diverged, sizeof(int), hipMemcpyDeviceToHost));
+        if (div) {
+            c->err = "Solver diverged: NaN or Inf detected in residual variable " + std::to_string(div - 1);
+            return TBFM_ERR_DIVERGED;
+        }
+        return TBFM_OK;
+    });
+}
+
+const char* tbfm_gpu_batch_last_error(const tbfm_batch* b) { return b ? b->err.c_str() : g_create_error.c_str(); }
+
+int tbfm_gpu_batch_create(tbfm_batch** out) {
+    if (!out) return failb(nullptr, TBFM_ERR_ARGUMENT, "null argument");
+    *out = nullptr;
+    auto b = std::make_unique<tbfm_batch>();
+    int rc = guarded((std::string*)nullptr, [&]() {
+        HIPCHK(hipStreamCreateWithFlags(&b->plan.stream, hipStreamNonBlocking));
+        b->plan.ownsCases = true;
+        return TBFM_OK;
+    });
+    if (rc != TBFM_OK) return rc;
+    *out = b.release();
+    return TBFM_OK;
+}
+
+int tbfm_gpu_batch_admit(tbfm_batch* b, tbfm_case* c) {
+    if (!b || !c) return failb(b, TBFM_ERR_ARGUMENT, "null argument");
+    if (c->batch) return failb(b, TBFM_ERR_ARGUMENT, "case is already admitted to a batch");
+    return guarded(&b->err, [&]() {
+        Plan& P = b->plan;
+        if (!P.members.empty() && P.members[0]->c.nstages != c->c.nstages)
+            throw std::runtime_error("cases of one batch must use the same RK scheme");
+        autotune(c);
+        c->batch = b;
+        P.members.push_back(c);
+        P.tablesValid = false;
+        P.graphValid = false;
+        return TBFM_OK;
+    });
+}
+
+int tbfm_gpu_batch_retire(tbfm_batch* b, tbfm_case* c) {
+    if (!b || !c) return failb(b, TBFM_ERR_ARGUMENT, "null argument");
+    if (c->batch != b) return failb(b, TBFM_ERR_ARGUMENT, "case is not a member of this batch");
+    return guarded(&b->err, [&]() {
+        Plan& P = b->plan;
+        HIPCHK(hipStreamSynchronize(P.stream));
+        P.members.erase(std::find(P.members.begin(), P.members.end(), c));
+        c->batch = nullptr;
+        P.tablesValid = false;
+        P.graphValid = false;
+        return TBFM_OK;
+    });
+}
+
+int tbfm_gpu_batch_size(const tbfm_batch* b) { return b ? (int)b->plan.members.size() : 0; }
+
+int tbfm_gpu_batch_advance(tbfm_batch* b, long n) {
+    if (!b || n < 0) return failb(b, TBFM_ERR_ARGUMENT, "bad argument");
+    return guarded(&b->err, [&]() {
+        Plan& P = b->plan;
+        if (P.members.empty() || n == 0) return TBFM_OK;
+        for (tbfm_case* m : P.members) growHistory(m, m->iter_host + n - 1);
+        if (!P.tablesValid) {
+            uploadCases(P);
+            buildTables(P);
+        }
+        capture(P);
+        for (long q = 0; q < n; ++q) HIPCHK(hipGraphLaunch(P.graph, P.stream));
+        HIPCHK(hipStreamSynchronize(P.stream));
+        for (tbfm_case* m : P.members) settle(m);
+        return TBFM_OK;
+    });
+}
+
+int tbfm_gpu_batch_destroy(tbfm_batch* b) {
+    if (!b) return TBFM_OK;
+    if (b->plan.stream) hipStreamSynchronize(b->plan.stream);
+    for (tbfm_case* m : b->plan.members) m->batch = nullptr;
+    b->plan.members.clear();
+    freePlan(b->plan);
+    if (b->plan.stream) hipStreamDestroy(b->plan.stream);
+    delete b;
+    return TBFM_OK;
 }
 
 }  // extern "C"

@@ -32,7 +32,22 @@ struct PatchDev {
 constexpr int MAX_PATCH = 32;
 constexpr int MAX_PROF = 4;
 
-// Everything a kernel needs; passed by value as the kernel argument.
+// Residual norms, baseline and convergence state of one case (solver.cpp:1426-1439,
+// 1488-1508, 2985-3015).
+struct ConvDev {
+    double* hist;        // [hist_len][5] log residuals of iterations 1..hist_len
+    double* init;        // [5] baseline (max over iterations 1-5)
+    long* conv_at;       // iteration of convergence, 0 if none
+    int* diverged;       // 1 + variable index
+    long hist_len;
+    double drop;         // orders of magnitude
+    int steady;
+    int density_only;    // 1: converged when the density residual alone has dropped
+};
+
+// Everything the kernels need about one case.  Kept in device memory (one entry per
+// case of a batch) and read by the kernels through the launch's case table, so that a
+// captured launch sequence sees later updates (history growth, tiling, stop flags).
 struct CaseDev {
     int ni, nj, nk;
     long N;
@@ -48,15 +63,21 @@ struct CaseDev {
     double dtheta;                // Gong: 2*pi/(NK-1)
     // geometry
     const double *six, *siy, *siz, *sjx, *sjy, *sjz, *skx, *sky, *skz;
-    // unit normals S/|S| and areas |S| of every face (same layout), computed on the host
-    // with the reference's own operations (Vector3D::magnitude, operator/)
-    const double *nix, *niy, *niz, *nia, *njx, *njy, *njz, *nja, *nkx, *nky, *nkz, *nka;
+    // unit normals S/|S| and areas |S| of every face, computed on the host with the
+    // reference's own operations (Vector3D::magnitude, operator/), in one allocation:
+    // per direction four planes (nx, ny, nz, |S|) of nfI / nfJ / nfK faces (layout as S),
+    // the I block at 0, the J block at noffJ, the K block at noffK
+    const double* nrm;
+    int nfI, nfJ, nfK, noffJ, noffK;
     const double *vol;
     const double *dux, *duy, *duz;     // dual nodes (NI+1)(NJ+1)(NK+1)
     // boundary faces: patch id and inlet direction per face; flux buffers
     const int *pI, *pJ, *pK;           // [2][...] patch index
     const double *dirI, *dirJ, *dirK;  // [2][3][...]
-    double *bI, *bJ, *bK;              // [2][5][...] flux*|S|
+    // boundary-face fluxes * |S|, one allocation: per direction [2 sides][5][faces of a
+    // side] (bszI = NJ*NK, bszJ = NI*NK, bszK = NI*NJ), the I block at 0, J at boffJ, K at boffK
+    double* bnd;
+    int boffJ, boffK;
     PatchDev patch[MAX_PATCH];
     // radial equilibrium profiles
     int nprof;
@@ -80,20 +101,40 @@ struct CaseDev {
     // convergence / control on device
     int* stop;                         // non-zero: skip all work
     long* iter;                        // current iteration counter (1-based during an iteration)
+    ConvDev conv;
+    // state buffers by role: 0 = A holds U^(0) at iteration start and receives the last
+    // stage in place; stages alternate between B (1) and C (2) in between
+    double* U[3];
+    int nstages;
+    double alpha[4];
+    // stage-kernel tiling: k planes per block and blocks along i, j, k
+    int kc, sgx, sgy, sgz;
+    double* rper;                      // [2][5][NI*NJ] residuals of the periodic planes
+    double* partial;                   // [stage blocks + periodic blocks][5] norm partial sums
+    int nparts_stage, nparts_per;
+    double* partial_min;               // [nblk_dt] per-block minima of the global time step
+    int nblk_dt;
 };
 
-struct StageDev {
-    const double* Uin;     // U^(k-1)
-    const double* U0;      // U^(0)
-    double* Uout;          // U^(k)
-    double alpha;
+// One block of a launch: the case it works on and its block coordinates in that case.
+struct Work {
+    int cs, x, y, z;
+};
+
+// Kernel argument of every launch: all cases of a batch advance in the same launch.
+struct Launch {
+    const CaseDev* cases;
+    const Work* work;
+    int in, out;           // state-buffer roles read and written
+    int stage;             // RK stage index
     int last;              // last stage: norms and side outputs
-    double* rper;          // [2][5][NI*NJ] residuals of the periodic planes
-    double* partial;       // [blocks][5] norm partial sums
-    int kc;                // k planes per block
-    const struct CaseDev* cdev;   // device-resident copy of the case description
     int rmode;             // write the residual instead of the update (residual entry point)
 };
+
+// RK stage s of ns reads role stage_in(s) and writes role stage_out(s, ns):
+// A -> B -> C -> B -> A (4 stages), A -> B -> C -> A (3 stages).
+__host__ __device__ inline int stage_in(int s) { return s == 0 ? 0 : (s % 2 == 1 ? 1 : 2); }
+__host__ __device__ inline int stage_out(int s, int ns) { return s == ns - 1 ? 0 : (s % 2 == 0 ? 1 : 2); }
 
 __device__ __forceinline__ double dmaxd(double a, double b) { return (a < b) ? b : a; }
 __device__ __forceinline__ double dmind(double a, double b) { return (b < a) ? b : a; }

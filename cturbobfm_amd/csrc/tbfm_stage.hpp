@@ -129,10 +129,14 @@ __device__ __noinline__ void node_sources(const CaseDev* __restrict__ cd, const 
 
 // --------------------------------------------------------------------- time step
 // Solver::computeTimestepArray, inviscid (src/solver.cpp:1531-1630)
-__global__ void __launch_bounds__(256) k_timestep(CaseDev c, const double* U, double* partial_min) {
+// Work: one entry per 256 nodes of a case (x = block index).
+__global__ void __launch_bounds__(256) k_timestep(Launch L) {
     __shared__ double smin[256];
+    const Work w = L.work[blockIdx.x];
+    const CaseDev& c = L.cases[w.cs];
     if (*c.stop) return;
-    const long n = (long)blockIdx.x * 256 + threadIdx.x;
+    const double* U = c.U[L.in];
+    const long n = (long)w.x * 256 + threadIdx.x;
     double dtv = 1.0e300;
     if (n < c.N) {
         if (c.dt_mode == 2) {
@@ -177,15 +181,18 @@ __global__ void __launch_bounds__(256) k_timestep(CaseDev c, const double* U, do
             if (threadIdx.x < s) smin[threadIdx.x] = dmind(smin[threadIdx.x], smin[threadIdx.x + s]);
             __syncthreads();
         }
-        if (threadIdx.x == 0) partial_min[blockIdx.x] = smin[0];
+        if (threadIdx.x == 0) c.partial_min[w.x] = smin[0];
     }
 }
 
-__global__ void __launch_bounds__(256) k_min_reduce(CaseDev c, const double* partial, int n) {
+// Work: one entry per case using the global time step.
+__global__ void __launch_bounds__(256) k_min_reduce(Launch L) {
     __shared__ double smin[256];
+    const Work w = L.work[blockIdx.x];
+    const CaseDev& c = L.cases[w.cs];
     if (*c.stop) return;
     double m = 1.0e300;
-    for (int q = threadIdx.x; q < n; q += 256) m = dmind(m, partial[q]);
+    for (int q = threadIdx.x; q < c.nblk_dt; q += 256) m = dmind(m, c.partial_min[q]);
     smin[threadIdx.x] = m;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
@@ -198,9 +205,13 @@ __global__ void __launch_bounds__(256) k_min_reduce(CaseDev c, const double* par
 // ------------------------------------------------------- radial equilibrium profile
 // Solver::updateRadialProfiles + integrateRadialEquilibrium, radial_equilibrium branch
 // (src/solver.cpp:2688-2787, src/math_utils.cpp:141-170).  One thread per profile: the
-// trapezoid scan over the outlet column is sequential in j.
-__global__ void k_radial_profile(CaseDev c, const double* U) {
+// trapezoid scan over the outlet column is sequential in j.  Work: one entry per case
+// with profiles.
+__global__ void k_radial_profile(Launch L) {
+    const Work w = L.work[blockIdx.x];
+    const CaseDev& c = L.cases[w.cs];
     if (*c.stop) return;
+    const double* U = c.U[L.in];
     const int q = threadIdx.x;
     if (q >= c.nprof) return;
     const int n = c.prof_n[q];
@@ -235,11 +246,15 @@ __global__ void k_radial_profile(CaseDev c, const double* U) {
 // ------------------------------------------------------------- boundary fluxes
 // All boundary faces of all sides, one thread per face.  Start faces use the
 // outward vector -S and the node at the face; end faces +S and the node before it
-// (solver.cpp:1911-1957).  Result already multiplied by |S|.
-__global__ void __launch_bounds__(256) k_boundary(CaseDev c, const double* U) {
+// (solver.cpp:1911-1957).  Result already multiplied by |S|.  Work: one entry per 256
+// boundary faces of a case.
+__global__ void __launch_bounds__(256) k_boundary(Launch L) {
+    const Work w = L.work[blockIdx.x];
+    const CaseDev& c = L.cases[w.cs];
     if (*c.stop) return;
+    const double* U = c.U[L.in];
     const long NI = c.ni, NJ = c.nj, NK = c.nk;
-    long t = (long)blockIdx.x * 256 + threadIdx.x;
+    long t = (long)w.x * 256 + threadIdx.x;
     const long nI = NJ * NK, nJ = NI * NK, nK = NI * NJ;
     const bool doJ = (c.topo != TOPO_1D), doK = (c.topo == TOPO_3D);
     int dir, side;
@@ -259,19 +274,19 @@ __global__ void __launch_bounds__(256) k_boundary(CaseDev c, const double* U) {
         fi = side ? (int)NI : 0; fj = j; fk = k;
         i = side ? (int)NI - 1 : 0;
         sidx = ((long)k * NJ + j) * (NI + 1) + fi;
-        sx = c.six; sy = c.siy; sz = c.siz; pid = c.pI; dirv = c.dirI; out = c.bI; nsz = nI;
+        sx = c.six; sy = c.siy; sz = c.siz; pid = c.pI; dirv = c.dirI; out = c.bnd; nsz = nI;
     } else if (dir == 1) {
         i = (int)(q % NI); k = (int)(q / NI);
         fi = i; fj = side ? (int)NJ : 0; fk = k;
         j = side ? (int)NJ - 1 : 0;
         sidx = ((long)k * (NJ + 1) + fj) * NI + i;
-        sx = c.sjx; sy = c.sjy; sz = c.sjz; pid = c.pJ; dirv = c.dirJ; out = c.bJ; nsz = nJ;
+        sx = c.sjx; sy = c.sjy; sz = c.sjz; pid = c.pJ; dirv = c.dirJ; out = c.bnd + c.boffJ; nsz = nJ;
     } else {
         i = (int)(q % NI); j = (int)(q / NI);
         fi = i; fj = j; fk = side ? (int)NK : 0;
         k = side ? (int)NK - 1 : 0;
         sidx = ((long)fk * NJ + j) * NI + i;
-        sx = c.skx; sy = c.sky; sz = c.skz; pid = c.pK; dirv = c.dirK; out = c.bK; nsz = nK;
+        sx = c.skx; sy = c.sky; sz = c.skz; pid = c.pK; dirv = c.dirK; out = c.bnd + c.boffK; nsz = nK;
     }
     double Sx = sx[sidx], Sy = sy[sidx], Sz = sz[sidx];
     if (!side) { Sx = -Sx; Sy = -Sy; Sz = -Sz; }

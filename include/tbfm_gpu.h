@@ -37,6 +37,7 @@ extern "C" {
 #define TBFM_ERR_UNSUPPORTED (-5)
 
 typedef struct tbfm_case tbfm_case;
+typedef struct tbfm_batch tbfm_batch;
 
 /* Everything the reference reads from disk, passed as arrays.
  *   x, y, z, field_data[f]: N = ni*nj*nk doubles each, reference order (k fastest);
@@ -136,6 +137,43 @@ const char* tbfm_gpu_tuning(const tbfm_case* c);
 
 /* Synchronises the case stream. */
 int tbfm_gpu_sync(tbfm_case* c);
+
+/* ---------------------------------------------------------------- batches and serving
+ * No counterpart in the reference, which runs one case per process (apps/turbobfm.cpp,
+ * Solver::solve, src/solver.cpp:1298-1424).  A batch advances many cases (operating
+ * points, grids of uneven size) together: every kernel of an iteration is one launch
+ * over all members, through device work tables, and the iteration is replayed as one
+ * captured HIP graph.  Each member keeps its own device memory, state, iteration
+ * counter, residual history and convergence test, so a case computes exactly the same
+ * numbers alone and in any mix.  Cases keep ownership: retire or destroy them freely
+ * between advances; a member is advanced only through its batch. */
+
+/* Device memory a case holds (state, geometry, boundary and reduction buffers): the
+ * admission footprint of a serving loop. */
+int tbfm_gpu_case_device_bytes(const tbfm_case* c, long* bytes);
+
+/* Replaces the convergence rule of Solver::checkConvergence (src/solver.cpp:2985-3015)
+ * for this case: drop = orders of magnitude below the baseline of iterations 1-5;
+ * density_only = 1 tests the density residual alone (the serving criterion). */
+int tbfm_gpu_case_set_convergence(tbfm_case* c, int density_only, double drop);
+
+/* Iterations executed so far, convergence iteration (0 if none), whether the case has
+ * stopped (converged or diverged), and the log residuals of its last iteration;
+ * TBFM_ERR_DIVERGED after a NaN/Inf residual (solver.cpp:1494-1496). */
+int tbfm_gpu_case_status(tbfm_case* c, long* iterations, long* converged_at, int* stopped,
+                         double* last_logres);
+
+int tbfm_gpu_batch_create(tbfm_batch** out);
+/* Admits a case (autotunes its tiling on first use); members must share the RK scheme. */
+int tbfm_gpu_batch_admit(tbfm_batch* b, tbfm_case* c);
+int tbfm_gpu_batch_retire(tbfm_batch* b, tbfm_case* c);
+int tbfm_gpu_batch_size(const tbfm_batch* b);
+/* Advances every member n iterations (stopped members are skipped on the device) and
+ * synchronises. */
+int tbfm_gpu_batch_advance(tbfm_batch* b, long n_iter);
+/* Retires every member (the cases stay valid) and frees the batch. */
+int tbfm_gpu_batch_destroy(tbfm_batch* b);
+const char* tbfm_gpu_batch_last_error(const tbfm_batch* b);
 
 #ifdef __cplusplus
 }
