@@ -68,9 +68,112 @@ def cpu_baseline(case, seconds: float = 15.0):
             % (n, stages, case["ni"], case["nj"], case["nk"], dt)}
 
 
+def bench_batch(args):
+    """Batched throttle sweep: rotor operating points on four grid sizes (outlet pressure
+    drawn uniformly from 90-110 kPa, seeded), FP32 flux / FP64 state, all advanced in one
+    batch (one launch per kernel, captured graph per iteration).  Compared with the plain
+    path: each case alone in FP64, one after another, for the same iterations."""
+    from cturbobfm_amd import Batch, Solver, cases
+    rng = np.random.default_rng(args.seed)
+    sizes = [(97, 25, 65), (129, 33, 97), (193, 49, 129), (257, 65, 193)]
+    per = args.batch_cases // len(sizes)
+    pts = [(sz, float(rng.uniform(90000.0, 110000.0))) for sz in sizes for _ in range(per)]
+    its = args.warmup + args.steps + 10
+    t_setup = time.perf_counter()
+    b = Batch()
+    members = []
+    for sz, p in pts:
+        s = Solver.from_case(cases.rotor3d(*sz, n_iterations=its, outlet_pressure=p), {"GPU_PRECISION": "mixed"})
+        b.admit(s)
+        members.append(s)
+    t_setup = time.perf_counter() - t_setup
+    nodes = sum(s.N for s in members)
+    b.advance(args.warmup)
+    t0 = time.perf_counter()
+    b.advance(args.steps)
+    wall = time.perf_counter() - t0
+    stopped = sum(1 for s in members if s.status()[2])
+    tiling = sorted(set(s.tuning().split("->")[-1].strip() for s in members))
+    b.close()
+    for s in members:
+        s.close()
+    # plain path: one case of each size alone in FP64 (the same work per case of a size)
+    seq = 0.0
+    for sz in sizes:
+        s = Solver.from_case(cases.rotor3d(*sz, n_iterations=its, outlet_pressure=95000.0), {"GPU_PRECISION": "fp64"})
+        s.advance(args.warmup)
+        s.sync()
+        t = time.perf_counter()
+        s.advance(args.steps)
+        s.sync()
+        seq += (time.perf_counter() - t) * per
+        s.close()
+    value = nodes * 4 * args.steps / wall
+    return {
+        "metric": "batched cell-stage updates/s (FP32 flux / FP64 state) vs plain FP64 one case at a time",
+        "value": value, "unit": "cell-stage updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64-state/f32-flux", "data": "synthetic rotor grids and blade fields, free-stream start",
+        "config": {"workload": "throttle sweep: %d rotor operating points, %d per size on %s, outlet 90-110 kPa "
+                               "(seed %d), JST, RK41A, one batch" % (len(pts), per,
+                                                                    ", ".join("%dx%dx%d" % z for z in sizes), args.seed),
+                   "nodes": nodes, "cases": len(pts), "stopped_cases": stopped, "stage_tiling": tiling,
+                   "setup_s": t_setup},
+        "plain_fp64_sequential_s": seq, "batched_s": wall, "speedup_vs_plain": seq / wall,
+    }
+
+
+def bench_serve(args):
+    """Served solves: a seeded Poisson stream (mean 2 cases/s) of 2-D bump and 3-D rotor
+    cases, admitted by device-memory footprint into one running batch (FP32 flux / FP64
+    state) and retired on a density-residual drop; against the plain FP64 path serving
+    the same stream one case at a time (its measured service times in a FIFO queue)."""
+    from cturbobfm_amd import serve
+    specs = serve.mixed_stream(args.serve_cases, seed=args.seed, precision="mixed")
+    arrivals = serve.poisson_arrivals(len(specs), rate=args.rate, seed=args.seed)
+    srv = serve.Server(drop=args.drop, max_iter=args.max_iter, chunk=100)
+    t0 = time.perf_counter()
+    res = srv.run(specs, arrivals)
+    wall = time.perf_counter() - t0
+    served = serve.summary(res, wall)
+    alone = serve.sequential(specs, drop=args.drop, max_iter=args.max_iter, precision="fp64")
+    # the plain path serving the same arrivals: one case at a time, first come first served
+    order = np.argsort(arrivals)
+    free = 0.0
+    ttc = np.zeros(len(specs))
+    for q in order:
+        service = alone[q].finished - alone[q].arrival
+        start = max(free, arrivals[q])
+        free = start + service
+        ttc[q] = free - arrivals[q]
+    plain_wall = free
+    plain = {"p50_s": float(np.percentile(ttc, 50)), "p99_s": float(np.percentile(ttc, 99)),
+             "cases_per_hour": len(specs) / plain_wall * 3600.0, "wall_s": plain_wall,
+             "converged": int(sum(r.converged for r in alone)), "iterations": int(sum(r.iterations for r in alone))}
+    return {
+        "metric": "served cases/hour; p50/p99 time-to-converge vs plain FP64",
+        "value": served["cases_per_hour"], "unit": "cases/hour", "n_gpus": 1, "steps": len(specs), "warmup": 0,
+        "ms_per_step": served["wall_s"] / len(specs) * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64-state/f32-flux",
+        "data": "synthetic bump and rotor cases (cturbobfm_amd/serve.py mixed_stream, seed %d)" % args.seed,
+        "config": {"workload": "%d cases, Poisson arrivals %.1f/s, retire at %.1f orders of density drop "
+                               "(cap %d iterations)" % (len(specs), args.rate, args.drop, args.max_iter)},
+        "served": served, "plain_fp64": plain,
+        "cases_per_hour_ratio": served["cases_per_hour"] / plain["cases_per_hour"],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--mode", choices=["stage", "batch", "serve"], default="stage",
+                    help="stage: the default single-case line; batch: throttle sweep; serve: served stream")
+    ap.add_argument("--seed", type=int, default=11)
+    ap.add_argument("--batch-cases", type=int, default=64)
+    ap.add_argument("--serve-cases", type=int, default=40)
+    ap.add_argument("--rate", type=float, default=2.0)
+    ap.add_argument("--drop", type=float, default=3.0)
+    ap.add_argument("--max-iter", type=int, default=20000)
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--size", default="193x49x129")
@@ -87,6 +190,13 @@ def main():
         sys.exit(2)
 
     from cturbobfm_amd import Solver, cases, copy_calibration
+
+    if args.mode == "batch":
+        print(json.dumps(bench_batch(args)), flush=True)
+        return
+    if args.mode == "serve":
+        print(json.dumps(bench_serve(args)), flush=True)
+        return
 
     if args.copy_calibration:
         ms = copy_calibration(args.copy_calibration, 20)

@@ -46,7 +46,7 @@ EXPORTS = ["tbfm_gpu_case_create", "tbfm_gpu_case_create_from_ini", "tbfm_gpu_ca
            "tbfm_gpu_tuning", "tbfm_gpu_case_device_bytes", "tbfm_gpu_case_set_convergence",
            "tbfm_gpu_case_status", "tbfm_gpu_batch_create", "tbfm_gpu_batch_admit", "tbfm_gpu_batch_retire",
            "tbfm_gpu_batch_size", "tbfm_gpu_batch_advance", "tbfm_gpu_batch_destroy",
-           "tbfm_gpu_batch_last_error"]
+           "tbfm_gpu_batch_last_error", "tbfm_gpu_device_memory"]
 
 _lib = None
 
@@ -95,6 +95,7 @@ def lib():
         L.tbfm_gpu_batch_destroy.argtypes = [vp]
         L.tbfm_gpu_batch_last_error.restype = C.c_char_p
         L.tbfm_gpu_batch_last_error.argtypes = [vp]
+        L.tbfm_gpu_device_memory.argtypes = [C.POINTER(C.c_long), C.POINTER(C.c_long)]
         _lib = L
     return _lib
 
@@ -110,6 +111,15 @@ def copy_calibration(nbytes: int, repeats: int = 10) -> float:
     if rc != TBFM_OK:
         raise SolverError(lib().tbfm_gpu_last_error(None).decode())
     return ms.value
+
+
+def device_memory():
+    """(free, total) bytes of device memory."""
+    f, t = C.c_long(), C.c_long()
+    rc = lib().tbfm_gpu_device_memory(C.byref(f), C.byref(t))
+    if rc != TBFM_OK:
+        raise SolverError(lib().tbfm_gpu_last_error(None).decode())
+    return f.value, t.value
 
 
 class Solver:

@@ -1396,6 +1396,17 @@ int tbfm_gpu_sync(tbfm_case* c) {
 
 
 // ------------------------------------------------------------------ batches
+int tbfm_gpu_device_memory(long* free_bytes, long* total_bytes) {
+    if (!free_bytes || !total_bytes) return fail(nullptr, TBFM_ERR_ARGUMENT, "null argument");
+    return guarded((std::string*)nullptr, [&]() {
+        size_t f = 0, t = 0;
+        HIPCHK(hipMemGetInfo(&f, &t));
+        *free_bytes = (long)f;
+        *total_bytes = (long)t;
+        return TBFM_OK;
+    });
+}
+
 int tbfm_gpu_case_device_bytes(const tbfm_case* c, long* bytes) {
     if (!c || !bytes) return fail(nullptr, TBFM_ERR_ARGUMENT, "null argument");
     *bytes = (long)c->dev_bytes;

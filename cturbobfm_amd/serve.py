@@ -1,0 +1,212 @@
+"""Solve-serving loop: steady cases arrive as a stream, are admitted to one device batch
+as memory frees up, advance together (one launch per kernel over all resident cases,
+replayed as a captured HIP graph), and retire when their density residual has dropped
+the requested number of orders below the baseline of iterations 1-5.
+
+The reference runs one case per process (apps/turbobfm.cpp; Solver::solve,
+src/solver.cpp:1298-1424) and has no serving layer; what is reused from it is the
+convergence rule (Solver::checkConvergence, solver.cpp:2985-3015) restricted to the
+density equation, evaluated on the device for every member.
+
+    specs = mixed_stream(40, seed=7)
+    arrivals = poisson_arrivals(len(specs), rate=2.0, seed=7)
+    report = Server(drop=3.0).run(specs, arrivals)
+
+Every case computes exactly the same numbers as it would alone (each keeps its own
+state, counters and reductions; tests/test_gpu_batch.py and tests/test_gpu_serve.py).
+"""
+from __future__ import annotations
+
+import dataclasses
+import time
+from collections import deque
+from typing import Callable, Dict, List, Optional
+
+import numpy as np
+
+from . import Batch, Solver, SolverError, cases, device_memory
+
+
+@dataclasses.dataclass
+class CaseSpec:
+    """One request: a synthetic case builder and its input.ini overrides."""
+    name: str
+    build: Callable[[], Dict]
+    overrides: Dict[str, str]
+    nodes: int
+    three_d: bool
+
+
+@dataclasses.dataclass
+class CaseResult:
+    name: str
+    nodes: int
+    arrival: float
+    admitted: float = -1.0
+    finished: float = -1.0
+    iterations: int = 0
+    converged: bool = False
+    diverged: bool = False
+    state: Optional[np.ndarray] = None
+
+    @property
+    def time_to_converge(self) -> float:
+        return self.finished - self.arrival
+
+
+def rotor_spec(ni: int, nj: int, nk: int, outlet_pressure: float, precision: str, scheme: str = "JST",
+               max_iter: int = 20000) -> CaseSpec:
+    def build():
+        return cases.rotor3d(ni, nj, nk, n_iterations=max_iter, scheme=scheme, outlet_pressure=outlet_pressure)
+    # the radial-equilibrium outlet takes its hub pressure from INIT_PRESSURE, which
+    # cases.rotor3d sets to the outlet pressure
+    return CaseSpec("rotor %dx%dx%d p_out=%.0f %s" % (ni, nj, nk, outlet_pressure, precision), build,
+                    {"GPU_PRECISION": precision}, ni * nj * nk, True)
+
+
+def bump_spec(ni: int, nj: int, mach_out: float, precision: str, max_iter: int = 20000) -> CaseSpec:
+    def build():
+        return cases.bump2d(ni, nj, n_iterations=max_iter, mach_out=mach_out)
+    return CaseSpec("bump %dx%d M_out=%.3f %s" % (ni, nj, mach_out, precision), build,
+                    {"GPU_PRECISION": precision}, ni * nj, False)
+
+
+def mixed_stream(n: int, seed: int = 7, precision: str = "mixed") -> List[CaseSpec]:
+    """A seeded mix of 2-D bump and 3-D rotor requests: half bumps (65x33 or 129x65,
+    outlet Mach 0.45-0.55), the rest rotors weighted toward the small sizes (97x25x65,
+    129x33x97, rarely 193x49x129) with outlet pressure drawn from 92-98 kPa."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        u = rng.random()
+        if u < 0.5:
+            ni, nj = (65, 33) if rng.random() < 0.6 else (129, 65)
+            out.append(bump_spec(ni, nj, float(rng.uniform(0.45, 0.55)), precision))
+        else:
+            v = rng.random()
+            size = (97, 25, 65) if v < 0.65 else ((129, 33, 97) if v < 0.95 else (193, 49, 129))
+            out.append(rotor_spec(*size, outlet_pressure=float(rng.uniform(92000.0, 98000.0)), precision=precision))
+    return out
+
+
+def poisson_arrivals(n: int, rate: float, seed: int = 7) -> np.ndarray:
+    """Arrival times (s) of a Poisson stream of the given mean rate."""
+    rng = np.random.default_rng(seed + 1)
+    return np.cumsum(rng.exponential(1.0 / rate, size=n))
+
+
+class Server:
+    """Continuous batching of steady solves on one GPU.
+
+    drop        orders of density-residual drop that retire a case
+    max_iter    iteration cap: a case that has not converged by then retires unconverged
+    chunk       iterations per batch advance between admission / retirement decisions
+    budget      device bytes the resident cases may hold (default: 80 % of free memory)
+    """
+
+    def __init__(self, drop: float = 8.0, max_iter: int = 20000, chunk: int = 100,
+                 budget: Optional[int] = None, keep_states: bool = False):
+        self.drop = drop
+        self.max_iter = max_iter
+        self.chunk = chunk
+        self.budget = budget if budget is not None else int(0.8 * device_memory()[0])
+        self.keep_states = keep_states
+        self.bytes_per_node = {True: 1500.0, False: 1500.0}   # refined from the first cases
+
+    def _estimate(self, spec: CaseSpec) -> int:
+        return int(self.bytes_per_node[spec.three_d] * spec.nodes) + (1 << 20)
+
+    def run(self, specs: List[CaseSpec], arrivals: np.ndarray) -> List[CaseResult]:
+        results = [CaseResult(s.name, s.nodes, float(a)) for s, a in zip(specs, arrivals)]
+        order = list(np.argsort(arrivals))
+        batch = Batch()
+        resident: Dict[int, Solver] = {}
+        footprint: Dict[int, int] = {}
+        queue: deque = deque()
+        used = 0
+        nxt = 0
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0
+        done = 0
+        while done < len(specs):
+            while nxt < len(order) and arrivals[order[nxt]] <= now():
+                queue.append(order[nxt])
+                nxt += 1
+            # admission, first come first served, by device-memory footprint
+            while queue and (not resident or used + self._estimate(specs[queue[0]]) <= self.budget):
+                q = queue.popleft()
+                sp = specs[q]
+                s = Solver.from_case(sp.build(), sp.overrides)
+                s.set_convergence(True, self.drop)
+                b = s.device_bytes()
+                self.bytes_per_node[sp.three_d] = b / sp.nodes
+                batch.admit(s)
+                resident[q] = s
+                footprint[q] = b
+                used += b
+                results[q].admitted = now()
+            if not resident:
+                if nxt < len(order):
+                    time.sleep(max(0.0, arrivals[order[nxt]] - now()))
+                continue
+            batch.advance(self.chunk)
+            for q in list(resident):
+                s = resident[q]
+                r = results[q]
+                try:
+                    it, conv, stopped, _ = s.status()
+                except SolverError:
+                    it, conv, stopped = s.iterations, 0, True
+                    r.diverged = True
+                if stopped or it >= self.max_iter:
+                    r.finished = now()
+                    r.iterations = it
+                    r.converged = conv > 0
+                    if self.keep_states:
+                        r.state = s.state()
+                    batch.retire(s)
+                    s.close()
+                    used -= footprint.pop(q)
+                    del resident[q]
+                    done += 1
+        batch.close()
+        return results
+
+
+def sequential(specs: List[CaseSpec], drop: float, max_iter: int = 20000, chunk: int = 500,
+               precision: Optional[str] = None, keep_states: bool = False) -> List[CaseResult]:
+    """The plain path: each case alone, one after another, from set-up to convergence
+    (precision overrides the spec's GPU_PRECISION when given)."""
+    out = []
+    t0 = time.perf_counter()
+    for sp in specs:
+        r = CaseResult(sp.name, sp.nodes, time.perf_counter() - t0)
+        ov = dict(sp.overrides)
+        if precision is not None:
+            ov["GPU_PRECISION"] = precision
+        s = Solver.from_case(sp.build(), ov)
+        s.set_convergence(True, drop)
+        r.admitted = time.perf_counter() - t0
+        try:
+            while s.iterations < max_iter and s.converged_at == 0:
+                s.advance(min(chunk, max_iter - s.iterations))
+        except SolverError:
+            r.diverged = True
+        r.iterations = s.iterations
+        r.converged = s.converged_at > 0
+        if keep_states:
+            r.state = s.state()
+        s.close()
+        r.finished = time.perf_counter() - t0
+        out.append(r)
+    return out
+
+
+def summary(results: List[CaseResult], wall: float) -> Dict:
+    ttc = np.array([r.time_to_converge for r in results])
+    return {"cases": len(results), "converged": int(sum(r.converged for r in results)),
+            "diverged": int(sum(r.diverged for r in results)),
+            "p50_s": float(np.percentile(ttc, 50)), "p99_s": float(np.percentile(ttc, 99)),
+            "cases_per_hour": len(results) / wall * 3600.0, "wall_s": wall,
+            "iterations": int(sum(r.iterations for r in results)),
+            "cell_iterations": float(sum(r.iterations * r.nodes for r in results))}

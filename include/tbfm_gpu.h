@@ -148,6 +148,10 @@ int tbfm_gpu_sync(tbfm_case* c);
  * numbers alone and in any mix.  Cases keep ownership: retire or destroy them freely
  * between advances; a member is advanced only through its batch. */
 
+/* Free and total device memory of the current GPU (hipMemGetInfo): the budget a
+ * serving loop admits cases against. */
+int tbfm_gpu_device_memory(long* free_bytes, long* total_bytes);
+
 /* Device memory a case holds (state, geometry, boundary and reduction buffers): the
  * admission footprint of a serving loop. */
 int tbfm_gpu_case_device_bytes(const tbfm_case* c, long* bytes);
