@@ -249,6 +249,38 @@ struct KQueue<1> {
     }
 };
 
+// Face fq of a tile: I faces first (row r, face m at i-index i0 + m), then J faces
+// (face row r at j-index j0 + r, column m); bnd marks a domain-boundary face (lo: the
+// start side) whose flux comes from the boundary buffer.
+template <class TL>
+struct FaceIdx {
+    bool isI, valid, bnd, lo;
+    int r, m, gi, gj;
+    __device__ __forceinline__ FaceIdx(int fq, int i0, int j0, int NI, int NJ) {
+        constexpr int TI = TL::TI, NFI = TL::TJ * (TL::TI + 1);
+        isI = fq < NFI;
+        const int qq = isI ? fq : fq - NFI;
+        r = isI ? qq / (TI + 1) : qq / TI;
+        m = isI ? qq % (TI + 1) : qq % TI;
+        gi = i0 + m;
+        gj = j0 + r;
+        valid = isI ? (gj < NJ && gi <= NI) : (gi < NI && gj <= NJ);
+        const int gf = isI ? gi : gj, nf = isI ? NI : NJ;
+        lo = gf == 0;
+        bnd = lo || gf == nf;
+    }
+};
+
+// Unit normal and area of an interior I or J face on plane k.
+template <class TL>
+__device__ __forceinline__ void face_normal(const CaseDev& c, const FaceIdx<TL>& f, int k, double nn[4]) {
+    const int NI = c.ni, NJ = c.nj;
+    const int fst = f.isI ? c.nfI : c.nfJ;
+    const int fo = f.isI ? (k * NJ + f.gj) * (NI + 1) + f.gi : c.noffJ + (k * (NJ + 1) + f.gj) * NI + f.gi;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) nn[v] = c.nrm[fo + v * fst];
+}
+
 // LDS position of halo slot h (0 .. NP-NT-1): two full rows above, two below, and the two
 // columns on each side of the tile rows.  Consecutive lanes take consecutive slots so the
 // halo work occupies as few wavefronts as possible.
@@ -290,6 +322,7 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
     const int N = (int)c.N;
     const int NIJ = NI * NJ;
     const int own = (tj + 2) * HI + (ti + 2);
+    const int nfaces = NFI + (doJ ? NFJ : 0);
     const bool bladecol = THREED && c.bfm_active && act && c.blade2d[j * NI + i];
     const bool srcOn = (c.topo == TOPO_AXI) || c.bfm_active;
     const double* __restrict__ Uin = st.Uin;
@@ -379,34 +412,27 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
         }
         for (int h = tid + NT; h < NP - NT; h += NT) fill_position<TL, SCH>(c, Uin, L, halo_pos<TL>(h), i0, j0, k);
         __syncthreads();
-        // ---- every I and J face of the tile once: q < NFI are I faces (row r, face m),
+        // ---- every I and J face of the tile once: fq < NFI are I faces (row r, face m),
         // then J faces (row r of faces, column m); one code path, no divergence
-        for (int fq = tid; fq < NFI + (doJ ? NFJ : 0); fq += NT) {
-            const bool isI = fq < NFI;
-            const int qq = isI ? fq : fq - NFI;
-            const int r = isI ? qq / (TI + 1) : qq / TI;
-            const int m = isI ? qq % (TI + 1) : qq % TI;
-            const int gi = i0 + m, gj = j0 + r;          // I: face i-index, row; J: column, face j-index
-            const bool valid = isI ? (gj < NJ && gi <= NI) : (gi < NI && gj <= NJ);
-            if (!valid) continue;
-            const int gf = isI ? gi : gj, nf = isI ? NI : NJ;
+        for (int fq = tid; fq < nfaces; fq += NT) {
+            const FaceIdx<TL> f(fq, i0, j0, NI, NJ);
+            if (!f.valid) continue;
             double F[5];
-            if (gf == 0 || gf == nf) {
-                const int nsz = isI ? NJ * NK : NI * NK;
-                const int bq = isI ? k * NJ + gj : k * NI + gi;
-                const int off = (isI ? 0 : c.boffJ) + ((gf == 0) ? 0 : 5 * nsz) + bq;
+            if (f.bnd) {
+                const int nsz = f.isI ? NJ * NK : NI * NK;
+                const int bq = f.isI ? k * NJ + f.gj : k * NI + f.gi;
+                const int off = (f.isI ? 0 : c.boffJ) + (f.lo ? 0 : 5 * nsz) + bq;
 #pragma unroll
                 for (int v = 0; v < 5; ++v) F[v] = c.bnd[off + v * nsz];
             } else {
-                const int fst = isI ? c.nfI : c.nfJ;
-                const int fo = isI ? (k * NJ + gj) * (NI + 1) + gi : c.noffJ + (k * (NJ + 1) + gj) * NI + gi;
-                const int l = isI ? (r + 2) * HI + (m + 1) : (r + 1) * HI + (m + 2);
-                lds_face<TL, SCH, FT>(c, L, l, isI ? 1 : HI, c.nrm[fo], c.nrm[fo + fst], c.nrm[fo + 2 * fst],
-                                      c.nrm[fo + 3 * fst], F);
+                double nn[4];
+                face_normal(c, f, k, nn);
+                const int l = f.isI ? (f.r + 2) * HI + (f.m + 1) : (f.r + 1) * HI + (f.m + 2);
+                lds_face<TL, SCH, FT>(c, L, l, f.isI ? 1 : HI, nn[0], nn[1], nn[2], nn[3], F);
             }
-            double* out = isI ? L.f.FI : L.f.FJ;
-            const int base = isI ? r * (TI + 1) + m : r * TI + m;
-            const int vstride = isI ? NFI : NFJ;
+            double* out = f.isI ? L.f.FI : L.f.FJ;
+            const int base = f.isI ? f.r * (TI + 1) + f.m : f.r * TI + f.m;
+            const int vstride = f.isI ? NFI : NFJ;
 #pragma unroll
             for (int v = 0; v < 5; ++v) out[v * vstride + base] = F[v];
         }
@@ -441,7 +467,16 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
                 }
                 R[v] = r;
             }
-            if (srcOn && i >= c.bi0 && i < c.bi1) node_sources<THREED>(cp, Uin, st.last, i, j, k, n, Wc, R);
+            if (srcOn && i >= c.bi0 && i < c.bi1) {
+                // geometric, Gong, body force, subtracted in that order
+                const int sl = (k * NJ + j) * (c.bi1 - c.bi0) + (i - c.bi0);
+                const double* sb = c.srcbuf + sl;
+                const int ns = c.ns;
+#pragma unroll
+                for (int t = 0; t < 3; ++t)
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) R[v] -= sb[(5 * t + v) * ns];
+            }
             if (parked) {
                 double* rp = c.rper + (k == 0 ? 0 : 5) * NIJ;
 #pragma unroll

@@ -81,7 +81,7 @@ struct Plan {
     CaseDev* dcases = nullptr;        // device case table
     size_t dcap = 0;
     bool ownsCases = false;
-    Table dt, mn, prof, bnd, per, norm;
+    Table dt, mn, prof, bnd, per, norm, src2, src3;
     std::vector<StageGroup> groups;
     int nstages = 0;
     hipGraphExec_t graph = nullptr;
@@ -454,6 +454,11 @@ void setup(tbfm_case* cs, const GridInput& grid) {
                 }
     if (bi1 <= bi0) { bi0 = 0; bi1 = 0; }
     c.bi0 = (int)bi0; c.bi1 = (int)bi1;
+    // the stage kernel takes the source terms of the slab from srcbuf (geometric sources
+    // exist for axisymmetric cases and 3-D blade rows, body forces with BFM_ACTIVE)
+    const bool srcOn = (cs->topo == Topology::Axisymmetric) || bfm;
+    c.ns = srcOn ? (int)((bi1 - bi0) * NJ * NK) : 0;
+    c.srcbuf = c.ns > 0 ? dalloc<double>(15L * c.ns, own) : nullptr;
     const long SW = bi1 - bi0, NS = std::max<long>(SW * NJ * NK, 1);
     std::vector<unsigned char> sflag(NS, 0);
     std::vector<double> s_cs(NS, 1.0), s_sn(NS, 0.0), s_r(NS, 0.0), s_om(NS, 0.0), s_b(NS, 1.0), s_gbx(NS, 0.0),
@@ -646,7 +651,7 @@ void freePlan(Plan& P) {
     if (P.stream) hipStreamSynchronize(P.stream);
     if (P.graph) hipGraphExecDestroy(P.graph);
     P.graph = nullptr;
-    for (Table* t : {&P.dt, &P.mn, &P.prof, &P.bnd, &P.per, &P.norm}) freeTable(*t);
+    for (Table* t : {&P.dt, &P.mn, &P.prof, &P.bnd, &P.per, &P.norm, &P.src2, &P.src3}) freeTable(*t);
     for (auto& g : P.groups) freeTable(g.t);
     P.groups.clear();
     if (P.ownsCases && P.dcases) hipFree(P.dcases);
@@ -684,7 +689,7 @@ void pushCase(tbfm_case* cs) {
 
 // Builds the work tables of every kernel from the members' current tiling.
 void buildTables(Plan& P) {
-    std::vector<dev::Work> dt, mn, prof, bnd, per, norm;
+    std::vector<dev::Work> dt, mn, prof, bnd, per, norm, src2, src3;
     for (auto& g : P.groups) freeTable(g.t);
     P.groups.clear();
     std::vector<std::vector<dev::Work>> gw;
@@ -700,6 +705,7 @@ void buildTables(Plan& P) {
                         ((cs->topo == Topology::ThreeD) ? 2 * cs->ni * cs->nj : 0);
         for (long b = 0; b < (nb + 255) / 256; ++b) bnd.push_back({m, (int)b, 0, 0});
         for (int b = 0; b < c.nparts_per; ++b) per.push_back({m, b, 0, 0});
+        for (int b = 0; b < (c.ns + 255) / 256; ++b) (cs->topo == Topology::ThreeD ? src3 : src2).push_back({m, b, 0, 0});
         norm.push_back({m, 0, 0, 0});
         const int sch = (cs->scheme == Scheme::JST) ? 0 : 1;
         const bool three = (cs->topo == Topology::ThreeD), mixed = (cs->precision == Precision::Mixed);
@@ -725,6 +731,8 @@ void buildTables(Plan& P) {
     uploadTable(P.bnd, bnd);
     uploadTable(P.per, per);
     uploadTable(P.norm, norm);
+    uploadTable(P.src2, src2);
+    uploadTable(P.src3, src3);
     for (size_t g = 0; g < P.groups.size(); ++g) uploadTable(P.groups[g].t, gw[g]);
     P.tablesValid = true;
     P.graphValid = false;
@@ -796,6 +804,13 @@ void launchStage(Plan& P, int s, int in, int out, bool last, bool rmode, tbfm_ca
         }
         HIPCHK(hipEventRecord(prof->pool[2 * prof->pool_used], P.stream));
     }
+    // source terms of the slab, then the fused stage kernel (both timed as the stage)
+    if (P.src3.n)
+        hipLaunchKernelGGL(dev::k_sources<true>, dim3((unsigned)P.src3.n), dim3(256), 0, P.stream,
+                           launchArg(P, P.src3, in, out, s, last ? 1 : 0, rmode ? 1 : 0));
+    if (P.src2.n)
+        hipLaunchKernelGGL(dev::k_sources<false>, dim3((unsigned)P.src2.n), dim3(256), 0, P.stream,
+                           launchArg(P, P.src2, in, out, s, last ? 1 : 0, rmode ? 1 : 0));
     launchStageGroups(P, in, out, s, last ? 1 : 0, rmode ? 1 : 0);
     if (prof) {
         HIPCHK(hipEventRecord(prof->pool[2 * prof->pool_used + 1], P.stream));

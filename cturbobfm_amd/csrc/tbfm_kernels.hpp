@@ -86,8 +86,10 @@ struct CaseDev {
     const double* prof_r;              // [MAX_PROF][NJ]
     const double* prof_cs;             // [MAX_PROF][NJ] cos(theta) at (i, j, 0)
     const double* prof_sn;
-    // body force slab: nodes with i in [bi0, bi1)
-    int bi0, bi1;
+    // body force slab: nodes with i in [bi0, bi1); ns = slab nodes; srcbuf holds the
+    // geometric, Gong and body-force source vectors of the current stage, [3][5][ns]
+    int bi0, bi1, ns;
+    double* srcbuf;
     const unsigned char* blade2d;      // (j*NI + i): bladePresent(i,j,0) > 0
     const unsigned char* sflag;        // bit0 present>0, bit1 |grad b|>1e-10, bit2 numberBlades != 0
     const double *s_cs, *s_sn, *s_r, *s_om, *s_b, *s_gbx, *s_gby, *s_gbz;

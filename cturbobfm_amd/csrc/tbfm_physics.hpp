@@ -343,7 +343,8 @@ __device__ __forceinline__ V3 cart_of(V3 v, double cs, double sn) { return v3(v.
 
 // Blockage + Hall / lift-drag source of one node (source_bfm_base.cpp:25-199,
 // source_bfm_hall.cpp:3-34, source_bfm_lift_drag.cpp:3-88).  s = slab index.
-__device__ inline void bfm_source(const CaseDev& c, long s, const double W[5], double V, double S[5],
+template <typename CT>
+__device__ inline void bfm_source(const CT& c, long s, const double W[5], double V, double S[5],
                                   double fi[3], double fv[3], double& dev) {
     double blk[5] = {0, 0, 0, 0, 0};
     fi[0] = fi[1] = fi[2] = 0.0;

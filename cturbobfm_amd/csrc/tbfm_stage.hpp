@@ -7,7 +7,8 @@ namespace tbfm {
 namespace dev {
 
 // Node and state offsets are 32-bit: the library rejects grids with 5*N >= 2^31.
-__device__ __forceinline__ int nid(const CaseDev& c, int i, int j, int k) {
+template <typename CT>
+__device__ __forceinline__ int nid(const CT& c, int i, int j, int k) {
     return (k * c.nj + j) * c.ni + i;
 }
 
@@ -53,17 +54,19 @@ __device__ inline void load_k(const CaseDev& c, const double* U, int i, int j, i
 
 // ------------------------------------------------------------------ node sources
 // Geometric (3-D blade rows: solver.cpp:2855-2867; axisymmetric: :2833-2852), Gong
-// (:2870-2876, :3074-3145) and body-force (:2878-2893) sources of one node, subtracted
-// from R in that order.  Out of line and reading the case from its device-resident
-// copy: only the body-force slab runs it, and keeping it out of the fused kernel's
-// register allocation doubles that kernel's occupancy.
+// (:2870-2876, :3074-3145) and body-force (:2878-2893) sources of one node.  The stage
+// kernel subtracts the three vectors from R in that order; an absent term is +0, and
+// x - (+0) == x for every x, so the order and the values are the reference's.
+// Also writes the body-force side outputs (inviscid / viscous force, deviation angle)
+// on the last stage.
 template <bool THREED>
-__device__ __noinline__ void node_sources(const CaseDev* __restrict__ cd, const double* __restrict__ Uin, int last,
-                                          int i, int j, int k, long n, const double* Wc, double* R) {
-    const CaseDev& c = *cd;
+__device__ inline void source_terms(const CaseDev& c, const double* __restrict__ Uin, int last, int i, int j,
+                                    int k, int n, long s, const double* Wc, double g[5], double gong[5],
+                                    double bf[5]) {
     const int NI = c.ni, NJ = c.nj, NK = c.nk;
-    const long N = c.N;
-    const long s = ((long)k * NJ + j) * (c.bi1 - c.bi0) + (i - c.bi0);
+    const int N = (int)c.N;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) { g[v] = 0.0; gong[v] = 0.0; bf[v] = 0.0; }
     const unsigned char fl = c.sflag[s];
     const double V = c.vol[n];
     const bool present = (fl & 1) != 0;
@@ -74,7 +77,6 @@ __device__ __noinline__ void node_sources(const CaseDev* __restrict__ cd, const 
         const double cs = c.s_cs[s], sn = c.s_sn[s];
         V3 vcy = cyl_of(v3(Wc[1], Wc[2], Wc[3]), cs, sn);
         V3 scy;
-        double g[5];
         if (c.topo == TOPO_AXI) {
             const double gf = c.axi_gf[(long)j * NI + i];
             scy = v3(0.0, (+Wc[0] * vcy.z * vcy.z + p) * gf, -Wc[0] * vcy.y * vcy.z * gf);
@@ -86,8 +88,6 @@ __device__ __noinline__ void node_sources(const CaseDev* __restrict__ cd, const 
             V3 sca = cart_of(scy, cs, sn);
             g[0] = 0.0 * V; g[1] = sca.x * V; g[2] = sca.y * V; g[3] = sca.z * V; g[4] = 0.0 * V;
         }
-#pragma unroll
-        for (int v = 0; v < 5; ++v) R[v] -= g[v];
     }
     if (THREED && present) {
         const double om = c.s_om[s];
@@ -110,20 +110,48 @@ __device__ __noinline__ void node_sources(const CaseDev* __restrict__ cd, const 
                 double d;
                 if (om > 0.0) d = (U0g[v] * 3.0 - U1g[v] * 4.0 + U2g[v]) / (2 * c.dtheta);
                 else d = (U0g[v] * (-3.0) + U1g[v] * 4.0 - U2g[v]) / (2 * c.dtheta);
-                R[v] -= (-d * om) * V;
+                gong[v] = (-d * om) * V;
             }
         }
     }
     if (c.bfm_active && (fl & 2)) {
-        double S[5], fi[3], fv[3], dv;
-        bfm_source(c, s, Wc, V, S, fi, fv, dv);
-#pragma unroll
-        for (int v = 0; v < 5; ++v) R[v] -= S[v];
+        double fi[3], fv[3], dv;
+        bfm_source(c, s, Wc, V, bf, fi, fv, dv);
         if (last && (fl & 4)) {
             c.finv[3 * n] = fi[0]; c.finv[3 * n + 1] = fi[1]; c.finv[3 * n + 2] = fi[2];
             c.fvisc[3 * n] = fv[0]; c.fvisc[3 * n + 1] = fv[1]; c.fvisc[3 * n + 2] = fv[2];
             c.dev[n] = dv;
         }
+    }
+}
+
+// Source terms of every node of the source slab (i in [bi0, bi1)) from the stage input,
+// stored as [geometric | Gong | body force][5][ns] for the stage kernel.  One thread per
+// slab node, run before the stage kernel: the transcendental-heavy blade-row work runs
+// at full occupancy and without diverging the stencil kernel's waves.  Work: one entry
+// per 256 slab nodes of a case with sources.
+template <bool THREED>
+__global__ void __launch_bounds__(256) k_sources(Launch La) {
+    const Work w = La.work[blockIdx.x];
+    const CaseDev& c = La.cases[w.cs];
+    if (*c.stop) return;
+    const int s = w.x * 256 + threadIdx.x;
+    if (s >= c.ns) return;
+    const int SW = c.bi1 - c.bi0, NJ = c.nj;
+    const int i = c.bi0 + s % SW, j = (s / SW) % NJ, k = s / (SW * NJ);
+    const int n = nid(c, i, j, k);
+    const double* Uin = c.U[La.in];
+    double U[5], W[5], p, rf;
+    load5(Uin, (int)c.N, n, U);
+    derive<double>(U, W, p, rf, c.gamma, c.gm1);
+    double g[5], go[5], bf[5];
+    source_terms<THREED>(c, Uin, La.last, i, j, k, n, s, W, g, go, bf);
+    const int ns = c.ns;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        c.srcbuf[v * ns + s] = g[v];
+        c.srcbuf[(5 + v) * ns + s] = go[v];
+        c.srcbuf[(10 + v) * ns + s] = bf[v];
     }
 }
 
