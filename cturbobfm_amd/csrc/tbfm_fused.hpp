@@ -624,10 +624,14 @@ __global__ void __launch_bounds__(256) k_norms(Launch La) {
     const ConvDev& cv = c.conv;
     const double* partial = c.partial;
     const int nparts = c.nparts_stage + c.nparts_per;
+    // one pass over the partials for all five equations (per equation the same order of
+    // additions as five separate passes)
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int q = threadIdx.x; q < nparts; q += 256)
+#pragma unroll
+        for (int v = 0; v < 5; ++v) acc[v] += partial[(long)q * 5 + v];
     for (int v = 0; v < 5; ++v) {
-        double s = 0.0;
-        for (int q = threadIdx.x; q < nparts; q += 256) s += partial[(long)q * 5 + v];
-        red[threadIdx.x] = s;
+        red[threadIdx.x] = acc[v];
         __syncthreads();
         for (int h = 128; h > 0; h >>= 1) {
             if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];

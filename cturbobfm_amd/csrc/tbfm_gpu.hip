@@ -81,7 +81,7 @@ struct Plan {
     CaseDev* dcases = nullptr;        // device case table
     size_t dcap = 0;
     bool ownsCases = false;
-    Table dt, mn, prof, bnd, per, norm, src2, src3;
+    Table dt, mn, prof, bnd, per, norm, start, pre;
     std::vector<StageGroup> groups;
     int nstages = 0;
     hipGraphExec_t graph = nullptr;
@@ -651,7 +651,7 @@ void freePlan(Plan& P) {
     if (P.stream) hipStreamSynchronize(P.stream);
     if (P.graph) hipGraphExecDestroy(P.graph);
     P.graph = nullptr;
-    for (Table* t : {&P.dt, &P.mn, &P.prof, &P.bnd, &P.per, &P.norm, &P.src2, &P.src3}) freeTable(*t);
+    for (Table* t : {&P.dt, &P.mn, &P.prof, &P.bnd, &P.per, &P.norm, &P.start, &P.pre}) freeTable(*t);
     for (auto& g : P.groups) freeTable(g.t);
     P.groups.clear();
     if (P.ownsCases && P.dcases) hipFree(P.dcases);
@@ -689,7 +689,7 @@ void pushCase(tbfm_case* cs) {
 
 // Builds the work tables of every kernel from the members' current tiling.
 void buildTables(Plan& P) {
-    std::vector<dev::Work> dt, mn, prof, bnd, per, norm, src2, src3;
+    std::vector<dev::Work> dt, mn, prof, bnd, per, norm, src;
     for (auto& g : P.groups) freeTable(g.t);
     P.groups.clear();
     std::vector<std::vector<dev::Work>> gw;
@@ -700,12 +700,12 @@ void buildTables(Plan& P) {
         if (c.nstages != P.nstages) throw std::runtime_error("cases of one batch must use the same RK scheme");
         for (int b = 0; b < c.nblk_dt; ++b) dt.push_back({m, b, 0, 0});
         if (c.dt_mode == 1) mn.push_back({m, 0, 0, 0});
-        if (c.nprof > 0) prof.push_back({m, 0, 0, 0});
+        if (c.nprof > 0) prof.push_back({m, 0, 0, 1});
         const long nb = 2 * cs->nj * cs->nk + ((cs->topo != Topology::OneD) ? 2 * cs->ni * cs->nk : 0) +
                         ((cs->topo == Topology::ThreeD) ? 2 * cs->ni * cs->nj : 0);
         for (long b = 0; b < (nb + 255) / 256; ++b) bnd.push_back({m, (int)b, 0, 0});
         for (int b = 0; b < c.nparts_per; ++b) per.push_back({m, b, 0, 0});
-        for (int b = 0; b < (c.ns + 255) / 256; ++b) (cs->topo == Topology::ThreeD ? src3 : src2).push_back({m, b, 0, 0});
+        for (int b = 0; b < (c.ns + 255) / 256; ++b) src.push_back({m, b, 0, (cs->topo == Topology::ThreeD) ? 1 : 2});
         norm.push_back({m, 0, 0, 0});
         const int sch = (cs->scheme == Scheme::JST) ? 0 : 1;
         const bool three = (cs->topo == Topology::ThreeD), mixed = (cs->precision == Precision::Mixed);
@@ -731,8 +731,12 @@ void buildTables(Plan& P) {
     uploadTable(P.bnd, bnd);
     uploadTable(P.per, per);
     uploadTable(P.norm, norm);
-    uploadTable(P.src2, src2);
-    uploadTable(P.src3, src3);
+    // combined launches: profiles then time-step blocks; boundary then source blocks
+    std::vector<dev::Work> start(prof), pre(bnd);
+    start.insert(start.end(), dt.begin(), dt.end());
+    pre.insert(pre.end(), src.begin(), src.end());
+    uploadTable(P.start, start);
+    uploadTable(P.pre, pre);
     for (size_t g = 0; g < P.groups.size(); ++g) uploadTable(P.groups[g].t, gw[g]);
     P.tablesValid = true;
     P.graphValid = false;
@@ -777,23 +781,25 @@ void launchStageGroups(const Plan& P, int in, int out, int stage, int last, int 
     }
 }
 
-void launchTimestep(const Plan& P, int in) {
-    if (P.dt.n) hipLaunchKernelGGL(dev::k_timestep, dim3((unsigned)P.dt.n), dim3(256), 0, P.stream, launchArg(P, P.dt, in, 0, 0, 0, 0));
-    if (P.mn.n) hipLaunchKernelGGL(dev::k_min_reduce, dim3((unsigned)P.mn.n), dim3(256), 0, P.stream, launchArg(P, P.mn, in, 0, 0, 0, 0));
+// k_iterstart over a table of time-step and / or profile blocks, then the global-step
+// minimum for the cases that use it.
+void launchStart(const Plan& P, const Table& t, int in, bool minimum) {
+    if (t.n) hipLaunchKernelGGL(dev::k_iterstart, dim3((unsigned)t.n), dim3(256), 0, P.stream, launchArg(P, t, in, 0, 0, 0, 0));
+    if (minimum && P.mn.n)
+        hipLaunchKernelGGL(dev::k_min_reduce, dim3((unsigned)P.mn.n), dim3(256), 0, P.stream, launchArg(P, P.mn, in, 0, 0, 0, 0));
 }
+void launchTimestep(const Plan& P, int in) { launchStart(P, P.dt, in, true); }
+void launchProfiles(const Plan& P, int in) { launchStart(P, P.prof, in, false); }
 
-void launchProfiles(const Plan& P, int in) {
-    if (P.prof.n) hipLaunchKernelGGL(dev::k_radial_profile, dim3((unsigned)P.prof.n), dim3(64), 0, P.stream, launchArg(P, P.prof, in, 0, 0, 0, 0));
+// k_prestage over a table of boundary and / or source blocks.
+void launchPre(const Plan& P, const Table& t, int in, int out, int stage, int last, int rmode) {
+    if (t.n) hipLaunchKernelGGL(dev::k_prestage, dim3((unsigned)t.n), dim3(256), 0, P.stream, launchArg(P, t, in, out, stage, last, rmode));
 }
-
-void launchBoundary(const Plan& P, int in) {
-    if (P.bnd.n) hipLaunchKernelGGL(dev::k_boundary, dim3((unsigned)P.bnd.n), dim3(256), 0, P.stream, launchArg(P, P.bnd, in, 0, 0, 0, 0));
-}
+void launchBoundary(const Plan& P, int in) { launchPre(P, P.bnd, in, 0, 0, 0, 0); }
 
 // One RK stage of every member: boundary fluxes, fused residual + update, periodic
 // closure, and after the last stage the norms and convergence test.
 void launchStage(Plan& P, int s, int in, int out, bool last, bool rmode, tbfm_case* prof) {
-    launchBoundary(P, in);
     if (prof) {
         if (prof->pool_used * 2 + 2 > prof->pool.size()) {
             hipEvent_t a, b;
@@ -804,13 +810,8 @@ void launchStage(Plan& P, int s, int in, int out, bool last, bool rmode, tbfm_ca
         }
         HIPCHK(hipEventRecord(prof->pool[2 * prof->pool_used], P.stream));
     }
-    // source terms of the slab, then the fused stage kernel (both timed as the stage)
-    if (P.src3.n)
-        hipLaunchKernelGGL(dev::k_sources<true>, dim3((unsigned)P.src3.n), dim3(256), 0, P.stream,
-                           launchArg(P, P.src3, in, out, s, last ? 1 : 0, rmode ? 1 : 0));
-    if (P.src2.n)
-        hipLaunchKernelGGL(dev::k_sources<false>, dim3((unsigned)P.src2.n), dim3(256), 0, P.stream,
-                           launchArg(P, P.src2, in, out, s, last ? 1 : 0, rmode ? 1 : 0));
+    // boundary fluxes and slab source terms, then the fused stage kernel (timed together)
+    launchPre(P, P.pre, in, out, s, last ? 1 : 0, rmode ? 1 : 0);
     launchStageGroups(P, in, out, s, last ? 1 : 0, rmode ? 1 : 0);
     if (prof) {
         HIPCHK(hipEventRecord(prof->pool[2 * prof->pool_used + 1], P.stream));
@@ -825,8 +826,7 @@ void launchStage(Plan& P, int s, int in, int out, bool last, bool rmode, tbfm_ca
 // One explicit iteration of every member (solver.cpp:1337-1381): time step, radial
 // profiles, then the RK stages A -> B -> C -> ... -> A with the state back in A.
 void launchIteration(Plan& P, tbfm_case* prof) {
-    launchTimestep(P, 0);
-    launchProfiles(P, 0);
+    launchStart(P, P.start, 0, true);
     for (int s = 0; s < P.nstages; ++s)
         launchStage(P, s, dev::stage_in(s), dev::stage_out(s, P.nstages), s + 1 == P.nstages, false, prof);
     HIPCHK(hipGetLastError());

@@ -131,11 +131,8 @@ __device__ inline void source_terms(const CaseDev& c, const double* __restrict__
 // at full occupancy and without diverging the stencil kernel's waves.  Work: one entry
 // per 256 slab nodes of a case with sources.
 template <bool THREED>
-__global__ void __launch_bounds__(256) k_sources(Launch La) {
-    const Work w = La.work[blockIdx.x];
-    const CaseDev& c = La.cases[w.cs];
-    if (*c.stop) return;
-    const int s = w.x * 256 + threadIdx.x;
+__device__ inline void sources_block(const CaseDev& c, const Launch& La, int bx) {
+    const int s = bx * 256 + threadIdx.x;
     if (s >= c.ns) return;
     const int SW = c.bi1 - c.bi0, NJ = c.nj;
     const int i = c.bi0 + s % SW, j = (s / SW) % NJ, k = s / (SW * NJ);
@@ -157,14 +154,10 @@ __global__ void __launch_bounds__(256) k_sources(Launch La) {
 
 // --------------------------------------------------------------------- time step
 // Solver::computeTimestepArray, inviscid (src/solver.cpp:1531-1630)
-// Work: one entry per 256 nodes of a case (x = block index).
-__global__ void __launch_bounds__(256) k_timestep(Launch L) {
+// One block of 256 nodes (bx = block index within the case).
+__device__ inline void timestep_block(const CaseDev& c, const double* U, int bx) {
     __shared__ double smin[256];
-    const Work w = L.work[blockIdx.x];
-    const CaseDev& c = L.cases[w.cs];
-    if (*c.stop) return;
-    const double* U = c.U[L.in];
-    const long n = (long)w.x * 256 + threadIdx.x;
+    const long n = (long)bx * 256 + threadIdx.x;
     double dtv = 1.0e300;
     if (n < c.N) {
         if (c.dt_mode == 2) {
@@ -209,7 +202,7 @@ __global__ void __launch_bounds__(256) k_timestep(Launch L) {
             if (threadIdx.x < s) smin[threadIdx.x] = dmind(smin[threadIdx.x], smin[threadIdx.x + s]);
             __syncthreads();
         }
-        if (threadIdx.x == 0) c.partial_min[w.x] = smin[0];
+        if (threadIdx.x == 0) c.partial_min[bx] = smin[0];
     }
 }
 
@@ -232,43 +225,60 @@ __global__ void __launch_bounds__(256) k_min_reduce(Launch L) {
 
 // ------------------------------------------------------- radial equilibrium profile
 // Solver::updateRadialProfiles + integrateRadialEquilibrium, radial_equilibrium branch
-// (src/solver.cpp:2688-2787, src/math_utils.cpp:141-170).  One thread per profile: the
-// trapezoid scan over the outlet column is sequential in j.  Work: one entry per case
-// with profiles.
-__global__ void k_radial_profile(Launch L) {
+// (src/solver.cpp:2688-2787, src/math_utils.cpp:141-170).  The terms rho*ut*ut/r of the
+// outlet column are evaluated by all threads in parallel; the trapezoid recurrence
+// P[m] = P[m-1] + 0.5*(d[m-1] + d[m])*dr is then run by one thread, in order, exactly
+// as the reference accumulates it.  One block per case with profiles.
+__device__ inline void radial_profile_block(const CaseDev& c, const double* U) {
+    __shared__ double dval[257];
+    const int tid = threadIdx.x;
+    for (int q = 0; q < c.nprof; ++q) {
+        const int n = c.prof_n[q];
+        if (n <= 1) continue;   // single point: profile keeps the hub value
+        const int i = c.prof_i[q];
+        double* P = c.prof_p + (long)q * c.nj;
+        const double* r = c.prof_r + (long)q * c.nj;
+        const double* cs = c.prof_cs + (long)q * c.nj;
+        const double* sn = c.prof_sn + (long)q * c.nj;
+        for (int base = 0; base < n; base += 256) {
+            // dval[0] carries d[base-1] of the previous chunk
+            const int m = base + tid;
+            if (m < n) {
+                const int j = c.prof_j0[q] + m;
+                double Un[5], W[5];
+                load5(U, (int)c.N, nid(c, i, j, 0), Un);
+                W[0] = Un[0];
+                W[1] = Un[1] / Un[0];
+                W[2] = Un[2] / Un[0];
+                W[3] = Un[3] / Un[0];
+                const double ut = fabs(-W[2] * sn[m] + W[3] * cs[m]);
+                const double rho = W[0];
+                dval[tid + 1] = rho * ut * ut / r[m];
+            }
+            __syncthreads();
+            if (tid == 0) {
+                const int top = min(n, base + 256);
+                for (int mm = (base == 0 ? 1 : base); mm < top; ++mm) {
+                    const double dr = r[mm] - r[mm - 1];
+                    P[mm] = P[mm - 1] + 0.5 * (dval[mm - base] + dval[mm - base + 1]) * dr;
+                }
+                dval[0] = dval[top - base];
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Start of an iteration: radial-equilibrium profiles (z = 1, one block per case with
+// profiles) and the time step (z = 0, one block per 256 nodes), both from the state in
+// role `in`; independent of each other, so they share one launch.
+__global__ void __launch_bounds__(256) k_iterstart(Launch L) {
     const Work w = L.work[blockIdx.x];
     const CaseDev& c = L.cases[w.cs];
     if (*c.stop) return;
     const double* U = c.U[L.in];
-    const int q = threadIdx.x;
-    if (q >= c.nprof) return;
-    const int n = c.prof_n[q];
-    const int i = c.prof_i[q];
-    double* P = c.prof_p + (long)q * c.nj;
-    const double* r = c.prof_r + (long)q * c.nj;
-    const double* cs = c.prof_cs + (long)q * c.nj;
-    const double* sn = c.prof_sn + (long)q * c.nj;
-    if (n <= 1) return;   // single point: profile keeps the hub value
-    double rho_prev = 0, ut_prev = 0;
-    for (int m = 0; m < n; ++m) {
-        const int j = c.prof_j0[q] + m;
-        double Un[5], W[5];
-        load5(U, c.N, nid(c, i, j, 0), Un);
-        W[0] = Un[0];
-        W[1] = Un[1] / Un[0];
-        W[2] = Un[2] / Un[0];
-        W[3] = Un[3] / Un[0];
-        double ut = fabs(-W[2] * sn[m] + W[3] * cs[m]);
-        double rho = W[0];
-        if (m > 0) {
-            double dr = r[m] - r[m - 1];
-            double d1 = rho_prev * ut_prev * ut_prev / r[m - 1];
-            double d2 = rho * ut * ut / r[m];
-            P[m] = P[m - 1] + 0.5 * (d1 + d2) * dr;
-        }
-        rho_prev = rho;
-        ut_prev = ut;
-    }
+    if (w.z == 1) radial_profile_block(c, U);
+    else timestep_block(c, U, w.x);
 }
 
 // ------------------------------------------------------------- boundary fluxes
@@ -276,13 +286,9 @@ __global__ void k_radial_profile(Launch L) {
 // outward vector -S and the node at the face; end faces +S and the node before it
 // (solver.cpp:1911-1957).  Result already multiplied by |S|.  Work: one entry per 256
 // boundary faces of a case.
-__global__ void __launch_bounds__(256) k_boundary(Launch L) {
-    const Work w = L.work[blockIdx.x];
-    const CaseDev& c = L.cases[w.cs];
-    if (*c.stop) return;
-    const double* U = c.U[L.in];
+__device__ inline void boundary_block(const CaseDev& c, const double* U, int bx) {
     const long NI = c.ni, NJ = c.nj, NK = c.nk;
-    long t = (long)w.x * 256 + threadIdx.x;
+    long t = (long)bx * 256 + threadIdx.x;
     const long nI = NJ * NK, nJ = NI * NK, nK = NI * NJ;
     const bool doJ = (c.topo != TOPO_1D), doK = (c.topo == TOPO_3D);
     int dir, side;
@@ -338,6 +344,19 @@ __global__ void __launch_bounds__(256) k_boundary(Launch L) {
     double* o = out + (long)side * 5 * nsz;
 #pragma unroll
     for (int v = 0; v < 5; ++v) o[(long)v * nsz + q] = F[v] * sm;
+}
+
+// Ahead of each stage kernel: boundary-face fluxes (z = 0, one block per 256 boundary
+// faces) and slab source terms (z = 1: 3-D case, z = 2: 2-D / axisymmetric; one block per
+// 256 slab nodes), all from the stage input.  Independent work in one launch: the
+// latency-bound boundary blocks (inlet bisection) overlap the throughput-bound sources.
+__global__ void __launch_bounds__(256) k_prestage(Launch La) {
+    const Work w = La.work[blockIdx.x];
+    const CaseDev& c = La.cases[w.cs];
+    if (*c.stop) return;
+    if (w.z == 0) boundary_block(c, c.U[La.in], w.x);
+    else if (w.z == 1) sources_block<true>(c, La, w.x);
+    else sources_block<false>(c, La, w.x);
 }
 
 }  // namespace dev
