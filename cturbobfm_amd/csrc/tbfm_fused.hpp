@@ -517,6 +517,7 @@ template <class TL, int SCH, bool THREED, typename FT>
 __global__ void __launch_bounds__(NT) k_stage(const CaseDev* __restrict__ cases, const Work* __restrict__ work,
                                               Launch La) {
     const Work w = work[blockIdx.x];
+    if (w.cs < 0) return;   // padding entry of the XCD-interleaved table
     const CaseDev* __restrict__ cp = cases + w.cs;
     if (*cp->stop) return;
     StageOps st;

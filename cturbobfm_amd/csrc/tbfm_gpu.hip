@@ -687,12 +687,19 @@ void pushCase(tbfm_case* cs) {
     }
 }
 
+constexpr int kXcds = 8;   // accelerator complex dies of an MI355X (each with its own L2)
+
 // Builds the work tables of every kernel from the members' current tiling.
 void buildTables(Plan& P) {
     std::vector<dev::Work> dt, mn, prof, bnd, per, norm, src;
     for (auto& g : P.groups) freeTable(g.t);
     P.groups.clear();
-    std::vector<std::vector<dev::Work>> gw;
+    // Stage-kernel blocks, XCD-aware: the GPU hands workgroup b to XCD b % kXcds, and each
+    // XCD has its own L2.  Every case's blocks are cut into kXcds contiguous k-major pieces
+    // (gl[group][xcd]); the table interleaves them so that each XCD walks one compact
+    // piece, and the planes and halos its neighbouring blocks share stay in its L2.
+    // Shorter pieces are padded with no-op entries (cs = -1).
+    std::vector<std::vector<std::vector<dev::Work>>> gl;
     P.nstages = P.members.empty() ? 0 : P.members[0]->c.nstages;
     for (int m = 0; m < (int)P.members.size(); ++m) {
         tbfm_case* cs = P.members[m];
@@ -718,12 +725,16 @@ void buildTables(Plan& P) {
             StageGroup g;
             g.tile = cs->tile; g.sch = sch; g.three = three; g.mixed = mixed;
             P.groups.push_back(g);
-            gw.emplace_back();
+            gl.emplace_back(kXcds);
         }
-        // k chunks outermost so that the blocks of one plane range run together
+        // the case's blocks in k-major order, cut into one contiguous piece per XCD
+        std::vector<dev::Work> seq;
         for (int z = 0; z < c.sgz; ++z)
             for (int y = 0; y < c.sgy; ++y)
-                for (int x = 0; x < c.sgx; ++x) gw[gi].push_back({m, x, y, z});
+                for (int x = 0; x < c.sgx; ++x) seq.push_back({m, x, y, z});
+        const size_t n = seq.size();
+        for (int x = 0; x < kXcds; ++x)
+            for (size_t q = n * x / kXcds; q < n * (x + 1) / kXcds; ++q) gl[gi][x].push_back(seq[q]);
     }
     uploadTable(P.dt, dt);
     uploadTable(P.mn, mn);
@@ -737,7 +748,14 @@ void buildTables(Plan& P) {
     pre.insert(pre.end(), src.begin(), src.end());
     uploadTable(P.start, start);
     uploadTable(P.pre, pre);
-    for (size_t g = 0; g < P.groups.size(); ++g) uploadTable(P.groups[g].t, gw[g]);
+    for (size_t g = 0; g < P.groups.size(); ++g) {
+        size_t len = 0;
+        for (auto& l : gl[g]) len = std::max(len, l.size());
+        std::vector<dev::Work> t(len * kXcds, dev::Work{-1, 0, 0, 0});
+        for (int x = 0; x < kXcds; ++x)
+            for (size_t q = 0; q < gl[g][x].size(); ++q) t[q * kXcds + x] = gl[g][x][q];
+        uploadTable(P.groups[g].t, t);
+    }
     P.tablesValid = true;
     P.graphValid = false;
 }
