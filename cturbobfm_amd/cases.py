@@ -258,3 +258,105 @@ def read_case(ini_path: str) -> Dict:
     return {"ni": ni, "nj": nj, "nk": nk, "x": arrs["x"], "y": arrs["y"], "z": arrs["z"],
             "fields": fields, "config": cfg, "patches": patches,
             "name": os.path.basename(base)}
+
+
+# --------------------------------------------------------------- real-gas table
+def covolume_lut(seed: int = 5, n_rho: int = 60, n_e: int = 60, n_pt: int = 40) -> Dict:
+    """A seeded smooth analytic CO2-like equation of state tabulated in the reference's
+    look-up-table layout (src/fluid_real.cpp:12-209).
+
+    Covolume (Noble-Abel) gas: p = (g-1) rho e / (1 - b rho), e = cv T, with
+    R ~ 188.9 J/kg/K, g ~ 1.29, b ~ 9.7e-4 m3/kg (each perturbed by the seed):
+      a^2 = g p / (rho (1 - b rho)),  dp/drho|e = (g-1) e / (1 - b rho)^2,
+      dp/de|rho = (g-1) rho / (1 - b rho),
+      s = cv ln(T/Tr) + R ln((1/rho - b)/(vr - b))   (Tr = 288.15 K, vr at 101325 Pa),
+    and the (p, T) and (p, s) tables from the closed-form inverses.  Returns a dict with
+    the header values and the three tables (arrays shaped (nx, ny, fields)).
+    """
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(-1.0, 1.0, size=3)
+    R = 188.92 * (1.0 + 0.01 * u[0])
+    g = 1.29 + 0.02 * u[1]
+    b = 9.7e-4 * (1.0 + 0.1 * u[2])
+    cv = R / (g - 1.0)
+    cp = cv + R
+    Tr, pr = 288.15, 101325.0
+    vr = R * Tr / pr + b
+
+    def props(rho, e):
+        T = e / cv
+        p = (g - 1.0) * rho * e / (1.0 - b * rho)
+        a = np.sqrt(g * p / (rho * (1.0 - b * rho)))
+        s = cv * np.log(T / Tr) + R * np.log((1.0 / rho - b) / (vr - b))
+        dpdrho = (g - 1.0) * e / (1.0 - b * rho) ** 2
+        dpde = (g - 1.0) * rho / (1.0 - b * rho)
+        return p, T, a, s, dpdrho, dpde
+
+    rho_min, rho_max = 0.4, 4.0
+    t_min, t_max = 150.0, 450.0
+    e_min, e_max = cv * t_min, cv * t_max
+    rho, e = np.meshgrid(np.linspace(rho_min, rho_max, n_rho), np.linspace(e_min, e_max, n_e), indexing="ij")
+    p, T, a, s, dpdrho, dpde = props(rho, e)
+    mu = np.full_like(p, 1.5e-5)
+    kappa = np.full_like(p, 0.017)
+    rho_e = np.stack([p, T, a, s, mu, kappa, dpdrho, dpde], axis=-1)
+
+    p_min, p_max = 2.0e4, 2.5e5
+    pp, tt = np.meshgrid(np.linspace(p_min, p_max, n_pt), np.linspace(t_min, t_max, n_pt), indexing="ij")
+    rho_pt = pp / (R * tt + b * pp)
+    e_pt = cv * tt
+    _, _, a_pt, s_pt, _, _ = props(rho_pt, e_pt)
+    p_t = np.stack([rho_pt, e_pt, a_pt, s_pt], axis=-1)
+
+    s_min, s_max = float(s_pt.min()), float(s_pt.max())
+    pp, ss = np.meshgrid(np.linspace(p_min, p_max, n_pt), np.linspace(s_min, s_max, n_pt), indexing="ij")
+    # s = cp ln T - cv ln Tr + R ln(R / (p (vr - b)))
+    T_ps = np.exp((ss + cv * math.log(Tr) - R * np.log(R / (pp * (vr - b)))) / cp)
+    rho_ps = pp / (R * T_ps + b * pp)
+    e_ps = cv * T_ps
+    p_s = np.stack([rho_ps, T_ps, e_ps], axis=-1)
+    return {"FLUID_NAME": "CO2_COVOLUME_SEED%d" % seed, "MOLAR_MASS": 8.314462618 / R,
+            "GAS_CONSTANT": R, "CRITICAL_PRESSURE": 7.3773e6, "CRITICAL_TEMPERATURE": 304.13,
+            "CRITICAL_DENSITY": 467.6, "NOMINAL_GAMMA": g,
+            "rho_e": (rho_min, rho_max, e_min, e_max, rho_e),
+            "p_t": (p_min, p_max, t_min, t_max, p_t),
+            "p_s": (p_min, p_max, s_min, s_max, p_s)}
+
+
+def write_lut(lut: Dict, path: str) -> str:
+    """Writes a table in the reference's .lut text format (fluid_real.cpp:12-209: header
+    KEY: value lines, then # SECTION: RHO_E / P_T / P_S blocks whose last bound key
+    triggers reading the rows)."""
+    def rows(f, arr):
+        nx, ny, nf = arr.shape
+        for i in range(nx):
+            for j in range(ny):
+                f.write(" ".join("%.17g" % v for v in arr[i, j]) + "\n")
+    with open(path, "w") as f:
+        for k in ("FLUID_NAME", "MOLAR_MASS", "GAS_CONSTANT", "CRITICAL_PRESSURE", "CRITICAL_TEMPERATURE",
+                  "CRITICAL_DENSITY", "NOMINAL_GAMMA"):
+            v = lut[k]
+            f.write("%s: %s\n" % (k, v if isinstance(v, str) else "%.17g" % v))
+        x0, x1, y0, y1, a = lut["rho_e"]
+        f.write("# SECTION: RHO_E\nN_RHO: %d\nN_E: %d\nRHO_MIN: %.17g\nRHO_MAX: %.17g\nE_MIN: %.17g\nE_MAX: %.17g\n"
+                % (a.shape[0], a.shape[1], x0, x1, y0, y1))
+        rows(f, a)
+        x0, x1, y0, y1, a = lut["p_t"]
+        f.write("# SECTION: P_T\nN_P: %d\nN_T: %d\nP_MIN: %.17g\nP_MAX: %.17g\nT_MIN: %.17g\nT_MAX: %.17g\n"
+                % (a.shape[0], a.shape[1], x0, x1, y0, y1))
+        rows(f, a)
+        x0, x1, y0, y1, a = lut["p_s"]
+        f.write("# SECTION: P_S\nN_P: %d\nN_S: %d\nP_MIN: %.17g\nP_MAX: %.17g\nS_MIN: %.17g\nS_MAX: %.17g\n"
+                % (a.shape[0], a.shape[1], x0, x1, y0, y1))
+        rows(f, a)
+    return path
+
+
+def with_real_gas(case: Dict, lut_path: str) -> Dict:
+    """The same case on the look-up-table fluid (FLUID_MODEL = real, FLUID_TABLE_FILE)."""
+    out = dict(case)
+    cfg = dict(case["config"])
+    cfg["FLUID_MODEL"] = "real"
+    cfg["FLUID_TABLE_FILE"] = os.path.abspath(lut_path)
+    out["config"] = cfg
+    return out

@@ -35,7 +35,70 @@ class Params(C.Structure):
                 ("ramp_iterations", C.c_double), ("bfm_active", C.c_int), ("bfm_model", C.c_int),
                 ("blockage_active", C.c_int), ("le_index", C.c_long), ("te_index", C.c_long),
                 ("omega_scaling", C.c_double), ("inlet_cylindrical", C.c_int), ("turbo_dir", C.c_int),
-                ("residual_drop", C.c_long), ("steady", C.c_int)]
+                ("residual_drop", C.c_long), ("steady", C.c_int), ("fluid_real", C.c_int)]
+
+
+class Table(C.Structure):
+    _fields_ = [("nx", C.c_long), ("ny", C.c_long), ("xmin", C.c_double), ("xmax", C.c_double),
+                ("ymin", C.c_double), ("ymax", C.c_double), ("data", C.POINTER(C.c_double))]
+
+
+LUT_COUNT = 15
+
+
+class Lut(C.Structure):
+    _fields_ = [("t", Table * LUT_COUNT), ("gamma_nominal", C.c_double)]
+
+
+def read_lut(path: str) -> Dict:
+    """Parses a .lut file as FluidReal::loadTable does (src/fluid_real.cpp:12-209): header
+    KEY: value lines before the first section; per section the bound keys, the last of
+    which (E_MAX, T_MAX, S_MAX) triggers reading nx*ny rows, skipping empty and '#' lines.
+    Returns {"header": {...}, "tables": [(nx, ny, xmin, xmax, ymin, ymax, array(nx*ny)), ...]}
+    in the oracle's table order (RHO_E: p T a s mu kappa dpdrho dpde; P_T: rho e a s;
+    P_S: rho T e)."""
+    lines = open(path).read().split("\n")
+    header, tables = {}, [None] * LUT_COUNT
+    section, meta, q = "", {}, 0
+    spec = {"RHO_E": (("N_RHO", "N_E", "RHO_MIN", "RHO_MAX", "E_MIN", "E_MAX"), 8, 0),
+            "P_T": (("N_P", "N_T", "P_MIN", "P_MAX", "T_MIN", "T_MAX"), 4, 8),
+            "P_S": (("N_P", "N_S", "P_MIN", "P_MAX", "S_MIN", "S_MAX"), 3, 12)}
+    while q < len(lines):
+        line = lines[q]
+        q += 1
+        if not line or line[0] == "\r":
+            continue
+        for name in spec:
+            if "# SECTION: " + name in line:
+                section, meta = name, {}
+                break
+        else:
+            if line[0] == "#":
+                continue
+            if ":" not in line:
+                continue
+            k, v = line.split(":", 1)
+            k, v = k.rstrip(" \t\r\n"), v.lstrip(" \t\r\n")
+            if not section:
+                header[k] = v
+                continue
+            keys, nf, base = spec[section]
+            meta[k] = v
+            if k == keys[5]:
+                nx, ny = int(meta[keys[0]]), int(meta[keys[1]])
+                rows = []
+                while len(rows) < nx * ny and q < len(lines):
+                    r = lines[q]
+                    q += 1
+                    if not r or r[0] in "#\r":
+                        continue
+                    rows.append([float(t) for t in r.split()[:nf]])
+                arr = np.array(rows, dtype=np.float64)
+                for f in range(nf):
+                    tables[base + f] = (nx, ny, float(meta[keys[2]]), float(meta[keys[3]]),
+                                        float(meta[keys[4]]), float(meta[keys[5]]),
+                                        np.ascontiguousarray(arr[:, f]))
+    return {"header": header, "tables": tables}
 
 
 class Patch(C.Structure):
@@ -88,6 +151,9 @@ def lib():
         L.tbo_bfm_source.argtypes = [C.c_void_p, C.c_long, C.c_long, C.c_long, dp, dp, dp, dp, dp]
         L.tbo_fluid_state.argtypes = [C.c_void_p, dp, dp]
         L.tbo_radial_equilibrium.argtypes = [dp, dp, dp, C.c_int, C.c_double, dp]
+        L.tbo_set_lut.argtypes = [C.c_void_p, C.POINTER(Lut)]
+        L.tbo_lut_eval.restype = C.c_double
+        L.tbo_lut_eval.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
         _lib = L
     return _lib
 
@@ -163,6 +229,13 @@ def params_from_config(case: Dict) -> Params:
     p.turbo_dir = {"i": 0, "+i": 0, "-i": 1, "j": 2, "+j": 2, "-j": 3}.get(td, 0)
     p.residual_drop = int(float(cfg.get("RESIDUALS_DROP_CONVERGENCE", "16")))
     p.steady = int(_cfg_bool(cfg, "SIMULATION_IS_STEADY", True))
+    fm = cfg.get("FLUID_MODEL", "ideal")
+    if fm in ("real", "Real", "REAL"):
+        p.fluid_real = 1
+    elif fm in ("None", "Ideal", "ideal", "IDEAL"):
+        p.fluid_real = 0
+    else:
+        raise ValueError('Invalid value for key "FLUID_MODEL" in configuration.')
     return p
 
 
@@ -206,6 +279,16 @@ class OracleCase:
                               self.patches, len(self.patches), err, 256)
         if not self.h:
             raise RuntimeError(err.value.decode())
+        if self.params.fluid_real:
+            cfg = case["config"]
+            path = cfg.get("FLUID_TABLE_FILE", "fluid_table_" + cfg.get("FLUID_NAME", cfg.get("FLUID", "")) + ".lut")
+            lut = read_lut(path)
+            self.lut = Lut()
+            for t, (nx, ny, x0, x1, y0, y1, a) in enumerate(lut["tables"]):
+                self._keep.append(a)
+                self.lut.t[t] = Table(nx, ny, x0, x1, y0, y1, _dp(a))
+            self.lut.gamma_nominal = float(lut["header"].get("NOMINAL_GAMMA", "1.4"))
+            L.tbo_set_lut(self.h, C.byref(self.lut))
         L.tbo_init_state(self.h)
 
     def __del__(self):
@@ -296,6 +379,9 @@ class OracleCase:
         lib().tbo_fluid_state(self.h, _dp(w), _dp(out))
         return out
 
+    def lut_eval(self, which: int, x: float, y: float) -> float:
+        return lib().tbo_lut_eval(self.h, which, x, y)
+
 
 def radial_equilibrium(rho, ut, r, hub):
     rho, ut, r = (np.ascontiguousarray(a, dtype=np.float64) for a in (rho, ut, r))
@@ -327,5 +413,7 @@ def ref_lib():
         L.ref_fluid_state.argtypes = [dp, dp]
         L.ref_init_state.argtypes = [C.c_double, C.c_double, C.c_double, dp, dp]
         L.ref_radial_equilibrium.argtypes = [dp, dp, dp, C.c_int, C.c_double, dp]
+        L.ref_fluid_eval.restype = C.c_double
+        L.ref_fluid_eval.argtypes = [C.c_int, C.c_double, C.c_double]
         _ref = L
     return _ref

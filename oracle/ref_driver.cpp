@@ -4,7 +4,8 @@
 // reference sources (src/config.cpp, src/input.cpp, src/mesh.cpp,
 // src/math_utils.cpp, src/fluid_ideal.cpp, src/advection_{base,jst,roe}.cpp,
 // src/boundary_{base,inlet,outlet,outlet_radial_equilibrium,inviscid_wall,
-// fake}.cpp, src/source_bfm_{base,hall,lift_drag}.cpp, src/input_table.cpp)
+// fake}.cpp, src/source_bfm_{base,hall,lift_drag}.cpp, src/input_table.cpp,
+// src/fluid_real.cpp)
 // into oracle/_ref/libtbfm_ref.so by oracle/Makefile.  Only tests/ and
 // tests/golden/gen_golden.py load it, to pin the CPU restatement in
 // oracle/tbfm_oracle.c and the product's host geometry.  src/solver.cpp is not
@@ -15,12 +16,13 @@
 //   AdvectionBase::computeFlux          include/advection_base.hpp:21-25
 //   BoundaryBase::computeBoundaryFlux   include/boundary_base.hpp:26-32
 //   SourceBFMBase::computeTotalSource   include/source_bfm_base.hpp:20-31
-//   FluidIdeal                          include/fluid_ideal.hpp
+//   FluidIdeal / FluidReal              include/fluid_ideal.hpp, include/fluid_real.hpp
 #include "types.hpp"
 #include "config.hpp"
 #include "mesh.hpp"
 #include "math_utils.hpp"
 #include "fluid_ideal.hpp"
+#include "fluid_real.hpp"
 #include "advection_jst.hpp"
 #include "advection_roe.hpp"
 #include "boundary_inlet.hpp"
@@ -38,7 +40,7 @@ namespace {
 struct RefCase {
     std::unique_ptr<Config> config;
     std::unique_ptr<Mesh> mesh;
-    std::unique_ptr<FluidIdeal> fluid;
+    std::unique_ptr<FluidBase> fluid;
     std::unique_ptr<AdvectionBase> advection;
     std::unique_ptr<SourceBFMBase> bfm;
     std::vector<FloatType> radialProfile;
@@ -69,8 +71,11 @@ int ref_case_load(const char* ini_path) {
         g_case = RefCase();
         g_case.config = std::make_unique<Config>(std::string(ini_path));
         g_case.mesh = std::make_unique<Mesh>(*g_case.config);
-        g_case.fluid = std::make_unique<FluidIdeal>(g_case.config->getFluidGamma(),
-                                                    g_case.config->getFluidGasConstant());
+        if (g_case.config->getFluidModel() == FluidModel::REAL)
+            g_case.fluid = std::make_unique<FluidReal>(g_case.config->getFluidTableFile());
+        else
+            g_case.fluid = std::make_unique<FluidIdeal>(g_case.config->getFluidGamma(),
+                                                        g_case.config->getFluidGasConstant());
         if (g_case.config->getAdvectionScheme() == AdvectionScheme::ROE)
             g_case.advection = std::make_unique<AdvectionRoe>(*g_case.config, *g_case.fluid);
         else
@@ -190,9 +195,9 @@ void ref_bfm_source(long i, long j, long k, const double* primitive, double time
     *dev = da(i, j, k);
 }
 
-// Ideal-gas helpers used on the path: out = {p, a, ht, T, pt, Tt, s}.
+// Fluid helpers used on the path: out = {p, a, ht, T, pt, Tt, s}.
 void ref_fluid_state(const double* primitive, double* out) {
-    const FluidIdeal& f = *g_case.fluid;
+    const FluidBase& f = *g_case.fluid;
     Vector3D u(primitive[1], primitive[2], primitive[3]);
     out[0] = f.computePressure_rho_u_et(primitive[0], u, primitive[4]);
     out[1] = f.computeSoundSpeed_rho_u_et(primitive[0], u, primitive[4]);
@@ -203,7 +208,35 @@ void ref_fluid_state(const double* primitive, double* out) {
     out[6] = f.computeEntropy_rho_u_et(primitive[0], u, primitive[4]);
 }
 
-// Initial free-stream state of computeInitFields (fluid_ideal.cpp) for a direction.
+// Point functions of the loaded fluid (public FluidBase members), numbered as the
+// oracle's tbo_lut_eval: 0 p(rho,e) 1 T(rho,e) 2 a(rho,e) 3 s(rho,e) 4 dp/drho|e
+// 5 dp/de|rho 6 e(p,rho) 7 a(p,rho) 8 rho(p,T) 9 s(p,T) 10 rho(p,s) 11 T(p,s) 12 e(p,s)
+// 13 h(p,s) 14 e(p,T) 15 a(p,T) 16 p(rho,T).
+double ref_fluid_eval(int which, double x, double y) {
+    const FluidBase& f = *g_case.fluid;
+    switch (which) {
+        case 0: return f.computePressure_rho_e(x, y);
+        case 1: return f.computeTemperature_rho_e(x, y);
+        case 2: return f.computeSoundSpeed_rho_e(x, y);
+        case 3: return f.computeEntropy_rho_e(x, y);
+        case 4: return f.computeDpDrho_e(x, y);
+        case 5: return f.computeDpDe_rho(x, y);
+        case 6: return f.computeStaticEnergy_p_rho(x, y);
+        case 7: return f.computeSoundSpeed_p_rho(x, y);
+        case 8: return f.computeDensity_p_T(x, y);
+        case 9: return f.computeEntropy_p_T(x, y);
+        case 10: return f.computeDensity_p_s(x, y);
+        case 11: return f.computeTemperature_p_s(x, y);
+        case 12: return f.computeInternalEnergy_p_s(x, y);
+        case 13: return f.computeEnthalpy_p_s(x, y);
+        case 14: return f.computeInternalEnergy_p_T(x, y);
+        case 15: return f.computeSoundSpeed_p_T(x, y);
+        case 16: return f.computePressure_rho_T(x, y);
+        default: return 0.0;
+    }
+}
+
+// Initial free-stream state of computeInitFields (fluid_ideal.cpp / fluid_real.cpp).
 void ref_init_state(double mach, double T, double p, const double* dir, double* U) {
     FloatType rho = 0, et = 0;
     Vector3D vel;

@@ -79,6 +79,10 @@ struct tbo_case {
     double *U0, *U, *R, *dt;
     double *finv, *fvisc, *dev;
     double t_phys;
+    /* look-up-table fluid (FLUID_MODEL = real) */
+    tbo_lut lut;
+    double dx[TBO_LUT_COUNT], dy[TBO_LUT_COUNT];
+    double* lutdata[TBO_LUT_COUNT];
 };
 
 #define FI(i, j, k) ((((long)(i) * c->p.nj + (j)) * c->p.nk + (k)) * 3)
@@ -88,42 +92,184 @@ struct tbo_case {
 
 static double* dalloc(long n) { return (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double)); }
 
-/* ---------------------------------------------------------------- fluid (ideal) */
-/* src/fluid_ideal.cpp:70-77 computeStaticEnergy_u_et: et - 0.5*pow(|u|,2) */
-static inline double f_e(v3 u, double et) { double m = mag3(u); return et - 0.5 * (m * m); }
-/* fluid_ideal.cpp:22-24 */
-static inline double f_p_rho_e(const tbo_case* c, double rho, double e) { return (c->p.gamma - 1) * rho * e; }
-/* fluid_ideal.cpp:90-92 */
+/* ------------------------------------------------------- fluid: look-up table */
+/* Table2D::interpolate (include/fluid_real.hpp:34-58): clamp, cell index by truncation,
+ * bilinear weights, the four terms summed left to right. */
+static double lut_interp(const tbo_case* c, int t, double x, double y) {
+    const tbo_table* T = &c->lut.t[t];
+    double xc = (x < T->xmin) ? T->xmin : ((T->xmax < x) ? T->xmax : x);   /* std::clamp */
+    double yc = (y < T->ymin) ? T->ymin : ((T->ymax < y) ? T->ymax : y);
+    double rx = (xc - T->xmin) / c->dx[t];
+    double ry = (yc - T->ymin) / c->dy[t];
+    size_t i = (size_t)rx, j = (size_t)ry;
+    if (i >= (size_t)T->nx - 1) i = (T->nx >= 2) ? (size_t)T->nx - 2 : 0;
+    if (j >= (size_t)T->ny - 1) j = (T->ny >= 2) ? (size_t)T->ny - 2 : 0;
+    double wx = rx - (double)i, wy = ry - (double)j;
+    const double* d = c->lutdata[t];
+    double f00 = d[i * T->ny + j], f10 = d[(i + 1) * T->ny + j];
+    double f01 = d[i * T->ny + (j + 1)], f11 = d[(i + 1) * T->ny + (j + 1)];
+    return (1.0 - wx) * (1.0 - wy) * f00 + wx * (1.0 - wy) * f10 + (1.0 - wx) * wy * f01 + wx * wy * f11;
+}
+/* FluidReal::computeStaticEnergy_p_rho (src/fluid_real.cpp:227-252): bisection on e */
+static double lut_e_p_rho(const tbo_case* c, double p, double rho) {
+    double e_low = c->lut.t[TBO_LUT_P].ymin, e_high = c->lut.t[TBO_LUT_P].ymax;
+    double p_low = lut_interp(c, TBO_LUT_P, rho, e_low);
+    double p_high = lut_interp(c, TBO_LUT_P, rho, e_high);
+    if (p <= p_low) return e_low;
+    if (p >= p_high) return e_high;
+    for (int it = 0; it < 30; ++it) {
+        double e_mid = 0.5 * (e_low + e_high);
+        double p_mid = lut_interp(c, TBO_LUT_P, rho, e_mid);
+        if (fabs(p_mid - p) <= 1e-4 * p) return e_mid;
+        if (p_mid < p) e_low = e_mid; else e_high = e_mid;
+    }
+    return 0.5 * (e_low + e_high);
+}
+/* FluidBase::computeEnthalpy_p_s (include/fluid_base.hpp) e(p,s) + p/rho(p,s) */
+static double lut_h_p_s(const tbo_case* c, double p, double s) {
+    return lut_interp(c, TBO_LUT_E_PS, p, s) + p / lut_interp(c, TBO_LUT_RHO_PS, p, s);
+}
+/* FluidReal::computePressure_rho_T (fluid_real.cpp:264-279) */
+static double lut_p_rho_T(const tbo_case* c, double rho, double T) {
+    double e_low = c->lut.t[TBO_LUT_T].ymin, e_high = c->lut.t[TBO_LUT_T].ymax;
+    for (int it = 0; it < 25; ++it) {
+        double e_mid = 0.5 * (e_low + e_high);
+        double T_mid = lut_interp(c, TBO_LUT_T, rho, e_mid);
+        if (T_mid < T) e_low = e_mid; else e_high = e_mid;
+    }
+    return lut_interp(c, TBO_LUT_P, rho, 0.5 * (e_low + e_high));
+}
+
+double tbo_lut_eval(const tbo_case* c, int which, double x, double y) {
+    switch (which) {
+        case 0: return lut_interp(c, TBO_LUT_P, x, y);
+        case 1: return lut_interp(c, TBO_LUT_T, x, y);
+        case 2: return lut_interp(c, TBO_LUT_A, x, y);
+        case 3: return lut_interp(c, TBO_LUT_S, x, y);
+        case 4: return lut_interp(c, TBO_LUT_DPDRHO, x, y);
+        case 5: return lut_interp(c, TBO_LUT_DPDE, x, y);
+        case 6: return lut_e_p_rho(c, x, y);
+        case 7: return lut_interp(c, TBO_LUT_A, y, lut_e_p_rho(c, x, y));
+        case 8: return lut_interp(c, TBO_LUT_RHO_PT, x, y);
+        case 9: return lut_interp(c, TBO_LUT_S_PT, x, y);
+        case 10: return lut_interp(c, TBO_LUT_RHO_PS, x, y);
+        case 11: return lut_interp(c, TBO_LUT_T_PS, x, y);
+        case 12: return lut_interp(c, TBO_LUT_E_PS, x, y);
+        case 13: return lut_h_p_s(c, x, y);
+        case 14: return lut_interp(c, TBO_LUT_E_PT, x, y);
+        case 15: return lut_interp(c, TBO_LUT_A_PT, x, y);
+        case 16: return lut_p_rho_T(c, x, y);
+        default: return NAN;
+    }
+}
+
+void tbo_set_lut(tbo_case* c, const tbo_lut* lut) {
+    c->lut = *lut;
+    for (int t = 0; t < TBO_LUT_COUNT; ++t) {
+        tbo_table* T = &c->lut.t[t];
+        long n = T->nx * T->ny;
+        free(c->lutdata[t]);
+        c->lutdata[t] = dalloc(n);
+        if (T->data && n > 0) memcpy(c->lutdata[t], T->data, sizeof(double) * (size_t)n);
+        T->data = c->lutdata[t];
+        /* Table2D::init (fluid_real.hpp:22-32) */
+        c->dx[t] = (T->nx > 1) ? (T->xmax - T->xmin) / (double)(T->nx - 1) : 1.0;
+        c->dy[t] = (T->ny > 1) ? (T->ymax - T->ymin) / (double)(T->ny - 1) : 1.0;
+    }
+}
+
+/* ------------------------------------------------------------------ fluid */
+/* Static energy from velocity and total energy.  Ideal: FluidIdeal override
+ * (src/fluid_ideal.cpp:70-77) et - 0.5*pow(|u|,2); real: FluidBase
+ * (include/fluid_base.hpp) et - 0.5*u.dot(u). */
+static inline double f_e_c(const tbo_case* c, v3 u, double et) {
+    if (c->p.fluid_real) return et - 0.5 * dot3(u, u);
+    double m = mag3(u);
+    return et - 0.5 * (m * m);
+}
+#define f_e(u, et) f_e_c(c, (u), (et))
+/* fluid_ideal.cpp:22-24 / fluid_real.cpp:211-213 */
+static inline double f_p_rho_e(const tbo_case* c, double rho, double e) {
+    if (c->p.fluid_real) return lut_interp(c, TBO_LUT_P, rho, e);
+    return (c->p.gamma - 1) * rho * e;
+}
+/* fluid_ideal.cpp:90-92 / FluidBase::computePressure_rho_u_et */
 static inline double f_p(const tbo_case* c, double rho, v3 u, double et) { return f_p_rho_e(c, rho, f_e(u, et)); }
-/* fluid_ideal.cpp:70-72 sqrt(gamma p / rho) */
-static inline double f_a_p_rho(const tbo_case* c, double p, double rho) { return sqrt(c->p.gamma * p / rho); }
+/* computeSoundSpeed_p_rho: ideal sqrt(gamma p / rho) (fluid_ideal.cpp:70-72); real
+ * a(rho, e(p, rho)) with the bisection for e (fluid_real.cpp:254-257) */
+static inline double f_a_p_rho(const tbo_case* c, double p, double rho) {
+    if (c->p.fluid_real) return lut_interp(c, TBO_LUT_A, rho, lut_e_p_rho(c, p, rho));
+    return sqrt(c->p.gamma * p / rho);
+}
+/* computeSoundSpeed_rho_u_et: ideal override via p; real a(rho, e) */
 static inline double f_a(const tbo_case* c, double rho, v3 u, double et) {
+    if (c->p.fluid_real) return lut_interp(c, TBO_LUT_A, rho, f_e(u, et));
     double p = f_p(c, rho, u, et);
     return f_a_p_rho(c, p, rho);
 }
-/* fluid_ideal.cpp:111-115 */
+/* computeSoundSpeed_rho_e: ideal sqrt(gamma (gamma-1) e); real a(rho, e) */
+static inline double f_a_rho_e(const tbo_case* c, double rho, double e) {
+    if (c->p.fluid_real) return lut_interp(c, TBO_LUT_A, rho, e);
+    return sqrt(c->p.gamma * (c->p.gamma - 1.0) * e);
+}
+/* computeDpDrho_e / computeDpDe_rho: ideal (gamma-1) e, (gamma-1) rho; real tables */
+static inline double f_chi(const tbo_case* c, double rho, double e) {
+    if (c->p.fluid_real) return lut_interp(c, TBO_LUT_DPDRHO, rho, e);
+    return (c->p.gamma - 1.0) * e;
+}
+static inline double f_kappa(const tbo_case* c, double rho, double e) {
+    if (c->p.fluid_real) return lut_interp(c, TBO_LUT_DPDE, rho, e);
+    return (c->p.gamma - 1.0) * rho;
+}
+/* fluid_ideal.cpp:111-115 / FluidBase::computeTotalEnthalpy_rho_u_et: et + p/rho */
 static inline double f_ht(const tbo_case* c, double rho, v3 u, double et) {
     double e = f_e(u, et);
     double p = f_p_rho_e(c, rho, e);
     return et + p / rho;
 }
 static inline double f_T(const tbo_case* c, double rho, v3 u, double et) {
+    if (c->p.fluid_real) return lut_interp(c, TBO_LUT_T, rho, f_e(u, et));
     return f_p(c, rho, u, et) / (rho * c->p.r_gas);
 }
 static inline double f_cp(const tbo_case* c) { return (c->p.gamma * c->p.r_gas) / (c->p.gamma - 1); }
+/* total pressure: ideal isentropic relation; real FluidReal::computeTotalPressure_rho_u_et
+ * (fluid_real.cpp:311-334): bisection on p_t >= p with h(p_t, s) = h_t */
 static double f_pt(const tbo_case* c, double rho, v3 u, double et) {
+    if (c->p.fluid_real) {
+        double e = f_e(u, et);
+        double p = lut_interp(c, TBO_LUT_P, rho, e);
+        double s = lut_interp(c, TBO_LUT_S, rho, e);
+        double ht = et + p / rho;
+        double p_low = p, p_high = c->lut.t[TBO_LUT_RHO_PS].xmax;
+        for (int it = 0; it < 25; ++it) {
+            double p_mid = 0.5 * (p_low + p_high);
+            double rho_mid = lut_interp(c, TBO_LUT_RHO_PS, p_mid, s);
+            double e_mid = lut_interp(c, TBO_LUT_E_PS, p_mid, s);
+            double h_mid = e_mid + p_mid / rho_mid;
+            if (h_mid < ht) p_low = p_mid; else p_high = p_mid;
+        }
+        return 0.5 * (p_low + p_high);
+    }
     double g = c->p.gamma;
     double p = f_p(c, rho, u, et);
     double M = mag3(u) / f_a(c, rho, u, et);
     return p * pow(1.0 + (g - 1.0) / 2.0 * M * M, g / (g - 1));
 }
+/* real: FluidReal::computeTotalTemperature_rho_u_et (fluid_real.cpp:336-341) */
 static double f_Tt(const tbo_case* c, double rho, v3 u, double et) {
+    if (c->p.fluid_real) {
+        double pt = f_pt(c, rho, u, et);
+        double e = f_e(u, et);
+        double s = lut_interp(c, TBO_LUT_S, rho, e);
+        return lut_interp(c, TBO_LUT_T_PS, pt, s);
+    }
     double g = c->p.gamma;
     double T = f_T(c, rho, u, et);
     double M = mag3(u) / f_a(c, rho, u, et);
     return T * (1.0 + (g - 1.0) / 2.0 * M * M);
 }
 static double f_s(const tbo_case* c, double rho, v3 u, double et) {
+    if (c->p.fluid_real) return lut_interp(c, TBO_LUT_S, rho, f_e(u, et));
     double p = f_p(c, rho, u, et);
     double T = f_T(c, rho, u, et);
     return f_cp(c) * log(T / 288.15) - c->p.r_gas * log(p / 101325);
@@ -299,8 +445,8 @@ static st5 roe_flux(const tbo_case* c, st5 Ull, st5 Ul, st5 Ur, st5 Urr, v3 S) {
     double eL = etL - 0.5 * (u1L * u1L + u2L * u2L + u3L * u3L);
     double eR = etR - 0.5 * (u1R * u1R + u2R * u2R + u3R * u3R);
     double pL = f_p_rho_e(c, rhoL, eL), pR = f_p_rho_e(c, rhoR, eR);
-    double chiL = (g - 1.0) * eL, kapL = (g - 1.0) * rhoL;
-    double chiR = (g - 1.0) * eR, kapR = (g - 1.0) * rhoR;
+    double chiL = f_chi(c, rhoL, eL), kapL = f_kappa(c, rhoL, eL);
+    double chiR = f_chi(c, rhoR, eR), kapR = f_kappa(c, rhoR, eR);
     double barChi = 0.5 * (chiL + chiR), barKap = 0.5 * (kapL + kapR);
     double barRho = 0.5 * (rhoL + rhoR), barE = 0.5 * (eL + eR);
     double dP = pR - pL, dRho = rhoR - rhoL, dE = eR - eL;
@@ -321,7 +467,7 @@ static st5 roe_flux(const tbo_case* c, st5 Ull, st5 Ul, st5 Ur, st5 Urr, v3 S) {
     double hA = htA - 0.5 * q2;
     double a2 = chiP + kapP * hA;
     if (a2 <= 0.0) {
-        double aL = sqrt(g * (g - 1.0) * eL), aR = sqrt(g * (g - 1.0) * eR);
+        double aL = f_a_rho_e(c, rhoL, eL), aR = f_a_rho_e(c, rhoR, eR);
         a2 = 0.5 * (aL * aL + aR * aR);
     }
     double aA = sqrt(a2);
@@ -390,6 +536,30 @@ static st5 inlet_subsonic(const tbo_case* c, st5 Uint, v3 S, double pt, double T
     double Cg = dot3(dir, n);
     if (Cg >= 0.0) Cg = -1.0;
     double Rout = pInt + Z * unInt;
+    if (c->p.fluid_real) {
+        /* boundary_base.cpp:77-111: entropy of the inlet total state, enthalpy search */
+        double sIn = lut_interp(c, TBO_LUT_S_PT, pt, Tt);
+        double htIn = lut_h_p_s(c, pt, sIn);
+        double pLow = 0.05 * pt, pHigh = 0.999999 * pt;
+        double pB = 0.5 * (pLow + pHigh);
+        for (int it = 0; it < 30; ++it) {
+            double pMid = 0.5 * (pLow + pHigh);
+            double hMid = lut_h_p_s(c, pMid, sIn);
+            double VMid = sqrt(2.0 * dmax(0.0, htIn - hMid));
+            double res = pMid + Z * Cg * VMid - Rout;
+            if (fabs(res) < 1e-4 * pt) { pB = pMid; break; }
+            if (res < 0.0) pLow = pMid; else pHigh = pMid;
+            pB = pMid;
+        }
+        double hB = lut_h_p_s(c, pB, sIn);
+        double VB = sqrt(2.0 * dmax(0.0, htIn - hB));
+        v3 vB = mul3(dir, VB);
+        double rhoB = lut_interp(c, TBO_LUT_RHO_PS, pB, sIn);
+        double eB = lut_interp(c, TBO_LUT_E_PS, pB, sIn);
+        double etB = eB + 0.5 * dot3(vB, vB);
+        st5 Wb = {{rhoB, vB.x, vB.y, vB.z, etB}};
+        return phys_flux(c, Wb, S);
+    }
     double cp = g * Rg / (g - 1.0);
     double htIn = cp * Tt;
     double pLow = 0.05 * pt, pHigh = 0.999999 * pt;
@@ -429,8 +599,17 @@ static st5 outlet_flux(const tbo_case* c, st5 Uint, v3 S, double pb, long it) {
     double a = f_a_p_rho(c, p, W.v[0]);
     if (mag3(u) >= a) return phys_flux(c, W, S);
     double pB = ramped_pressure(c, it, pb);
-    double rhoB = pB * rho / p;
-    double eB = pB / ((c->p.gamma - 1) * rhoB);
+    double rhoB, eB;
+    if (c->p.fluid_real) {
+        /* boundary_base.cpp:205-211: isentropic extrapolation along the interior entropy */
+        double e_int = W.v[4] - 0.5 * dot3(u, u);
+        double s_int = lut_interp(c, TBO_LUT_S, rho, e_int);
+        rhoB = lut_interp(c, TBO_LUT_RHO_PS, pB, s_int);
+        eB = lut_interp(c, TBO_LUT_E_PS, pB, s_int);
+    } else {
+        rhoB = pB * rho / p;
+        eB = pB / ((c->p.gamma - 1) * rhoB);
+    }
     double etB = eB + 0.5 * dot3(u, u);
     st5 Wb = {{rhoB, u.x, u.y, u.z, etB}};
     return phys_flux(c, Wb, S);
@@ -1195,10 +1374,10 @@ void tbo_turbo(tbo_case* c, const double* U, double* out) {
     if (c->p.topology == TBO_TOPO_AXI) m *= 2.0 * M_PI / c->wedge_angle;
     else if (c->per_angle_deg != 0.0) m *= 360.0 / c->per_angle_deg;
     out[6] = m;
-    double ptm[2], ttm[2];
+    double ptm[2], ttm[2], htm[2], sm[2];
     for (int s = 0; s < 2; ++s) {
         int b = bi[s];
-        double sp = 0.0, stt = 0.0;
+        double sp = 0.0, stt = 0.0, sht = 0.0, ssv = 0.0;
         long n1 = (b < 2) ? NJ : NI, n2 = NK;
         for (long p = 0; p < n1; ++p)
             for (long q = 0; q < n2; ++q) {
@@ -1214,21 +1393,40 @@ void tbo_turbo(tbo_case* c, const double* U, double* out) {
                 double Tt = f_Tt(c, W.v[0], u, W.v[4]);
                 sp += S[0] * (W.v[0] * u.x * pt) + S[1] * (W.v[0] * u.y * pt) + S[2] * (W.v[0] * u.z * pt);
                 stt += S[0] * (W.v[0] * u.x * Tt) + S[1] * (W.v[0] * u.y * Tt) + S[2] * (W.v[0] * u.z * Tt);
+                if (c->p.fluid_real) {
+                    double ht = f_ht(c, W.v[0], u, W.v[4]);
+                    double sv = f_s(c, W.v[0], u, W.v[4]);
+                    sht += S[0] * (W.v[0] * u.x * ht) + S[1] * (W.v[0] * u.y * ht) + S[2] * (W.v[0] * u.z * ht);
+                    ssv += S[0] * (W.v[0] * u.x * sv) + S[1] * (W.v[0] * u.y * sv) + S[2] * (W.v[0] * u.z * sv);
+                }
             }
         double mfl = mf[b];
         if (fabs(mfl) < 1e-12) mfl = (mfl >= 0.0 ? 1e-12 : -1e-12);
         ptm[s] = sp / mfl;
         ttm[s] = stt / mfl;
+        htm[s] = sht / mfl;
+        sm[s] = ssv / mfl;
     }
     double PR = (fabs(ptm[0]) > 1e-12) ? ptm[1] / ptm[0] : 1.0;
     double TR = (fabs(ttm[0]) > 1e-12) ? ttm[1] / ttm[0] : 1.0;
     out[7] = PR;
     out[8] = TR;
-    double g = c->p.gamma;
-    double dT = TR - 1.0, dTs = pow(PR, (g - 1.0) / g) - 1.0, eta;
-    if (fabs(dT) < 1e-12) eta = 1.0;
-    else if (dT >= 0.0) eta = dTs / dT;
-    else eta = (fabs(dTs) < 1e-12) ? 1.0 : dT / dTs;
+    double eta;
+    if (c->p.fluid_real) {
+        /* solver.cpp:1769-1776, FluidBase::computeTotalEfficiency_h */
+        double ht_in = htm[0], s_in = sm[0], pt_out = ptm[1], ht_out = htm[1];
+        double ht_out_s = lut_h_p_s(c, pt_out, s_in);
+        double dh = ht_out - ht_in, dhs = ht_out_s - ht_in;
+        if (fabs(dh) < 1e-12) eta = 1.0;
+        else if (dh >= 0.0) eta = dhs / dh;
+        else eta = (fabs(dhs) < 1e-12) ? 1.0 : dh / dhs;
+    } else {
+        double g = c->p.gamma;
+        double dT = TR - 1.0, dTs = pow(PR, (g - 1.0) / g) - 1.0;
+        if (fabs(dT) < 1e-12) eta = 1.0;
+        else if (dT >= 0.0) eta = dTs / dT;
+        else eta = (fabs(dTs) < 1e-12) ? 1.0 : dT / dTs;
+    }
     out[9] = eta;
 }
 
@@ -1317,6 +1515,7 @@ tbo_case* tbo_create(const tbo_params* p, const double* x, const double* y, cons
 }
 
 void tbo_destroy(tbo_case* c) {
+    if (c) for (int t = 0; t < TBO_LUT_COUNT; ++t) free(c->lutdata[t]);
     if (!c) return;
     double* ptrs[] = {c->x, c->y, c->z, c->dual, c->sI, c->sJ, c->sK, c->cI, c->cJ, c->cK, c->vol,
                       c->radius, c->theta, c->gradb, c->blockage, c->nax, c->nrad, c->ntan, c->rpm,
@@ -1348,12 +1547,26 @@ static void init_profiles(tbo_case* c) {
 void tbo_init_state(tbo_case* c) {
     const double g = c->p.gamma, Rg = c->p.r_gas;
     v3 dir = mk3(c->p.init_dir[0], c->p.init_dir[1], c->p.init_dir[2]);
-    double a = sqrt(g * Rg * c->p.init_temperature);
-    v3 vel = mul3(mul3(div3(dir, mag3(dir)), a), c->p.init_mach);
-    double rho = c->p.init_pressure / (Rg * c->p.init_temperature);
-    double e = c->p.init_pressure / ((g - 1) * rho);
-    double vm = mag3(vel);
-    double et = e + 0.5 * (vm * vm);
+    double rho, et;
+    v3 vel;
+    if (c->p.fluid_real) {
+        /* FluidReal::computeInitFields (src/fluid_real.cpp:375-393) */
+        const double p0 = c->p.init_pressure, T0 = c->p.init_temperature;
+        rho = lut_interp(c, TBO_LUT_RHO_PT, p0, T0);
+        double e = lut_interp(c, TBO_LUT_E_PT, p0, T0);
+        double a = lut_interp(c, TBO_LUT_A, rho, e);
+        double dm = mag3(dir);
+        v3 dn = (dm > 1e-12) ? div3(dir, dm) : mk3(1.0, 0.0, 0.0);
+        vel = mul3(mul3(dn, a), c->p.init_mach);
+        et = e + 0.5 * dot3(vel, vel);
+    } else {
+        double a = sqrt(g * Rg * c->p.init_temperature);
+        vel = mul3(mul3(div3(dir, mag3(dir)), a), c->p.init_mach);
+        rho = c->p.init_pressure / (Rg * c->p.init_temperature);
+        double e = c->p.init_pressure / ((g - 1) * rho);
+        double vm = mag3(vel);
+        et = e + 0.5 * (vm * vm);
+    }
     for (long n = 0; n < c->N; ++n) {
         c->U0[n] = rho;
         c->U0[c->N + n] = rho * vel.x;

@@ -41,7 +41,27 @@ typedef struct {
     int turbo_dir;                   /* 0: i, 1: -i, 2: j, 3: -j */
     long residual_drop;              /* RESIDUALS_DROP_CONVERGENCE */
     int steady;
+    int fluid_real;                  /* FLUID_MODEL = real: look-up-table fluid (tbo_set_lut) */
 } tbo_params;
+
+/* A fluid look-up table (include/fluid_real.hpp:11-60 Table2D): nx*ny values, row-major
+ * data[i*ny + j] over a uniform grid x in [xmin, xmax], y in [ymin, ymax]. */
+typedef struct {
+    long nx, ny;
+    double xmin, xmax, ymin, ymax;
+    const double* data;
+} tbo_table;
+
+/* The tables of a FluidReal (src/fluid_real.cpp:12-209), in this order:
+ * RHO_E (x = rho, y = e): p, T, a, s, mu, kappa, dp/drho|e, dp/de|rho;
+ * P_T (x = p, y = T): rho, e, a, s;  P_S (x = p, y = s): rho, T, e. */
+enum { TBO_LUT_P, TBO_LUT_T, TBO_LUT_A, TBO_LUT_S, TBO_LUT_MU, TBO_LUT_KAPPA, TBO_LUT_DPDRHO,
+       TBO_LUT_DPDE, TBO_LUT_RHO_PT, TBO_LUT_E_PT, TBO_LUT_A_PT, TBO_LUT_S_PT, TBO_LUT_RHO_PS,
+       TBO_LUT_T_PS, TBO_LUT_E_PS, TBO_LUT_COUNT };
+typedef struct {
+    tbo_table t[TBO_LUT_COUNT];
+    double gamma_nominal;            /* NOMINAL_GAMMA */
+} tbo_lut;
 
 typedef struct {
     int type;
@@ -63,6 +83,14 @@ tbo_case* tbo_create(const tbo_params* p, const double* x, const double* y, cons
                      const tbo_fields* f, const tbo_patch* patches, int npatches,
                      char* err, int errlen);
 void tbo_destroy(tbo_case* c);
+/* Installs the look-up-table fluid of a FLUID_MODEL = real case (copies the tables);
+ * call before tbo_init_state. */
+void tbo_set_lut(tbo_case* c, const tbo_lut* lut);
+/* FluidReal point functions of an installed table, for per-function pinning:
+ * which 0 p(rho,e) 1 T(rho,e) 2 a(rho,e) 3 s(rho,e) 4 dp/drho|e 5 dp/de|rho 6 e(p,rho)
+ * 7 a(p,rho) 8 rho(p,T) 9 s(p,T) 10 rho(p,s) 11 T(p,s) 12 e(p,s) 13 h(p,s)
+ * 14 e(p,T) 15 a(p,T) 16 p(rho,T); for (x, y) in that order. */
+double tbo_lut_eval(const tbo_case* c, int which, double x, double y);
 long tbo_num_nodes(const tbo_case* c);
 
 /* state: 5*N doubles, variable-major (rho[N], rhoU[N], ...), k-fastest */

@@ -10,6 +10,10 @@ Each fixture holds seeded inputs and the reference's outputs:
                           (+ blockage), with inviscid/viscous force and deviation angle
   fluid.npz               FluidIdeal p, a, h_t, T, p_t, T_t, s
   radial_eq.npz           integrateRadialEquilibrium
+  lut.npz                 FluidReal on a seeded covolume-gas table (the table itself is
+                          stored in the fixture): the 17 point functions, JST and
+                          Roe-MUSCL face fluxes, inlet / outlet / wall fluxes, p a h_t T
+                          p_t T_t s, and the initial state
 
 Run from the repository root:  python tests/golden/gen_golden.py
 """
@@ -153,7 +157,91 @@ def main():
                 devs.append(dv.value)
             np.savez_compressed(os.path.join(HERE, "bfm_%s.npz" % name), idx=np.array(idx), W=np.array(Ws),
                                 S=np.array(outs), finv=np.array(fis), fvisc=np.array(fvs), dev=np.array(devs))
+    lut_fixtures(R, rng, tmp)
     print("golden fixtures written to", HERE)
+
+
+def real_gas_states(rng, n):
+    """Conserved states inside the covolume table's (rho, e) range."""
+    rho = rng.uniform(0.8, 3.0, n)
+    u = rng.uniform(-150, 150, (n, 3))
+    e = rng.uniform(1.2e5, 2.5e5, n)
+    et = e + 0.5 * np.sum(u * u, axis=1)
+    return np.stack([rho, rho * u[:, 0], rho * u[:, 1], rho * u[:, 2], rho * et], axis=1)
+
+
+def lut_fixtures(R, rng, tmp):
+    """FluidReal fixtures on the seeded covolume-gas table."""
+    lut = cases.covolume_lut(seed=5)
+    os.makedirs(tmp, exist_ok=True)
+    path = cases.write_lut(lut, os.path.join(tmp, "covolume.lut"))
+    out = {"rho_e": lut["rho_e"][4], "p_t": lut["p_t"][4], "p_s": lut["p_s"][4],
+           "bounds": np.array([lut["rho_e"][:4], lut["p_t"][:4], lut["p_s"][:4]]),
+           "header": np.array([lut["GAS_CONSTANT"], lut["NOMINAL_GAMMA"], lut["MOLAR_MASS"]])}
+    n = 200
+    pts = np.zeros((17, n, 3))
+    for which in range(17):
+        for q in range(n):
+            if which <= 5:
+                x, y = rng.uniform(0.2, 4.5), rng.uniform(8e4, 3e5)
+            elif which in (6, 7):
+                x, y = rng.uniform(1e4, 3e5), rng.uniform(0.3, 4.2)
+            elif which in (8, 9, 14, 15):
+                x, y = rng.uniform(1e4, 3e5), rng.uniform(120, 480)
+            elif which == 16:
+                x, y = rng.uniform(0.3, 4.2), rng.uniform(120, 480)
+            else:
+                x, y = rng.uniform(1e4, 3e5), rng.uniform(-400, 400)
+            pts[which, q] = (x, y, 0.0)
+    nflux = 128
+    for label, kw in (("jst", {}), ("roe", {"scheme": "ROE", "muscl": True, "limiter": "van albada"})):
+        case = cases.with_real_gas(cases.rotor3d(11, 5, 7, **kw), path)
+        ini = cases.write_case(case, os.path.join(tmp, "lut_" + label))
+        assert R.ref_case_load(ini.encode()) == 0
+        if label == "jst":
+            for which in range(17):
+                for q in range(n):
+                    pts[which, q, 2] = R.ref_fluid_eval(which, pts[which, q, 0], pts[which, q, 1])
+        U4 = [real_gas_states(rng, nflux) for _ in range(4)]
+        S = rng.normal(size=(nflux, 3))
+        F = np.zeros((nflux, 5))
+        for q in range(nflux):
+            R.ref_face_flux(*[dp(np.ascontiguousarray(u[q])) for u in U4], dp(np.ascontiguousarray(S[q])), dp(F[q]))
+        out["flux_%s_U" % label] = np.stack(U4)
+        out["flux_%s_S" % label] = S
+        out["flux_%s_F" % label] = F
+    # boundary fluxes, fluid state and initial state on the JST case's fluid
+    case = cases.with_real_gas(cases.rotor3d(11, 5, 7), path)
+    ini = cases.write_case(case, os.path.join(tmp, "lut_jst"))
+    assert R.ref_case_load(ini.encode()) == 0
+    nj = case["nj"]
+    prof = rng.uniform(9.0e4, 1.0e5, nj)
+    recs = []
+    for q in range(nflux):
+        Ui = real_gas_states(rng, 1)[0]
+        Sv = rng.normal(size=3)
+        face = np.array([0, rng.integers(0, nj), rng.integers(0, case["nk"])], dtype=np.int64)
+        for kind, vals in ((0, [101325.0, 288.15, 1.0, 0.0, 0.0]), (0, [1.2e5, 300.0, 1.0, 1.0, 1.0]),
+                           (1, [95000.0]), (2, [0.0]), (3, [])):
+            o = np.zeros(5)
+            v = np.array(vals + [0.0] * (8 - len(vals)))
+            R.ref_boundary_flux(kind, dp(v), len(vals), dp(Ui), dp(Sv), face.ctypes.data_as(C.POINTER(C.c_long)),
+                                3, dp(prof), nj, dp(o))
+            recs.append((kind, v, Ui, Sv, face.astype(np.float64), o))
+    out.update(bnd_kind=np.array([r[0] for r in recs]), bnd_values=np.array([r[1] for r in recs]),
+               bnd_U=np.array([r[2] for r in recs]), bnd_S=np.array([r[3] for r in recs]),
+               bnd_face=np.array([r[4] for r in recs]), bnd_F=np.array([r[5] for r in recs]),
+               bnd_profile=prof, bnd_iteration=3)
+    Us = real_gas_states(rng, nflux)
+    W = np.stack([Us[:, 0], Us[:, 1] / Us[:, 0], Us[:, 2] / Us[:, 0], Us[:, 3] / Us[:, 0], Us[:, 4] / Us[:, 0]], axis=1)
+    fs = np.zeros((nflux, 7))
+    for q in range(nflux):
+        R.ref_fluid_state(dp(np.ascontiguousarray(W[q])), dp(fs[q]))
+    out.update(fluid_W=W, fluid_out=fs)
+    init = np.zeros(5)
+    R.ref_init_state(0.4, 290.0, 95000.0, dp(np.array([1.0, 0.2, 0.0])), dp(init))
+    out.update(init=init, points=pts)
+    np.savez_compressed(os.path.join(HERE, "lut.npz"), **out)
 
 
 if __name__ == "__main__":
