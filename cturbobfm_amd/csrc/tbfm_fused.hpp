@@ -91,7 +91,7 @@ __device__ __forceinline__ bool fetch_position(const CaseDev& c, const double* _
 
 // Stores what the faces read at LDS position `pos` (JST: state, primitives, pressure,
 // spectral factor; Roe: primitives); unused positions get harmless constants.
-template <class TL, int SCH>
+template <class TL, int SCH, int FL>
 __device__ __forceinline__ void commit_position(const CaseDev& c, PlaneLds<TL>& L, int pos, bool ok,
                                                 const double Un[5]) {
     if (!ok) {
@@ -106,7 +106,7 @@ __device__ __forceinline__ void commit_position(const CaseDev& c, PlaneLds<TL>& 
         return;
     }
     double W[5], p, rf;
-    derive<double>(Un, W, p, rf, c.gamma, c.gm1);
+    derive<double, FL>(Un, W, p, rf, Gas<FL>(c));
 #pragma unroll
     for (int v = 0; v < 5; ++v) L.d.W[v][pos] = W[v];
     if constexpr (SCH == 0) {
@@ -117,17 +117,17 @@ __device__ __forceinline__ void commit_position(const CaseDev& c, PlaneLds<TL>& 
     }
 }
 
-template <class TL, int SCH>
+template <class TL, int SCH, int FL>
 __device__ inline void fill_position(const CaseDev& c, const double* __restrict__ U, PlaneLds<TL>& L, int pos,
                                      int i0, int j0, int k) {
     double Un[5];
     const bool ok = fetch_position<TL>(c, U, pos, i0, j0, k, Un);
-    commit_position<TL, SCH>(c, L, pos, ok, Un);
+    commit_position<TL, SCH, FL>(c, L, pos, ok, Un);
 }
 
 // Face flux*|S| between LDS positions l and r (with ll = l - st, rr = r + st) along i
 // or j; n and area are the face's unit normal and |S|.
-template <class TL, int SCH, typename FT>
+template <class TL, int SCH, typename FT, int FL>
 __device__ __forceinline__ void lds_face(const CaseDev& c, const PlaneLds<TL>& L, int l, int st, double nx, double ny,
                                          double nz, double area, double out[5]) {
     const int r = l + st, ll = l - st, rr = r + st;
@@ -145,7 +145,7 @@ __device__ __forceinline__ void lds_face(const CaseDev& c, const PlaneLds<TL>& L
         FT t4 = FT(0.02) * rr_ - FT(2.0) * psi2;
         FT psi4 = (FT(0.0) < t4) ? t4 : FT(0.0);
         FT F[5];
-        phys_flux_n<FT>(Wavg, FT(nx), FT(ny), FT(nz), FT(c.gm1), F);
+        phys_flux_n<FT, FL>(Wavg, FT(nx), FT(ny), FT(nz), Gas<FL>(c), F);
         const FT sm = FT(area);
 #pragma unroll
         for (int v = 0; v < 5; ++v) {
@@ -161,7 +161,7 @@ __device__ __forceinline__ void lds_face(const CaseDev& c, const PlaneLds<TL>& L
         for (int v = 0; v < 5; ++v) {
             Wll[v] = L.d.W[v][ll]; Wl[v] = L.d.W[v][l]; Wr[v] = L.d.W[v][r]; Wrr[v] = L.d.W[v][rr];
         }
-        roe_face<FT>(Wll, Wl, Wr, Wrr, c.muscl, c.limiter, nx, ny, nz, area, c.gamma, out);
+        roe_face<FT, FL>(Wll, Wl, Wr, Wrr, c.muscl, c.limiter, nx, ny, nz, area, Gas<FL>(c), out);
     }
 }
 
@@ -170,22 +170,22 @@ __device__ __forceinline__ void lds_face(const CaseDev& c, const PlaneLds<TL>& L
 // spectral factor of slots 1 and 2 only (those of a new plane are derived when it
 // reaches slot 2); Roe keeps the primitives of all four.  Unused slot values (planes
 // beyond -1 .. NK) are never read by a face.
-template <int SCH>
+template <int SCH, int FL>
 struct KQueue;
 
-template <>
-struct KQueue<0> {
+template <int FL>
+struct KQueue<0, FL> {
     // primitives of slots 1, 2 without the density, which is U[slot][0]
     double U[4][5], P[4], Wv[2][4], RF[2];
     __device__ __forceinline__ void put(const CaseDev& c, int s, const double* u) {
         double w[5], rf;
 #pragma unroll
         for (int v = 0; v < 5; ++v) U[s][v] = u[v];
-        derive<double>(U[s], w, P[s], rf, c.gamma, c.gm1);
+        derive<double, FL>(U[s], w, P[s], rf, Gas<FL>(c));
     }
     __device__ __forceinline__ void prim(const CaseDev& c, int s, int d) {   // primitives of slot s into d
         double w[5], p;
-        derive<double>(U[s], w, p, RF[d], c.gamma, c.gm1);
+        derive<double, FL>(U[s], w, p, RF[d], Gas<FL>(c));
 #pragma unroll
         for (int v = 0; v < 4; ++v) Wv[d][v] = w[v + 1];
     }
@@ -216,19 +216,20 @@ struct KQueue<0> {
         const double sr = jst_sensor<double>(P[1], P[2], P[3]);
         const double Wl[5] = {U[1][0], Wv[0][0], Wv[0][1], Wv[0][2], Wv[0][3]};
         const double Wr[5] = {U[2][0], Wv[1][0], Wv[1][1], Wv[1][2], Wv[1][3]};
-        jst_face<FT>(U[0], U[1], U[2], U[3], Wl, Wr, RF[0], RF[1], sl, sr, c.nrm[c.noffK + f], c.nrm[c.noffK + c.nfK + f],
-                     c.nrm[c.noffK + 2 * c.nfK + f], c.nrm[c.noffK + 3 * c.nfK + f], c.gm1, out);
+        jst_face<FT, FL>(U[0], U[1], U[2], U[3], Wl, Wr, RF[0], RF[1], sl, sr, c.nrm[c.noffK + f],
+                         c.nrm[c.noffK + c.nfK + f], c.nrm[c.noffK + 2 * c.nfK + f], c.nrm[c.noffK + 3 * c.nfK + f],
+                         Gas<FL>(c), out);
     }
 };
 
-template <>
-struct KQueue<1> {
+template <int FL>
+struct KQueue<1, FL> {
     double W[4][5];
     __device__ __forceinline__ void put(const CaseDev& c, int s, const double* u) {
         double uu[5], p, rf;
 #pragma unroll
         for (int v = 0; v < 5; ++v) uu[v] = u[v];
-        derive<double>(uu, W[s], p, rf, c.gamma, c.gm1);
+        derive<double, FL>(uu, W[s], p, rf, Gas<FL>(c));
     }
     __device__ __forceinline__ void init_end(const CaseDev&) {}
     __device__ __forceinline__ void shift() {
@@ -244,8 +245,8 @@ struct KQueue<1> {
     }
     template <typename FT>
     __device__ __forceinline__ void face(const CaseDev& c, int f, double out[5]) const {
-        roe_face<FT>(W[0], W[1], W[2], W[3], c.muscl, c.limiter, c.nrm[c.noffK + f], c.nrm[c.noffK + c.nfK + f],
-                     c.nrm[c.noffK + 2 * c.nfK + f], c.nrm[c.noffK + 3 * c.nfK + f], c.gamma, out);
+        roe_face<FT, FL>(W[0], W[1], W[2], W[3], c.muscl, c.limiter, c.nrm[c.noffK + f], c.nrm[c.noffK + c.nfK + f],
+                         c.nrm[c.noffK + 2 * c.nfK + f], c.nrm[c.noffK + 3 * c.nfK + f], Gas<FL>(c), out);
     }
 };
 
@@ -304,7 +305,7 @@ struct StageOps {
     int last, rmode;
 };
 
-template <class TL, int SCH, bool THREED, typename FT>
+template <class TL, int SCH, bool THREED, typename FT, int FL>
 __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, const Work w, const StageOps st) {
     constexpr int TI = TL::TI, TJ = TL::TJ, HI = TL::HI, NP = TL::NP;
     constexpr int NFI = TJ * (TI + 1), NFJ = (TJ + 1) * TI;
@@ -328,7 +329,7 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
     const double* __restrict__ Uin = st.Uin;
 
     // K queue of the column (planes k-1 .. k+2 at the top of step k)
-    KQueue<SCH> q;
+    KQueue<SCH, FL> q;
     double FKl[5] = {0, 0, 0, 0, 0};
     if (THREED && act) {
 #pragma unroll
@@ -394,7 +395,7 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
         } else if (act) {
             double Uc[5], Pc, RFc;
             load5(Uin, N, nid(c, i, j, k), Uc);
-            derive<double>(Uc, Wc, Pc, RFc, c.gamma, c.gm1);
+            derive<double, FL>(Uc, Wc, Pc, RFc, Gas<FL>(c));
 #pragma unroll
             for (int v = 0; v < 5; ++v) L.d.W[v][own] = Wc[v];
             if constexpr (SCH == 0) {
@@ -405,12 +406,12 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
             }
         }
         // tile positions outside the domain (possibly ghosts) and the halo by contiguous lanes
-        if (!act) fill_position<TL, SCH>(c, Uin, L, own, i0, j0, k);
+        if (!act) fill_position<TL, SCH, FL>(c, Uin, L, own, i0, j0, k);
         if (hown) {
-            commit_position<TL, SCH>(c, L, hpos, hok, hU);
+            commit_position<TL, SCH, FL>(c, L, hpos, hok, hU);
             if (k + 1 < kc1) hok = fetch_position<TL>(c, Uin, hpos, i0, j0, k + 1, hU);
         }
-        for (int h = tid + NT; h < NP - NT; h += NT) fill_position<TL, SCH>(c, Uin, L, halo_pos<TL>(h), i0, j0, k);
+        for (int h = tid + NT; h < NP - NT; h += NT) fill_position<TL, SCH, FL>(c, Uin, L, halo_pos<TL>(h), i0, j0, k);
         __syncthreads();
         // ---- every I and J face of the tile once: fq < NFI are I faces (row r, face m),
         // then J faces (row r of faces, column m); one code path, no divergence
@@ -428,7 +429,7 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
                 double nn[4];
                 face_normal(c, f, k, nn);
                 const int l = f.isI ? (f.r + 2) * HI + (f.m + 1) : (f.r + 1) * HI + (f.m + 2);
-                lds_face<TL, SCH, FT>(c, L, l, f.isI ? 1 : HI, nn[0], nn[1], nn[2], nn[3], F);
+                lds_face<TL, SCH, FT, FL>(c, L, l, f.isI ? 1 : HI, nn[0], nn[1], nn[2], nn[3], F);
             }
             double* out = f.isI ? L.f.FI : L.f.FJ;
             const int base = f.isI ? f.r * (TI + 1) + f.m : f.r * TI + f.m;
@@ -513,7 +514,7 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
 // Work: one entry per (i, j) tile and k chunk of a case.  The case and work tables are
 // also passed as restrict kernel parameters (read-only for the launch), which lets the
 // compiler keep case fields in scalar registers or reload them freely.
-template <class TL, int SCH, bool THREED, typename FT>
+template <class TL, int SCH, bool THREED, typename FT, int FL>
 __global__ void __launch_bounds__(NT) k_stage(const CaseDev* __restrict__ cases, const Work* __restrict__ work,
                                               Launch La) {
     const Work w = work[blockIdx.x];
@@ -527,7 +528,7 @@ __global__ void __launch_bounds__(NT) k_stage(const CaseDev* __restrict__ cases,
     st.alpha = cp->alpha[La.stage];
     st.last = La.last;
     st.rmode = La.rmode;
-    stage_block<TL, SCH, THREED, FT>(cp, w, st);
+    stage_block<TL, SCH, THREED, FT, FL>(cp, w, st);
 }
 
 // Periodic closure of the k = 0 / k = NK-1 node-mates: residual averaging

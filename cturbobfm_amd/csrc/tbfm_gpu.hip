@@ -68,7 +68,7 @@ struct Table {
 // Stage-kernel instantiation shared by the cases of one group.
 struct StageGroup {
     int tile = 0, sch = 0;
-    bool three = false, mixed = false;
+    bool three = false, mixed = false, real = false;
     Table t;
 };
 
@@ -84,6 +84,7 @@ struct Plan {
     Table dt, mn, prof, bnd, per, norm, start, pre;
     std::vector<StageGroup> groups;
     int nstages = 0;
+    int fluids = 0;                   // bit 0: an ideal-gas member, bit 1: a table-fluid member
     hipGraphExec_t graph = nullptr;
     bool graphValid = false;
     bool tablesValid = false;
@@ -97,6 +98,9 @@ struct tbfm_case {
     Topology topo = Topology::TwoD;
     Scheme scheme = Scheme::JST;
     Precision precision = Precision::FP64;
+    FluidModel fluid = FluidModel::Ideal;
+    LutSet lut{};                     // host copy of the fluid tables (FLUID_MODEL = real)
+    std::vector<double> lutValues;    // their values, all tables back to back
     std::vector<double> rk;
     Geometry geo;
     std::vector<Patch> patches;
@@ -209,9 +213,55 @@ void setup(tbfm_case* cs, const GridInput& grid) {
     c.scheme = (int)cs->scheme;
     c.muscl = cfg.getMUSCLreconstruction() ? 1 : 0;
     c.limiter = (int)cfg.getFluxLimiter();
-    c.gamma = cfg.getFluidGamma();
+    cs->fluid = cfg.getFluidModel();
+    c.real = 0;
+    c.lut = nullptr;
+    if (cs->fluid == FluidModel::Real) {
+        // Solver::buildFluidModel (solver.cpp:49-74); no table generation here: the
+        // table file has to exist
+        if (cs->precision == Precision::Mixed)
+            throw std::runtime_error("GPU_PRECISION = mixed needs FLUID_MODEL = ideal: the look-up-table fluid runs in FP64");
+        std::string path = cfg.getFluidTableFile();
+        if (!path.empty() && path[0] != '/') path = cfg.baseDir + "/" + path;
+        const FluidTable ft = readFluidTable(path);
+        static_assert(FluidTable::kTables == LUT_COUNT, "table count");
+        size_t total = 0;
+        for (int t = 0; t < LUT_COUNT; ++t) total += ft.t[t].v.size();
+        cs->lutValues.resize(total);
+        size_t off[LUT_COUNT];
+        size_t o = 0;
+        for (int t = 0; t < LUT_COUNT; ++t) {
+            const FluidTable::Tab& F = ft.t[t];
+            LutTab& T = cs->lut.t[t];
+            T.nx = (int)F.nx; T.ny = (int)F.ny;
+            T.xmin = F.xmin; T.xmax = F.xmax; T.ymin = F.ymin; T.ymax = F.ymax;
+            // Table2D::init (fluid_real.hpp:22-32)
+            T.dx = (F.nx > 1) ? (F.xmax - F.xmin) / (double)(F.nx - 1) : 1.0;
+            T.dy = (F.ny > 1) ? (F.ymax - F.ymin) / (double)(F.ny - 1) : 1.0;
+            std::copy(F.v.begin(), F.v.end(), cs->lutValues.begin() + o);
+            off[t] = o;
+            o += F.v.size();
+        }
+        double* dv = upload(cs->lutValues, cs->owned);
+        LutSet dset = cs->lut;
+        for (int t = 0; t < LUT_COUNT; ++t) {
+            cs->lut.t[t].d = cs->lutValues.data() + off[t];
+            dset.t[t].d = dv + off[t];
+        }
+        c.lut = upload(std::vector<LutSet>{dset}, cs->owned);
+        c.real = 1;
+        auto hdr = [&](const char* k, double def) {
+            auto it = ft.header.find(k);
+            return it == ft.header.end() ? def : std::stod(it->second);
+        };
+        // gamma and R only feed ideal-gas formulas, which this fluid never reaches
+        c.gamma = cfg.has("FLUID_GAMMA") ? cfg.getFluidGamma() : hdr("NOMINAL_GAMMA", 1.4);
+        c.rgas = cfg.has("FLUID_R_CONSTANT") ? cfg.getFluidGasConstant() : hdr("GAS_CONSTANT", 287.0);
+    } else {
+        c.gamma = cfg.getFluidGamma();
+        c.rgas = cfg.getFluidGasConstant();
+    }
     c.gm1 = c.gamma - 1;
-    c.rgas = cfg.getFluidGasConstant();
     c.cp = c.gamma * c.rgas / (c.gamma - 1.0);
     c.cfl = cfg.getCFL();
     c.dt_mode = (int)cfg.getTimeStepMethod();
@@ -582,13 +632,25 @@ void setup(tbfm_case* cs, const GridInput& grid) {
         std::vector<double> dir = cfg.getInitDirection();
         const double gm = c.gamma, Rg = c.rgas, T = cfg.getInitTemperature(), p = cfg.getInitPressure();
         double dm = std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
-        double a = std::sqrt(gm * Rg * T);
         double M = cfg.getInitMachNumber();
-        double vx = dir[0] / dm * a * M, vy = dir[1] / dm * a * M, vz = dir[2] / dm * a * M;
-        double rho = p / (Rg * T);
-        double e = p / ((gm - 1) * rho);
-        double vm = std::sqrt(vx * vx + vy * vy + vz * vz);
-        double et = e + 0.5 * (vm * vm);
+        double rho, vx, vy, vz, et;
+        if (c.real) {
+            // FluidReal::computeInitFields (src/fluid_real.cpp:375-393)
+            rho = lut_at(cs->lut, LUT_RHO_PT, p, T);
+            double e = lut_at(cs->lut, LUT_E_PT, p, T);
+            double a = lut_at(cs->lut, LUT_A, rho, e);
+            double dx = 1.0, dy = 0.0, dz = 0.0;
+            if (dm > 1e-12) { dx = dir[0] / dm; dy = dir[1] / dm; dz = dir[2] / dm; }
+            vx = dx * a * M; vy = dy * a * M; vz = dz * a * M;
+            et = e + 0.5 * (vx * vx + vy * vy + vz * vz);
+        } else {
+            double a = std::sqrt(gm * Rg * T);
+            vx = dir[0] / dm * a * M; vy = dir[1] / dm * a * M; vz = dir[2] / dm * a * M;
+            rho = p / (Rg * T);
+            double e = p / ((gm - 1) * rho);
+            double vm = std::sqrt(vx * vx + vy * vy + vz * vz);
+            et = e + 0.5 * (vm * vm);
+        }
         for (long n = 0; n < N; ++n) {
             U0[n] = rho; U0[N + n] = rho * vx; U0[2 * N + n] = rho * vy; U0[3 * N + n] = rho * vz; U0[4 * N + n] = rho * et;
         }
@@ -694,6 +756,7 @@ void buildTables(Plan& P) {
     std::vector<dev::Work> dt, mn, prof, bnd, per, norm, src;
     for (auto& g : P.groups) freeTable(g.t);
     P.groups.clear();
+    P.fluids = 0;
     // Stage-kernel blocks, XCD-aware: the GPU hands workgroup b to XCD b % kXcds, and each
     // XCD has its own L2.  Every case's blocks are cut into kXcds contiguous k-major pieces
     // (gl[group][xcd]); the table interleaves them so that each XCD walks one compact
@@ -716,14 +779,16 @@ void buildTables(Plan& P) {
         norm.push_back({m, 0, 0, 0});
         const int sch = (cs->scheme == Scheme::JST) ? 0 : 1;
         const bool three = (cs->topo == Topology::ThreeD), mixed = (cs->precision == Precision::Mixed);
+        const bool real = (c.real != 0);
+        P.fluids |= real ? 2 : 1;
         size_t gi = 0;
         for (; gi < P.groups.size(); ++gi) {
             const StageGroup& g = P.groups[gi];
-            if (g.tile == cs->tile && g.sch == sch && g.three == three && g.mixed == mixed) break;
+            if (g.tile == cs->tile && g.sch == sch && g.three == three && g.mixed == mixed && g.real == real) break;
         }
         if (gi == P.groups.size()) {
             StageGroup g;
-            g.tile = cs->tile; g.sch = sch; g.three = three; g.mixed = mixed;
+            g.tile = cs->tile; g.sch = sch; g.three = three; g.mixed = mixed; g.real = real;
             P.groups.push_back(g);
             gl.emplace_back(kXcds);
         }
@@ -772,10 +837,12 @@ template <class TL, int SCH, bool THREED>
 void launchStageT(const Plan& P, const StageGroup& g, const dev::Launch& L) {
     const CaseDev* cs = L.cases;
     const dev::Work* wk = L.work;
-    if (g.mixed)
-        hipLaunchKernelGGL((dev::k_stage<TL, SCH, THREED, float>), dim3((unsigned)g.t.n), dim3(dev::NT), 0, P.stream, cs, wk, L);
+    if (g.real)   // the look-up-table fluid: FP64 throughout
+        hipLaunchKernelGGL((dev::k_stage<TL, SCH, THREED, double, 1>), dim3((unsigned)g.t.n), dim3(dev::NT), 0, P.stream, cs, wk, L);
+    else if (g.mixed)
+        hipLaunchKernelGGL((dev::k_stage<TL, SCH, THREED, float, 0>), dim3((unsigned)g.t.n), dim3(dev::NT), 0, P.stream, cs, wk, L);
     else
-        hipLaunchKernelGGL((dev::k_stage<TL, SCH, THREED, double>), dim3((unsigned)g.t.n), dim3(dev::NT), 0, P.stream, cs, wk, L);
+        hipLaunchKernelGGL((dev::k_stage<TL, SCH, THREED, double, 0>), dim3((unsigned)g.t.n), dim3(dev::NT), 0, P.stream, cs, wk, L);
 }
 
 template <int SCH, bool THREED>
@@ -811,7 +878,10 @@ void launchProfiles(const Plan& P, int in) { launchStart(P, P.prof, in, false); 
 
 // k_prestage over a table of boundary and / or source blocks.
 void launchPre(const Plan& P, const Table& t, int in, int out, int stage, int last, int rmode) {
-    if (t.n) hipLaunchKernelGGL(dev::k_prestage, dim3((unsigned)t.n), dim3(256), 0, P.stream, launchArg(P, t, in, out, stage, last, rmode));
+    if (!t.n) return;
+    const dev::Launch L = launchArg(P, t, in, out, stage, last, rmode);
+    if (P.fluids & 1) hipLaunchKernelGGL(dev::k_prestage<0>, dim3((unsigned)t.n), dim3(256), 0, P.stream, L);
+    if (P.fluids & 2) hipLaunchKernelGGL(dev::k_prestage<1>, dim3((unsigned)t.n), dim3(256), 0, P.stream, L);
 }
 void launchBoundary(const Plan& P, int in) { launchPre(P, P.bnd, in, 0, 0, 0, 0); }
 
@@ -1070,9 +1140,13 @@ int tbfm_gpu_set_state(tbfm_case* c, const double* U) {
                 long n = ((long)c->c.prof_i[q] * c->nj + c->c.prof_j0[q] + m) * c->nk + 0;
                 double rho = U[n], W1 = U[c->N + n] / rho, W2 = U[2 * c->N + n] / rho, W3 = U[3 * c->N + n] / rho,
                        W4 = U[4 * c->N + n] / rho;
-                double vm = std::sqrt(W1 * W1 + W2 * W2 + W3 * W3);
-                double e = W4 - 0.5 * (vm * vm);
-                prof[q * c->nj + m] = (g - 1) * rho * e;
+                if (c->c.real) {
+                    prof[q * c->nj + m] = lut_at(c->lut, LUT_P, rho, W4 - 0.5 * (W1 * W1 + W2 * W2 + W3 * W3));
+                } else {
+                    double vm = std::sqrt(W1 * W1 + W2 * W2 + W3 * W3);
+                    double e = W4 - 0.5 * (vm * vm);
+                    prof[q * c->nj + m] = (g - 1) * rho * e;
+                }
             }
         HIPCHK(hipMemcpyAsync(c->c.prof_p, prof.data(), prof.size() * 8, hipMemcpyHostToDevice, st));
         HIPCHK(hipStreamSynchronize(st));
@@ -1288,18 +1362,34 @@ int tbfm_gpu_get_turbo(tbfm_case* c, double* out) {
         if (c->topo == Topology::Axisymmetric) m *= 2.0 * M_PI / g.wedgeAngle;
         else if (perDeg != 0.0) m *= 360.0 / perDeg;
         out[6] = m;
-        double ptm[2], ttm[2];
+        const bool real = (c->c.real != 0);
+        double ptm[2], ttm[2], htm[2] = {0.0, 0.0}, sm[2] = {0.0, 0.0};
         for (int s = 0; s < 2; ++s) {
             int b = bi[s];
             long n1, n2;
             extent(b, n1, n2);
-            double sp = 0.0, st = 0.0;
+            double sp = 0.0, st = 0.0, sh = 0.0, ss = 0.0;
             for (long p = 0; p < n1; ++p)
                 for (long q = 0; q < n2; ++q) {
                     const double* S = surf(b, p, q);
                     long n = node(b, p, q);
                     double rho = U[n], ux = U[N + n] / rho, uy = U[2 * N + n] / rho, uz = U[3 * N + n] / rho,
                            et = U[4 * N + n] / rho;
+                    if (real) {
+                        // FluidReal total pressure / temperature (fluid_real.cpp:311-341),
+                        // h_t = e_t + p / rho and s(rho, e) of the same state
+                        const double qq = ux * ux + uy * uy + uz * uz;
+                        const double e = et - 0.5 * qq;
+                        const double pt = lut_total_pressure(c->lut, rho, qq, et);
+                        const double Tt = lut_total_temperature(c->lut, rho, qq, et);
+                        const double ht = et + lut_at(c->lut, LUT_P, rho, e) / rho;
+                        const double sv = lut_at(c->lut, LUT_S, rho, e);
+                        sp += S[0] * (rho * ux * pt) + S[1] * (rho * uy * pt) + S[2] * (rho * uz * pt);
+                        st += S[0] * (rho * ux * Tt) + S[1] * (rho * uy * Tt) + S[2] * (rho * uz * Tt);
+                        sh += S[0] * (rho * ux * ht) + S[1] * (rho * uy * ht) + S[2] * (rho * uz * ht);
+                        ss += S[0] * (rho * ux * sv) + S[1] * (rho * uy * sv) + S[2] * (rho * uz * sv);
+                        continue;
+                    }
                     double um = std::sqrt(ux * ux + uy * uy + uz * uz);
                     double e = et - 0.5 * (um * um);
                     double pr = (gm - 1) * rho * e;
@@ -1315,15 +1405,30 @@ int tbfm_gpu_get_turbo(tbfm_case* c, double* out) {
             if (std::abs(mfl) < 1e-12) mfl = (mfl >= 0.0 ? 1e-12 : -1e-12);
             ptm[s] = sp / mfl;
             ttm[s] = st / mfl;
+            htm[s] = sh / mfl;
+            sm[s] = ss / mfl;
         }
         double PR = (std::abs(ptm[0]) > 1e-12) ? ptm[1] / ptm[0] : 1.0;
         double TR = (std::abs(ttm[0]) > 1e-12) ? ttm[1] / ttm[0] : 1.0;
         out[7] = PR;
         out[8] = TR;
-        double dT = TR - 1.0, dTs = std::pow(PR, (gm - 1.0) / gm) - 1.0, eta;
-        if (std::abs(dT) < 1e-12) eta = 1.0;
-        else if (dT >= 0.0) eta = dTs / dT;
-        else eta = (std::abs(dTs) < 1e-12) ? 1.0 : dT / dTs;
+        double eta;
+        if (real) {
+            // enthalpy-based total-to-total efficiency (solver.cpp:1766-1776,
+            // FluidBase::computeTotalEfficiency_h) with the isentropic outlet enthalpy
+            // h(p_t,out, s_in)
+            const double ht_in = htm[0], ht_out = htm[1];
+            const double ht_out_s = lut_h_p_s(c->lut, ptm[1], sm[0]);
+            const double dh = ht_out - ht_in, dhs = ht_out_s - ht_in;
+            if (std::abs(dh) < 1e-12) eta = 1.0;
+            else if (dh >= 0.0) eta = dhs / dh;
+            else eta = (std::abs(dhs) < 1e-12) ? 1.0 : dh / dhs;
+        } else {
+            double dT = TR - 1.0, dTs = std::pow(PR, (gm - 1.0) / gm) - 1.0;
+            if (std::abs(dT) < 1e-12) eta = 1.0;
+            else if (dT >= 0.0) eta = dTs / dT;
+            else eta = (std::abs(dTs) < 1e-12) ? 1.0 : dT / dTs;
+        }
         out[9] = eta;
         return TBFM_OK;
     });

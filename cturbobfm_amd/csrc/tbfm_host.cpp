@@ -174,6 +174,97 @@ Precision Config::getGpuPrecision() const {
     throw std::runtime_error("Invalid value for key \"GPU_PRECISION\" (fp64 | mixed)");
 }
 
+// Config::getFluidModel / getFluidName / getFluidTableFile (src/config.cpp:468-497)
+FluidModel Config::getFluidModel() const {
+    if (!has("FLUID_MODEL")) return FluidModel::Ideal;
+    std::string v = get("FLUID_MODEL");
+    if (v == "None" || v == "Ideal" || v == "ideal" || v == "IDEAL") return FluidModel::Ideal;
+    if (v == "real" || v == "Real" || v == "REAL") return FluidModel::Real;
+    throw std::runtime_error("Invalid value for key \"FLUID_MODEL\" in configuration.");
+}
+std::string Config::getFluidName() const {
+    if (has("FLUID_NAME")) return get("FLUID_NAME");
+    if (has("FLUID")) return get("FLUID");
+    return "CO2";
+}
+std::string Config::getFluidTableFile() const {
+    if (has("FLUID_TABLE_FILE")) return get("FLUID_TABLE_FILE");
+    return "fluid_table_" + getFluidName() + ".lut";
+}
+
+FluidTable readFluidTable(const std::string& path) {
+    std::ifstream f(path);
+    if (!f.is_open()) throw std::runtime_error("FluidReal: Could not open table file at: " + path);
+    FluidTable out;
+    // section name, its six bound keys (the last one starts the rows), fields, first table
+    struct Sec {
+        const char* name;
+        const char* keys[6];
+        int nf, base;
+    };
+    static const Sec secs[3] = {
+        {"RHO_E", {"N_RHO", "N_E", "RHO_MIN", "RHO_MAX", "E_MIN", "E_MAX"}, 8, 0},
+        {"P_T", {"N_P", "N_T", "P_MIN", "P_MAX", "T_MIN", "T_MAX"}, 4, 8},
+        {"P_S", {"N_P", "N_S", "P_MIN", "P_MAX", "S_MIN", "S_MAX"}, 3, 12},
+    };
+    const Sec* cur = nullptr;
+    std::map<std::string, std::string> meta;
+    auto trimKey = [](std::string k) { k.erase(k.find_last_not_of(" \t\r\n") + 1); return k; };
+    auto trimVal = [](std::string v) {
+        size_t a = v.find_first_not_of(" \t\r\n");
+        return a == std::string::npos ? std::string() : v.substr(a);
+    };
+    std::string line;
+    while (std::getline(f, line)) {
+        if (line.empty() || line[0] == '\r') continue;
+        bool marker = false;
+        for (const Sec& s : secs)
+            if (line.find(std::string("# SECTION: ") + s.name) != std::string::npos) {
+                cur = &s;
+                meta.clear();
+                marker = true;
+                break;
+            }
+        if (marker || line[0] == '#') continue;
+        const size_t colon = line.find(':');
+        if (colon == std::string::npos) continue;
+        const std::string key = trimKey(line.substr(0, colon)), val = trimVal(line.substr(colon + 1));
+        if (!cur) {
+            out.header[key] = val;
+            continue;
+        }
+        meta[key] = val;
+        if (key != cur->keys[5]) continue;
+        const long nx = std::stol(meta.at(cur->keys[0])), ny = std::stol(meta.at(cur->keys[1]));
+        // bilinear interpolation reads the cell (i, j)..(i+1, j+1): two nodes per direction
+        if (nx < 2 || ny < 2)
+            throw std::runtime_error("FluidReal: the " + std::string(cur->name) + " table in " + path +
+                                     " needs at least 2 x 2 nodes");
+        for (int q = 0; q < cur->nf; ++q) {
+            FluidTable::Tab& T = out.t[cur->base + q];
+            T.nx = nx; T.ny = ny;
+            T.xmin = std::stod(meta.at(cur->keys[2])); T.xmax = std::stod(meta.at(cur->keys[3]));
+            T.ymin = std::stod(meta.at(cur->keys[4])); T.ymax = std::stod(val);
+            T.v.assign((size_t)(nx * ny), 0.0);
+        }
+        // nx * ny rows; empty, '#' and '\r' lines in between are skipped
+        for (long r = 0; r < nx * ny; ++r) {
+            if (!std::getline(f, line)) break;
+            while (line.empty() || line[0] == '#' || line[0] == '\r')
+                if (!std::getline(f, line)) break;
+            std::istringstream iss(line);
+            for (int q = 0; q < cur->nf; ++q) {
+                double x = 0.0;
+                iss >> x;
+                out.t[cur->base + q].v[(size_t)r] = x;
+            }
+        }
+    }
+    for (int t = 0; t < FluidTable::kTables; ++t)
+        if (out.t[t].v.empty()) throw std::runtime_error("FluidReal: table file " + path + " lacks a section");
+    return out;
+}
+
 PatchType patchTypeFromString(const std::string& s) {
     std::string k = lower(s);
     if (k == "inlet") return PatchType::Inlet;

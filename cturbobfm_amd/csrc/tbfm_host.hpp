@@ -25,6 +25,7 @@ enum class TimeStepMethod { Local = 0, Global = 1, Fixed = 2 };
 enum class BfmModel { None = 0, Hall = 1, LiftDrag = 2, Blockage = 3 };
 enum class PatchType { Inlet = 0, Outlet = 1, RadialEquilibrium = 2, Wall = 3, Periodic = 4, Wedge = 5 };
 enum class Precision { FP64 = 0, Mixed = 1 };
+enum class FluidModel { Ideal = 0, Real = 1 };
 
 class Config {
 public:
@@ -47,6 +48,9 @@ public:
     TimeStepMethod getTimeStepMethod() const;
     double getCFL() const;
     double getFixedTimeStep() const { return getFloat("FIXED_TIME_STEP", 0.0); }
+    FluidModel getFluidModel() const;
+    std::string getFluidName() const;
+    std::string getFluidTableFile() const;
     double getFluidGamma() const { return getFloat("FLUID_GAMMA"); }
     double getFluidGasConstant() const { return getFloat("FLUID_R_CONSTANT"); }
     double getInitMachNumber() const { return getFloat("INIT_MACH_NUMBER"); }
@@ -94,6 +98,22 @@ struct GridInput {
 };
 
 GridInput readGridCsv(const std::string& path);
+
+// A fluid look-up table file (src/fluid_real.cpp:12-209): header KEY: value lines, then
+// the sections RHO_E (p T a s mu kappa dp/drho dp/de over rho, e), P_T (rho e a s over
+// p, T) and P_S (rho T e over p, s), each with its node counts and bounds.  The fifteen
+// tables are stored in that order, value (i, j) of a table at v[i * ny + j].
+struct FluidTable {
+    struct Tab {
+        long nx = 0, ny = 0;
+        double xmin = 0, xmax = 0, ymin = 0, ymax = 0;
+        std::vector<double> v;
+    };
+    static constexpr int kTables = 15;
+    std::map<std::string, std::string> header;
+    Tab t[kTables];
+};
+FluidTable readFluidTable(const std::string& path);
 std::vector<Patch> readBoundaryCsv(const std::string& path, const Config& cfg);
 PatchType patchTypeFromString(const std::string& s);
 

@@ -11,9 +11,12 @@
 // that FP64 results match the reference CPU build bit for bit wherever no libm
 // transcendental is involved.  The flux type FT is double (plain FP64 path) or float
 // (mixed precision: flux, dissipation and source arithmetic in FP32, state
-// differences, RK accumulation, norms and convergence in FP64).
+// differences, RK accumulation, norms and convergence in FP64).  The fluid is the ideal
+// gas or the look-up-table fluid (FLUID_MODEL = real, FP64 only).
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include "tbfm_fluid.hpp"
 
 namespace tbfm {
 namespace dev {
@@ -54,6 +57,8 @@ struct CaseDev {
     int topo, scheme, muscl, limiter;
     int periodic;
     double gamma, gm1, rgas, cp;
+    int real;                     // FLUID_MODEL = real: thermodynamics from the tables in lut
+    const LutSet* lut;            // device copy (device pointers), null for the ideal gas
     double per_c, per_s;          // cos/sin of +angle
     double per_cm, per_sm;        // cos/sin of -angle
     double cfl, fixed_dt;

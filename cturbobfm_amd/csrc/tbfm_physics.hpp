@@ -15,53 +15,107 @@ __device__ __forceinline__ T static_energy(T u, T v, T w, T et) {
     return et - T(0.5) * (m * m);
 }
 
-// W = (rho, rhoU/rho, rhoV/rho, rhoW/rho, rhoE/rho) (src/math_utils.cpp:49-57), then
-// p = (g-1) rho e (fluid_ideal.cpp:22-24) and the JST spectral factor |u| + a with
-// a = sqrt(g p / rho) (advection_jst.cpp:40-45, fluid_ideal.cpp:83-87)
-template <typename T>
-__device__ __forceinline__ void derive(const T U[5], T W[5], T& p, T& rf, T g, T gm1) {
+// Fluid of a kernel instance: FL = 0 the ideal gas of constant gamma
+// (src/fluid_ideal.cpp), FL = 1 the look-up-table fluid (src/fluid_real.cpp), FP64 only.
+template <int FL>
+struct Gas;
+template <>
+struct Gas<0> {
+    double g, gm1;
+    __device__ __forceinline__ explicit Gas(const CaseDev& c) : g(c.gamma), gm1(c.gm1) {}
+};
+template <>
+struct Gas<1> {
+    const LutSet* L;
+    __device__ __forceinline__ explicit Gas(const CaseDev& c) : L(c.lut) {}
+    __device__ __forceinline__ double at(int t, double x, double y) const { return lut_interp(L->t[t], x, y); }
+};
+
+// Static pressure of primitive state W.  Ideal: (g-1) rho (et - 0.5 pow(|u|,2))
+// (fluid_ideal.cpp:22-24, 70-77); real: p(rho, et - 0.5 u.u) (FluidBase composition,
+// fluid_real.cpp:211-213).
+template <typename T, int FL>
+__device__ __forceinline__ T pressure_w(const Gas<FL>& gs, const T W[5]) {
+    if constexpr (FL == 0) {
+        T e = static_energy(W[1], W[2], W[3], W[4]);
+        return T(gs.gm1) * W[0] * e;
+    } else {
+        static_assert(sizeof(T) == 8, "the look-up-table fluid runs in FP64");
+        double e = W[4] - 0.5 * (W[1] * W[1] + W[2] * W[2] + W[3] * W[3]);
+        return gs.at(LUT_P, W[0], e);
+    }
+}
+
+// Speed of sound from rho, u, et (computeSoundSpeed_rho_u_et): ideal sqrt(g p / rho) with
+// the pressure p of the same state (fluid_ideal.cpp:83-87); real a(rho, et - 0.5 u.u).
+template <int FL>
+__device__ __forceinline__ double sound_speed_w(const Gas<FL>& gs, const double W[5], double p) {
+    if constexpr (FL == 0) return sqrt(gs.g * p / W[0]);
+    else return gs.at(LUT_A, W[0], W[4] - 0.5 * (W[1] * W[1] + W[2] * W[2] + W[3] * W[3]));
+}
+
+// Speed of sound from p and rho (computeSoundSpeed_p_rho): ideal sqrt(g p / rho); real
+// a(rho, e(p, rho)) with the bisection for e (fluid_real.cpp:254-257).
+template <int FL>
+__device__ __forceinline__ double sound_speed_p_rho(const Gas<FL>& gs, double p, double rho) {
+    if constexpr (FL == 0) return sqrt(gs.g * p / rho);
+    else return gs.at(LUT_A, rho, lut_e_p_rho(*gs.L, p, rho));
+}
+
+// W = (rho, rhoU/rho, rhoV/rho, rhoW/rho, rhoE/rho) (src/math_utils.cpp:49-57), the
+// static pressure p and the JST spectral factor |u| + a (advection_jst.cpp:40-45).
+// Ideal: p = (g-1) rho e, a = sqrt(g p / rho) (fluid_ideal.cpp:22-24, 83-87); real:
+// p(rho, e), a(rho, e) with e = et - 0.5 u.u.
+template <typename T, int FL>
+__device__ __forceinline__ void derive(const T U[5], T W[5], T& p, T& rf, const Gas<FL>& gs) {
     W[0] = U[0];
     W[1] = U[1] / U[0];
     W[2] = U[2] / U[0];
     W[3] = U[3] / U[0];
     W[4] = U[4] / U[0];
-    T m = sqrt(W[1] * W[1] + W[2] * W[2] + W[3] * W[3]);
-    T e = W[4] - T(0.5) * (m * m);
-    p = gm1 * W[0] * e;
-    T a = sqrt(g * p / W[0]);
-    rf = m + a;
+    if constexpr (FL == 0) {
+        T m = sqrt(W[1] * W[1] + W[2] * W[2] + W[3] * W[3]);
+        T e = W[4] - T(0.5) * (m * m);
+        p = T(gs.gm1) * W[0] * e;
+        T a = sqrt(T(gs.g) * p / W[0]);
+        rf = m + a;
+    } else {
+        static_assert(sizeof(T) == 8, "the look-up-table fluid runs in FP64");
+        double q = W[1] * W[1] + W[2] * W[2] + W[3] * W[3];
+        double m = sqrt(q);
+        double e = W[4] - 0.5 * q;
+        p = gs.at(LUT_P, W[0], e);
+        rf = m + gs.at(LUT_A, W[0], e);
+    }
 }
 
-// Physical flux through S of primitive state W (src/math_utils.cpp:71-86).  Returns
-// |S| so the caller can scale by it, as the residual assembly does.
+// Physical flux through the unit normal n of primitive state W with static pressure p
+// (src/math_utils.cpp:71-86; h_t = e_t + p / rho, fluid_ideal.cpp:111-115).
 template <typename T>
-__device__ __forceinline__ T phys_flux(const T W[5], T sx, T sy, T sz, T gm1, T F[5]) {
-    T sm = sqrt(sx * sx + sy * sy + sz * sz);
-    T nx = sx / sm, ny = sy / sm, nz = sz / sm;
+__device__ __forceinline__ void phys_flux_pn(const T W[5], T nx, T ny, T nz, T p, T F[5]) {
     T un = W[1] * nx + W[2] * ny + W[3] * nz;
-    T e = static_energy(W[1], W[2], W[3], W[4]);
-    T p = gm1 * W[0] * e;
     T ht = W[4] + p / W[0];
     F[0] = W[0] * un;
     F[1] = W[0] * un * W[1] + p * nx;
     F[2] = W[0] * un * W[2] + p * ny;
     F[3] = W[0] * un * W[3] + p * nz;
     F[4] = W[0] * un * ht;
+}
+
+// Physical flux through S (n = S/|S|).  Returns |S| so the caller can scale by it, as the
+// residual assembly does.
+template <typename T, int FL>
+__device__ __forceinline__ T phys_flux(const T W[5], T sx, T sy, T sz, const Gas<FL>& gs, T F[5]) {
+    T sm = sqrt(sx * sx + sy * sy + sz * sz);
+    T nx = sx / sm, ny = sy / sm, nz = sz / sm;
+    phys_flux_pn<T>(W, nx, ny, nz, pressure_w<T, FL>(gs, W), F);
     return sm;
 }
 
 // Same with the unit normal n = S/|S| supplied (precomputed with identical operations).
-template <typename T>
-__device__ __forceinline__ void phys_flux_n(const T W[5], T nx, T ny, T nz, T gm1, T F[5]) {
-    T un = W[1] * nx + W[2] * ny + W[3] * nz;
-    T e = static_energy(W[1], W[2], W[3], W[4]);
-    T p = gm1 * W[0] * e;
-    T ht = W[4] + p / W[0];
-    F[0] = W[0] * un;
-    F[1] = W[0] * un * W[1] + p * nx;
-    F[2] = W[0] * un * W[2] + p * ny;
-    F[3] = W[0] * un * W[3] + p * nz;
-    F[4] = W[0] * un * ht;
+template <typename T, int FL>
+__device__ __forceinline__ void phys_flux_n(const T W[5], T nx, T ny, T nz, const Gas<FL>& gs, T F[5]) {
+    phys_flux_pn<T>(W, nx, ny, nz, pressure_w<T, FL>(gs, W), F);
 }
 
 // JST pressure sensor |p1 - 2 p2 + p3| / (p1 + 2 p2 + p3)   (advection_jst.cpp:47-57)
@@ -73,11 +127,11 @@ __device__ __forceinline__ T jst_sensor(T p1, T p2, T p3) {
 // JST face flux times |S| (advection_jst.cpp:4-38; k2 = 0.5, k4 = 0.02, c4 = 2,
 // advection_jst.hpp:23).  Conserved differences are formed in double; the flux type FT
 // carries the rest.
-template <typename FT>
+template <typename FT, int FL>
 __device__ __forceinline__ void jst_face(const double Ull[5], const double Ul[5], const double Ur[5],
                                          const double Urr[5], const double Wl[5], const double Wr[5],
                                          double rfl, double rfr, double sl, double sr, double nx, double ny,
-                                         double nz, double area, double gm1, double out[5]) {
+                                         double nz, double area, const Gas<FL>& gs, double out[5]) {
     FT Wavg[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) Wavg[v] = (FT(Wl[v]) + FT(Wr[v])) * FT(0.5);
@@ -87,7 +141,7 @@ __device__ __forceinline__ void jst_face(const double Ull[5], const double Ul[5]
     FT t4 = FT(0.02) * r - FT(2.0) * psi2;
     FT psi4 = (FT(0.0) < t4) ? t4 : FT(0.0);
     FT F[5];
-    phys_flux_n<FT>(Wavg, FT(nx), FT(ny), FT(nz), FT(gm1), F);
+    phys_flux_n<FT, FL>(Wavg, FT(nx), FT(ny), FT(nz), gs, F);
     const FT sm = FT(area);
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
@@ -126,11 +180,10 @@ __device__ __forceinline__ T roe_avg(T rl, T rr, T pl, T pr) {
 
 // Roe flux with Vinokur-Montagne averaging and entropy fix 0.01, times |S|
 // (src/advection_roe.cpp:6-221, include/advection_roe.hpp:63)
-template <typename FT>
+template <typename FT, int FL>
 __device__ __forceinline__ void roe_face(const double Wll_[5], const double Wl_[5], const double Wr_[5],
                                          const double Wrr_[5], int muscl, int lim, double nx_, double ny_,
-                                         double nz_, double area_, double g_, double out[5]) {
-    const FT g = FT(g_), gm1 = FT(g_ - 1.0);
+                                         double nz_, double area_, const Gas<FL>& gs, double out[5]) {
     FT Wl[5], Wr[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) { Wl[v] = FT(Wl_[v]); Wr[v] = FT(Wr_[v]); }
@@ -166,21 +219,34 @@ __device__ __forceinline__ void roe_face(const double Wll_[5], const double Wl_[
     R[1] = Wr[1] * n1x + Wr[2] * n1y + Wr[3] * n1z;
     R[2] = Wr[1] * n2x + Wr[2] * n2y + Wr[3] * n2z;
     R[3] = Wr[1] * n3x + Wr[2] * n3y + Wr[3] * n3z;
-    // Roe averages (:66-144)
+    // Roe averages (:66-144).  The ideal gas forms h_t from the pressure of
+    // FluidIdeal::computeStaticEnergy_u_et (pow(|u|,2)) and p, chi, kappa from e of the
+    // explicit formula; for the table fluid both energies are et - 0.5 u.u.
     const FT rhoL = L[0], rhoR = R[0];
-    FT pLt = gm1 * rhoL * static_energy(L[1], L[2], L[3], L[4]);
-    FT pRt = gm1 * rhoR * static_energy(R[1], R[2], R[3], R[4]);
+    FT eL = L[4] - FT(0.5) * (L[1] * L[1] + L[2] * L[2] + L[3] * L[3]);
+    FT eR = R[4] - FT(0.5) * (R[1] * R[1] + R[2] * R[2] + R[3] * R[3]);
+    FT pLt, pRt, pL, pR, chiL, kapL, chiR, kapR;
+    if constexpr (FL == 0) {
+        const FT g = FT(gs.g), gm1 = FT(gs.gm1);
+        pLt = gm1 * rhoL * static_energy(L[1], L[2], L[3], L[4]);
+        pRt = gm1 * rhoR * static_energy(R[1], R[2], R[3], R[4]);
+        pL = gm1 * rhoL * eL;
+        pR = gm1 * rhoR * eR;
+        chiL = (g - FT(1.0)) * eL; kapL = (g - FT(1.0)) * rhoL;
+        chiR = (g - FT(1.0)) * eR; kapR = (g - FT(1.0)) * rhoR;
+    } else {
+        static_assert(sizeof(FT) == 8, "the look-up-table fluid runs in FP64");
+        pLt = pL = gs.at(LUT_P, rhoL, eL);
+        pRt = pR = gs.at(LUT_P, rhoR, eR);
+        chiL = gs.at(LUT_DPDRHO, rhoL, eL); kapL = gs.at(LUT_DPDE, rhoL, eL);
+        chiR = gs.at(LUT_DPDRHO, rhoR, eR); kapR = gs.at(LUT_DPDE, rhoR, eR);
+    }
     FT htL = L[4] + pLt / rhoL, htR = R[4] + pRt / rhoR;
     FT rhoA = sqrt(rhoL * rhoR);
     FT u1A = roe_avg(rhoL, rhoR, L[1], R[1]);
     FT u2A = roe_avg(rhoL, rhoR, L[2], R[2]);
     FT u3A = roe_avg(rhoL, rhoR, L[3], R[3]);
     FT htA = roe_avg(rhoL, rhoR, htL, htR);
-    FT eL = L[4] - FT(0.5) * (L[1] * L[1] + L[2] * L[2] + L[3] * L[3]);
-    FT eR = R[4] - FT(0.5) * (R[1] * R[1] + R[2] * R[2] + R[3] * R[3]);
-    FT pL = gm1 * rhoL * eL, pR = gm1 * rhoR * eR;
-    FT chiL = (g - FT(1.0)) * eL, kapL = (g - FT(1.0)) * rhoL;
-    FT chiR = (g - FT(1.0)) * eR, kapR = (g - FT(1.0)) * rhoR;
     FT barChi = FT(0.5) * (chiL + chiR), barKap = FT(0.5) * (kapL + kapR);
     FT barRho = FT(0.5) * (rhoL + rhoR), barE = FT(0.5) * (eL + eR);
     FT dP = pR - pL, dRho = rhoR - rhoL, dE = eR - eL;
@@ -201,7 +267,15 @@ __device__ __forceinline__ void roe_face(const double Wll_[5], const double Wl_[
     FT hA = htA - FT(0.5) * q2;
     FT a2 = chiP + kapP * hA;
     if (a2 <= FT(0.0)) {
-        FT aL = sqrt(g * (g - FT(1.0)) * eL), aR = sqrt(g * (g - FT(1.0)) * eR);
+        FT aL, aR;
+        if constexpr (FL == 0) {
+            const FT g = FT(gs.g);
+            aL = sqrt(g * (g - FT(1.0)) * eL);
+            aR = sqrt(g * (g - FT(1.0)) * eR);
+        } else {
+            aL = gs.at(LUT_A, rhoL, eL);
+            aR = gs.at(LUT_A, rhoR, eR);
+        }
         a2 = FT(0.5) * (aL * aL + aR * aR);
     }
     FT aA = sqrt(a2);
@@ -223,13 +297,14 @@ __device__ __forceinline__ void roe_face(const double Wll_[5], const double Wl_[
     w[3] = rhoA * dU3;
     w[4] = FT(1.0) / (FT(2.0) * aA * aA) * (dP2 + rhoA * aA * dU1);
     // assembly (:184-221): physical fluxes along (1,0,0) in the rotated frame
-    FT FL[5], FR[5];
+    // (the pressure of either physical flux is pLt / pRt: the same expression)
+    FT FLf[5], FRf[5];
     FT one = FT(1.0), zero = FT(0.0);
-    phys_flux<FT>(L, one, zero, zero, gm1, FL);
-    phys_flux<FT>(R, one, zero, zero, gm1, FR);
+    phys_flux_pn<FT>(L, one, zero, zero, pLt, FLf);
+    phys_flux_pn<FT>(R, one, zero, zero, pRt, FRf);
     FT Fr[5];
 #pragma unroll
-    for (int i = 0; i < 5; ++i) Fr[i] = (FL[i] + FR[i]) * FT(0.5);
+    for (int i = 0; i < 5; ++i) Fr[i] = (FLf[i] + FRf[i]) * FT(0.5);
 #pragma unroll
     for (int i = 0; i < 5; ++i)
 #pragma unroll
@@ -251,15 +326,16 @@ __device__ __forceinline__ void roe_face(const double Wll_[5], const double Wl_[
 }
 
 // -------------------------------------------------------------- boundary fluxes
-// Subsonic inlet, ideal gas (src/boundary_base.cpp:7-124)
-__device__ inline void inlet_flux(const CaseDev& c, const double U[5], double sx, double sy, double sz,
-                                  double pt, double Tt, double dx, double dy, double dz, double F[5]) {
-    const double g = c.gamma, Rg = c.rgas;
+// Subsonic inlet (src/boundary_base.cpp:7-124): the outgoing Riemann invariant of the
+// interior state and a bisection on the boundary pressure along the inlet total state,
+// isentropic for the ideal gas, along the entropy s(p0, T0) for the table fluid.
+template <int FL>
+__device__ inline void inlet_flux(const CaseDev& c, const Gas<FL>& gs, const double U[5], double sx, double sy,
+                                  double sz, double pt, double Tt, double dx, double dy, double dz, double F[5]) {
     double W[5], p, rf;
-    derive<double>(U, W, p, rf, c.gamma, c.gm1);
-    double aInt = rf - sqrt(W[1] * W[1] + W[2] * W[2] + W[3] * W[3]);
-    // sound speed recomputed exactly as computeSoundSpeed_rho_u_et
-    aInt = sqrt(g * p / W[0]);
+    derive<double, FL>(U, W, p, rf, gs);
+    // computeSoundSpeed_rho_u_et
+    const double aInt = sound_speed_w<FL>(gs, W, p);
     double sm = sqrt(sx * sx + sy * sy + sz * sz);
     double nx = sx / sm, ny = sy / sm, nz = sz / sm;
     double unInt = W[1] * nx + W[2] * ny + W[3] * nz;
@@ -267,61 +343,99 @@ __device__ inline void inlet_flux(const CaseDev& c, const double U[5], double sx
     double Cg = dx * nx + dy * ny + dz * nz;
     if (Cg >= 0.0) Cg = -1.0;
     double Rout = p + Z * unInt;
-    double cp = g * Rg / (g - 1.0);
-    double htIn = cp * Tt;
     double pLow = 0.05 * pt, pHigh = 0.999999 * pt;
     double pB = 0.5 * (pLow + pHigh);
-    const double ex = (g - 1.0) / g;
-    for (int it = 0; it < 30; ++it) {
-        double pMid = 0.5 * (pLow + pHigh);
-        double TMid = Tt * pow(pMid / pt, ex);
-        double hMid = cp * TMid;
-        double VMid = sqrt(2.0 * dmaxd(0.0, htIn - hMid));
-        double res = pMid + Z * Cg * VMid - Rout;
-        if (fabs(res) < 1e-4 * pt) { pB = pMid; break; }
-        if (res < 0.0) pLow = pMid; else pHigh = pMid;
-        pB = pMid;
+    double Wb[5];
+    if constexpr (FL == 0) {
+        const double g = c.gamma, Rg = c.rgas;
+        double cp = g * Rg / (g - 1.0);
+        double htIn = cp * Tt;
+        const double ex = (g - 1.0) / g;
+        for (int it = 0; it < 30; ++it) {
+            double pMid = 0.5 * (pLow + pHigh);
+            double TMid = Tt * pow(pMid / pt, ex);
+            double hMid = cp * TMid;
+            double VMid = sqrt(2.0 * dmaxd(0.0, htIn - hMid));
+            double res = pMid + Z * Cg * VMid - Rout;
+            if (fabs(res) < 1e-4 * pt) { pB = pMid; break; }
+            if (res < 0.0) pLow = pMid; else pHigh = pMid;
+            pB = pMid;
+        }
+        double TB = Tt * pow(pB / pt, ex);
+        double hB = cp * TB;
+        double VB = sqrt(2.0 * dmaxd(0.0, htIn - hB));
+        double vx = dx * VB, vy = dy * VB, vz = dz * VB;
+        double rhoB = pB / (Rg * TB);
+        double eB = pB / ((g - 1) * rhoB);
+        double etB = eB + 0.5 * (vx * vx + vy * vy + vz * vz);
+        Wb[0] = rhoB; Wb[1] = vx; Wb[2] = vy; Wb[3] = vz; Wb[4] = etB;
+    } else {
+        // boundary_base.cpp:77-111
+        const LutSet& L = *gs.L;
+        double sIn = gs.at(LUT_S_PT, pt, Tt);
+        double htIn = lut_h_p_s(L, pt, sIn);
+        for (int it = 0; it < 30; ++it) {
+            double pMid = 0.5 * (pLow + pHigh);
+            double hMid = lut_h_p_s(L, pMid, sIn);
+            double VMid = sqrt(2.0 * dmaxd(0.0, htIn - hMid));
+            double res = pMid + Z * Cg * VMid - Rout;
+            if (fabs(res) < 1e-4 * pt) { pB = pMid; break; }
+            if (res < 0.0) pLow = pMid; else pHigh = pMid;
+            pB = pMid;
+        }
+        double hB = lut_h_p_s(L, pB, sIn);
+        double VB = sqrt(2.0 * dmaxd(0.0, htIn - hB));
+        double vx = dx * VB, vy = dy * VB, vz = dz * VB;
+        double rhoB = gs.at(LUT_RHO_PS, pB, sIn);
+        double eB = gs.at(LUT_E_PS, pB, sIn);
+        double etB = eB + 0.5 * (vx * vx + vy * vy + vz * vz);
+        Wb[0] = rhoB; Wb[1] = vx; Wb[2] = vy; Wb[3] = vz; Wb[4] = etB;
     }
-    double TB = Tt * pow(pB / pt, ex);
-    double hB = cp * TB;
-    double VB = sqrt(2.0 * dmaxd(0.0, htIn - hB));
-    double vx = dx * VB, vy = dy * VB, vz = dz * VB;
-    double rhoB = pB / (Rg * TB);
-    double eB = pB / ((g - 1) * rhoB);
-    double etB = eB + 0.5 * (vx * vx + vy * vy + vz * vz);
-    double Wb[5] = {rhoB, vx, vy, vz, etB};
-    phys_flux<double>(Wb, sx, sy, sz, c.gm1, F);
+    phys_flux<double, FL>(Wb, sx, sy, sz, gs, F);
 }
 
 // Outlet with static pressure and optional ramp (boundary_base.cpp:181-224,
-// config.cpp:430-439)
-__device__ inline void outlet_flux(const CaseDev& c, const double U[5], double sx, double sy, double sz,
-                                   double pout, long it, double F[5]) {
+// config.cpp:430-439); supersonic outflow takes the interior flux.  The subsonic
+// boundary state keeps the interior velocity and, for the table fluid, the interior
+// entropy (boundary_base.cpp:205-211).
+template <int FL>
+__device__ inline void outlet_flux(const CaseDev& c, const Gas<FL>& gs, const double U[5], double sx, double sy,
+                                   double sz, double pout, long it, double F[5]) {
     double W[5], p, rf;
-    derive<double>(U, W, p, rf, c.gamma, c.gm1);
-    double a = sqrt(c.gamma * p / W[0]);
+    derive<double, FL>(U, W, p, rf, gs);
+    double a = sound_speed_p_rho<FL>(gs, p, W[0]);
     double m = sqrt(W[1] * W[1] + W[2] * W[2] + W[3] * W[3]);
     if (m >= a) {
-        phys_flux<double>(W, sx, sy, sz, c.gm1, F);
+        phys_flux<double, FL>(W, sx, sy, sz, gs, F);
         return;
     }
     double ratio = double(it) / c.ramp_iter;
     double pB = (ratio >= 1.0) ? pout : c.init_p + (pout - c.init_p) * ratio;
-    double rhoB = pB * W[0] / p;
-    double eB = pB / ((c.gamma - 1) * rhoB);
+    double rhoB, eB;
+    if constexpr (FL == 0) {
+        rhoB = pB * W[0] / p;
+        eB = pB / ((c.gamma - 1) * rhoB);
+    } else {
+        double e_int = W[4] - 0.5 * (W[1] * W[1] + W[2] * W[2] + W[3] * W[3]);
+        double s_int = gs.at(LUT_S, W[0], e_int);
+        rhoB = gs.at(LUT_RHO_PS, pB, s_int);
+        eB = gs.at(LUT_E_PS, pB, s_int);
+    }
     double etB = eB + 0.5 * (W[1] * W[1] + W[2] * W[2] + W[3] * W[3]);
     double Wb[5] = {rhoB, W[1], W[2], W[3], etB};
-    phys_flux<double>(Wb, sx, sy, sz, c.gm1, F);
+    phys_flux<double, FL>(Wb, sx, sy, sz, gs, F);
 }
 
 // Inviscid wall (src/boundary_inviscid_wall.cpp:5-30)
-__device__ inline void wall_flux(const CaseDev& c, const double U[5], double sx, double sy, double sz, double F[5]) {
+template <int FL>
+__device__ inline void wall_flux(const Gas<FL>& gs, const double U[5], double sx, double sy, double sz,
+                                 double F[5]) {
     double W[5], p, rf;
-    derive<double>(U, W, p, rf, c.gamma, c.gm1);
+    derive<double, FL>(U, W, p, rf, gs);
     double sm = sqrt(sx * sx + sy * sy + sz * sz);
     double nx = sx / sm, ny = sy / sm, nz = sz / sm;
     double un = W[1] * nx + W[2] * ny + W[3] * nz;
-    double a = sqrt(c.gamma * p / W[0]);
+    double a = sound_speed_p_rho<FL>(gs, p, W[0]);
     double pw = p + W[0] * un * a;
     if (pw < 0.01 * p) pw = 0.01 * p;
     F[0] = 0.0; F[1] = pw * nx; F[2] = pw * ny; F[3] = pw * nz; F[4] = 0.0;
@@ -343,16 +457,15 @@ __device__ __forceinline__ V3 cart_of(V3 v, double cs, double sn) { return v3(v.
 
 // Blockage + Hall / lift-drag source of one node (source_bfm_base.cpp:25-199,
 // source_bfm_hall.cpp:3-34, source_bfm_lift_drag.cpp:3-88).  s = slab index.
-template <typename CT>
-__device__ inline void bfm_source(const CT& c, long s, const double W[5], double V, double S[5],
+template <int FL, typename CT>
+__device__ inline void bfm_source(const CT& c, const Gas<FL>& gs, long s, const double W[5], double V, double S[5],
                                   double fi[3], double fv[3], double& dev) {
     double blk[5] = {0, 0, 0, 0, 0};
     fi[0] = fi[1] = fi[2] = 0.0;
     fv[0] = fv[1] = fv[2] = 0.0;
     dev = 0.0;
     if (c.blockage_active) {
-        double e = static_energy<double>(W[1], W[2], W[3], W[4]);
-        double p = c.gm1 * W[0] * e;
+        double p = pressure_w<double, FL>(gs, W);
         double ht = W[4] + p / W[0];
         double b = c.s_b[s];
         double dd = W[1] * c.s_gbx[s] + W[2] * c.s_gby[s] + W[3] * c.s_gbz[s];

@@ -59,8 +59,8 @@ __device__ inline void load_k(const CaseDev& c, const double* U, int i, int j, i
 // x - (+0) == x for every x, so the order and the values are the reference's.
 // Also writes the body-force side outputs (inviscid / viscous force, deviation angle)
 // on the last stage.
-template <bool THREED>
-__device__ inline void source_terms(const CaseDev& c, const double* __restrict__ Uin, int last, int i, int j,
+template <bool THREED, int FL>
+__device__ inline void source_terms(const CaseDev& c, const Gas<FL>& gs, const double* __restrict__ Uin, int last, int i, int j,
                                     int k, int n, long s, const double* Wc, double g[5], double gong[5],
                                     double bf[5]) {
     const int NI = c.ni, NJ = c.nj, NK = c.nk;
@@ -72,8 +72,7 @@ __device__ inline void source_terms(const CaseDev& c, const double* __restrict__
     const bool present = (fl & 1) != 0;
     const bool geo = (c.topo == TOPO_AXI) || (THREED && present);
     if (geo) {
-        double e = static_energy<double>(Wc[1], Wc[2], Wc[3], Wc[4]);
-        double p = c.gm1 * Wc[0] * e;
+        double p = pressure_w<double, FL>(gs, Wc);
         const double cs = c.s_cs[s], sn = c.s_sn[s];
         V3 vcy = cyl_of(v3(Wc[1], Wc[2], Wc[3]), cs, sn);
         V3 scy;
@@ -116,7 +115,7 @@ __device__ inline void source_terms(const CaseDev& c, const double* __restrict__
     }
     if (c.bfm_active && (fl & 2)) {
         double fi[3], fv[3], dv;
-        bfm_source(c, s, Wc, V, bf, fi, fv, dv);
+        bfm_source<FL>(c, gs, s, Wc, V, bf, fi, fv, dv);
         if (last && (fl & 4)) {
             c.finv[3 * n] = fi[0]; c.finv[3 * n + 1] = fi[1]; c.finv[3 * n + 2] = fi[2];
             c.fvisc[3 * n] = fv[0]; c.fvisc[3 * n + 1] = fv[1]; c.fvisc[3 * n + 2] = fv[2];
@@ -130,7 +129,7 @@ __device__ inline void source_terms(const CaseDev& c, const double* __restrict__
 // slab node, run before the stage kernel: the transcendental-heavy blade-row work runs
 // at full occupancy and without diverging the stencil kernel's waves.  Work: one entry
 // per 256 slab nodes of a case with sources.
-template <bool THREED>
+template <bool THREED, int FL>
 __device__ inline void sources_block(const CaseDev& c, const Launch& La, int bx) {
     const int s = bx * 256 + threadIdx.x;
     if (s >= c.ns) return;
@@ -138,11 +137,14 @@ __device__ inline void sources_block(const CaseDev& c, const Launch& La, int bx)
     const int i = c.bi0 + s % SW, j = (s / SW) % NJ, k = s / (SW * NJ);
     const int n = nid(c, i, j, k);
     const double* Uin = c.U[La.in];
-    double U[5], W[5], p, rf;
+    const Gas<FL> gs(c);
+    double U[5], W[5];
     load5(Uin, (int)c.N, n, U);
-    derive<double>(U, W, p, rf, c.gamma, c.gm1);
+    W[0] = U[0];
+#pragma unroll
+    for (int v = 1; v < 5; ++v) W[v] = U[v] / U[0];
     double g[5], go[5], bf[5];
-    source_terms<THREED>(c, Uin, La.last, i, j, k, n, s, W, g, go, bf);
+    source_terms<THREED, FL>(c, gs, Uin, La.last, i, j, k, n, s, W, g, go, bf);
     const int ns = c.ns;
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
@@ -155,6 +157,7 @@ __device__ inline void sources_block(const CaseDev& c, const Launch& La, int bx)
 // --------------------------------------------------------------------- time step
 // Solver::computeTimestepArray, inviscid (src/solver.cpp:1531-1630)
 // One block of 256 nodes (bx = block index within the case).
+template <int FL>
 __device__ inline void timestep_block(const CaseDev& c, const double* U, int bx) {
     __shared__ double smin[256];
     const long n = (long)bx * 256 + threadIdx.x;
@@ -175,10 +178,11 @@ __device__ inline void timestep_block(const CaseDev& c, const double* U, int bx)
             double dsI = sqrt(eIx * eIx + eIy * eIy + eIz * eIz);
             double dsJ = sqrt(eJx * eJx + eJy * eJy + eJz * eJz);
             double dsK = sqrt(eKx * eKx + eKy * eKy + eKz * eKz);
+            const Gas<FL> gs(c);
             double Un[5], W[5], p, rf;
             load5(U, c.N, n, Un);
-            derive<double>(Un, W, p, rf, c.gamma, c.gm1);
-            double a = sqrt(c.gamma * p / W[0]);
+            derive<double, FL>(Un, W, p, rf, gs);
+            const double a = sound_speed_w<FL>(gs, W, p);
             double ui = fabs(W[1] * (eIx / dsI) + W[2] * (eIy / dsI) + W[3] * (eIz / dsI));
             double lam = (ui + a) / dsI;
             if (c.topo != TOPO_1D) {
@@ -278,7 +282,8 @@ __global__ void __launch_bounds__(256) k_iterstart(Launch L) {
     if (*c.stop) return;
     const double* U = c.U[L.in];
     if (w.z == 1) radial_profile_block(c, U);
-    else timestep_block(c, U, w.x);
+    else if (c.real) timestep_block<1>(c, U, w.x);
+    else timestep_block<0>(c, U, w.x);
 }
 
 // ------------------------------------------------------------- boundary fluxes
@@ -286,6 +291,7 @@ __global__ void __launch_bounds__(256) k_iterstart(Launch L) {
 // outward vector -S and the node at the face; end faces +S and the node before it
 // (solver.cpp:1911-1957).  Result already multiplied by |S|.  Work: one entry per 256
 // boundary faces of a case.
+template <int FL>
 __device__ inline void boundary_block(const CaseDev& c, const double* U, int bx) {
     const long NI = c.ni, NJ = c.nj, NK = c.nk;
     long t = (long)bx * 256 + threadIdx.x;
@@ -330,15 +336,16 @@ __device__ inline void boundary_block(const CaseDev& c, const double* U, int bx)
     const PatchDev pa = c.patch[p];
     double F[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     const long it = *c.iter;
+    const Gas<FL> gs(c);
     if (pa.type == PT_INLET) {
         const double* d = dirv + (long)side * 3 * nsz;
-        inlet_flux(c, Un, Sx, Sy, Sz, pa.v0, pa.v1, d[q], d[nsz + q], d[2 * nsz + q], F);
+        inlet_flux<FL>(c, gs, Un, Sx, Sy, Sz, pa.v0, pa.v1, d[q], d[nsz + q], d[2 * nsz + q], F);
     } else if (pa.type == PT_OUTLET) {
-        outlet_flux(c, Un, Sx, Sy, Sz, pa.v0, it, F);
+        outlet_flux<FL>(c, gs, Un, Sx, Sy, Sz, pa.v0, it, F);
     } else if (pa.type == PT_RADIAL) {
-        outlet_flux(c, Un, Sx, Sy, Sz, c.prof_p[(long)pa.profile * NJ + fj], it, F);
+        outlet_flux<FL>(c, gs, Un, Sx, Sy, Sz, c.prof_p[(long)pa.profile * NJ + fj], it, F);
     } else if (pa.type == PT_WALL) {
-        wall_flux(c, Un, Sx, Sy, Sz, F);
+        wall_flux<FL>(gs, Un, Sx, Sy, Sz, F);
     }
     const double sm = sqrt(Sx * Sx + Sy * Sy + Sz * Sz);
     double* o = out + (long)side * 5 * nsz;
@@ -350,13 +357,16 @@ __device__ inline void boundary_block(const CaseDev& c, const double* U, int bx)
 // faces) and slab source terms (z = 1: 3-D case, z = 2: 2-D / axisymmetric; one block per
 // 256 slab nodes), all from the stage input.  Independent work in one launch: the
 // latency-bound boundary blocks (inlet bisection) overlap the throughput-bound sources.
+// One instance per fluid (the table fluid's boundary states would otherwise cost the
+// ideal-gas blocks occupancy); each skips the cases of the other fluid.
+template <int FL>
 __global__ void __launch_bounds__(256) k_prestage(Launch La) {
     const Work w = La.work[blockIdx.x];
     const CaseDev& c = La.cases[w.cs];
-    if (*c.stop) return;
-    if (w.z == 0) boundary_block(c, c.U[La.in], w.x);
-    else if (w.z == 1) sources_block<true>(c, La, w.x);
-    else sources_block<false>(c, La, w.x);
+    if (*c.stop || c.real != FL) return;
+    if (w.z == 0) boundary_block<FL>(c, c.U[La.in], w.x);
+    else if (w.z == 1) sources_block<true, FL>(c, La, w.x);
+    else sources_block<false, FL>(c, La, w.x);
 }
 
 }  // namespace dev
