@@ -128,15 +128,25 @@ def bench_serve(args):
     cases, admitted by device-memory footprint into one running batch (FP32 flux / FP64
     state) and retired on a density-residual drop; against the plain FP64 path serving
     the same stream one case at a time (its measured service times in a FIFO queue)."""
+    import tempfile
     from cturbobfm_amd import serve
-    specs = serve.mixed_stream(args.serve_cases, seed=args.seed, precision="mixed")
+    tmp = tempfile.mkdtemp(prefix="tbfm_serve_")
+    lut = serve.stream_table(tmp)
+    specs = serve.mixed_stream(args.serve_cases, seed=args.seed, precision="mixed", real_fraction=0.05,
+                               lut_path=lut)
     arrivals = serve.poisson_arrivals(len(specs), rate=args.rate, seed=args.seed)
-    srv = serve.Server(drop=args.drop, max_iter=args.max_iter, chunk=100)
+    srv = serve.Server(drop=args.drop, max_iter=args.max_iter, chunk=100, keep_turbo=True)
     t0 = time.perf_counter()
     res = srv.run(specs, arrivals)
     wall = time.perf_counter() - t0
     served = serve.summary(res, wall)
-    alone = serve.sequential(specs, drop=args.drop, max_iter=args.max_iter, precision="fp64")
+    alone = serve.sequential(specs, drop=args.drop, max_iter=args.max_iter, precision="fp64", keep_turbo=True)
+    # converged mass flow and total-pressure ratio, served (FP32 flux) against plain FP64
+    dm, dpr = [], []
+    for r, a in zip(res, alone):
+        if r.turbo is not None and a.turbo is not None and r.converged and a.converged:
+            dm.append(abs(r.turbo[6] - a.turbo[6]) / abs(a.turbo[6]))
+            dpr.append(abs(r.turbo[7] - a.turbo[7]) / abs(a.turbo[7]))
     # the plain path serving the same arrivals: one case at a time, first come first served
     order = np.argsort(arrivals)
     free = 0.0
@@ -159,6 +169,12 @@ def bench_serve(args):
         "config": {"workload": "%d cases, Poisson arrivals %.1f/s, retire at %.1f orders of density drop "
                                "(cap %d iterations)" % (len(specs), args.rate, args.drop, args.max_iter)},
         "served": served, "plain_fp64": plain,
+        "table_fluid_cases": int(sum("table-fluid" in sp.name for sp in specs)),
+        "converged_rel_diff_vs_plain": {"mass_flow_max": max(dm) if dm else None,
+                                        "mass_flow_median": float(np.median(dm)) if dm else None,
+                                        "prtt_max": max(dpr) if dpr else None,
+                                        "prtt_median": float(np.median(dpr)) if dpr else None,
+                                        "cases": len(dm)},
         "cases_per_hour_ratio": served["cases_per_hour"] / plain["cases_per_hour"],
     }
 
@@ -247,6 +263,7 @@ def main():
                                % (ni, nj, nk, args.scheme, args.precision),
                    "nodes": nodes, "stages_per_step": stages, "body_force_nodes": nbf,
                    "stage_tiling": s.tuning().split("->")[-1].strip(),
+                   "autotune_ms": s.tuning().split("->")[0].strip(),
                    "ms_per_step_graph": (g1 - g0) / args.steps * 1e3},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

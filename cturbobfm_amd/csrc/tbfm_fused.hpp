@@ -43,7 +43,6 @@ struct PlaneLds {
         double FI[TL::NFI];   // [v][tj][m]  m = 0..TI, I face of node i0+m (its left face)
         double FJ[TL::NFJ];   // [v][r][ti] r = 0..TJ
     } f;
-    double red[5][NT];        // per-thread residual-norm accumulators
 };
 
 template <class TL>
@@ -216,9 +215,10 @@ struct KQueue<0, FL> {
         const double sr = jst_sensor<double>(P[1], P[2], P[3]);
         const double Wl[5] = {U[1][0], Wv[0][0], Wv[0][1], Wv[0][2], Wv[0][3]};
         const double Wr[5] = {U[2][0], Wv[1][0], Wv[1][1], Wv[1][2], Wv[1][3]};
-        jst_face<FT, FL>(U[0], U[1], U[2], U[3], Wl, Wr, RF[0], RF[1], sl, sr, c.nrm[c.noffK + f],
-                         c.nrm[c.noffK + c.nfK + f], c.nrm[c.noffK + 2 * c.nfK + f], c.nrm[c.noffK + 3 * c.nfK + f],
-                         Gas<FL>(c), out);
+        const double* nk = c.nrm + c.noffK;
+        const int nf = c.nfK;
+        jst_face<FT, FL>(U[0], U[1], U[2], U[3], Wl, Wr, RF[0], RF[1], sl, sr, gld(nk, f), gld(nk + nf, f),
+                         gld(nk + 2 * nf, f), gld(nk + 3 * nf, f), Gas<FL>(c), out);
     }
 };
 
@@ -245,8 +245,10 @@ struct KQueue<1, FL> {
     }
     template <typename FT>
     __device__ __forceinline__ void face(const CaseDev& c, int f, double out[5]) const {
-        roe_face<FT, FL>(W[0], W[1], W[2], W[3], c.muscl, c.limiter, c.nrm[c.noffK + f], c.nrm[c.noffK + c.nfK + f],
-                         c.nrm[c.noffK + 2 * c.nfK + f], c.nrm[c.noffK + 3 * c.nfK + f], Gas<FL>(c), out);
+        const double* nk = c.nrm + c.noffK;
+        const int nf = c.nfK;
+        roe_face<FT, FL>(W[0], W[1], W[2], W[3], c.muscl, c.limiter, gld(nk, f), gld(nk + nf, f),
+                         gld(nk + 2 * nf, f), gld(nk + 3 * nf, f), Gas<FL>(c), out);
     }
 };
 
@@ -279,7 +281,7 @@ __device__ __forceinline__ void face_normal(const CaseDev& c, const FaceIdx<TL>&
     const int fst = f.isI ? c.nfI : c.nfJ;
     const int fo = f.isI ? (k * NJ + f.gj) * (NI + 1) + f.gi : c.noffJ + (k * (NJ + 1) + f.gj) * NI + f.gi;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) nn[v] = c.nrm[fo + v * fst];
+    for (int v = 0; v < 4; ++v) nn[v] = gld(c.nrm + (long)v * fst, (unsigned)fo);
 }
 
 // LDS position of halo slot h (0 .. NP-NT-1): two full rows above, two below, and the two
@@ -317,14 +319,15 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
     const int i = i0 + ti, j = j0 + tj;
     const int NI = c.ni, NJ = c.nj, NK = c.nk;
     const bool act = (i < NI) && (j < NJ);
-    const int kc0 = w.z * c.kc;
-    const int kc1 = min(NK, kc0 + c.kc);
+    // the block's k planes: piece w.z of sgz balanced pieces of [0, NK)
+    const int kc0 = (w.z * NK) / c.sgz;
+    const int kc1 = ((w.z + 1) * NK) / c.sgz;
     const bool doJ = (c.topo != TOPO_1D);
     const int N = (int)c.N;
     const int NIJ = NI * NJ;
     const int own = (tj + 2) * HI + (ti + 2);
     const int nfaces = NFI + (doJ ? NFJ : 0);
-    const bool bladecol = THREED && c.bfm_active && act && c.blade2d[j * NI + i];
+    const bool bladecol = THREED && c.bfm_active && act && gld(c.blade2d, (unsigned)(j * NI + i));
     const bool srcOn = (c.topo == TOPO_AXI) || c.bfm_active;
     const double* __restrict__ Uin = st.Uin;
 
@@ -343,15 +346,13 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
         if (!bladecol) {
             if (kc0 == 0) {
 #pragma unroll
-                for (int v = 0; v < 5; ++v) FKl[v] = c.bnd[c.boffK + v * NIJ + j * NI + i];
+                for (int v = 0; v < 5; ++v) FKl[v] = gld(c.bnd + c.boffK + (long)v * NIJ, (unsigned)(j * NI + i));
             } else {
                 q.template face<FT>(c, (kc0 * NJ + j) * NI + i, FKl);
             }
         }
     }
-    if (st.last)
-#pragma unroll
-        for (int v = 0; v < 5; ++v) L.red[v][tid] = 0.0;
+    double nsq[5] = {0.0, 0.0, 0.0, 0.0, 0.0};   // residual-norm accumulators of the column
     // Software pipeline: the next step's column plane and halo slot are read while the
     // current plane's faces are evaluated.
     double pre[5];
@@ -370,8 +371,8 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
         double u0v[5], dtv = 0.0, volv = 1.0;
         if (act && !parked && !st.rmode) {
             load5(st.U0, N, n, u0v);
-            dtv = (c.dt_mode == 1) ? c.dtmin[0] : c.dt[n];
-            volv = c.vol[n];
+            dtv = (c.dt_mode == 1) ? gld(c.dtmin, 0u) : gld(c.dt, (unsigned)n);
+            volv = gld(c.vol, (unsigned)n);
         }
         double Wc[5];
         if (THREED) {
@@ -424,7 +425,7 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
                 const int bq = f.isI ? k * NJ + f.gj : k * NI + f.gi;
                 const int off = (f.isI ? 0 : c.boffJ) + (f.lo ? 0 : 5 * nsz) + bq;
 #pragma unroll
-                for (int v = 0; v < 5; ++v) F[v] = c.bnd[off + v * nsz];
+                for (int v = 0; v < 5; ++v) F[v] = gld(c.bnd, (unsigned)(off + v * nsz));
             } else {
                 double nn[4];
                 face_normal(c, f, k, nn);
@@ -442,7 +443,8 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
         if (THREED && act && !bladecol) {
             if (k + 1 == NK) {
 #pragma unroll
-                for (int v = 0; v < 5; ++v) FKr[v] = c.bnd[c.boffK + (5 + v) * NIJ + j * NI + i];
+                for (int v = 0; v < 5; ++v)
+                    FKr[v] = gld(c.bnd + c.boffK + (long)(5 + v) * NIJ, (unsigned)(j * NI + i));
             } else {
                 q.template face<FT>(c, ((k + 1) * NJ + j) * NI + i, FKr);
             }
@@ -471,43 +473,50 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
             if (srcOn && i >= c.bi0 && i < c.bi1) {
                 // geometric, Gong, body force, subtracted in that order
                 const int sl = (k * NJ + j) * (c.bi1 - c.bi0) + (i - c.bi0);
-                const double* sb = c.srcbuf + sl;
                 const int ns = c.ns;
 #pragma unroll
                 for (int t = 0; t < 3; ++t)
 #pragma unroll
-                    for (int v = 0; v < 5; ++v) R[v] -= sb[(5 * t + v) * ns];
+                    for (int v = 0; v < 5; ++v) R[v] -= gld(c.srcbuf + (long)(5 * t + v) * ns, (unsigned)sl);
             }
             if (parked) {
                 double* rp = c.rper + (k == 0 ? 0 : 5) * NIJ;
 #pragma unroll
-                for (int v = 0; v < 5; ++v) rp[v * NIJ + j * NI + i] = R[v];
+                for (int v = 0; v < 5; ++v) gst(rp + (long)v * NIJ, (unsigned)(j * NI + i), R[v]);
             } else if (st.rmode) {
 #pragma unroll
-                for (int v = 0; v < 5; ++v) st.Uout[v * N + n] = R[v];
+                for (int v = 0; v < 5; ++v) gst(st.Uout + (long)v * N, (unsigned)n, R[v]);
             } else {
                 const double factor = st.alpha * dtv / volv;
 #pragma unroll
-                for (int v = 0; v < 5; ++v) st.Uout[v * N + n] = u0v[v] - R[v] * factor;
+                for (int v = 0; v < 5; ++v) gst(st.Uout + (long)v * N, (unsigned)n, u0v[v] - R[v] * factor);
                 if (st.last)
 #pragma unroll
-                    for (int v = 0; v < 5; ++v) L.red[v][tid] += R[v] * R[v];
+                    for (int v = 0; v < 5; ++v) nsq[v] += R[v] * R[v];
             }
         }
 #pragma unroll
         for (int v = 0; v < 5; ++v) FKl[v] = FKr[v];
-        __syncthreads();
+        // No barrier here: the next plane's staging writes the plane buffers, whose last
+        // readers (this plane's faces) finished before the barrier above, and its faces
+        // overwrite the face buffers only after the staging barrier, which every thread
+        // reaches after its assembly.
     }
-    // ---- per-block norm partial sums (deterministic tree)
+    // ---- per-block norm partial sums (deterministic tree), in the primitive buffer: its
+    // last readers finished before the last face barrier
     if (st.last) {
+        double (*red)[TL::NP] = L.d.W;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) red[v][tid] = nsq[v];
+        __syncthreads();
         const long b = ((long)w.z * c.sgy + w.y) * c.sgx + w.x;
         for (int s = NT / 2; s > 0; s >>= 1) {
             if (tid < s)
 #pragma unroll
-                for (int v = 0; v < 5; ++v) L.red[v][tid] += L.red[v][tid + s];
+                for (int v = 0; v < 5; ++v) red[v][tid] += red[v][tid + s];
             __syncthreads();
         }
-        if (tid < 5) c.partial[b * 5 + tid] = L.red[tid][0];
+        if (tid < 5) gst(c.partial, (unsigned)(b * 5 + tid), red[tid][0]);
     }
 }
 

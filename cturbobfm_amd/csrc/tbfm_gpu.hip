@@ -166,14 +166,16 @@ void faceSoA(const std::vector<double>& src, long ai, long aj, long ak, std::vec
 
 int tileI(int t) { return t == 0 ? 32 : (t == 1 ? 16 : 64); }
 
-void setTiling(tbfm_case* cs, int tile, int kc) {
+// Tile shape and the number of k pieces: block z of a column of tiles marches planes
+// [z*NK/pieces, (z+1)*NK/pieces).
+void setTiling(tbfm_case* cs, int tile, int pieces) {
     const int ti = tileI(tile), tj = 256 / ti;
     CaseDev& c = cs->c;
     cs->tile = tile;
-    c.kc = kc;
+    c.sgz = std::max(1, std::min(pieces, (int)cs->nk));
+    c.kc = (int)((cs->nk + c.sgz - 1) / c.sgz);
     c.sgx = (int)((cs->ni + ti - 1) / ti);
     c.sgy = (int)((cs->nj + tj - 1) / tj);
-    c.sgz = (int)((cs->nk + kc - 1) / kc);
     c.nparts_stage = c.sgx * c.sgy * c.sgz;
 }
 
@@ -964,7 +966,7 @@ void autotune(tbfm_case* cs) {
         if (it != g_tuned.end()) {
             setTiling(cs, it->second.first, it->second.second);
             cs->tuning = "cached -> " + std::to_string(tileI(cs->tile)) + "x" + std::to_string(256 / tileI(cs->tile)) +
-                         "/kc" + std::to_string(cs->c.kc);
+                         "/z" + std::to_string(cs->c.sgz);
             pushCase(cs);
             return;
         }
@@ -973,16 +975,30 @@ void autotune(tbfm_case* cs) {
         pushCase(cs);
         return;
     }
-    std::vector<int> kcs = (cs->topo == Topology::ThreeD) ? std::vector<int>{2, 4, 8, 16} : std::vector<int>{1};
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     float best = 1e30f;
-    int bt = cs->tile, bk = cs->c.kc;
+    int bt = cs->tile, bk = cs->c.sgz;
     char buf[96];
+    // candidates: per tile shape, k piece counts that fill whole rounds of block slots
+    // (m * 256 blocks: one or two resident blocks per CU of 256) and the plain planes-per-
+    // piece choices 2, 4, 8, 16
     std::vector<std::pair<int, int>> cand;
-    for (int t = 0; t < 3; ++t)
-        for (int kc : kcs) cand.push_back({t, kc});
+    for (int t = 0; t < 3; ++t) {
+        const long tiles = ((cs->ni + tileI(t) - 1) / tileI(t)) * ((cs->nj + 256 / tileI(t) - 1) / (256 / tileI(t)));
+        std::vector<int> zs;
+        if (cs->topo == Topology::ThreeD) {
+            for (int m = 1; m <= 8; ++m) zs.push_back((int)(m * 256 / tiles));
+            for (int kc : {2, 4, 8, 16}) zs.push_back((int)((cs->nk + kc - 1) / kc));
+        } else {
+            zs.push_back(1);
+        }
+        std::sort(zs.begin(), zs.end());
+        zs.erase(std::unique(zs.begin(), zs.end()), zs.end());
+        for (int z : zs)
+            if (z >= 1 && z <= cs->nk / 2 + (cs->nk == 1)) cand.push_back({t, z});
+    }
     std::vector<float> tmin(cand.size(), 1e30f);
     for (int round = 0; round < 2; ++round)
         for (size_t q = 0; q < cand.size(); ++q) {
@@ -1003,7 +1019,7 @@ void autotune(tbfm_case* cs) {
             }
         }
     for (size_t q = 0; q < cand.size(); ++q) {
-        std::snprintf(buf, sizeof buf, "%dx%d/kc%d:%.4fms ", tileI(cand[q].first), 256 / tileI(cand[q].first),
+        std::snprintf(buf, sizeof buf, "%dx%d/z%d:%.4fms ", tileI(cand[q].first), 256 / tileI(cand[q].first),
                       cand[q].second, tmin[q]);
         cs->tuning += buf;
         if (tmin[q] < best) { best = tmin[q]; bt = cand[q].first; bk = cand[q].second; }
@@ -1013,7 +1029,7 @@ void autotune(tbfm_case* cs) {
     setTiling(cs, bt, bk);
     pushCase(cs);
     buildTables(P);
-    std::snprintf(buf, sizeof buf, "-> %dx%d/kc%d", tileI(bt), 256 / tileI(bt), bk);
+    std::snprintf(buf, sizeof buf, "-> %dx%d/z%d", tileI(bt), 256 / tileI(bt), bk);
     cs->tuning += buf;
     std::lock_guard<std::mutex> lk(g_tuned_mu);
     g_tuned[key] = {bt, bk};

@@ -143,6 +143,22 @@ struct Launch {
 __host__ __device__ inline int stage_in(int s) { return s == 0 ? 0 : (s % 2 == 1 ? 1 : 2); }
 __host__ __device__ inline int stage_out(int s, int ns) { return s == ns - 1 ? 0 : (s % 2 == 0 ? 1 : 2); }
 
+// Global-memory access by a uniform base pointer and a 32-bit element index.  The device
+// arrays live in global memory, but pointers read from the case table are generic ones,
+// which the compiler would access with flat instructions (slower, and counted with LDS
+// operations so that every LDS wait also waits for them).  The cast makes them global
+// loads and stores with the base in scalar registers and a 32-bit byte offset per lane;
+// the library keeps every array below 4 GiB.
+#define TBFM_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ T gld(const T* base, unsigned idx) {
+    return *(const TBFM_GLOBAL T*)((const TBFM_GLOBAL char*)base + idx * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void gst(T* base, unsigned idx, T v) {
+    *(TBFM_GLOBAL T*)((TBFM_GLOBAL char*)base + idx * (unsigned)sizeof(T)) = v;
+}
+
 __device__ __forceinline__ double dmaxd(double a, double b) { return (a < b) ? b : a; }
 __device__ __forceinline__ double dmind(double a, double b) { return (b < a) ? b : a; }
 

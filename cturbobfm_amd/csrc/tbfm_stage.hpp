@@ -14,7 +14,7 @@ __device__ __forceinline__ int nid(const CT& c, int i, int j, int k) {
 
 __device__ __forceinline__ void load5(const double* U, int N, int n, double out[5]) {
 #pragma unroll
-    for (int v = 0; v < 5; ++v) out[v] = U[v * N + n];
+    for (int v = 0; v < 5; ++v) out[v] = gld(U + (long)v * N, (unsigned)n);
 }
 
 // rotateStateVectorAlongXAxis (src/math_utils.cpp:109-122)

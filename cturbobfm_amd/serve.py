@@ -48,44 +48,66 @@ class CaseResult:
     converged: bool = False
     diverged: bool = False
     state: Optional[np.ndarray] = None
+    turbo: Optional[np.ndarray] = None     # Solver.turbo() at retirement
 
     @property
     def time_to_converge(self) -> float:
         return self.finished - self.arrival
 
 
+def _fluid(case: Dict, lut_path: Optional[str]) -> Dict:
+    return cases.with_real_gas(case, lut_path) if lut_path else case
+
+
 def rotor_spec(ni: int, nj: int, nk: int, outlet_pressure: float, precision: str, scheme: str = "JST",
-               max_iter: int = 20000) -> CaseSpec:
+               max_iter: int = 20000, lut_path: Optional[str] = None) -> CaseSpec:
     def build():
-        return cases.rotor3d(ni, nj, nk, n_iterations=max_iter, scheme=scheme, outlet_pressure=outlet_pressure)
+        return _fluid(cases.rotor3d(ni, nj, nk, n_iterations=max_iter, scheme=scheme,
+                                    outlet_pressure=outlet_pressure), lut_path)
     # the radial-equilibrium outlet takes its hub pressure from INIT_PRESSURE, which
     # cases.rotor3d sets to the outlet pressure
-    return CaseSpec("rotor %dx%dx%d p_out=%.0f %s" % (ni, nj, nk, outlet_pressure, precision), build,
+    return CaseSpec("rotor %dx%dx%d p_out=%.0f %s%s" % (ni, nj, nk, outlet_pressure, precision,
+                                                        " table-fluid" if lut_path else ""), build,
                     {"GPU_PRECISION": precision}, ni * nj * nk, True)
 
 
-def bump_spec(ni: int, nj: int, mach_out: float, precision: str, max_iter: int = 20000) -> CaseSpec:
+def bump_spec(ni: int, nj: int, mach_out: float, precision: str, max_iter: int = 20000,
+              lut_path: Optional[str] = None) -> CaseSpec:
     def build():
-        return cases.bump2d(ni, nj, n_iterations=max_iter, mach_out=mach_out)
-    return CaseSpec("bump %dx%d M_out=%.3f %s" % (ni, nj, mach_out, precision), build,
+        return _fluid(cases.bump2d(ni, nj, n_iterations=max_iter, mach_out=mach_out), lut_path)
+    return CaseSpec("bump %dx%d M_out=%.3f %s%s" % (ni, nj, mach_out, precision,
+                                                    " table-fluid" if lut_path else ""), build,
                     {"GPU_PRECISION": precision}, ni * nj, False)
 
 
-def mixed_stream(n: int, seed: int = 7, precision: str = "mixed") -> List[CaseSpec]:
+def stream_table(directory: str, seed: int = 5) -> str:
+    """Writes the stream's look-up table (a 60 x 60 rho-e covolume CO2-like gas from a
+    seeded smooth analytic equation of state, cases.covolume_lut) and returns its path."""
+    import os
+    return cases.write_lut(cases.covolume_lut(seed=seed), os.path.join(directory, "covolume_seed%d.lut" % seed))
+
+
+def mixed_stream(n: int, seed: int = 7, precision: str = "mixed", real_fraction: float = 0.05,
+                 lut_path: Optional[str] = None) -> List[CaseSpec]:
     """A seeded mix of 2-D bump and 3-D rotor requests: half bumps (65x33 or 129x65,
     outlet Mach 0.45-0.55), the rest rotors weighted toward the small sizes (97x25x65,
-    129x33x97, rarely 193x49x129) with outlet pressure drawn from 92-98 kPa."""
+    129x33x97, rarely 193x49x129) with outlet pressure drawn from 92-98 kPa.  A fraction
+    real_fraction of the requests (drawn from the same seed) run on the look-up-table
+    fluid of lut_path; these compute in FP64, the look-up-table path having no FP32 form."""
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(n):
         u = rng.random()
+        table = lut_path if (lut_path and rng.random() < real_fraction) else None
+        prec = "fp64" if table else precision
         if u < 0.5:
             ni, nj = (65, 33) if rng.random() < 0.6 else (129, 65)
-            out.append(bump_spec(ni, nj, float(rng.uniform(0.45, 0.55)), precision))
+            out.append(bump_spec(ni, nj, float(rng.uniform(0.45, 0.55)), prec, lut_path=table))
         else:
             v = rng.random()
             size = (97, 25, 65) if v < 0.65 else ((129, 33, 97) if v < 0.95 else (193, 49, 129))
-            out.append(rotor_spec(*size, outlet_pressure=float(rng.uniform(92000.0, 98000.0)), precision=precision))
+            out.append(rotor_spec(*size, outlet_pressure=float(rng.uniform(92000.0, 98000.0)), precision=prec,
+                                  lut_path=table))
     return out
 
 
@@ -105,12 +127,13 @@ class Server:
     """
 
     def __init__(self, drop: float = 8.0, max_iter: int = 20000, chunk: int = 100,
-                 budget: Optional[int] = None, keep_states: bool = False):
+                 budget: Optional[int] = None, keep_states: bool = False, keep_turbo: bool = False):
         self.drop = drop
         self.max_iter = max_iter
         self.chunk = chunk
         self.budget = budget if budget is not None else int(0.8 * device_memory()[0])
         self.keep_states = keep_states
+        self.keep_turbo = keep_turbo
         self.bytes_per_node = {True: 1500.0, False: 1500.0}   # refined from the first cases
 
     def _estimate(self, spec: CaseSpec) -> int:
@@ -164,6 +187,8 @@ class Server:
                     r.converged = conv > 0
                     if self.keep_states:
                         r.state = s.state()
+                    if self.keep_turbo and not r.diverged:
+                        r.turbo = s.turbo()
                     batch.retire(s)
                     s.close()
                     used -= footprint.pop(q)
@@ -174,7 +199,8 @@ class Server:
 
 
 def sequential(specs: List[CaseSpec], drop: float, max_iter: int = 20000, chunk: int = 500,
-               precision: Optional[str] = None, keep_states: bool = False) -> List[CaseResult]:
+               precision: Optional[str] = None, keep_states: bool = False,
+               keep_turbo: bool = False) -> List[CaseResult]:
     """The plain path: each case alone, one after another, from set-up to convergence
     (precision overrides the spec's GPU_PRECISION when given)."""
     out = []
@@ -196,6 +222,8 @@ def sequential(specs: List[CaseSpec], drop: float, max_iter: int = 20000, chunk:
         r.converged = s.converged_at > 0
         if keep_states:
             r.state = s.state()
+        if keep_turbo and not r.diverged:
+            r.turbo = s.turbo()
         s.close()
         r.finished = time.perf_counter() - t0
         out.append(r)

@@ -12,18 +12,27 @@ from cturbobfm_amd import serve
 pytestmark = pytest.mark.gpu
 
 
-def small_stream():
+@pytest.fixture(scope="module")
+def lut_path(tmp_path_factory):
+    return serve.stream_table(str(tmp_path_factory.mktemp("lut")))
+
+
+def small_stream(lut_path=None):
     specs = [serve.bump_spec(33, 17, 0.5, "fp64", max_iter=1200),
              serve.rotor_spec(25, 9, 17, 95000.0, "fp64", max_iter=1200),
              serve.bump_spec(33, 17, 0.45, "mixed", max_iter=1200),
              serve.rotor_spec(33, 9, 17, 93000.0, "mixed", max_iter=1200),
              serve.bump_spec(49, 25, 0.55, "fp64", max_iter=1200),
              serve.rotor_spec(25, 9, 17, 97000.0, "mixed", max_iter=1200)]
+    if lut_path:
+        # look-up-table fluid requests in the same stream
+        specs.insert(2, serve.rotor_spec(25, 9, 17, 95000.0, "fp64", max_iter=1200, lut_path=lut_path))
+        specs.append(serve.bump_spec(33, 17, 0.5, "fp64", max_iter=1200, lut_path=lut_path))
     return specs
 
 
-def test_served_equals_alone():
-    specs = small_stream()
+def test_served_equals_alone(lut_path):
+    specs = small_stream(lut_path)
     arrivals = serve.poisson_arrivals(len(specs), rate=50.0, seed=3)
     srv = serve.Server(drop=1.5, max_iter=1200, chunk=100, keep_states=True)
     served = srv.run(specs, arrivals)
