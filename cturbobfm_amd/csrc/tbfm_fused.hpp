@@ -367,6 +367,12 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
     for (int k = kc0; k < kc1; ++k) {
         const int n = (k * NJ + j) * NI + i;
         const bool parked = THREED && c.periodic && (k == 0 || k == NK - 1);
+        // unit normal and area of the thread's first I/J face of this plane
+        double nnx[4] = {1.0, 0.0, 0.0, 1.0};
+        {
+            const FaceIdx<TL> f(tid, i0, j0, NI, NJ);
+            if (tid < nfaces && f.valid && !f.bnd) face_normal(c, f, k, nnx);
+        }
         // update operands, read early and consumed at the update
         double u0v[5], dtv = 0.0, volv = 1.0;
         if (act && !parked && !st.rmode) {
@@ -415,9 +421,16 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
         for (int h = tid + NT; h < NP - NT; h += NT) fill_position<TL, SCH, FL>(c, Uin, L, halo_pos<TL>(h), i0, j0, k);
         __syncthreads();
         // ---- every I and J face of the tile once: fq < NFI are I faces (row r, face m),
-        // then J faces (row r of faces, column m); one code path, no divergence
+        // then J faces (row r of faces, column m); one code path, no divergence.  The unit
+        // normal and area of the thread's next face are requested before the current face
+        // is evaluated (those of its first face before the plane was staged).
         for (int fq = tid; fq < nfaces; fq += NT) {
             const FaceIdx<TL> f(fq, i0, j0, NI, NJ);
+            double nn[4] = {nnx[0], nnx[1], nnx[2], nnx[3]};
+            if (fq + NT < nfaces) {
+                const FaceIdx<TL> g(fq + NT, i0, j0, NI, NJ);
+                if (g.valid && !g.bnd) face_normal(c, g, k, nnx);
+            }
             if (!f.valid) continue;
             double F[5];
             if (f.bnd) {
@@ -427,8 +440,6 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
 #pragma unroll
                 for (int v = 0; v < 5; ++v) F[v] = gld(c.bnd, (unsigned)(off + v * nsz));
             } else {
-                double nn[4];
-                face_normal(c, f, k, nn);
                 const int l = f.isI ? (f.r + 2) * HI + (f.m + 1) : (f.r + 1) * HI + (f.m + 2);
                 lds_face<TL, SCH, FT, FL>(c, L, l, f.isI ? 1 : HI, nn[0], nn[1], nn[2], nn[3], F);
             }

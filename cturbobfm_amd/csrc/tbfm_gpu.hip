@@ -753,6 +753,19 @@ void pushCase(tbfm_case* cs) {
 
 constexpr int kXcds = 8;   // accelerator complex dies of an MI355X (each with its own L2)
 
+// XCD-aware order of a sequence of blocks whose neighbours share data (planes of the
+// state, dual nodes): the GPU hands workgroup b to XCD b % kXcds, so the sequence is cut
+// into kXcds contiguous pieces and the table interleaves them; each XCD then walks one
+// compact piece and keeps what neighbouring blocks share in its own L2.  Shorter pieces
+// are padded with no-op entries (cs = -1).
+std::vector<dev::Work> xcdOrder(const std::vector<dev::Work>& seq) {
+    const size_t n = seq.size(), len = (n + kXcds - 1) / kXcds;
+    std::vector<dev::Work> t(len * kXcds, dev::Work{-1, 0, 0, 0});
+    for (int x = 0; x < kXcds; ++x)
+        for (size_t q = n * x / kXcds; q < n * (x + 1) / kXcds; ++q) t[(q - n * x / kXcds) * kXcds + x] = seq[q];
+    return t;
+}
+
 // Builds the work tables of every kernel from the members' current tiling.
 void buildTables(Plan& P) {
     std::vector<dev::Work> dt, mn, prof, bnd, per, norm, src;
@@ -809,10 +822,18 @@ void buildTables(Plan& P) {
     uploadTable(P.bnd, bnd);
     uploadTable(P.per, per);
     uploadTable(P.norm, norm);
-    // combined launches: profiles then time-step blocks; boundary then source blocks
-    std::vector<dev::Work> start(prof), pre(bnd);
-    start.insert(start.end(), dt.begin(), dt.end());
-    pre.insert(pre.end(), src.begin(), src.end());
+    // combined launches: profiles then time-step blocks; boundary then source blocks.
+    // Profile and boundary blocks go first, round robin over the XCDs (the latency-bound
+    // inlet bisection spread out); the time-step and source blocks follow in XCD order
+    // (shared dual nodes; the Gong source reads the two neighbouring planes).
+    auto padded = [](std::vector<dev::Work> v) {
+        while (v.size() % kXcds) v.push_back(dev::Work{-1, 0, 0, 0});
+        return v;
+    };
+    std::vector<dev::Work> start = padded(prof), pre = padded(bnd);
+    const std::vector<dev::Work> dto = xcdOrder(dt), srco = xcdOrder(src);
+    start.insert(start.end(), dto.begin(), dto.end());
+    pre.insert(pre.end(), srco.begin(), srco.end());
     uploadTable(P.start, start);
     uploadTable(P.pre, pre);
     for (size_t g = 0; g < P.groups.size(); ++g) {

@@ -278,6 +278,7 @@ __device__ inline void radial_profile_block(const CaseDev& c, const double* U) {
 // role `in`; independent of each other, so they share one launch.
 __global__ void __launch_bounds__(256) k_iterstart(Launch L) {
     const Work w = L.work[blockIdx.x];
+    if (w.cs < 0) return;   // padding entry of an XCD-ordered table
     const CaseDev& c = L.cases[w.cs];
     if (*c.stop) return;
     const double* U = c.U[L.in];
@@ -362,6 +363,7 @@ __device__ inline void boundary_block(const CaseDev& c, const double* U, int bx)
 template <int FL>
 __global__ void __launch_bounds__(256) k_prestage(Launch La) {
     const Work w = La.work[blockIdx.x];
+    if (w.cs < 0) return;   // padding entry of an XCD-ordered table
     const CaseDev& c = La.cases[w.cs];
     if (*c.stop || c.real != FL) return;
     if (w.z == 0) boundary_block<FL>(c, c.U[La.in], w.x);
