@@ -210,15 +210,13 @@ struct KQueue<0, FL> {
         for (int v = 0; v < 4; ++v) w[v + 1] = Wv[0][v];
     }
     template <typename FT>
-    __device__ __forceinline__ void face(const CaseDev& c, int f, double out[5]) const {
+    __device__ __forceinline__ void face(const CaseDev& c, const double nn[4], double out[5]) const {
         const double sl = jst_sensor<double>(P[0], P[1], P[2]);
         const double sr = jst_sensor<double>(P[1], P[2], P[3]);
         const double Wl[5] = {U[1][0], Wv[0][0], Wv[0][1], Wv[0][2], Wv[0][3]};
         const double Wr[5] = {U[2][0], Wv[1][0], Wv[1][1], Wv[1][2], Wv[1][3]};
-        const double* nk = c.nrm + c.noffK;
-        const int nf = c.nfK;
-        jst_face<FT, FL>(U[0], U[1], U[2], U[3], Wl, Wr, RF[0], RF[1], sl, sr, gld(nk, f), gld(nk + nf, f),
-                         gld(nk + 2 * nf, f), gld(nk + 3 * nf, f), Gas<FL>(c), out);
+        jst_face<FT, FL>(U[0], U[1], U[2], U[3], Wl, Wr, RF[0], RF[1], sl, sr, nn[0], nn[1], nn[2], nn[3],
+                         Gas<FL>(c), out);
     }
 };
 
@@ -244,11 +242,8 @@ struct KQueue<1, FL> {
         for (int v = 0; v < 5; ++v) w[v] = W[1][v];
     }
     template <typename FT>
-    __device__ __forceinline__ void face(const CaseDev& c, int f, double out[5]) const {
-        const double* nk = c.nrm + c.noffK;
-        const int nf = c.nfK;
-        roe_face<FT, FL>(W[0], W[1], W[2], W[3], c.muscl, c.limiter, gld(nk, f), gld(nk + nf, f),
-                         gld(nk + 2 * nf, f), gld(nk + 3 * nf, f), Gas<FL>(c), out);
+    __device__ __forceinline__ void face(const CaseDev& c, const double nn[4], double out[5]) const {
+        roe_face<FT, FL>(W[0], W[1], W[2], W[3], c.muscl, c.limiter, nn[0], nn[1], nn[2], nn[3], Gas<FL>(c), out);
     }
 };
 
@@ -273,6 +268,14 @@ struct FaceIdx {
         bnd = lo || gf == nf;
     }
 };
+
+// Unit normal and area of K face f (index (k*NJ + j)*NI + i: the face below plane k).
+__device__ __forceinline__ void kface_normal(const CaseDev& c, int f, double nn[4]) {
+    const double* nk = c.nrm + c.noffK;
+    const long nf = c.nfK;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) nn[v] = gld(nk + v * nf, (unsigned)f);
+}
 
 // Unit normal and area of an interior I or J face on plane k.
 template <class TL>
@@ -348,7 +351,9 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
 #pragma unroll
                 for (int v = 0; v < 5; ++v) FKl[v] = gld(c.bnd + c.boffK + (long)v * NIJ, (unsigned)(j * NI + i));
             } else {
-                q.template face<FT>(c, (kc0 * NJ + j) * NI + i, FKl);
+                double nk[4];
+                kface_normal(c, (kc0 * NJ + j) * NI + i, nk);
+                q.template face<FT>(c, nk, FKl);
             }
         }
     }
@@ -367,7 +372,10 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
     for (int k = kc0; k < kc1; ++k) {
         const int n = (k * NJ + j) * NI + i;
         const bool parked = THREED && c.periodic && (k == 0 || k == NK - 1);
-        // unit normal and area of the thread's first I/J face of this plane
+        // unit normal and area of the K face above the node and of the thread's first I/J
+        // face of this plane, requested before the plane is staged
+        double nnk[4] = {1.0, 0.0, 0.0, 1.0};
+        if (THREED && act && !bladecol && k + 1 < NK) kface_normal(c, ((k + 1) * NJ + j) * NI + i, nnk);
         double nnx[4] = {1.0, 0.0, 0.0, 1.0};
         {
             const FaceIdx<TL> f(tid, i0, j0, NI, NJ);
@@ -457,7 +465,7 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
                 for (int v = 0; v < 5; ++v)
                     FKr[v] = gld(c.bnd + c.boffK + (long)(5 + v) * NIJ, (unsigned)(j * NI + i));
             } else {
-                q.template face<FT>(c, ((k + 1) * NJ + j) * NI + i, FKr);
+                q.template face<FT>(c, nnk, FKr);
             }
         }
         __syncthreads();

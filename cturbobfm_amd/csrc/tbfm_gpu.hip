@@ -616,6 +616,17 @@ void setup(tbfm_case* cs, const GridInput& grid) {
     c.conv.density_only = 0;
     long one = 1;
     HIPCHK(hipMemcpy(c.iter, &one, sizeof(long), hipMemcpyHostToDevice));
+    // bisection tables of the ideal-gas inlet patches (exact: the same device operations)
+    if (!c.real)
+        for (size_t q = 0; q < cs->patches.size(); ++q)
+            if (c.patch[q].type == dev::PT_INLET) {
+                double* tab = dalloc<double>(2 * (size_t)dev::INLET_NODES, own);
+                hipLaunchKernelGGL(dev::k_inlet_table, dim3((dev::INLET_NODES + 255) / 256), dim3(256), 0,
+                                   cs->solo.stream, tab, c.patch[q].v0, c.patch[q].v1, c.gamma, c.rgas);
+                HIPCHK(hipGetLastError());
+                c.patch[q].tab = tab;
+            }
+    HIPCHK(hipStreamSynchronize(cs->solo.stream));
     cs->cdev = dalloc<CaseDev>(1, own);
     HIPCHK(hipMemcpy(cs->cdev, &c, sizeof(CaseDev), hipMemcpyHostToDevice));
     g_bytes = nullptr;

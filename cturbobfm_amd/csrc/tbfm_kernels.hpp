@@ -30,7 +30,17 @@ struct PatchDev {
     int type;
     int profile;        // radial-equilibrium profile index
     double v0, v1;      // inlet: p0, T0; outlet: p
+    const double* tab;  // ideal-gas inlet: bisection table (below), null otherwise
 };
+
+// Ideal-gas subsonic inlet: along the pressure bisection of boundary_base.cpp:51-68 the
+// static temperature T(p) and speed V(p) of the inlet total state depend only on the
+// patch's p0, T0 and on the visited midpoint, never on the interior state.  The first
+// INLET_LEVELS levels of the bisection tree (node q has children 2q+1: p_low moves up,
+// and 2q+2: p_high moves down) are tabulated once per patch: T at tab[q], V at
+// tab[INLET_NODES + q], computed with the same operations as the on-the-fly values.
+constexpr int INLET_LEVELS = 16;
+constexpr int INLET_NODES = (1 << INLET_LEVELS) - 1;
 
 constexpr int MAX_PATCH = 32;
 constexpr int MAX_PROF = 4;

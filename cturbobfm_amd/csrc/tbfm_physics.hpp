@@ -330,8 +330,9 @@ __device__ __forceinline__ void roe_face(const double Wll_[5], const double Wl_[
 // interior state and a bisection on the boundary pressure along the inlet total state,
 // isentropic for the ideal gas, along the entropy s(p0, T0) for the table fluid.
 template <int FL>
-__device__ inline void inlet_flux(const CaseDev& c, const Gas<FL>& gs, const double U[5], double sx, double sy,
-                                  double sz, double pt, double Tt, double dx, double dy, double dz, double F[5]) {
+__device__ inline void inlet_flux(const CaseDev& c, const Gas<FL>& gs, const double* tab, const double U[5], double sx,
+                                  double sy, double sz, double pt, double Tt, double dx, double dy, double dz,
+                                  double F[5]) {
     double W[5], p, rf;
     derive<double, FL>(U, W, p, rf, gs);
     // computeSoundSpeed_rho_u_et
@@ -351,19 +352,35 @@ __device__ inline void inlet_flux(const CaseDev& c, const Gas<FL>& gs, const dou
         double cp = g * Rg / (g - 1.0);
         double htIn = cp * Tt;
         const double ex = (g - 1.0) / g;
+        // tree node of the midpoint while inside the patch table, and that of pB
+        int qn = 0, qB = -1;
         for (int it = 0; it < 30; ++it) {
             double pMid = 0.5 * (pLow + pHigh);
-            double TMid = Tt * pow(pMid / pt, ex);
-            double hMid = cp * TMid;
-            double VMid = sqrt(2.0 * dmaxd(0.0, htIn - hMid));
+            const bool inTab = tab && it < INLET_LEVELS;
+            double VMid;
+            if (inTab) {
+                VMid = gld(tab + INLET_NODES, (unsigned)qn);
+            } else {
+                double TMid = Tt * pow(pMid / pt, ex);
+                double hMid = cp * TMid;
+                VMid = sqrt(2.0 * dmaxd(0.0, htIn - hMid));
+            }
             double res = pMid + Z * Cg * VMid - Rout;
-            if (fabs(res) < 1e-4 * pt) { pB = pMid; break; }
-            if (res < 0.0) pLow = pMid; else pHigh = pMid;
+            if (fabs(res) < 1e-4 * pt) { pB = pMid; qB = inTab ? qn : -1; break; }
             pB = pMid;
+            qB = inTab ? qn : -1;
+            if (res < 0.0) { pLow = pMid; qn = 2 * qn + 1; } else { pHigh = pMid; qn = 2 * qn + 2; }
+            if (!inTab) qn = 0;
         }
-        double TB = Tt * pow(pB / pt, ex);
-        double hB = cp * TB;
-        double VB = sqrt(2.0 * dmaxd(0.0, htIn - hB));
+        double TB, VB;
+        if (qB >= 0) {
+            TB = gld(tab, (unsigned)qB);
+            VB = gld(tab + INLET_NODES, (unsigned)qB);
+        } else {
+            TB = Tt * pow(pB / pt, ex);
+            double hB = cp * TB;
+            VB = sqrt(2.0 * dmaxd(0.0, htIn - hB));
+        }
         double vx = dx * VB, vy = dy * VB, vz = dz * VB;
         double rhoB = pB / (Rg * TB);
         double eB = pB / ((g - 1) * rhoB);

@@ -340,7 +340,7 @@ __device__ inline void boundary_block(const CaseDev& c, const double* U, int bx)
     const Gas<FL> gs(c);
     if (pa.type == PT_INLET) {
         const double* d = dirv + (long)side * 3 * nsz;
-        inlet_flux<FL>(c, gs, Un, Sx, Sy, Sz, pa.v0, pa.v1, d[q], d[nsz + q], d[2 * nsz + q], F);
+        inlet_flux<FL>(c, gs, pa.tab, Un, Sx, Sy, Sz, pa.v0, pa.v1, d[q], d[nsz + q], d[2 * nsz + q], F);
     } else if (pa.type == PT_OUTLET) {
         outlet_flux<FL>(c, gs, Un, Sx, Sy, Sz, pa.v0, it, F);
     } else if (pa.type == PT_RADIAL) {
@@ -352,6 +352,29 @@ __device__ inline void boundary_block(const CaseDev& c, const double* U, int bx)
     double* o = out + (long)side * 5 * nsz;
 #pragma unroll
     for (int v = 0; v < 5; ++v) o[(long)v * nsz + q] = F[v] * sm;
+}
+
+// Bisection table of an ideal-gas inlet patch (p0, T0): node q of the first INLET_LEVELS
+// levels, reached from the root by the halvings its heap index encodes; T and V of its
+// midpoint exactly as the inlet flux computes them (boundary_base.cpp:51-68).
+__global__ void __launch_bounds__(256) k_inlet_table(double* tab, double pt, double Tt, double g, double Rg) {
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= INLET_NODES) return;
+    const double cp = g * Rg / (g - 1.0);
+    const double htIn = cp * Tt;
+    const double ex = (g - 1.0) / g;
+    double pLow = 0.05 * pt, pHigh = 0.999999 * pt;
+    const int level = 31 - __clz(q + 1);
+    const int pos = q + 1 - (1 << level);
+    for (int s = level - 1; s >= 0; --s) {
+        const double pMid = 0.5 * (pLow + pHigh);
+        if ((pos >> s) & 1) pHigh = pMid; else pLow = pMid;
+    }
+    const double pMid = 0.5 * (pLow + pHigh);
+    const double TMid = Tt * pow(pMid / pt, ex);
+    const double hMid = cp * TMid;
+    tab[q] = TMid;
+    tab[INLET_NODES + q] = sqrt(2.0 * dmaxd(0.0, htIn - hMid));
 }
 
 // Ahead of each stage kernel: boundary-face fluxes (z = 0, one block per 256 boundary
