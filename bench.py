@@ -82,10 +82,11 @@ def bench_batch(args):
     t_setup = time.perf_counter()
     b = Batch()
     members = []
-    for sz, p in pts:
+    for q, (sz, p) in enumerate(pts):
         s = Solver.from_case(cases.rotor3d(*sz, n_iterations=its, outlet_pressure=p), {"GPU_PRECISION": "mixed"})
         b.admit(s)
         members.append(s)
+        print("admitted %d/%d: rotor %dx%dx%d p_out %.0f" % (q + 1, len(pts), *sz, p), file=sys.stderr, flush=True)
     t_setup = time.perf_counter() - t_setup
     nodes = sum(s.N for s in members)
     b.advance(args.warmup)
@@ -135,12 +136,14 @@ def bench_serve(args):
     specs = serve.mixed_stream(args.serve_cases, seed=args.seed, precision="mixed", real_fraction=0.05,
                                lut_path=lut)
     arrivals = serve.poisson_arrivals(len(specs), rate=args.rate, seed=args.seed)
-    srv = serve.Server(drop=args.drop, max_iter=args.max_iter, chunk=100, keep_turbo=True)
+    progress = lambda m: print(m, file=sys.stderr, flush=True)
+    srv = serve.Server(drop=args.drop, max_iter=args.max_iter, chunk=100, keep_turbo=True, log=progress)
     t0 = time.perf_counter()
     res = srv.run(specs, arrivals)
     wall = time.perf_counter() - t0
     served = serve.summary(res, wall)
-    alone = serve.sequential(specs, drop=args.drop, max_iter=args.max_iter, precision="fp64", keep_turbo=True)
+    alone = serve.sequential(specs, drop=args.drop, max_iter=args.max_iter, precision="fp64", keep_turbo=True,
+                             log=progress)
     # converged mass flow and total-pressure ratio, served (FP32 flux) against plain FP64
     dm, dpr = [], []
     for r, a in zip(res, alone):

@@ -36,6 +36,14 @@ struct HipError : std::runtime_error {
 
 thread_local size_t* g_bytes = nullptr;   // device-byte counter of the case being set up
 
+// Serialises the structural device work of all host threads: allocation, release and
+// initialisation of device memory, work-table builds and stream capture.  Cases and
+// batches may be driven from several host threads (the serving loop sets requests up
+// on a worker while batches advance); replays of captured graphs and stream waits run
+// outside the lock, so the GPU work itself still overlaps.
+std::recursive_mutex g_dev_mu;
+using DevLock = std::lock_guard<std::recursive_mutex>;
+
 template <typename T>
 T* dalloc(size_t n, std::vector<void*>& owned) {
     void* p = nullptr;
@@ -1070,6 +1078,7 @@ void autotune(tbfm_case* cs) {
 // Grows a case's device residual history to hold iteration `last`.
 void growHistory(tbfm_case* c, long last) {
     if (last <= c->hist_cap) return;
+    DevLock lk(g_dev_mu);
     const long cap = std::max(last, 2 * c->hist_cap);
     size_t* saved = g_bytes;
     g_bytes = &c->dev_bytes;
@@ -1103,6 +1112,7 @@ void tbfm_gpu_dims(const tbfm_case* c, long* ni, long* nj, long* nk) {
 
 int tbfm_gpu_case_create(const tbfm_case_desc* d, tbfm_case** out) {
     if (!d || !out) return fail(nullptr, TBFM_ERR_ARGUMENT, "null argument");
+    DevLock lk(g_dev_mu);
     *out = nullptr;
     auto cs = std::make_unique<tbfm_case>();
     int rc = guarded((std::string*)nullptr, [&]() {
@@ -1138,6 +1148,7 @@ int tbfm_gpu_case_create(const tbfm_case_desc* d, tbfm_case** out) {
 
 int tbfm_gpu_case_create_from_ini(const char* ini, tbfm_case** out) {
     if (!ini || !out) return fail(nullptr, TBFM_ERR_ARGUMENT, "null argument");
+    DevLock lk(g_dev_mu);
     *out = nullptr;
     auto cs = std::make_unique<tbfm_case>();
     int rc = guarded((std::string*)nullptr, [&]() {
@@ -1161,6 +1172,7 @@ int tbfm_gpu_batch_retire(tbfm_batch* b, tbfm_case* c);
 
 int tbfm_gpu_case_destroy(tbfm_case* c) {
     if (!c) return TBFM_OK;
+    DevLock lk(g_dev_mu);
     if (c->batch) tbfm_gpu_batch_retire(c->batch, c);
     freePlan(c->solo);
     for (void* p : c->owned) hipFree(p);
@@ -1173,6 +1185,7 @@ int tbfm_gpu_case_destroy(tbfm_case* c) {
 int tbfm_gpu_set_state(tbfm_case* c, const double* U) {
     if (!c || !U) return fail(c, TBFM_ERR_ARGUMENT, "null argument");
     return guarded(c, [&]() {
+        DevLock lk(g_dev_mu);
         std::vector<double> dU(5 * c->N), tmp;
         for (int v = 0; v < 5; ++v) {
             toDevice(c, U + v * c->N, tmp);
@@ -1205,6 +1218,7 @@ int tbfm_gpu_set_state(tbfm_case* c, const double* U) {
 int tbfm_gpu_get_state(tbfm_case* c, double* U) {
     if (!c || !U) return fail(c, TBFM_ERR_ARGUMENT, "null argument");
     return guarded(c, [&]() {
+        DevLock lk(g_dev_mu);
         std::vector<double> dU(5 * c->N);
         HIPCHK(hipMemcpyAsync(dU.data(), c->c.U[0], dU.size() * 8, hipMemcpyDeviceToHost, c->solo.stream));
         HIPCHK(hipStreamSynchronize(c->solo.stream));
@@ -1242,21 +1256,26 @@ int tbfm_gpu_advance(tbfm_case* c, long n, double* logres, long* converged_at) {
     if (c->batch) return fail(c, TBFM_ERR_ARGUMENT, "case is admitted to a batch: advance the batch");
     return guarded(c, [&]() {
         const long first = c->iter_host;
-        growHistory(c, first + n - 1);
-        autotune(c);
         Plan& P = c->solo;
-        if (!P.tablesValid) buildTables(P);
-        int stopped = 0;
-        HIPCHK(hipMemcpy(&stopped, c->c.stop, sizeof(int), hipMemcpyDeviceToHost));
-        if (!stopped) {
-            if (c->prof) {
-                for (long q = 0; q < n; ++q) launchIteration(P, c);
-            } else {
-                capture(P);
-                for (long q = 0; q < n; ++q) HIPCHK(hipGraphLaunch(P.graph, P.stream));
-            }
+        {
+            DevLock lk(g_dev_mu);
+            growHistory(c, first + n - 1);
+            autotune(c);
+            if (!P.tablesValid) buildTables(P);
+            if (!c->prof) capture(P);
         }
+        int stopped = 0;
+        {
+            DevLock lk(g_dev_mu);
+            HIPCHK(hipMemcpy(&stopped, c->c.stop, sizeof(int), hipMemcpyDeviceToHost));
+            if (!stopped && c->prof)
+                for (long q = 0; q < n; ++q) launchIteration(P, c);
+        }
+        // graph replays and the wait run outside the lock
+        if (!stopped && !c->prof)
+            for (long q = 0; q < n; ++q) HIPCHK(hipGraphLaunch(P.graph, P.stream));
         HIPCHK(hipStreamSynchronize(P.stream));
+        DevLock lk(g_dev_mu);
         if (c->prof) drainProfile(c);
         // iterations launched after the device stopped (convergence or divergence) were
         // no-ops; the state in buffer A is the one after the last executed iteration
@@ -1268,6 +1287,7 @@ int tbfm_gpu_advance(tbfm_case* c, long n, double* logres, long* converged_at) {
 int tbfm_gpu_compute_timestep(tbfm_case* c, const double* U, double* dt) {
     if (!c || !U || !dt) return fail(c, TBFM_ERR_ARGUMENT, "null argument");
     return guarded(c, [&]() {
+        DevLock lk(g_dev_mu);
         std::vector<double> dU(5 * c->N), tmp;
         for (int v = 0; v < 5; ++v) {
             toDevice(c, U + v * c->N, tmp);
@@ -1296,6 +1316,7 @@ int tbfm_gpu_compute_timestep(tbfm_case* c, const double* U, double* dt) {
 int tbfm_gpu_compute_residual(tbfm_case* c, const double* U, long it, int update_profiles, double* R) {
     if (!c || !U || !R) return fail(c, TBFM_ERR_ARGUMENT, "null argument");
     return guarded(c, [&]() {
+        DevLock lk(g_dev_mu);
         std::vector<double> dU(5 * c->N), tmp;
         for (int v = 0; v < 5; ++v) {
             toDevice(c, U + v * c->N, tmp);
@@ -1326,6 +1347,7 @@ int tbfm_gpu_compute_residual(tbfm_case* c, const double* U, long it, int update
 int tbfm_gpu_get_side_outputs(tbfm_case* c, double* finv, double* fvisc, double* devang, double* dt) {
     if (!c) return fail(c, TBFM_ERR_ARGUMENT, "null argument");
     return guarded(c, [&]() {
+        DevLock lk(g_dev_mu);
         HIPCHK(hipStreamSynchronize(c->solo.stream));
         std::vector<double> h3(3 * c->N), h(c->N), comp(c->N), out(c->N);
         auto vec3 = [&](const double* d, double* dst) {
@@ -1359,6 +1381,7 @@ int tbfm_gpu_get_side_outputs(tbfm_case* c, double* finv, double* fvisc, double*
 int tbfm_gpu_get_turbo(tbfm_case* c, double* out) {
     if (!c || !out) return fail(c, TBFM_ERR_ARGUMENT, "null argument");
     return guarded(c, [&]() {
+        DevLock lk(g_dev_mu);
         // boundary slices are small: evaluate on the host from the device state
         std::vector<double> U(5 * c->N);
         int rc = tbfm_gpu_get_state(c, U.data());
@@ -1516,6 +1539,7 @@ void drainProfile(tbfm_case* c) {
 int tbfm_gpu_profile(tbfm_case* c, int enable, int reset) {
     if (!c) return fail(c, TBFM_ERR_ARGUMENT, "null argument");
     return guarded(c, [&]() {
+        DevLock lk(g_dev_mu);
         drainProfile(c);
         c->prof = enable != 0;
         if (reset) { c->prof_ms = 0.0; c->prof_launches = 0; }
@@ -1526,6 +1550,7 @@ int tbfm_gpu_profile(tbfm_case* c, int enable, int reset) {
 int tbfm_gpu_profile_get(tbfm_case* c, double* ms, long* launches, double* bytes) {
     if (!c) return fail(c, TBFM_ERR_ARGUMENT, "null argument");
     return guarded(c, [&]() {
+        DevLock lk(g_dev_mu);
         drainProfile(c);
         if (ms) *ms = c->prof_ms;
         if (launches) *launches = c->prof_launches;
@@ -1546,6 +1571,7 @@ __global__ void __launch_bounds__(256) k_copy_calibration(const double* __restri
 int tbfm_gpu_copy_calibration(long nbytes, int repeats, double* ms_per_copy) {
     if (nbytes <= 0 || repeats <= 0) return fail(nullptr, TBFM_ERR_ARGUMENT, "bad argument");
     return guarded((std::string*)nullptr, [&]() {
+        DevLock lk(g_dev_mu);
         std::vector<void*> own;
         const long n = nbytes / 8;
         double* a = dalloc<double>(n, own);
@@ -1575,6 +1601,7 @@ const char* tbfm_gpu_tuning(const tbfm_case* c) { return c ? c->tuning.c_str() :
 int tbfm_gpu_sync(tbfm_case* c) {
     if (!c) return fail(c, TBFM_ERR_ARGUMENT, "null argument");
     return guarded(c, [&]() {
+        DevLock lk(g_dev_mu);
         HIPCHK(hipStreamSynchronize(c->solo.stream));
         return TBFM_OK;
     });
@@ -1585,6 +1612,7 @@ int tbfm_gpu_sync(tbfm_case* c) {
 int tbfm_gpu_device_memory(long* free_bytes, long* total_bytes) {
     if (!free_bytes || !total_bytes) return fail(nullptr, TBFM_ERR_ARGUMENT, "null argument");
     return guarded((std::string*)nullptr, [&]() {
+        DevLock lk(g_dev_mu);
         size_t f = 0, t = 0;
         HIPCHK(hipMemGetInfo(&f, &t));
         *free_bytes = (long)f;
@@ -1602,6 +1630,7 @@ int tbfm_gpu_case_device_bytes(const tbfm_case* c, long* bytes) {
 int tbfm_gpu_case_set_convergence(tbfm_case* c, int density_only, double drop) {
     if (!c || !(drop > 0.0)) return fail(c, TBFM_ERR_ARGUMENT, "bad argument");
     return guarded(c, [&]() {
+        DevLock lk(g_dev_mu);
         c->c.conv.density_only = density_only ? 1 : 0;
         c->c.conv.drop = drop;
         c->c.conv.steady = 1;
@@ -1613,6 +1642,7 @@ int tbfm_gpu_case_set_convergence(tbfm_case* c, int density_only, double drop) {
 int tbfm_gpu_case_status(tbfm_case* c, long* iterations, long* converged_at, int* stopped, double* last_logres) {
     if (!c) return fail(c, TBFM_ERR_ARGUMENT, "null argument");
     return guarded(c, [&]() {
+        DevLock lk(g_dev_mu);
         long conv = 0;
         int st = 0;
         HIPCHK(hipMemcpy(&conv, c->c.conv.conv_at, sizeof(long), hipMemcpyDeviceToHost));
@@ -1643,6 +1673,7 @@ int tbfm_gpu_batch_create(tbfm_batch** out) {
     *out = nullptr;
     auto b = std::make_unique<tbfm_batch>();
     int rc = guarded((std::string*)nullptr, [&]() {
+        DevLock lk(g_dev_mu);
         HIPCHK(hipStreamCreateWithFlags(&b->plan.stream, hipStreamNonBlocking));
         b->plan.ownsCases = true;
         return TBFM_OK;
@@ -1656,6 +1687,7 @@ int tbfm_gpu_batch_admit(tbfm_batch* b, tbfm_case* c) {
     if (!b || !c) return failb(b, TBFM_ERR_ARGUMENT, "null argument");
     if (c->batch) return failb(b, TBFM_ERR_ARGUMENT, "case is already admitted to a batch");
     return guarded(&b->err, [&]() {
+        DevLock lk(g_dev_mu);
         Plan& P = b->plan;
         if (!P.members.empty() && P.members[0]->c.nstages != c->c.nstages)
             throw std::runtime_error("cases of one batch must use the same RK scheme");
@@ -1672,6 +1704,7 @@ int tbfm_gpu_batch_retire(tbfm_batch* b, tbfm_case* c) {
     if (!b || !c) return failb(b, TBFM_ERR_ARGUMENT, "null argument");
     if (c->batch != b) return failb(b, TBFM_ERR_ARGUMENT, "case is not a member of this batch");
     return guarded(&b->err, [&]() {
+        DevLock lk(g_dev_mu);
         Plan& P = b->plan;
         HIPCHK(hipStreamSynchronize(P.stream));
         P.members.erase(std::find(P.members.begin(), P.members.end(), c));
@@ -1689,14 +1722,18 @@ int tbfm_gpu_batch_advance(tbfm_batch* b, long n) {
     return guarded(&b->err, [&]() {
         Plan& P = b->plan;
         if (P.members.empty() || n == 0) return TBFM_OK;
-        for (tbfm_case* m : P.members) growHistory(m, m->iter_host + n - 1);
-        if (!P.tablesValid) {
-            uploadCases(P);
-            buildTables(P);
+        {
+            DevLock lk(g_dev_mu);
+            for (tbfm_case* m : P.members) growHistory(m, m->iter_host + n - 1);
+            if (!P.tablesValid) {
+                uploadCases(P);
+                buildTables(P);
+            }
+            capture(P);
         }
-        capture(P);
         for (long q = 0; q < n; ++q) HIPCHK(hipGraphLaunch(P.graph, P.stream));
         HIPCHK(hipStreamSynchronize(P.stream));
+        DevLock lk(g_dev_mu);
         for (tbfm_case* m : P.members) settle(m);
         return TBFM_OK;
     });
@@ -1704,6 +1741,7 @@ int tbfm_gpu_batch_advance(tbfm_batch* b, long n) {
 
 int tbfm_gpu_batch_destroy(tbfm_batch* b) {
     if (!b) return TBFM_OK;
+    DevLock lk(g_dev_mu);
     if (b->plan.stream) hipStreamSynchronize(b->plan.stream);
     for (tbfm_case* m : b->plan.members) m->batch = nullptr;
     b->plan.members.clear();

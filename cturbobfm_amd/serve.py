@@ -124,83 +124,159 @@ class Server:
     max_iter    iteration cap: a case that has not converged by then retires unconverged
     chunk       iterations per batch advance between admission / retirement decisions
     budget      device bytes the resident cases may hold (default: 80 % of free memory)
+    lanes       node-count upper bounds of the batches that run side by side: a request
+                joins the first lane its grid fits.  Each lane is one batch advanced by its
+                own host thread on its own stream, so cheap small grids iterate at their own
+                rate instead of at the pace of the largest resident grid.
     """
 
     def __init__(self, drop: float = 8.0, max_iter: int = 20000, chunk: int = 100,
-                 budget: Optional[int] = None, keep_states: bool = False, keep_turbo: bool = False):
+                 budget: Optional[int] = None, keep_states: bool = False, keep_turbo: bool = False,
+                 log: Optional[Callable[[str], None]] = None, lanes=(50000, None)):
         self.drop = drop
         self.max_iter = max_iter
         self.chunk = chunk
         self.budget = budget if budget is not None else int(0.8 * device_memory()[0])
         self.keep_states = keep_states
         self.keep_turbo = keep_turbo
+        self.log = log
+        self.lanes = list(lanes)
         self.bytes_per_node = {True: 1500.0, False: 1500.0}   # refined from the first cases
 
     def _estimate(self, spec: CaseSpec) -> int:
         return int(self.bytes_per_node[spec.three_d] * spec.nodes) + (1 << 20)
 
+    def _lane_of(self, spec: CaseSpec) -> int:
+        for q, cap in enumerate(self.lanes):
+            if cap is None or spec.nodes <= cap:
+                return q
+        return len(self.lanes) - 1
+
+    def _build(self, spec: CaseSpec) -> Solver:
+        s = Solver.from_case(spec.build(), spec.overrides)
+        s.set_convergence(True, self.drop)
+        return s
+
     def run(self, specs: List[CaseSpec], arrivals: np.ndarray) -> List[CaseResult]:
+        """Serves the stream.  The main thread takes arrivals and sets requests up (grid,
+        geometry, device buffers) on a worker pool as soon as memory allows; each lane
+        thread admits its set-up requests in arrival order at its chunk boundaries,
+        advances its batch, and retires converged, diverged or capped cases."""
+        import threading
+        from concurrent.futures import ThreadPoolExecutor
         results = [CaseResult(s.name, s.nodes, float(a)) for s, a in zip(specs, arrivals)]
         order = list(np.argsort(arrivals))
-        batch = Batch()
-        resident: Dict[int, Solver] = {}
-        footprint: Dict[int, int] = {}
-        queue: deque = deque()
-        used = 0
-        nxt = 0
+        nl = len(self.lanes)
+        lock = threading.Lock()
+        queues = [deque() for _ in range(nl)]       # arrived requests of each lane, in order
+        building: Dict[int, object] = {}            # request -> future of its Solver
+        reserved: Dict[int, int] = {}               # estimated bytes of requests in set-up
+        state = {"used": 0, "done": 0, "error": None}
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0
-        done = 0
-        while done < len(specs):
-            while nxt < len(order) and arrivals[order[nxt]] <= now():
-                queue.append(order[nxt])
-                nxt += 1
-            # admission, first come first served, by device-memory footprint
-            while queue and (not resident or used + self._estimate(specs[queue[0]]) <= self.budget):
-                q = queue.popleft()
-                sp = specs[q]
-                s = Solver.from_case(sp.build(), sp.overrides)
-                s.set_convergence(True, self.drop)
-                b = s.device_bytes()
-                self.bytes_per_node[sp.three_d] = b / sp.nodes
-                batch.admit(s)
-                resident[q] = s
-                footprint[q] = b
-                used += b
-                results[q].admitted = now()
-            if not resident:
-                if nxt < len(order):
-                    time.sleep(max(0.0, arrivals[order[nxt]] - now()))
-                continue
-            batch.advance(self.chunk)
-            for q in list(resident):
-                s = resident[q]
-                r = results[q]
-                try:
-                    it, conv, stopped, _ = s.status()
-                except SolverError:
-                    it, conv, stopped = s.iterations, 0, True
-                    r.diverged = True
-                if stopped or it >= self.max_iter:
-                    r.finished = now()
-                    r.iterations = it
-                    r.converged = conv > 0
-                    if self.keep_states:
-                        r.state = s.state()
-                    if self.keep_turbo and not r.diverged:
-                        r.turbo = s.turbo()
-                    batch.retire(s)
-                    s.close()
-                    used -= footprint.pop(q)
-                    del resident[q]
-                    done += 1
-        batch.close()
+        pool = ThreadPoolExecutor(max_workers=2)
+
+        def lane_main(li: int):
+            batch = Batch()
+            resident: Dict[int, Solver] = {}
+            footprint: Dict[int, int] = {}
+            try:
+                while True:
+                    with lock:
+                        if state["done"] == len(specs) or state["error"]:
+                            return
+                        ready = []
+                        while queues[li] and queues[li][0] in building and building[queues[li][0]].done():
+                            q = queues[li].popleft()
+                            ready.append((q, building.pop(q)))
+                            reserved.pop(q)
+                    for q, fut in ready:
+                        s = fut.result()
+                        sp = specs[q]
+                        b = s.device_bytes()
+                        batch.admit(s)
+                        with lock:
+                            self.bytes_per_node[sp.three_d] = b / sp.nodes
+                            resident[q] = s
+                            footprint[q] = b
+                            state["used"] += b
+                            results[q].admitted = now()
+                    if not resident:
+                        time.sleep(0.002)
+                        continue
+                    batch.advance(self.chunk)
+                    for q in list(resident):
+                        s = resident[q]
+                        r = results[q]
+                        try:
+                            it, conv, stopped, _ = s.status()
+                        except SolverError:
+                            it, conv, stopped = s.iterations, 0, True
+                            r.diverged = True
+                        if stopped or it >= self.max_iter:
+                            r.finished = now()
+                            r.iterations = it
+                            r.converged = conv > 0
+                            if self.keep_states:
+                                r.state = s.state()
+                            if self.keep_turbo and not r.diverged:
+                                r.turbo = s.turbo()
+                            batch.retire(s)
+                            s.close()
+                            del resident[q]
+                            with lock:
+                                state["used"] -= footprint.pop(q)
+                                state["done"] += 1
+                                done = state["done"]
+                            if self.log:
+                                self.log("served %d/%d: %s, %d iterations, %.2f s after arrival"
+                                         % (done, len(specs), r.name, r.iterations, r.time_to_converge))
+            except BaseException as e:   # surfaced by the main thread
+                with lock:
+                    state["error"] = e
+            finally:
+                batch.close()
+
+        threads = [threading.Thread(target=lane_main, args=(li,), daemon=True) for li in range(nl)]
+        for t in threads:
+            t.start()
+        nxt = 0
+        try:
+            while True:
+                with lock:
+                    if state["error"]:
+                        raise state["error"]
+                    if state["done"] == len(specs):
+                        break
+                    while nxt < len(order) and arrivals[order[nxt]] <= now():
+                        q = order[nxt]
+                        queues[self._lane_of(specs[q])].append(q)
+                        nxt += 1
+                    # set-up ahead of admission, first come first served over all lanes,
+                    # while the estimated footprint of resident and in-set-up cases fits
+                    pending = sorted((q for qu in queues for q in qu if q not in building),
+                                     key=lambda q: arrivals[q])
+                    for q in pending:
+                        est = self._estimate(specs[q])
+                        busy = state["used"] > 0 or bool(building)
+                        if busy and state["used"] + sum(reserved.values()) + est > self.budget:
+                            break
+                        building[q] = pool.submit(self._build, specs[q])
+                        reserved[q] = est
+                time.sleep(0.001)
+        finally:
+            with lock:
+                if state["done"] != len(specs) and not state["error"]:
+                    state["error"] = RuntimeError("serving stopped")
+            for t in threads:
+                t.join()
+            pool.shutdown(wait=True)
         return results
 
 
 def sequential(specs: List[CaseSpec], drop: float, max_iter: int = 20000, chunk: int = 500,
                precision: Optional[str] = None, keep_states: bool = False,
-               keep_turbo: bool = False) -> List[CaseResult]:
+               keep_turbo: bool = False, log: Optional[Callable[[str], None]] = None) -> List[CaseResult]:
     """The plain path: each case alone, one after another, from set-up to convergence
     (precision overrides the spec's GPU_PRECISION when given)."""
     out = []
@@ -227,6 +303,9 @@ def sequential(specs: List[CaseSpec], drop: float, max_iter: int = 20000, chunk:
         s.close()
         r.finished = time.perf_counter() - t0
         out.append(r)
+        if log:
+            log("alone %d/%d: %s, %d iterations, %.2f s" % (len(out), len(specs), r.name, r.iterations,
+                                                            r.finished - r.admitted))
     return out
 
 
