@@ -232,7 +232,8 @@ void setup(tbfm_case* cs, const GridInput& grid) {
         if (cs->precision == Precision::Mixed)
             throw std::runtime_error("GPU_PRECISION = mixed needs FLUID_MODEL = ideal: the look-up-table fluid runs in FP64");
         std::string path = cfg.getFluidTableFile();
-        if (!path.empty() && path[0] != '/') path = cfg.baseDir + "/" + path;
+        // relative to the input.ini folder when read from one, else to the working folder
+        if (!path.empty() && path[0] != '/' && !cfg.baseDir.empty()) path = cfg.baseDir + "/" + path;
         const FluidTable ft = readFluidTable(path);
         static_assert(FluidTable::kTables == LUT_COUNT, "table count");
         size_t total = 0;

@@ -19,8 +19,14 @@
  *   - Every entry point returns TBFM_OK (0) or a negative code and never throws; the
  *     message of the last failure is tbfm_gpu_last_error(case) (or NULL case for
  *     creation failures).
- *   - One host thread may use a case at a time; each case owns its HIP stream, and
- *     different cases may be used concurrently.
+ *   - One host thread may use a case (or batch) at a time; each case and batch owns its
+ *     HIP stream, and different cases and batches may be driven from different host
+ *     threads concurrently (allocation, stream capture and synchronous copies are
+ *     serialised inside the library; graph replays overlap).
+ *   - Fluid: FLUID_MODEL = ideal (FLUID_GAMMA, FLUID_R_CONSTANT) or real, the look-up
+ *     table of FLUID_TABLE_FILE (default fluid_table_<FLUID_NAME>.lut; a relative path
+ *     is taken from the input.ini folder, or from the working folder for a case built
+ *     from a descriptor) in the reference's .lut format (src/fluid_real.cpp:12-209).  The table fluid computes in FP64 (GPU_PRECISION = mixed is refused).
  */
 #ifndef TBFM_GPU_H
 #define TBFM_GPU_H
