@@ -90,8 +90,12 @@ def rotor3d(ni: int = 193, nj: int = 49, nk: int = 129, n_iterations: int = 2000
             scheme: str = "JST", bfm_model: str = "Hall", cfl: float = 2.0,
             muscl: bool = False, limiter: str = "None", outlet_pressure: float = 95000.0,
             length: float = 0.8, blockage: bool = True, time_integration: int = 0,
-            time_step: str = "local", outlet: str = "radial_equilibrium") -> Dict:
-    """Annular rotor; the last k plane is a bitwise copy of k = 0 (full annulus)."""
+            time_step: str = "local", outlet: str = "radial_equilibrium", off_design: bool = False,
+            stall: bool = False) -> Dict:
+    """Annular rotor; the last k plane is a bitwise copy of k = 0 (full annulus).
+    bfm_model Hall, Lift-Drag or Hall-Thollet (K_n = K_f = K_d = 1, nu = 1.48e-5 as the
+    reference's GER4 cases; off_design adds the deviation-pivot term); stall switches on
+    STALLED_BFM_ACTIVE with a stall deviation of 0.12 rad."""
     p0, t0 = 101325.0, 288.15
     mach = math.sqrt(max(((p0 / outlet_pressure) ** ((GAMMA - 1.0) / GAMMA) - 1.0) * 2.0 / (GAMMA - 1.0), 1e-4))
     _, t_init = _static_from_total(p0, t0, mach)
@@ -133,6 +137,8 @@ def rotor3d(ni: int = 193, nj: int = 49, nk: int = 129, n_iterations: int = 2000
         "numberBlades": field(np.where(inside, 20.0, 0.0)),
         "liftDragBeta0": field(kappa),
         "liftDragKpEtamax": field(np.where(inside, 0.05, 0.0)),
+        "deviationAnglePivot": field(np.where(inside, 0.02, 0.0)),
+        "deviationAngleStall": field(np.where(inside, 0.12, 0.0)),
     }
     config = {
         "KIND_SOLVER": "Euler", "TOPOLOGY": "3D",
@@ -155,6 +161,13 @@ def rotor3d(ni: int = 193, nj: int = 49, nk: int = 129, n_iterations: int = 2000
     }
     if time_step == "fixed":
         config["FIXED_TIME_STEP"] = "1e-7"
+    if bfm_model == "Hall-Thollet":
+        config.update({"HALL_THOLLET_COEFFICIENT_KN": "1.0", "HALL_THOLLET_COEFFICIENT_KF": "1.0",
+                       "HALL_THOLLET_COEFFICIENT_KD": "1.0",
+                       "HALL_THOLLET_OFF_DESIGN_ACTIVE": "yes" if off_design else "no",
+                       "FLUID_KINEMATIC_VISCOSITY": "1.48E-5"})
+    if stall:
+        config["STALLED_BFM_ACTIVE"] = "yes"
     patches = [("INFLOW", 0, 0, 0, nj, 0, nk), ("OUTFLOW", ni, ni, 0, nj, 0, nk),
                ("HUB", 0, ni, 0, 0, 0, nk), ("SHROUD", 0, ni, nj, nj, 0, nk),
                ("PER1", 0, ni, 0, nj, 0, 0), ("PER2", 0, ni, 0, nj, nk, nk)]

@@ -22,7 +22,7 @@ TOPO = {"2D": 0, "2d": 0, "3D": 1, "3d": 1, "axisymmetric": 2, "Axisymmetric": 2
 LIMITER = {"None": 0, "van albada": 1, "van leer": 2, "min mod": 3}
 BC = {"inlet": 0, "outlet": 1, "radial_equilibrium": 2, "inviscid_wall": 3, "no_slip_wall": 3,
       "periodic": 4, "wedge": 5}
-BFM = {"None": 0, "Hall": 1, "Lift-Drag": 2, "Blockage": 3}
+BFM = {"None": 0, "Hall": 1, "Lift-Drag": 2, "Blockage": 3, "Hall-Thollet": 4}
 
 
 class Params(C.Structure):
@@ -35,7 +35,9 @@ class Params(C.Structure):
                 ("ramp_iterations", C.c_double), ("bfm_active", C.c_int), ("bfm_model", C.c_int),
                 ("blockage_active", C.c_int), ("le_index", C.c_long), ("te_index", C.c_long),
                 ("omega_scaling", C.c_double), ("inlet_cylindrical", C.c_int), ("turbo_dir", C.c_int),
-                ("residual_drop", C.c_long), ("steady", C.c_int), ("fluid_real", C.c_int)]
+                ("residual_drop", C.c_long), ("steady", C.c_int), ("fluid_real", C.c_int),
+                ("kn", C.c_double), ("kf", C.c_double), ("kd", C.c_double), ("thollet_off_design", C.c_int),
+                ("stall_active", C.c_int), ("nu", C.c_double)]
 
 
 class Table(C.Structure):
@@ -110,14 +112,16 @@ class Patch(C.Structure):
 class Fields(C.Structure):
     _fields_ = [(n, C.POINTER(C.c_double)) for n in (
         "blockage", "normal_axial", "normal_radial", "normal_tangential", "rpm", "blade_present",
-        "number_blades", "metal_angle", "streamwise_length", "lift_drag_beta0", "lift_drag_kp")]
+        "number_blades", "metal_angle", "streamwise_length", "lift_drag_beta0", "lift_drag_kp",
+        "deviation_angle_pivot", "deviation_angle_stall")]
 
 
 FIELD_COLUMNS = {"blockage": "blockage", "normal_axial": "normalAxial",
                  "normal_radial": "normalRadial", "normal_tangential": "normalTangential",
                  "rpm": "rpm", "blade_present": "bladePresent", "number_blades": "numberBlades",
                  "metal_angle": "bladeMetalAngle", "streamwise_length": "streamwiseLength",
-                 "lift_drag_beta0": "liftDragBeta0", "lift_drag_kp": "liftDragKpEtamax"}
+                 "lift_drag_beta0": "liftDragBeta0", "lift_drag_kp": "liftDragKpEtamax",
+                 "deviation_angle_pivot": "deviationAnglePivot", "deviation_angle_stall": "deviationAngleStall"}
 
 _lib = None
 
@@ -229,6 +233,16 @@ def params_from_config(case: Dict) -> Params:
     p.turbo_dir = {"i": 0, "+i": 0, "-i": 1, "j": 2, "+j": 2, "-j": 3}.get(td, 0)
     p.residual_drop = int(float(cfg.get("RESIDUALS_DROP_CONVERGENCE", "16")))
     p.steady = int(_cfg_bool(cfg, "SIMULATION_IS_STEADY", True))
+    if p.bfm_active and p.bfm_model == BFM["Hall-Thollet"]:
+        # source_bfm_thollet.hpp:47-50 (config.hpp:295-299), config.cpp:441-450
+        p.kn = float(cfg["HALL_THOLLET_COEFFICIENT_KN"])
+        p.kf = float(cfg["HALL_THOLLET_COEFFICIENT_KF"])
+        p.kd = float(cfg["HALL_THOLLET_COEFFICIENT_KD"])
+        p.thollet_off_design = int(_cfg_bool(cfg, "HALL_THOLLET_OFF_DESIGN_ACTIVE", False))
+        if "FLUID_KINEMATIC_VISCOSITY" not in cfg:
+            raise ValueError("Hall-Thollet needs FLUID_KINEMATIC_VISCOSITY on the inviscid path")
+        p.nu = float(cfg["FLUID_KINEMATIC_VISCOSITY"])
+    p.stall_active = int(_cfg_bool(cfg, "STALLED_BFM_ACTIVE", False))
     fm = cfg.get("FLUID_MODEL", "ideal")
     if fm in ("real", "Real", "REAL"):
         p.fluid_real = 1

@@ -4,7 +4,7 @@
 // reference sources (src/config.cpp, src/input.cpp, src/mesh.cpp,
 // src/math_utils.cpp, src/fluid_ideal.cpp, src/advection_{base,jst,roe}.cpp,
 // src/boundary_{base,inlet,outlet,outlet_radial_equilibrium,inviscid_wall,
-// fake}.cpp, src/source_bfm_{base,hall,lift_drag}.cpp, src/input_table.cpp,
+// fake}.cpp, src/source_bfm_{base,hall,lift_drag,thollet}.cpp, src/input_table.cpp,
 // src/fluid_real.cpp)
 // into oracle/_ref/libtbfm_ref.so by oracle/Makefile.  Only tests/ and
 // tests/golden/gen_golden.py load it, to pin the CPU restatement in
@@ -31,6 +31,7 @@
 #include "boundary_inviscid_wall.hpp"
 #include "source_bfm_hall.hpp"
 #include "source_bfm_lift_drag.hpp"
+#include "source_bfm_thollet.hpp"
 
 #include <memory>
 #include <cstdio>
@@ -85,6 +86,8 @@ int ref_case_load(const char* ini_path) {
             g_case.bfm = std::make_unique<SourceBFMHall>(*g_case.config, *g_case.fluid, *g_case.mesh);
         else if (m == BodyForceModel::LIFT_DRAG)
             g_case.bfm = std::make_unique<SourceBFMLiftDrag>(*g_case.config, *g_case.fluid, *g_case.mesh);
+        else if (m == BodyForceModel::HALL_THOLLET)
+            g_case.bfm = std::make_unique<SourceBFMThollet>(*g_case.config, *g_case.fluid, *g_case.mesh);
         else
             g_case.bfm = std::make_unique<SourceBFMBase>(*g_case.config, *g_case.fluid, *g_case.mesh);
         std::fflush(stdout);

@@ -63,6 +63,7 @@ struct tbo_case {
     double *radius, *theta;
     double *gradb;                   /* blockage gradient 3*N (BFM only) */
     double *blockage, *nax, *nrad, *ntan, *rpm, *present, *nblades, *metal, *slen, *beta0, *kp;
+    double *pivot, *stall;                 /* deviationAnglePivot, deviationAngleStall */
     tbo_patch *patches;
     int npatches;
     int *mapI, *mapJ, *mapK;         /* patch index per boundary face, 2 layers each */
@@ -908,7 +909,8 @@ int tbo_geometry(const tbo_case* c, int which, double* out) {
 
 /* ------------------------------------------------------------------ BFM source */
 typedef struct {
-    double radius, cs, sn, omega, pitch, stag_pitch, flow_angle, dev, ntan;
+    double radius, cs, sn, omega, pitch, stag_pitch, flow_angle, dev, ntan, blockage;
+    int stalled;
     v3 rel_cyl, rel_cart, inv_dir_cyl, visc_dir_cyl;
 } flow_state;
 
@@ -922,10 +924,11 @@ static double deviation_angle(v3 rel, v3 nc) {
     double nv = dot3(vin, n);
     return -asin(nv / mag3(vin));
 }
-/* src/source_bfm_base.cpp:168-199 (stall model off) */
-static v3 inviscid_dir(v3 rel, v3 nc) {
+/* src/source_bfm_base.cpp:168-199; stalled: the camber normal itself once the deviation
+ * exceeds the stall deviation (STALLED_BFM_ACTIVE) */
+static v3 inviscid_dir(v3 rel, v3 nc, int stalled) {
     v3 rd = nrm3(rel), nd = nrm3(nc);
-    v3 ver = nrm3(sub3(nd, mul3(rd, dot3(nd, rd))));
+    v3 ver = stalled ? nd : nrm3(sub3(nd, mul3(rd, dot3(nd, rd))));
     v3 vm = mk3(rel.x, rel.y, 0.0);
     v3 sd = nrm3(mk3(vm.y, -vm.x, 0.0));
     return nrm3(sub3(ver, mul3(sd, dot3(ver, sd))));
@@ -952,12 +955,26 @@ static flow_state compute_flow_state(const tbo_case* c, long idx, st5 W) {
     s.stag_pitch = s.pitch * cos(metal);
     s.flow_angle = atan2(s.rel_cyl.z, vmer);
     s.dev = deviation_angle(s.rel_cyl, nc);
-    s.inv_dir_cyl = inviscid_dir(s.rel_cyl, nc);
+    s.stalled = c->p.stall_active && s.dev > (c->stall ? c->stall[idx] : 0.0);
+    s.blockage = c->blockage ? c->blockage[idx] : 1.0;
+    s.inv_dir_cyl = inviscid_dir(s.rel_cyl, nc, s.stalled);
     s.visc_dir_cyl = neg3(nrm3(s.rel_cyl));
     return s;
 }
 
 static st5 scale5(st5 a, double s) { for (int v = 0; v < 5; ++v) a.v[v] *= s; return a; }
+
+/* SourceBFMThollet::computeCompressibilityCorrection (source_bfm_thollet.cpp:50-71):
+ * relative Mach with the sound speed of computeSoundSpeed_rho_u_et, min(3, K') */
+static double thollet_kmach(const tbo_case* c, v3 rel_cyl, st5 W) {
+    double a = f_a(c, W.v[0], mk3(W.v[1], W.v[2], W.v[3]), W.v[4]);
+    double m = mag3(rel_cyl) / a;
+    if (fabs(m - 1.0) < 1E-03) m = 0.999;
+    double kp = 0.0;
+    if (m < 1.0) kp = 1.0 / sqrt(1.0 - m * m);
+    else kp = 2.0 / M_PI / sqrt(m * m - 1.0);
+    return (kp < 3.0) ? kp : 3.0;   /* std::min(3.0, kp) */
+}
 
 /* src/source_bfm_base.cpp:25-84 + source_bfm_hall.cpp:3-34 + source_bfm_lift_drag.cpp:3-88 */
 static st5 bfm_total_source(tbo_case* c, long i, long j, long k, st5 W, int write_side) {
@@ -1020,6 +1037,49 @@ static st5 bfm_total_source(tbo_case* c, long i, long j, long k, st5 W, int writ
         for (int v = 0; v < 5; ++v) body.v[v] = si.v[v] + sv.v[v];
         dev = s.dev;
     }
+    else if (c->p.bfm_model == TBO_BFM_THOLLET) {
+        flow_state s = compute_flow_state(c, idx, W);
+        /* inviscid component (source_bfm_thollet.cpp:23-48) */
+        double kmach = thollet_kmach(c, s.rel_cyl, W);
+        double fm = c->p.kn * kmach * dot3(s.rel_cyl, s.rel_cyl) * M_PI * s.dev / s.pitch / fabs(s.ntan) / s.blockage;
+        v3 fcyl = mul3(s.inv_dir_cyl, fm);
+        v3 fcar = cart_from_cyl(fcyl, s.cs, s.sn);
+        finv = fcar;
+        st5 si = {{0, fcar.x, fcar.y, fcar.z, fcyl.z * s.omega * s.radius}};
+        si = scale5(scale5(si, V), W.v[0]);
+        /* viscous component (:73-138): none once stalled, the side output keeping its
+         * previous value */
+        st5 sv = {{0, 0, 0, 0, 0}};
+        int keep_visc = 0;
+        if (s.stalled) {
+            keep_visc = 1;
+        } else {
+            double stwl = c->slen[idx];
+            if (stwl < 1E-06) {
+                if (i + 1 < c->p.ni) stwl = (c->slen[IDX(c, i + 1, j, k)] + c->slen[idx]) / 2.0;
+                if (stwl < 1E-06) stwl = 1E-06;
+            }
+            double rex = mag3(s.rel_cyl) * stwl / c->p.nu;
+            double cf = 0.0592 * pow(rex, -0.2);
+            double fm2;
+            if (!c->p.thollet_off_design) {
+                fm2 = dot3(s.rel_cyl, s.rel_cyl) / (s.pitch * s.blockage * fabs(s.ntan)) * (c->p.kf * cf);
+            } else {
+                double pivot = c->pivot[idx];
+                double km = thollet_kmach(c, s.rel_cyl, W);
+                fm2 = dot3(s.rel_cyl, s.rel_cyl) / (s.pitch * s.blockage * fabs(s.ntan)) *
+                      (c->p.kf * cf + M_PI * km * pow(fabs(s.dev - pivot), 2.0 * c->p.kd));
+            }
+            v3 vcyl = mul3(s.visc_dir_cyl, fm2);
+            v3 vcar = cart_from_cyl(vcyl, s.cs, s.sn);
+            fvisc = vcar;
+            st5 t = {{0, vcar.x, vcar.y, vcar.z, vcyl.z * s.omega * s.radius}};
+            sv = scale5(scale5(t, V), W.v[0]);
+        }
+        for (int v = 0; v < 5; ++v) body.v[v] = si.v[v] + sv.v[v];
+        dev = s.dev;
+        if (keep_visc && write_side) fvisc = mk3(c->fvisc[3 * idx], c->fvisc[3 * idx + 1], c->fvisc[3 * idx + 2]);
+    }
     if (write_side) {
         c->finv[3 * idx] = finv.x; c->finv[3 * idx + 1] = finv.y; c->finv[3 * idx + 2] = finv.z;
         c->fvisc[3 * idx] = fvisc.x; c->fvisc[3 * idx + 1] = fvisc.y; c->fvisc[3 * idx + 2] = fvisc.z;
@@ -1033,6 +1093,11 @@ void tbo_bfm_source(tbo_case* c, long i, long j, long k, const double* w, double
                     double* finv, double* fvisc, double* dev) {
     st5 W;
     memcpy(W.v, w, 40);
+    /* a single evaluation on fresh side-output arrays (a stalled node leaves its viscous
+     * force untouched, i.e. zero here; in a run it keeps the previous iteration's) */
+    long id0 = IDX(c, i, j, k);
+    for (int q = 0; q < 3; ++q) c->finv[3 * id0 + q] = c->fvisc[3 * id0 + q] = 0.0;
+    c->dev[id0] = 0.0;
     st5 s = bfm_total_source(c, i, j, k, W, 1);
     memcpy(out, s.v, 40);
     long idx = IDX(c, i, j, k);
@@ -1470,6 +1535,7 @@ tbo_case* tbo_create(const tbo_params* p, const double* x, const double* y, cons
     CPF(blockage, blockage); CPF(nax, normal_axial); CPF(nrad, normal_radial); CPF(ntan, normal_tangential);
     CPF(rpm, rpm); CPF(present, blade_present); CPF(nblades, number_blades); CPF(metal, metal_angle);
     CPF(slen, streamwise_length); CPF(beta0, lift_drag_beta0); CPF(kp, lift_drag_kp);
+    CPF(pivot, deviation_angle_pivot); CPF(stall, deviation_angle_stall);
 #undef CPF
     c->patches = (tbo_patch*)malloc(sizeof(tbo_patch) * (npatches > 0 ? npatches : 1));
     memcpy(c->patches, patches, sizeof(tbo_patch) * npatches);
@@ -1520,7 +1586,7 @@ void tbo_destroy(tbo_case* c) {
     double* ptrs[] = {c->x, c->y, c->z, c->dual, c->sI, c->sJ, c->sK, c->cI, c->cJ, c->cK, c->vol,
                       c->radius, c->theta, c->gradb, c->blockage, c->nax, c->nrad, c->ntan, c->rpm,
                       c->present, c->nblades, c->metal, c->slen, c->beta0, c->kp, c->U0, c->U, c->R,
-                      c->dt, c->finv, c->fvisc, c->dev};
+                      c->dt, c->finv, c->fvisc, c->dev, c->pivot, c->stall};
     for (size_t q = 0; q < sizeof(ptrs) / sizeof(ptrs[0]); ++q) free(ptrs[q]);
     for (int q = 0; q < c->nprof; ++q) { free(c->prof_p[q]); free(c->prof_r[q]); }
     free(c->mapI); free(c->mapJ); free(c->mapK); free(c->patches);

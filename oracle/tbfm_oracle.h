@@ -20,7 +20,7 @@ enum { TBO_TOPO_2D = 0, TBO_TOPO_3D = 1, TBO_TOPO_AXI = 2, TBO_TOPO_1D = 3 };
 enum { TBO_JST = 0, TBO_ROE = 1 };
 enum { TBO_LIM_NONE = 0, TBO_LIM_VAN_ALBADA = 1, TBO_LIM_VAN_LEER = 2, TBO_LIM_MIN_MOD = 3 };
 enum { TBO_DT_LOCAL = 0, TBO_DT_GLOBAL = 1, TBO_DT_FIXED = 2 };
-enum { TBO_BFM_NONE = 0, TBO_BFM_HALL = 1, TBO_BFM_LIFT_DRAG = 2, TBO_BFM_BLOCKAGE = 3 };
+enum { TBO_BFM_NONE = 0, TBO_BFM_HALL = 1, TBO_BFM_LIFT_DRAG = 2, TBO_BFM_BLOCKAGE = 3, TBO_BFM_THOLLET = 4 };
 enum { TBO_BC_INLET = 0, TBO_BC_OUTLET = 1, TBO_BC_RADIAL_EQ = 2, TBO_BC_WALL = 3,
        TBO_BC_PERIODIC = 4, TBO_BC_WEDGE = 5 };
 
@@ -42,6 +42,11 @@ typedef struct {
     long residual_drop;              /* RESIDUALS_DROP_CONVERGENCE */
     int steady;
     int fluid_real;                  /* FLUID_MODEL = real: look-up-table fluid (tbo_set_lut) */
+    /* Hall-Thollet (include/source_bfm_thollet.hpp:47-50, config.hpp:295-313) */
+    double kn, kf, kd;               /* HALL_THOLLET_COEFFICIENT_KN / KF / KD */
+    int thollet_off_design;          /* HALL_THOLLET_OFF_DESIGN_ACTIVE */
+    int stall_active;                /* STALLED_BFM_ACTIVE (all models' force direction) */
+    double nu;                       /* FLUID_KINEMATIC_VISCOSITY */
 } tbo_params;
 
 /* A fluid look-up table (include/fluid_real.hpp:11-60 Table2D): nx*ny values, row-major
@@ -75,6 +80,7 @@ typedef struct {
     const double *blockage, *normal_axial, *normal_radial, *normal_tangential;
     const double *rpm, *blade_present, *number_blades, *metal_angle;
     const double *streamwise_length, *lift_drag_beta0, *lift_drag_kp;
+    const double *deviation_angle_pivot, *deviation_angle_stall;
 } tbo_fields;
 
 typedef struct tbo_case tbo_case;

@@ -6,7 +6,8 @@ Each fixture holds seeded inputs and the reference's outputs:
   flux_<scheme>.npz       AdvectionJst / AdvectionRoe(+MUSCL, 4 limiters) face fluxes
   boundary.npz            inlet (cartesian, face-normal, cylindrical), outlet (+ramp),
                           radial-equilibrium outlet and inviscid-wall fluxes
-  bfm_<model>.npz         SourceBFMBase::computeTotalSource for Hall and lift-drag
+  bfm_<model>.npz         SourceBFMBase::computeTotalSource for Hall, lift-drag and
+                          Hall-Thollet (on-design; off-design with the stall model)
                           (+ blockage), with inviscid/viscous force and deviation angle
   fluid.npz               FluidIdeal p, a, h_t, T, p_t, T_t, s
   radial_eq.npz           integrateRadialEquilibrium
@@ -158,7 +159,37 @@ def main():
             np.savez_compressed(os.path.join(HERE, "bfm_%s.npz" % name), idx=np.array(idx), W=np.array(Ws),
                                 S=np.array(outs), finv=np.array(fis), fvisc=np.array(fvs), dev=np.array(devs))
     lut_fixtures(R, rng, tmp)
+    thollet_fixtures(R, tmp)
     print("golden fixtures written to", HERE)
+
+
+def thollet_cases():
+    """Hall-Thollet rotors: on-design, and off-design with the stall model."""
+    return {"rotor3d_thollet": cases.rotor3d(11, 5, 7, bfm_model="Hall-Thollet"),
+            "rotor3d_thollet_od_stall": cases.rotor3d(11, 5, 7, bfm_model="Hall-Thollet", off_design=True,
+                                                      stall=True)}
+
+
+def thollet_fixtures(R, tmp, n=64):
+    """SourceBFMBase::computeTotalSource of the reference's SourceBFMThollet at blade nodes
+    (the leading-edge column exercises the streamwise-length fallback) for random states;
+    repeated calls at one node check that a stalled node keeps its viscous side output."""
+    rng = np.random.default_rng(20261204)
+    for name, case in thollet_cases().items():
+        ini = cases.write_case(case, os.path.join(tmp, name))
+        assert R.ref_case_load(ini.encode()) == 0
+        idx, Ws, outs, fis, fvs, devs = [], [], [], [], [], []
+        cand = np.argwhere(case["fields"]["bladePresent"] > 0)
+        for q in range(n):
+            i, j, k = cand[rng.integers(0, len(cand))]
+            _, W = random_states(rng, 1)
+            W = W[0]
+            out, fi, fv, dv = np.zeros(5), np.zeros(3), np.zeros(3), C.c_double()
+            R.ref_bfm_source(int(i), int(j), int(k), dp(W), 0.0, dp(out), dp(fi), dp(fv), C.byref(dv))
+            idx.append((i, j, k)); Ws.append(W); outs.append(out); fis.append(fi); fvs.append(fv)
+            devs.append(dv.value)
+        np.savez_compressed(os.path.join(HERE, "bfm_%s.npz" % name), idx=np.array(idx), W=np.array(Ws),
+                            S=np.array(outs), finv=np.array(fis), fvisc=np.array(fvs), dev=np.array(devs))
 
 
 def real_gas_states(rng, n):

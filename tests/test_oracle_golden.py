@@ -24,6 +24,10 @@ def fixture_case(name):
         return cases.rotor3d(11, 5, 7, bfm_model="Hall")
     if name == "rotor3d_liftdrag":
         return cases.rotor3d(11, 5, 7, bfm_model="Lift-Drag", scheme="ROE", muscl=True, limiter="van albada")
+    if name == "rotor3d_thollet":
+        return cases.rotor3d(11, 5, 7, bfm_model="Hall-Thollet")
+    if name == "rotor3d_thollet_od_stall":
+        return cases.rotor3d(11, 5, 7, bfm_model="Hall-Thollet", off_design=True, stall=True)
     raise KeyError(name)
 
 
@@ -64,7 +68,7 @@ def test_boundary_fluxes():
         assert np.array_equal(out, b["F"][q]), (q, int(b["kind"][q]))
 
 
-@pytest.mark.parametrize("name", ["rotor3d_hall", "rotor3d_liftdrag"])
+@pytest.mark.parametrize("name", ["rotor3d_hall", "rotor3d_liftdrag", "rotor3d_thollet", "rotor3d_thollet_od_stall"])
 def test_bfm_source(name):
     b = load("bfm_%s.npz" % name)
     oc = O.OracleCase(fixture_case(name))
