@@ -493,7 +493,8 @@ void setup(tbfm_case* cs, const GridInput& grid) {
     std::vector<unsigned char> blade2d(NI * NJ, 0);
     long bi0 = NI, bi1 = 0;
     const bool axi = (cs->topo == Topology::Axisymmetric);
-    const bool hall = bfm && (c.bfm_model == 1 || c.bfm_model == 2);
+    const bool hall = bfm && (c.bfm_model == 1 || c.bfm_model == 2 || c.bfm_model == 4);
+    const bool thollet = bfm && c.bfm_model == 4;
     const std::vector<double>* present = bfm ? field("bladePresent", cs->topo == Topology::ThreeD) : nullptr;
     const std::vector<double>* nblades = bfm ? field("numberBlades", true) : nullptr;
     const std::vector<double>* blockage = bfm ? field("blockage", true) : nullptr;
@@ -524,13 +525,31 @@ void setup(tbfm_case* cs, const GridInput& grid) {
     std::vector<unsigned char> sflag(NS, 0);
     std::vector<double> s_cs(NS, 1.0), s_sn(NS, 0.0), s_r(NS, 0.0), s_om(NS, 0.0), s_b(NS, 1.0), s_gbx(NS, 0.0),
         s_gby(NS, 0.0), s_gbz(NS, 0.0), s_nax(NS, 0.0), s_nrad(NS, 0.0), s_ntan(NS, 1.0), s_pitch(NS, 1.0),
-        s_spitch(NS, 1.0), s_sol(NS, 0.0), s_beta0(NS, 0.0), s_kp(NS, 0.0);
+        s_spitch(NS, 1.0), s_sol(NS, 0.0), s_beta0(NS, 0.0), s_kp(NS, 0.0), s_stwl(NS, 1.0), s_piv(NS, 0.0),
+        s_stall(NS, 0.0);
     const std::vector<double>* rpm = (bfm && (hall || cs->topo == Topology::ThreeD)) ? field("rpm", hall) : nullptr;
     const std::vector<double>* nax = hall ? field("normalAxial", true) : nullptr;
     const std::vector<double>* nrad = hall ? field("normalRadial", true) : nullptr;
     const std::vector<double>* ntan = hall ? field("normalTangential", true) : nullptr;
     const std::vector<double>* metal = hall ? field("bladeMetalAngle", false) : nullptr;
-    const std::vector<double>* slen = (c.bfm_model == 2) ? field("streamwiseLength", true) : nullptr;
+    const std::vector<double>* slen = (c.bfm_model == 2 || thollet) ? field("streamwiseLength", true) : nullptr;
+    // Hall-Thollet constants (source_bfm_thollet.hpp:48-51); the reference reads each
+    // key without a default, so a missing one is an error
+    c.stall = (hall && cfg.isStalledBfmActive()) ? 1 : 0;
+    if (thollet) {
+        c.th_kn = cfg.getHallTholletCoefficient_KN();
+        c.th_kf = cfg.getHallTholletCoefficient_KF();
+        c.th_kd = cfg.getHallTholletCoefficient_KD();
+        c.th_off_design = cfg.getHallTholletOffDesignActive() ? 1 : 0;
+        // source_bfm_thollet.cpp:85-95: without FLUID_KINEMATIC_VISCOSITY the reference
+        // takes mu(rho, e)/rho from the fluid's viscosity model
+        if (!cfg.has("FLUID_KINEMATIC_VISCOSITY"))
+            throw std::runtime_error("BFM_MODEL = Hall-Thollet needs FLUID_KINEMATIC_VISCOSITY on the GPU path "
+                                     "(the molecular-viscosity model is outside it)");
+        c.th_nu = cfg.getFloat("FLUID_KINEMATIC_VISCOSITY");
+    }
+    const std::vector<double>* pivot = (thollet && c.th_off_design) ? field("deviationAnglePivot", true) : nullptr;
+    const std::vector<double>* stallf = c.stall ? field("deviationAngleStall", true) : nullptr;
     const std::vector<double>* beta0 = (c.bfm_model == 2) ? field("liftDragBeta0", true) : nullptr;
     const std::vector<double>* kp = (c.bfm_model == 2) ? field("liftDragKpEtamax", true) : nullptr;
     const long le = cfg.getLeadingEdgeIndex(), te = cfg.getTrailingEdgeIndex();
@@ -573,6 +592,18 @@ void setup(tbfm_case* cs, const GridInput& grid) {
                         s_beta0[s] = (*beta0)[n];
                         s_kp[s] = (*kp)[n];
                     }
+                    if (thollet) {
+                        // streamwise length with the leading-edge fallback of
+                        // source_bfm_thollet.cpp:97-106
+                        double stwl = (*slen)[n];
+                        if (stwl < 1E-06) {
+                            if (i + 1 < NI) stwl = ((*slen)[((i + 1) * NJ + j) * NK + k] + (*slen)[n]) / 2.0;
+                            if (stwl < 1E-06) stwl = 1E-06;
+                        }
+                        s_stwl[s] = stwl;
+                        if (pivot) s_piv[s] = (*pivot)[n];
+                    }
+                    if (stallf) s_stall[s] = (*stallf)[n];
                 }
             }
     std::vector<double> axi_gf(NI * NJ, 0.0);
@@ -589,6 +620,7 @@ void setup(tbfm_case* cs, const GridInput& grid) {
     c.s_nax = upload(s_nax, own); c.s_nrad = upload(s_nrad, own); c.s_ntan = upload(s_ntan, own);
     c.s_pitch = upload(s_pitch, own); c.s_spitch = upload(s_spitch, own); c.s_sol = upload(s_sol, own);
     c.s_beta0 = upload(s_beta0, own); c.s_kp = upload(s_kp, own);
+    c.s_stwl = upload(s_stwl, own); c.s_piv = upload(s_piv, own); c.s_stall = upload(s_stall, own);
     c.axi_gf = upload(axi_gf, own);
     c.finv = dalloc<double>(3 * N, own);
     c.fvisc = dalloc<double>(3 * N, own);

@@ -150,8 +150,9 @@ BfmModel Config::getBFMModel() const {
     if (v == "Lift-Drag") return BfmModel::LiftDrag;
     if (v == "None") return BfmModel::None;
     if (v == "Blockage") return BfmModel::Blockage;
-    if (v == "Hall-Thollet" || v == "Chima" || v == "Correlations")
-        throw std::runtime_error("BFM_MODEL = " + v + " is outside the GPU residual path (Hall, Lift-Drag)");
+    if (v == "Hall-Thollet") return BfmModel::HallThollet;
+    if (v == "Chima" || v == "Correlations")
+        throw std::runtime_error("BFM_MODEL = " + v + " is outside the GPU residual path (Hall, Hall-Thollet, Lift-Drag)");
     throw std::runtime_error("Invalid value for key \"BFM_MODEL\" in configuration.");
 }
 bool Config::isInletCylindrical() const {

@@ -22,7 +22,7 @@ enum class Topology { TwoD, ThreeD, Axisymmetric, OneD };
 enum class Scheme { JST, ROE };
 enum class Limiter { None = 0, VanAlbada = 1, VanLeer = 2, MinMod = 3 };
 enum class TimeStepMethod { Local = 0, Global = 1, Fixed = 2 };
-enum class BfmModel { None = 0, Hall = 1, LiftDrag = 2, Blockage = 3 };
+enum class BfmModel { None = 0, Hall = 1, LiftDrag = 2, Blockage = 3, HallThollet = 4 };
 enum class PatchType { Inlet = 0, Outlet = 1, RadialEquilibrium = 2, Wall = 3, Periodic = 4, Wedge = 5 };
 enum class Precision { FP64 = 0, Mixed = 1 };
 enum class FluidModel { Ideal = 0, Real = 1 };
@@ -61,6 +61,16 @@ public:
     bool isBFMActive() const { return getBool("BFM_ACTIVE", false); }
     bool isBlockageActive() const { return getBool("BLOCKAGE_ACTIVE", false); }
     BfmModel getBFMModel() const;
+    // Hall-Thollet (include/config.hpp:295-313, src/config.cpp:441-450)
+    double getHallTholletCoefficient_KN() const { return getFloat("HALL_THOLLET_COEFFICIENT_KN"); }
+    double getHallTholletCoefficient_KF() const { return getFloat("HALL_THOLLET_COEFFICIENT_KF"); }
+    double getHallTholletCoefficient_KD() const { return getFloat("HALL_THOLLET_COEFFICIENT_KD"); }
+    bool getHallTholletOffDesignActive() const {
+        if (!has("HALL_THOLLET_OFF_DESIGN_ACTIVE"))
+            throw std::runtime_error("Not found value for key \"HALL_THOLLET_OFF_DESIGN_ACTIVE\" in configuration.");
+        return getBool("HALL_THOLLET_OFF_DESIGN_ACTIVE", false);
+    }
+    bool isStalledBfmActive() const { return getBool("STALLED_BFM_ACTIVE", false); }
     long getLeadingEdgeIndex() const { return (long)getFloat("LEADING_EDGE_INDEX", 0.0); }
     long getTrailingEdgeIndex() const { return (long)getFloat("TRAILING_EDGE_INDEX", 0.0); }
     double getRotationalSpeedScalingFactor() const { return getFloat("ROTATIONAL_SPEED_SCALING_FACTOR", 1.0); }

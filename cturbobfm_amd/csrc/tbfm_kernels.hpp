@@ -109,6 +109,11 @@ struct CaseDev {
     const unsigned char* sflag;        // bit0 present>0, bit1 |grad b|>1e-10, bit2 numberBlades != 0
     const double *s_cs, *s_sn, *s_r, *s_om, *s_b, *s_gbx, *s_gby, *s_gbz;
     const double *s_nax, *s_nrad, *s_ntan, *s_pitch, *s_spitch, *s_sol, *s_beta0, *s_kp;
+    // Hall-Thollet: streamwise length with the reference's leading-edge fallback,
+    // deviation pivot (off-design) and stall deviation (STALLED_BFM_ACTIVE), per slab node
+    const double *s_stwl, *s_piv, *s_stall;
+    double th_kn, th_kf, th_kd, th_nu;
+    int th_off_design, stall;
     const double* axi_gf;              // axisymmetric: |S_K(i,j,0)| * wedge angle, (j*NI + i)
     // side outputs
     double *finv, *fvisc, *dev;

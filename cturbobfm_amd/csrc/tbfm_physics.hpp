@@ -472,15 +472,31 @@ __device__ __forceinline__ V3 vnrm(V3 a) { double m = vmag(a); return (m == 0) ?
 __device__ __forceinline__ V3 cyl_of(V3 v, double cs, double sn) { return v3(v.x, v.y * cs + v.z * sn, -v.y * sn + v.z * cs); }
 __device__ __forceinline__ V3 cart_of(V3 v, double cs, double sn) { return v3(v.x, v.y * cs - v.z * sn, v.y * sn + v.z * cs); }
 
-// Blockage + Hall / lift-drag source of one node (source_bfm_base.cpp:25-199,
-// source_bfm_hall.cpp:3-34, source_bfm_lift_drag.cpp:3-88).  s = slab index.
+// Compressibility correction of the Hall-Thollet model
+// (SourceBFMThollet::computeCompressibilityCorrection, source_bfm_thollet.cpp:50-71):
+// relative Mach number m, 1/sqrt(1-m^2) subsonic, 2/(pi sqrt(m^2-1)) supersonic,
+// m within 1e-3 of 1 taken as 0.999, capped at 3.
+__device__ inline double thollet_kmach(double wrel, double a) {
+    double m = wrel / a;
+    if (fabs(m - 1.0) < 1E-03) m = 0.999;
+    double kp;
+    if (m < 1.0) kp = 1.0 / sqrt(1.0 - m * m);
+    else kp = 2.0 / M_PI / sqrt(m * m - 1.0);
+    return (kp < 3.0) ? kp : 3.0;
+}
+
+// Blockage + Hall / lift-drag / Hall-Thollet source of one node (source_bfm_base.cpp:25-199,
+// source_bfm_hall.cpp:3-34, source_bfm_lift_drag.cpp:3-88, source_bfm_thollet.cpp:3-138).
+// s = slab index.  keepv: the node is stalled under Hall-Thollet, which leaves the
+// viscous-force side output at its previous value (source_bfm_thollet.cpp:79-84).
 template <int FL, typename CT>
 __device__ inline void bfm_source(const CT& c, const Gas<FL>& gs, long s, const double W[5], double V, double S[5],
-                                  double fi[3], double fv[3], double& dev) {
+                                  double fi[3], double fv[3], double& dev, bool& keepv) {
     double blk[5] = {0, 0, 0, 0, 0};
     fi[0] = fi[1] = fi[2] = 0.0;
     fv[0] = fv[1] = fv[2] = 0.0;
     dev = 0.0;
+    keepv = false;
     if (c.blockage_active) {
         double p = pressure_w<double, FL>(gs, W);
         double ht = W[4] + p / W[0];
@@ -494,7 +510,7 @@ __device__ inline void bfm_source(const CT& c, const Gas<FL>& gs, long s, const 
         blk[4] = common * ht * V;
     }
     double body[5] = {0, 0, 0, 0, 0};
-    if ((c.sflag[s] & 4) && (c.bfm_model == 1 || c.bfm_model == 2)) {
+    if ((c.sflag[s] & 4) && (c.bfm_model == 1 || c.bfm_model == 2 || c.bfm_model == 4)) {
         // computeFlowState (source_bfm_base.cpp:104-149)
         const double cs = c.s_cs[s], sn = c.s_sn[s], r = c.s_r[s], om = c.s_om[s];
         V3 vcy = cyl_of(v3(W[1], W[2], W[3]), cs, sn);
@@ -506,9 +522,11 @@ __device__ inline void bfm_source(const CT& c, const Gas<FL>& gs, long s, const 
         sp = vdiv(sp, vmag(sp));
         V3 vin = vsub(rel, vmul(sp, vdot(rel, sp)));
         double dv = -asin(vdot(vin, n) / vmag(vin));
-        // inviscid force direction (:168-199, stall model off)
+        // inviscid force direction (:168-199): the camber normal itself once the
+        // deviation exceeds the stall deviation (STALLED_BFM_ACTIVE, :178-184)
+        const bool stalled = c.stall && dv > c.s_stall[s];
         V3 rd = vnrm(rel), nd = vnrm(nc);
-        V3 ver = vnrm(vsub(nd, vmul(rd, vdot(nd, rd))));
+        V3 ver = stalled ? nd : vnrm(vsub(nd, vmul(rd, vdot(nd, rd))));
         V3 sd = vnrm(v3(rel.y, -rel.x, 0.0));
         V3 idir = vnrm(vsub(ver, vmul(sd, vdot(ver, sd))));
         dev = dv;
@@ -520,6 +538,37 @@ __device__ inline void bfm_source(const CT& c, const Gas<FL>& gs, long s, const 
             double sv[5] = {0, fca.x, fca.y, fca.z, fcy.z * om * r};
 #pragma unroll
             for (int v = 0; v < 5; ++v) body[v] = sv[v] * V * W[0];
+        } else if (c.bfm_model == 4) {
+            // inviscid component (source_bfm_thollet.cpp:23-48)
+            const double kmach = thollet_kmach(vmag(rel), sound_speed_w<FL>(gs, W, pressure_w<double, FL>(gs, W)));
+            const double rr = vdot(rel, rel);
+            const double pitch = c.s_pitch[s], b = c.s_b[s], antan = fabs(c.s_ntan[s]);
+            double fm = c.th_kn * kmach * rr * M_PI * dv / pitch / antan / b;
+            V3 fcy = vmul(idir, fm);
+            V3 fca = cart_of(fcy, cs, sn);
+            fi[0] = fca.x; fi[1] = fca.y; fi[2] = fca.z;
+            double si[5] = {0, fca.x, fca.y, fca.z, fcy.z * om * r};
+            double sv[5] = {0, 0, 0, 0, 0};
+            if (stalled) {
+                keepv = true;
+            } else {
+                // viscous component (:73-138): flat-plate skin friction on the streamwise
+                // length (leading-edge fallback applied on the host), plus the off-design
+                // loss about the pivot deviation
+                double rex = vmag(rel) * c.s_stwl[s] / c.th_nu;
+                double cf = 0.0592 * pow(rex, -0.2);
+                double coef;
+                if (!c.th_off_design) coef = c.th_kf * cf;
+                else coef = c.th_kf * cf + M_PI * kmach * pow(fabs(dv - c.s_piv[s]), 2.0 * c.th_kd);
+                double fm2 = rr / (pitch * b * antan) * coef;
+                V3 vdir = v3(-vnrm(rel).x, -vnrm(rel).y, -vnrm(rel).z);
+                V3 vcyl = vmul(vdir, fm2);
+                V3 vcar = cart_of(vcyl, cs, sn);
+                fv[0] = vcar.x; fv[1] = vcar.y; fv[2] = vcar.z;
+                sv[1] = vcar.x; sv[2] = vcar.y; sv[3] = vcar.z; sv[4] = vcyl.z * om * r;
+            }
+#pragma unroll
+            for (int v = 0; v < 5; ++v) body[v] = si[v] * V * W[0] + sv[v] * V * W[0];
         } else {
             double vmer = sqrt(vcy.x * vcy.x + vcy.y * vcy.y);
             double flow_angle = atan2(rel.z, vmer);

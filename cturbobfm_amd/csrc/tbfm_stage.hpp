@@ -115,11 +115,17 @@ __device__ inline void source_terms(const CaseDev& c, const Gas<FL>& gs, const d
     }
     if (c.bfm_active && (fl & 2)) {
         double fi[3], fv[3], dv;
-        bfm_source<FL>(c, gs, s, Wc, V, bf, fi, fv, dv);
+        bool keepv;
+        bfm_source<FL>(c, gs, s, Wc, V, bf, fi, fv, dv, keepv);
         if (last && (fl & 4)) {
             c.finv[3 * n] = fi[0]; c.finv[3 * n + 1] = fi[1]; c.finv[3 * n + 2] = fi[2];
-            c.fvisc[3 * n] = fv[0]; c.fvisc[3 * n + 1] = fv[1]; c.fvisc[3 * n + 2] = fv[2];
             c.dev[n] = dv;
+        }
+        // a stalled Hall-Thollet node keeps the viscous force of its latest unstalled
+        // evaluation, which may be an earlier stage's: with the stall model on, every
+        // stage stores it
+        if ((fl & 4) && !keepv && (last || c.stall)) {
+            c.fvisc[3 * n] = fv[0]; c.fvisc[3 * n + 1] = fv[1]; c.fvisc[3 * n + 2] = fv[2];
         }
     }
 }

@@ -170,7 +170,7 @@ class Server:
         lock = threading.Lock()
         queues = [deque() for _ in range(nl)]       # arrived requests of each lane, in order
         building: Dict[int, object] = {}            # request -> future of its Solver
-        reserved: Dict[int, int] = {}               # estimated bytes of requests in set-up
+        reserved: Dict[int, int] = {}               # estimated bytes of requests in set-up or not yet admitted
         state = {"used": 0, "done": 0, "error": None}
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0
@@ -189,7 +189,6 @@ class Server:
                         while queues[li] and queues[li][0] in building and building[queues[li][0]].done():
                             q = queues[li].popleft()
                             ready.append((q, building.pop(q)))
-                            reserved.pop(q)
                     for q, fut in ready:
                         s = fut.result()
                         sp = specs[q]
@@ -199,7 +198,10 @@ class Server:
                             self.bytes_per_node[sp.three_d] = b / sp.nodes
                             resident[q] = s
                             footprint[q] = b
+                            # the estimate stays reserved until the measured footprint
+                            # is counted, so the budget never sees the case as absent
                             state["used"] += b
+                            reserved.pop(q)
                             results[q].admitted = now()
                     if not resident:
                         time.sleep(0.002)
@@ -258,7 +260,7 @@ class Server:
                                      key=lambda q: arrivals[q])
                     for q in pending:
                         est = self._estimate(specs[q])
-                        busy = state["used"] > 0 or bool(building)
+                        busy = state["used"] > 0 or bool(reserved)
                         if busy and state["used"] + sum(reserved.values()) + est > self.budget:
                             break
                         building[q] = pool.submit(self._build, specs[q])

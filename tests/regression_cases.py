@@ -36,6 +36,19 @@ CONFIGS = {
                                         CONVECTION_SCHEME="JST"),
 }
 
+# GER4 compressor rotor with the Hall-Thollet body force (blockage on, on-design); the
+# axisymmetric and full-annulus (360-degree periodic) variants share these keys
+GER4 = dict(COMMON, BFM_ACTIVE="yes", BLOCKAGE_ACTIVE="yes", BFM_MODEL="Hall-Thollet",
+            HALL_THOLLET_COEFFICIENT_KN="1.0", HALL_THOLLET_COEFFICIENT_KF="1.0",
+            HALL_THOLLET_COEFFICIENT_KD="1.0", HALL_THOLLET_OFF_DESIGN_ACTIVE="no",
+            FLUID_KINEMATIC_VISCOSITY="1.48E-5", FLUID_NAME="Air", INFLOW="inlet, 101300, 288.15, 1, 0, 0",
+            OUTFLOW="radial_equilibrium, 100000", INIT_MACH_NUMBER="0.45", INIT_TEMPERATURE="288.15",
+            INIT_PRESSURE="101325", INIT_DIRECTION="1, 0, 0", ROTATION_AXIS="1, 0, 0", CFL="1.3",
+            RESIDUALS_DROP_CONVERGENCE="4", TURBO_OUTPUT="yes", TURBO_DIRECTION="i", CONVECTION_SCHEME="JST")
+CONFIGS["GER4_Axisymmetric_HallThollet"] = dict(GER4, TOPOLOGY="axisymmetric")
+CONFIGS["GER4_FullAnnulus"] = dict(GER4, TOPOLOGY="3D", PER1="periodic, 0, 0, 360.0",
+                                   PER2="periodic, 1, 0, 360.0")
+
 
 def load_case(name):
     folder = os.path.join(HERE, name)
@@ -54,13 +67,16 @@ def load_case(name):
                 nk = int(line[3:])
             elif line.startswith("x,y,z"):
                 break
+        header = line.strip().split(",")
     data = np.loadtxt(grid, delimiter=",", skiprows=skip, ndmin=2)
     X, Y, Z = (data[:, c].reshape(ni, nj, nk).copy() for c in range(3))
+    # body-force input fields (blockage, camber normal, rpm, ...) after x, y, z
+    fields = {h: data[:, c].reshape(ni, nj, nk).copy() for c, h in enumerate(header) if c >= 3}
     patches = [("INFLOW", 0, 0, 0, nj, 0, nk), ("OUTFLOW", ni, ni, 0, nj, 0, nk),
                ("LOWER_WALL", 0, ni, 0, 0, 0, nk), ("UPPER_WALL", 0, ni, nj, nj, 0, nk)]
     if nk > 1:
         patches += [("PER1", 0, ni, 0, nj, 0, 0), ("PER2", 0, ni, 0, nj, nk, nk)]
-    return {"ni": ni, "nj": nj, "nk": nk, "x": X, "y": Y, "z": Z, "fields": {}, "config": cfg,
+    return {"ni": ni, "nj": nj, "nk": nk, "x": X, "y": Y, "z": Z, "fields": fields, "config": cfg,
             "patches": patches, "name": name}
 
 

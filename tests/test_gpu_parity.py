@@ -40,6 +40,11 @@ CASES = {
     "rotor3d_jst_hall": lambda: cases.rotor3d(25, 9, 17, n_iterations=200),
     "rotor3d_roe_muscl_liftdrag": lambda: cases.rotor3d(25, 9, 17, n_iterations=200, scheme="ROE", muscl=True,
                                                           limiter="van leer", bfm_model="Lift-Drag", cfl=1.2),
+    "rotor3d_jst_thollet": lambda: cases.rotor3d(25, 9, 17, n_iterations=200, bfm_model="Hall-Thollet"),
+    # off-design loss and the stall model: about 160 of the 1071 blade nodes run stalled
+    "rotor3d_roe_muscl_thollet_od_stall": lambda: cases.rotor3d(25, 9, 17, n_iterations=200, scheme="ROE", muscl=True,
+                                                                  limiter="van albada", bfm_model="Hall-Thollet",
+                                                                  off_design=True, stall=True, cfl=1.2),
 }
 
 
@@ -124,3 +129,29 @@ def test_determinism_bitwise():
     la, lb = a.advance(20), b.advance(20)
     assert np.array_equal(la, lb)
     assert np.array_equal(a.state(), b.state())
+
+
+def test_thollet_stall_exercised():
+    """The stalled-blade branch is taken (camber-normal force direction, no viscous
+    force) at nodes whose deviation exceeds the stall deviation, and the GPU agrees with
+    the restatement on which nodes those are."""
+    case = CASES["rotor3d_roe_muscl_thollet_od_stall"]()
+    s = Solver.from_case(case)
+    oc = O.OracleCase(case)
+    oc.run(60, threads=8)
+    s.advance(60)
+    dv_g = s.side_outputs()[2]
+    dv_o = oc.side_outputs()[2]
+    stall = 0.12
+    n_stalled = int(np.sum(dv_o > stall))
+    print("stalled nodes", n_stalled, "of", int(np.sum(dv_o != 0)), flush=True)
+    assert n_stalled > 50
+    assert np.array_equal(dv_g > stall, dv_o > stall)
+
+
+def test_thollet_missing_coefficient():
+    case = CASES["rotor3d_jst_thollet"]()
+    del case["config"]["HALL_THOLLET_COEFFICIENT_KF"]
+    from cturbobfm_amd import SolverError
+    with pytest.raises(SolverError, match="HALL_THOLLET_COEFFICIENT_KF"):
+        Solver.from_case(case)

@@ -1,6 +1,8 @@
 """The reference's regression cases (CircularBump2D: Roe 2-D; CircularBump3D_PeriodicSwirl:
 JST 3-D, 45-degree sector, cylindrical inlet, radial-equilibrium outlet;
-AxisymmetricSwirlingChannel: JST axisymmetric wedge) run to the reference's convergence
+AxisymmetricSwirlingChannel: JST axisymmetric wedge; GER4_Axisymmetric_HallThollet and
+GER4_FullAnnulus: the GER4 compressor rotor with the Hall-Thollet body force, axisymmetric
+and 360-degree periodic with the Gong rotor source) run to the reference's convergence
 test with the CPU restatement and compared with the reference's reference.csv on
 Density, Pressure and Temperature by relative L2, with the reference's own tolerance
 3e-2 (test/regression_tests/test_cases.py:82-83)."""
