@@ -1,5 +1,6 @@
-/* C-ABI of the MI355X explicit residual path (JST / Roe-MUSCL fluxes, Hall / lift-drag
- * body forces, local / global / fixed time step, RK41A / RK31B stage update).
+/* C-ABI of the MI355X explicit residual path (JST / Roe-MUSCL fluxes, Hall / lift-drag /
+ * Hall-Thollet body forces with the stalled-blade model, local / global / fixed time step,
+ * RK41A / RK31B stage update).
  *
  * The reference has no FFI; its operator seams are per-element C++ virtuals
  * (AdvectionBase::computeFlux, include/advection_base.hpp:21-25;
