@@ -46,7 +46,8 @@ EXPORTS = ["tbfm_gpu_case_create", "tbfm_gpu_case_create_from_ini", "tbfm_gpu_ca
            "tbfm_gpu_tuning", "tbfm_gpu_case_device_bytes", "tbfm_gpu_case_set_convergence",
            "tbfm_gpu_case_status", "tbfm_gpu_batch_create", "tbfm_gpu_batch_admit", "tbfm_gpu_batch_retire",
            "tbfm_gpu_batch_size", "tbfm_gpu_batch_advance", "tbfm_gpu_batch_destroy",
-           "tbfm_gpu_batch_last_error", "tbfm_gpu_device_memory"]
+           "tbfm_gpu_batch_last_error", "tbfm_gpu_device_memory", "tbfm_gpu_write_solution",
+           "tbfm_write_solution_from_ini", "tbfm_volume_file_name", "tbfm_initial_state_from_ini"]
 
 _lib = None
 
@@ -96,6 +97,10 @@ def lib():
         L.tbfm_gpu_batch_last_error.restype = C.c_char_p
         L.tbfm_gpu_batch_last_error.argtypes = [vp]
         L.tbfm_gpu_device_memory.argtypes = [C.POINTER(C.c_long), C.POINTER(C.c_long)]
+        L.tbfm_gpu_write_solution.argtypes = [vp, C.c_char_p]
+        L.tbfm_write_solution_from_ini.argtypes = [C.c_char_p, dp, dp, dp, dp, C.c_char_p]
+        L.tbfm_volume_file_name.argtypes = [C.c_char_p, C.c_long, C.c_char_p, C.c_long]
+        L.tbfm_initial_state_from_ini.argtypes = [C.c_char_p, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -120,6 +125,58 @@ def device_memory():
     if rc != TBFM_OK:
         raise SolverError(lib().tbfm_gpu_last_error(None).decode())
     return f.value, t.value
+
+
+def _host_error():
+    return SolverError(lib().tbfm_gpu_last_error(None).decode())
+
+
+def _grid_dims(ini_path: str):
+    """NI, NJ, NK of the grid file an input.ini names (header lines of the grid CSV)."""
+    from .cases import read_ini
+    cfg = read_ini(ini_path)
+    grid = cfg["GRID_FILE"]
+    if not os.path.isabs(grid):
+        grid = os.path.join(os.path.dirname(os.path.abspath(ini_path)), grid)
+    dims = {}
+    with open(grid) as f:
+        for line in f:
+            for key in ("NI", "NJ", "NK"):
+                if line.startswith(key + "="):
+                    dims[key] = int(line.split("=")[1])
+            if len(dims) == 3:
+                break
+    return dims["NI"], dims["NJ"], dims["NK"]
+
+
+def initial_state_from_ini(ini_path: str):
+    """(U, finv, fvisc) a case created from input.ini starts from: the uniform INIT_*
+    state or the restart file (``tbfm_initial_state_from_ini``; host only)."""
+    ni, nj, nk = _grid_dims(ini_path)
+    U = np.zeros((5, ni, nj, nk))
+    fi = np.zeros((ni, nj, nk, 3))
+    fv = np.zeros((ni, nj, nk, 3))
+    if lib().tbfm_initial_state_from_ini(os.fsencode(ini_path), _dp(U), _dp(fi), _dp(fv)) != TBFM_OK:
+        raise _host_error()
+    return U, fi, fv
+
+
+def write_solution_from_ini(ini_path: str, U: np.ndarray, path: str, finv=None, fvisc=None, dev=None) -> None:
+    """The reference's volume solution file of a given state (``tbfm_write_solution_from_ini``;
+    host only)."""
+    keep = [np.ascontiguousarray(a, dtype=np.float64) if a is not None else None for a in (U, finv, fvisc, dev)]
+    ptrs = [_dp(a) if a is not None else None for a in keep]
+    if lib().tbfm_write_solution_from_ini(os.fsencode(ini_path), ptrs[0], ptrs[1], ptrs[2], ptrs[3],
+                                          os.fsencode(path)) != TBFM_OK:
+        raise _host_error()
+
+
+def volume_file_name(ini_path: str, iteration: int) -> str:
+    """Volume_CSV file name (without folder and extension) at an iteration."""
+    buf = C.create_string_buffer(4096)
+    if lib().tbfm_volume_file_name(os.fsencode(ini_path), iteration, buf, len(buf)) != TBFM_OK:
+        raise _host_error()
+    return buf.value.decode()
 
 
 class Solver:
@@ -238,6 +295,11 @@ class Solver:
         dt = np.zeros_like(dev)
         self._check(lib().tbfm_gpu_get_side_outputs(self.h, _dp(finv), _dp(fvisc), _dp(dev), _dp(dt)))
         return finv, fvisc, dev, dt
+
+    def write_solution(self, path: str) -> None:
+        """The reference's volume solution file (Output::writeSolution) of the current
+        state and side outputs."""
+        self._check(lib().tbfm_gpu_write_solution(self.h, os.fsencode(path)))
 
     def turbo(self) -> np.ndarray:
         out = np.zeros(10)

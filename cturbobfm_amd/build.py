@@ -37,7 +37,8 @@ def _run(cmd, cwd=None):
 
 
 def build_gpu(verbose: bool = False) -> None:
-    srcs = [os.path.join(CSRC, "tbfm_gpu.hip"), os.path.join(CSRC, "tbfm_host.cpp")]
+    srcs = [os.path.join(CSRC, "tbfm_gpu.hip"), os.path.join(CSRC, "tbfm_host.cpp"),
+            os.path.join(CSRC, "tbfm_io.cpp")]
     objs = []
     for s in srcs:
         o = os.path.join(CSRC, os.path.basename(s) + ".o")

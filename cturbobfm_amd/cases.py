@@ -223,8 +223,8 @@ def write_case(case: Dict, folder: str) -> str:
     return ini
 
 
-def read_case(ini_path: str) -> Dict:
-    """Reads input.ini + grid CSV + boundaries CSV back into a case dict."""
+def read_ini(ini_path: str) -> Dict[str, str]:
+    """KEY = value pairs of an input.ini (comment and section lines skipped)."""
     cfg: Dict[str, str] = {}
     with open(ini_path) as f:
         for line in f:
@@ -233,6 +233,12 @@ def read_case(ini_path: str) -> Dict:
                 continue
             k, v = line.split("=", 1)
             cfg[k.strip()] = v.strip()
+    return cfg
+
+
+def read_case(ini_path: str) -> Dict:
+    """Reads input.ini + grid CSV + boundaries CSV back into a case dict."""
+    cfg = read_ini(ini_path)
     base = os.path.dirname(os.path.abspath(ini_path))
 
     def resolve(p):

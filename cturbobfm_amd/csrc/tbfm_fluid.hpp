@@ -55,6 +55,19 @@ __host__ __device__ inline double lut_interp(const LutTab& T, double x, double y
 
 __host__ __device__ inline double lut_at(const LutSet& L, int t, double x, double y) { return lut_interp(L.t[t], x, y); }
 
+// FluidReal::computePressure_rho_T (fluid_real.cpp:262-277): 25 halvings of the T
+// table's energy range for T(rho, e) = T, then p(rho, e) at the midpoint.
+__host__ __device__ inline double lut_p_rho_T(const LutSet& L, double rho, double T) {
+    double e_low = L.t[LUT_T].ymin, e_high = L.t[LUT_T].ymax;
+    for (int it = 0; it < 25; ++it) {
+        const double e_mid = 0.5 * (e_low + e_high);
+        const double T_mid = lut_interp(L.t[LUT_T], rho, e_mid);
+        if (T_mid < T) e_low = e_mid;
+        else e_high = e_mid;
+    }
+    return lut_interp(L.t[LUT_P], rho, 0.5 * (e_low + e_high));
+}
+
 // FluidReal::computeStaticEnergy_p_rho: bisection on e over the RHO_E energy range,
 // at most 30 halvings, tolerance 1e-4 p.
 __host__ __device__ inline double lut_e_p_rho(const LutSet& L, double p, double rho) {

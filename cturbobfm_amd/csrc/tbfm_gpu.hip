@@ -15,6 +15,7 @@
 #include "../../include/tbfm_gpu.h"
 #include "tbfm_fused.hpp"
 #include "tbfm_host.hpp"
+#include "tbfm_io.hpp"
 
 using namespace tbfm;
 using tbfm::dev::CaseDev;
@@ -107,8 +108,9 @@ struct tbfm_case {
     Scheme scheme = Scheme::JST;
     Precision precision = Precision::FP64;
     FluidModel fluid = FluidModel::Ideal;
-    LutSet lut{};                     // host copy of the fluid tables (FLUID_MODEL = real)
-    std::vector<double> lutValues;    // their values, all tables back to back
+    HostFluid hf;                     // the fluid on the host (tables of FLUID_MODEL = real)
+    LutSet lut{};                     // host copy of the fluid tables (points into hf.values)
+    GridInput grid;                   // grid points and the blockage / rpm fields, for output files
     std::vector<double> rk;
     Geometry geo;
     std::vector<Patch> patches;
@@ -226,52 +228,21 @@ void setup(tbfm_case* cs, const GridInput& grid) {
     cs->fluid = cfg.getFluidModel();
     c.real = 0;
     c.lut = nullptr;
-    if (cs->fluid == FluidModel::Real) {
-        // Solver::buildFluidModel (solver.cpp:49-74); no table generation here: the
-        // table file has to exist
-        if (cs->precision == Precision::Mixed)
-            throw std::runtime_error("GPU_PRECISION = mixed needs FLUID_MODEL = ideal: the look-up-table fluid runs in FP64");
-        std::string path = cfg.getFluidTableFile();
-        // relative to the input.ini folder when read from one, else to the working folder
-        if (!path.empty() && path[0] != '/' && !cfg.baseDir.empty()) path = cfg.baseDir + "/" + path;
-        const FluidTable ft = readFluidTable(path);
-        static_assert(FluidTable::kTables == LUT_COUNT, "table count");
-        size_t total = 0;
-        for (int t = 0; t < LUT_COUNT; ++t) total += ft.t[t].v.size();
-        cs->lutValues.resize(total);
-        size_t off[LUT_COUNT];
-        size_t o = 0;
-        for (int t = 0; t < LUT_COUNT; ++t) {
-            const FluidTable::Tab& F = ft.t[t];
-            LutTab& T = cs->lut.t[t];
-            T.nx = (int)F.nx; T.ny = (int)F.ny;
-            T.xmin = F.xmin; T.xmax = F.xmax; T.ymin = F.ymin; T.ymax = F.ymax;
-            // Table2D::init (fluid_real.hpp:22-32)
-            T.dx = (F.nx > 1) ? (F.xmax - F.xmin) / (double)(F.nx - 1) : 1.0;
-            T.dy = (F.ny > 1) ? (F.ymax - F.ymin) / (double)(F.ny - 1) : 1.0;
-            std::copy(F.v.begin(), F.v.end(), cs->lutValues.begin() + o);
-            off[t] = o;
-            o += F.v.size();
-        }
-        double* dv = upload(cs->lutValues, cs->owned);
+    if (cs->fluid == FluidModel::Real && cs->precision == Precision::Mixed)
+        throw std::runtime_error("GPU_PRECISION = mixed needs FLUID_MODEL = ideal: the look-up-table fluid runs in FP64");
+    // relative table path from the input.ini folder when read from one, else from the
+    // working folder
+    cs->hf.load(cfg, cfg.baseDir);
+    if (cs->hf.real) {
+        cs->lut = cs->hf.lut;
+        double* dv = upload(cs->hf.values, cs->owned);
         LutSet dset = cs->lut;
-        for (int t = 0; t < LUT_COUNT; ++t) {
-            cs->lut.t[t].d = cs->lutValues.data() + off[t];
-            dset.t[t].d = dv + off[t];
-        }
+        for (int t = 0; t < LUT_COUNT; ++t) dset.t[t].d = dv + (cs->lut.t[t].d - cs->hf.values.data());
         c.lut = upload(std::vector<LutSet>{dset}, cs->owned);
         c.real = 1;
-        auto hdr = [&](const char* k, double def) {
-            auto it = ft.header.find(k);
-            return it == ft.header.end() ? def : std::stod(it->second);
-        };
-        // gamma and R only feed ideal-gas formulas, which this fluid never reaches
-        c.gamma = cfg.has("FLUID_GAMMA") ? cfg.getFluidGamma() : hdr("NOMINAL_GAMMA", 1.4);
-        c.rgas = cfg.has("FLUID_R_CONSTANT") ? cfg.getFluidGasConstant() : hdr("GAS_CONSTANT", 287.0);
-    } else {
-        c.gamma = cfg.getFluidGamma();
-        c.rgas = cfg.getFluidGasConstant();
     }
+    c.gamma = cs->hf.gamma;
+    c.rgas = cs->hf.R;
     c.gm1 = c.gamma - 1;
     c.cp = c.gamma * c.rgas / (c.gamma - 1.0);
     c.cfl = cfg.getCFL();
@@ -680,35 +651,27 @@ void setup(tbfm_case* cs, const GridInput& grid) {
     const int dirs = (cs->topo == Topology::ThreeD) ? 3 : (cs->topo == Topology::OneD ? 1 : 2);
     cs->bytes_per_node = 8.0 * (15 + 2 + 3 * dirs);
 
-    // ---- initial solution (solver.cpp:853-888, fluid_ideal.cpp:155-170)
-    std::vector<double> U0(5 * N);
-    {
-        std::vector<double> dir = cfg.getInitDirection();
-        const double gm = c.gamma, Rg = c.rgas, T = cfg.getInitTemperature(), p = cfg.getInitPressure();
-        double dm = std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
-        double M = cfg.getInitMachNumber();
-        double rho, vx, vy, vz, et;
-        if (c.real) {
-            // FluidReal::computeInitFields (src/fluid_real.cpp:375-393)
-            rho = lut_at(cs->lut, LUT_RHO_PT, p, T);
-            double e = lut_at(cs->lut, LUT_E_PT, p, T);
-            double a = lut_at(cs->lut, LUT_A, rho, e);
-            double dx = 1.0, dy = 0.0, dz = 0.0;
-            if (dm > 1e-12) { dx = dir[0] / dm; dy = dir[1] / dm; dz = dir[2] / dm; }
-            vx = dx * a * M; vy = dy * a * M; vz = dz * a * M;
-            et = e + 0.5 * (vx * vx + vy * vy + vz * vz);
-        } else {
-            double a = std::sqrt(gm * Rg * T);
-            vx = dir[0] / dm * a * M; vy = dir[1] / dm * a * M; vz = dir[2] / dm * a * M;
-            rho = p / (Rg * T);
-            double e = p / ((gm - 1) * rho);
-            double vm = std::sqrt(vx * vx + vy * vy + vz * vz);
-            et = e + 0.5 * (vm * vm);
-        }
-        for (long n = 0; n < N; ++n) {
-            U0[n] = rho; U0[N + n] = rho * vx; U0[2 * N + n] = rho * vy; U0[3 * N + n] = rho * vz; U0[4 * N + n] = rho * et;
+    // ---- initial solution: uniform (solver.cpp:853-888) or from a restart file
+    // (solver.cpp:890-1295), with the restart's body forces as the side outputs' start
+    std::vector<double> U0, fi0, fv0;
+    bool restartForces = false;
+    initialState(cfg, cfg.baseDir, cs->hf, grid, cs->topo, periodic, U0, fi0, fv0, restartForces);
+    if (restartForces) {
+        std::vector<double> comp(N), dev(N), h3(3 * N);
+        for (const auto& pr : {std::make_pair(&fi0, c.finv), std::make_pair(&fv0, c.fvisc)}) {
+            for (int q = 0; q < 3; ++q) {
+                for (long n = 0; n < N; ++n) comp[n] = (*pr.first)[3 * n + q];
+                toDevice(cs, comp.data(), dev);
+                for (long n = 0; n < N; ++n) h3[3 * n + q] = dev[n];
+            }
+            HIPCHK(hipMemcpy(pr.second, h3.data(), h3.size() * 8, hipMemcpyHostToDevice));
         }
     }
+    // what the volume file needs later
+    cs->grid.ni = grid.ni; cs->grid.nj = grid.nj; cs->grid.nk = grid.nk;
+    cs->grid.x = grid.x; cs->grid.y = grid.y; cs->grid.z = grid.z;
+    for (const char* f : {"blockage", "rpm"})
+        if (const std::vector<double>* v = grid.field(f)) cs->grid.fields[f] = *v;
     int rc = tbfm_gpu_set_state(cs, U0.data());
     if (rc != TBFM_OK) throw std::runtime_error(cs->err);
 }
@@ -1785,3 +1748,88 @@ int tbfm_gpu_batch_destroy(tbfm_batch* b) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- on-disk formats
+
+namespace {
+
+// Config, grid, boundaries and fluid of an input.ini, host side only
+struct HostCase {
+    Config cfg;
+    GridInput grid;
+    std::vector<Patch> patches;
+    HostFluid hf;
+    explicit HostCase(const char* ini) : cfg(ini) {
+        auto resolve = [&](const std::string& p) { return (!p.empty() && p[0] == '/') ? p : cfg.baseDir + "/" + p; };
+        grid = readGridCsv(resolve(cfg.getGridFile()));
+        patches = readBoundaryCsv(resolve(cfg.getBoundaryFile()), cfg);
+        hf.load(cfg, cfg.baseDir);
+    }
+    bool periodic() const {
+        for (auto& p : patches)
+            if (p.type == PatchType::Periodic && !p.values.empty() && p.values[0] == 0) return true;
+        return false;
+    }
+};
+
+VolumeData volumeData(const GridInput& g, const Geometry* geo, const double* U, const double* finv,
+                      const double* fvisc, const double* dev) {
+    VolumeData d;
+    d.ni = g.ni; d.nj = g.nj; d.nk = g.nk;
+    d.x = g.x.data(); d.y = g.y.data(); d.z = g.z.data();
+    d.U = U; d.finv = finv; d.fvisc = fvisc; d.dev = dev;
+    if (const std::vector<double>* b = g.field("blockage")) d.blockage = b->data();
+    if (const std::vector<double>* r = g.field("rpm")) d.rpm = r->data();
+    if (geo) { d.radius = geo->radius.data(); d.theta = geo->theta.data(); }
+    return d;
+}
+
+}  // namespace
+
+int tbfm_gpu_write_solution(tbfm_case* c, const char* path) {
+    if (!c || !path) return fail(c, TBFM_ERR_ARGUMENT, "null argument");
+    return guarded(c, [&]() {
+        std::vector<double> U(5 * c->N), fi(3 * c->N), fv(3 * c->N), dv(c->N);
+        int rc = tbfm_gpu_get_state(c, U.data());
+        if (rc == TBFM_OK) rc = tbfm_gpu_get_side_outputs(c, fi.data(), fv.data(), dv.data(), nullptr);
+        if (rc != TBFM_OK) return rc;
+        writeVolumeCsv(path, c->cfg, c->hf, volumeData(c->grid, &c->geo, U.data(), fi.data(), fv.data(), dv.data()));
+        return TBFM_OK;
+    });
+}
+
+int tbfm_write_solution_from_ini(const char* ini, const double* U, const double* finv, const double* fvisc,
+                                 const double* dev, const char* path) {
+    if (!ini || !U || !path) return fail(nullptr, TBFM_ERR_ARGUMENT, "null argument");
+    return guarded((std::string*)nullptr, [&]() {
+        HostCase h(ini);
+        const Geometry geo = buildGeometry(h.grid, h.cfg.getTopology(), false);
+        writeVolumeCsv(path, h.cfg, h.hf, volumeData(h.grid, &geo, U, finv, fvisc, dev));
+        return TBFM_OK;
+    });
+}
+
+int tbfm_volume_file_name(const char* ini, long iteration, char* name, long size) {
+    if (!ini || !name || size <= 0) return fail(nullptr, TBFM_ERR_ARGUMENT, "null argument");
+    return guarded((std::string*)nullptr, [&]() {
+        const std::string f = volumeFileName(Config(ini), iteration);
+        if ((long)f.size() + 1 > size) throw std::invalid_argument("name buffer too small");
+        std::memcpy(name, f.c_str(), f.size() + 1);
+        return TBFM_OK;
+    });
+}
+
+int tbfm_initial_state_from_ini(const char* ini, double* U, double* finv, double* fvisc) {
+    if (!ini || !U) return fail(nullptr, TBFM_ERR_ARGUMENT, "null argument");
+    return guarded((std::string*)nullptr, [&]() {
+        HostCase h(ini);
+        std::vector<double> u, fi, fv;
+        bool forces = false;
+        initialState(h.cfg, h.cfg.baseDir, h.hf, h.grid, h.cfg.getTopology(), h.periodic(), u, fi, fv, forces);
+        std::copy(u.begin(), u.end(), U);
+        if (finv) std::copy(fi.begin(), fi.end(), finv);
+        if (fvisc) std::copy(fv.begin(), fv.end(), fvisc);
+        return TBFM_OK;
+    });
+}
+

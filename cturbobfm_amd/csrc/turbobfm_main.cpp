@@ -1,8 +1,16 @@
 // `turbobfm input.ini` for the GPU path (replaces src/turbobfm.cpp:7-39 when the
-// explicit residual path runs on the device).  Reads the same input.ini, grid CSV and
-// boundaries CSV, advances up to N_ITERATIONS iterations (stopping at the reference's
-// convergence test), prints the residual table like Solver::printLogResiduals
-// (solver.cpp:1473-1484) and writes residuals.csv and the final state.
+// explicit residual path runs on the device).  Reads the same input.ini, grid CSV,
+// boundaries CSV and restart file, advances up to N_ITERATIONS iterations (stopping at
+// the reference's convergence test), prints the residual table like
+// Solver::printLogResiduals (solver.cpp:1473-1484) and writes the reference's files in
+// the working folder:
+//   Volume_CSV/<SOLUTION_NAME>[_<iteration>].csv  every SAVE_ITERATIONS_INTERVAL
+//       iterations, at N_ITERATIONS and at convergence (solver.cpp:1400-1413,
+//       output.cpp:367-447);
+//   residuals.csv  "Rho,RhoU,RhoV,RhoW,RhoE" and one row of log residuals per iteration,
+//       six significant digits (Solver::writeLogResidualsToCsvFile, solver.cpp:2520-2556).
+#include <sys/stat.h>
+
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -10,41 +18,76 @@
 
 #include "../../include/tbfm_gpu.h"
 
+namespace {
+
+// KEY = value of input.ini ("" when absent)
+std::string iniValue(const char* ini, const char* key) {
+    FILE* f = std::fopen(ini, "r");
+    char line[4096];
+    std::string out;
+    while (f && std::fgets(line, sizeof line, f)) {
+        std::string s(line);
+        if (s.empty() || s[0] == ';' || s[0] == '[') continue;
+        size_t eq = s.find('=');
+        if (eq == std::string::npos) continue;
+        std::string k = s.substr(0, eq);
+        k.erase(0, k.find_first_not_of(" \t"));
+        k.erase(k.find_last_not_of(" \t") + 1);
+        if (k == key) {
+            out = s.substr(eq + 1);
+            out.erase(0, out.find_first_not_of(" \t"));
+            out.erase(out.find_last_not_of(" \t\r\n") + 1);
+        }
+    }
+    if (f) std::fclose(f);
+    return out;
+}
+
+bool writeVolume(tbfm_case* c, const char* ini, long it) {
+    char name[4096];
+    if (tbfm_volume_file_name(ini, it, name, sizeof name) != TBFM_OK) {
+        std::fprintf(stderr, "Solver execution halted: %s\n", tbfm_gpu_last_error(nullptr));
+        return false;
+    }
+    mkdir("Volume_CSV", 0755);
+    const std::string path = std::string("Volume_CSV/") + name + ".csv";
+    if (tbfm_gpu_write_solution(c, path.c_str()) != TBFM_OK) {
+        std::fprintf(stderr, "Solver execution halted: %s\n", tbfm_gpu_last_error(c));
+        return false;
+    }
+    std::printf("\nSolution written to file: %s\n\n", name);
+    return true;
+}
+
+}  // namespace
+
 int main(int argc, char** argv) {
     if (argc < 2) {
         std::fprintf(stderr, "usage: %s input.ini [max_iterations]\n", argv[0]);
         return 1;
     }
+    const char* ini = argv[1];
     tbfm_case* c = nullptr;
-    if (tbfm_gpu_case_create_from_ini(argv[1], &c) != TBFM_OK) {
+    if (tbfm_gpu_case_create_from_ini(ini, &c) != TBFM_OK) {
         std::fprintf(stderr, "Solver execution halted: %s\n", tbfm_gpu_last_error(nullptr));
         return 1;
     }
-    long ni, nj, nk;
-    tbfm_gpu_dims(c, &ni, &nj, &nk);
-    long nmax = 0;
-    {
-        // N_ITERATIONS from input.ini, overridable on the command line
-        FILE* f = std::fopen(argv[1], "r");
-        char line[4096];
-        while (f && std::fgets(line, sizeof line, f)) {
-            std::string s(line);
-            size_t eq = s.find('=');
-            if (eq == std::string::npos) continue;
-            std::string k = s.substr(0, eq);
-            k.erase(0, k.find_first_not_of(" \t"));
-            k.erase(k.find_last_not_of(" \t") + 1);
-            if (k == "N_ITERATIONS") nmax = (long)std::atof(s.c_str() + eq + 1);
-        }
-        if (f) std::fclose(f);
-    }
+    long nmax = (long)std::atof(iniValue(ini, "N_ITERATIONS").c_str());
     if (argc >= 3) nmax = std::atol(argv[2]);
-    const long chunk = 100;
+    const std::string every = iniValue(ini, "SAVE_ITERATIONS_INTERVAL");
+    const long interval = every.empty() ? 0 : (long)std::atof(every.c_str());
+    if (interval <= 0) {
+        std::fprintf(stderr, "Solver execution halted: SAVE_ITERATIONS_INTERVAL must be a positive number\n");
+        tbfm_gpu_case_destroy(c);
+        return 1;
+    }
     std::vector<double> hist;
     long conv = 0, done = 0;
     std::printf("|%14s|%14s|%14s|%14s|%14s|%14s|\n", "Iteration", "rms[Rho]", "rms[RhoU]", "rms[RhoV]", "rms[RhoW]", "rms[RhoE]");
+    int status = 0;
     while (done < nmax && conv == 0) {
-        long n = std::min(chunk, nmax - done);
+        // advance to the next output iteration (or the end) in one call
+        long n = std::min(interval - done % interval, nmax - done);
         std::vector<double> lr(5 * n, 0.0);
         int rc = tbfm_gpu_advance(c, n, lr.data(), &conv);
         long rows = conv ? conv - done : n;
@@ -55,34 +98,27 @@ int main(int argc, char** argv) {
         }
         if (rc != TBFM_OK) {
             std::fprintf(stderr, "Solver execution halted: %s\n", tbfm_gpu_last_error(c));
-            tbfm_gpu_case_destroy(c);
-            return 1;
+            status = 1;
+            break;
         }
         done += rows;
+        if (conv) std::printf("\nConvergence reached at iteration %ld\n", conv);
+        if (conv || done % interval == 0 || done == nmax)
+            if (!writeVolume(c, ini, done)) { status = 1; break; }
     }
-    if (conv) std::printf("\nConvergence reached at iteration %ld\n", conv);
     FILE* f = std::fopen("residuals.csv", "w");
     if (f) {
-        std::fprintf(f, "Iteration,rms[Rho],rms[RhoU],rms[RhoV],rms[RhoW],rms[RhoE]\n");
+        std::fprintf(f, "Rho,RhoU,RhoV,RhoW,RhoE\n");
         for (size_t r = 0; r < hist.size() / 5; ++r)
-            std::fprintf(f, "%zu,%.6g,%.6g,%.6g,%.6g,%.6g\n", r + 1, hist[5 * r], hist[5 * r + 1], hist[5 * r + 2],
-                         hist[5 * r + 3], hist[5 * r + 4]);
+            std::fprintf(f, "%g,%g,%g,%g,%g\n", hist[5 * r], hist[5 * r + 1], hist[5 * r + 2], hist[5 * r + 3],
+                         hist[5 * r + 4]);
         std::fclose(f);
-    }
-    const long N = ni * nj * nk;
-    std::vector<double> U(5 * N);
-    tbfm_gpu_get_state(c, U.data());
-    f = std::fopen("solution.csv", "w");
-    if (f) {
-        std::fprintf(f, "NI=%ld\nNJ=%ld\nNK=%ld\nrho,rhoU,rhoV,rhoW,rhoE\n", ni, nj, nk);
-        for (long n = 0; n < N; ++n)
-            std::fprintf(f, "%.16e,%.16e,%.16e,%.16e,%.16e\n", U[n], U[N + n], U[2 * N + n], U[3 * N + n], U[4 * N + n]);
-        std::fclose(f);
+        std::printf("\nLog residuals appended to residuals.csv\n\n");
     }
     double turbo[10];
-    if (tbfm_gpu_get_turbo(c, turbo) == TBFM_OK)
+    if (status == 0 && tbfm_gpu_get_turbo(c, turbo) == TBFM_OK)
         std::printf("\nMass Flow [kg/s]: %.6g\nTotal Pressure Ratio [-]: %.6g\nTotal Temperature Ratio [-]: %.6g\n",
                     turbo[6], turbo[7], turbo[8]);
     tbfm_gpu_case_destroy(c);
-    return 0;
+    return status;
 }

@@ -186,6 +186,38 @@ int tbfm_gpu_batch_advance(tbfm_batch* b, long n_iter);
 int tbfm_gpu_batch_destroy(tbfm_batch* b);
 const char* tbfm_gpu_batch_last_error(const tbfm_batch* b);
 
+/* ---- on-disk formats */
+
+/* Output::writeSolution (src/output.cpp:367-447) of the case's current state and side
+ * outputs: "NI=", "NJ=", "NK=" lines, the header "x,y,z" plus the field names of
+ * OUTPUT_FIELDS_TYPE (primary; secondary, the default; turbo_bfm) in name order, one row
+ * per node with k fastest, 16 significant digits.  path is the file to write (the
+ * reference writes Volume_CSV/<SOLUTION_NAME>[_<iteration>].csv, see
+ * tbfm_volume_file_name). */
+int tbfm_gpu_write_solution(tbfm_case* c, const char* path);
+
+/* The same file for a state given by the caller (U: 5*N, finv / fvisc: 3*N xyz
+ * triples, dev: N, reference layout; the side outputs may be NULL for zeros), with the
+ * grid, fluid and output keys read from input.ini.  Host only: no device is used. */
+int tbfm_write_solution_from_ini(const char* ini_path, const double* U, const double* finv, const double* fvisc,
+                                 const double* dev, const char* path);
+
+/* Output::getOutputFilename (src/output.cpp:352-364): SOLUTION_NAME, with
+ * "_<6-digit iteration>" when SAVE_UNSTEADY (default yes); written into name (size
+ * bytes, NUL-terminated).  Host only. */
+int tbfm_volume_file_name(const char* ini_path, long iteration, char* name, long size);
+
+/* Initial state the case created from input.ini starts from (replaces
+ * Solver::initializeSolutionArrays' state part, solver.cpp:777-1295): the uniform
+ * INIT_* state (INIT_DIRECTION = 0,0,0: the grid's i-direction per node), or with
+ * RESTART_SOLUTION = yes the restart file RESTART_SOLUTION_FILEPATH: same dimensions ->
+ * standard restart; RESTART_TYPE = axisymmetric with equal NI, NJ -> the k = 0 plane
+ * rotated to every node; nearest_neighbor[_axisymmetric] or other dimensions -> the
+ * nearest file node (in x, y, z, or in x, r with the velocity rotated).  U is 5*N;
+ * finv / fvisc (3*N, may be NULL) receive the restart's body forces (zeros otherwise).
+ * Host only. */
+int tbfm_initial_state_from_ini(const char* ini_path, double* U, double* finv, double* fvisc);
+
 #ifdef __cplusplus
 }
 #endif

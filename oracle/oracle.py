@@ -429,5 +429,7 @@ def ref_lib():
         L.ref_radial_equilibrium.argtypes = [dp, dp, dp, C.c_int, C.c_double, dp]
         L.ref_fluid_eval.restype = C.c_double
         L.ref_fluid_eval.argtypes = [C.c_int, C.c_double, C.c_double]
+        L.ref_write_solution.restype = C.c_int
+        L.ref_write_solution.argtypes = [dp, dp, dp, dp, C.c_long, C.c_char_p]
         _ref = L
     return _ref

@@ -5,7 +5,7 @@
 // src/math_utils.cpp, src/fluid_ideal.cpp, src/advection_{base,jst,roe}.cpp,
 // src/boundary_{base,inlet,outlet,outlet_radial_equilibrium,inviscid_wall,
 // fake}.cpp, src/source_bfm_{base,hall,lift_drag,thollet}.cpp, src/input_table.cpp,
-// src/fluid_real.cpp)
+// src/fluid_real.cpp, src/output.cpp, src/turbulence_model_base.cpp)
 // into oracle/_ref/libtbfm_ref.so by oracle/Makefile.  Only tests/ and
 // tests/golden/gen_golden.py load it, to pin the CPU restatement in
 // oracle/tbfm_oracle.c and the product's host geometry.  src/solver.cpp is not
@@ -17,6 +17,7 @@
 //   BoundaryBase::computeBoundaryFlux   include/boundary_base.hpp:26-32
 //   SourceBFMBase::computeTotalSource   include/source_bfm_base.hpp:20-31
 //   FluidIdeal / FluidReal              include/fluid_ideal.hpp, include/fluid_real.hpp
+//   Output::writeSolution               include/output.hpp:30-34
 #include "types.hpp"
 #include "config.hpp"
 #include "mesh.hpp"
@@ -32,6 +33,10 @@
 #include "source_bfm_hall.hpp"
 #include "source_bfm_lift_drag.hpp"
 #include "source_bfm_thollet.hpp"
+#include "output.hpp"
+#include "turbulence_model_none.hpp"
+
+#include <filesystem>
 
 #include <memory>
 #include <cstdio>
@@ -254,6 +259,48 @@ void ref_radial_equilibrium(const double* rho, const double* ut, const double* r
     FloatType h = hub;
     integrateRadialEquilibrium(a, b, c, out, h);
     for (int i = 0; i < n; ++i) p[i] = out[i];
+}
+
+// Volume solution file through the reference's own writer: Output::writeSolution
+// (src/output.cpp:367-447) in folder `dir` (Volume_CSV/<SOLUTION_NAME>[_<it>].csv) for
+// the state U (5 x N, k fastest) and side outputs finv / fvisc (xyz triples) and
+// devang.  No boundaries, no turbulence model (TurbulenceModelNone), as on the Euler
+// path.  Returns 0 on success.
+int ref_write_solution(const double* U, const double* finv, const double* fvisc, const double* devang, long it,
+                       const char* dir) {
+    try {
+        const Mesh& m = *g_case.mesh;
+        const size_t ni = m.getNumberPointsI(), nj = m.getNumberPointsJ(), nk = m.getNumberPointsK();
+        const size_t N = ni * nj * nk;
+        FlowSolution sol(ni, nj, nk);
+        Matrix3D<Vector3D> inv(ni, nj, nk), visc(ni, nj, nk);
+        Matrix3D<FloatType> dev(ni, nj, nk), wall(ni, nj, nk);
+        for (size_t n = 0; n < N; ++n) {
+            sol._rho[n] = U[n];
+            sol._rhoU[n] = U[N + n];
+            sol._rhoV[n] = U[2 * N + n];
+            sol._rhoW[n] = U[3 * N + n];
+            sol._rhoE[n] = U[4 * N + n];
+            inv[n] = Vector3D(finv[3 * n], finv[3 * n + 1], finv[3 * n + 2]);
+            visc[n] = Vector3D(fvisc[3 * n], fvisc[3 * n + 1], fvisc[3 * n + 2]);
+            dev[n] = devang[n];
+        }
+        std::map<SolutionName, Matrix3D<Vector3D>> grad;
+        std::vector<Boundary> boundaries;
+        TurbulenceModelNone turb(*g_case.config, *g_case.fluid, m, boundaries, wall, sol);
+        const std::filesystem::path here = std::filesystem::current_path();
+        std::filesystem::current_path(dir);
+        {
+            Output out(*g_case.config, m, sol, grad, *g_case.fluid, boundaries, turb, inv, visc, dev, wall);
+            out.writeSolution((size_t)it);
+        }
+        std::filesystem::current_path(here);
+        std::fflush(stdout);
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "ref_write_solution: %s\n", e.what());
+        return -1;
+    }
+    return 0;
 }
 
 }  // extern "C"

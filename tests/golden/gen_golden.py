@@ -11,6 +11,9 @@ Each fixture holds seeded inputs and the reference's outputs:
                           (+ blockage), with inviscid/viscous force and deviation angle
   fluid.npz               FluidIdeal p, a, h_t, T, p_t, T_t, s
   radial_eq.npz           integrateRadialEquilibrium
+  volume_<case>.npz       Output::writeSolution: file name and text of the volume solution
+                          file for seeded states and side outputs (primary, secondary and
+                          turbo_bfm fields; ideal and table fluid)
   lut.npz                 FluidReal on a seeded covolume-gas table (the table itself is
                           stored in the fixture): the 17 point functions, JST and
                           Roe-MUSCL face fluxes, inlet / outlet / wall fluxes, p a h_t T
@@ -160,7 +163,49 @@ def main():
                                 S=np.array(outs), finv=np.array(fis), fvisc=np.array(fvs), dev=np.array(devs))
     lut_fixtures(R, rng, tmp)
     thollet_fixtures(R, tmp)
+    volume_fixtures(R, tmp)
     print("golden fixtures written to", HERE)
+
+
+def volume_cases(lut_path):
+    """Cases whose volume solution file the reference writes: every OUTPUT_FIELDS_TYPE,
+    both fluids, steady and unsteady file names."""
+    rotor = cases.rotor3d(9, 4, 5, bfm_model="Hall-Thollet")
+    rotor["config"].update({"OUTPUT_FIELDS_TYPE": "turbo_bfm", "SOLUTION_NAME": "results", "SAVE_UNSTEADY": "no",
+                            "ROTATIONAL_SPEED_SCALING_FACTOR": "0.97"})
+    bump = cases.bump2d(9, 6)
+    bump["config"].update({"SOLUTION_NAME": "flow"})
+    prim = cases.bump2d(9, 6)
+    prim["config"].update({"OUTPUT_FIELDS_TYPE": "primary", "SOLUTION_NAME": "results", "SAVE_UNSTEADY": "no"})
+    real = cases.with_real_gas(cases.bump2d(9, 6), lut_path)
+    real["config"].update({"SOLUTION_NAME": "results", "SAVE_UNSTEADY": "no"})
+    return {"rotor3d_turbo_bfm": rotor, "bump2d_secondary_unsteady": bump, "bump2d_primary": prim,
+            "bump2d_real_secondary": real}
+
+
+def volume_fixtures(R, tmp, iteration=42):
+    """Output::writeSolution of the reference (src/output.cpp) for seeded states and side
+    outputs: the file name and its full text."""
+    rng = np.random.default_rng(20261205)
+    lut_path = cases.write_lut(cases.covolume_lut(seed=5), os.path.join(tmp, "covolume_volume.lut"))
+    for name, case in volume_cases(lut_path).items():
+        folder = os.path.join(tmp, "volume_" + name)
+        ini = cases.write_case(case, folder)
+        assert R.ref_case_load(ini.encode()) == 0
+        oc = O.OracleCase(case)
+        U = np.ascontiguousarray(cases.perturb_state(case, oc.state(), 0.03, int(rng.integers(1, 1000))))
+        N = case["ni"] * case["nj"] * case["nk"]
+        fi = rng.normal(0.0, 300.0, 3 * N)
+        fv = rng.normal(0.0, 30.0, 3 * N)
+        dv = rng.uniform(-0.3, 0.3, N)
+        assert R.ref_write_solution(dp(U), dp(fi), dp(fv), dp(dv), iteration, folder.encode()) == 0
+        files = os.listdir(os.path.join(folder, "Volume_CSV"))
+        assert len(files) == 1, files
+        with open(os.path.join(folder, "Volume_CSV", files[0]), "rb") as f:
+            text = f.read()
+        np.savez_compressed(os.path.join(HERE, "volume_%s.npz" % name), U=U, finv=fi, fvisc=fv, dev=dv,
+                            iteration=np.array(iteration), fname=np.array(files[0]),
+                            text=np.frombuffer(text, dtype=np.uint8))
 
 
 def thollet_cases():

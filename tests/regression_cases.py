@@ -14,7 +14,8 @@ HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "regre
 
 COMMON = {"KIND_SOLVER": "Euler", "FLUID_GAMMA": "1.4", "FLUID_R_CONSTANT": "287.05", "FLUID_MODEL": "Ideal",
           "TIME_INTEGRATION_TYPE": "0", "TIME_STEP_METHOD": "local", "N_ITERATIONS": "99999",
-          "SAVE_ITERATIONS_INTERVAL": "250", "LOWER_WALL": "inviscid_wall", "UPPER_WALL": "inviscid_wall"}
+          "SAVE_ITERATIONS_INTERVAL": "250", "LOWER_WALL": "inviscid_wall", "UPPER_WALL": "inviscid_wall",
+          "SOLUTION_NAME": "results", "SAVE_UNSTEADY": "no"}
 
 CONFIGS = {
     "CircularBump2D": dict(COMMON, TOPOLOGY="2D", INFLOW="inlet, 140513.23, 316.224, 1, 0, 0",

@@ -17,7 +17,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_symbols():
     txt = open(os.path.join(ROOT, "include", "tbfm_gpu.h")).read()
-    return sorted(set(re.findall(r"\b(tbfm_gpu_\w+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(tbfm_\w+)\s*\(", txt)))
 
 
 def test_library_exports_every_declared_symbol():
