@@ -47,7 +47,8 @@ EXPORTS = ["tbfm_gpu_case_create", "tbfm_gpu_case_create_from_ini", "tbfm_gpu_ca
            "tbfm_gpu_case_status", "tbfm_gpu_batch_create", "tbfm_gpu_batch_admit", "tbfm_gpu_batch_retire",
            "tbfm_gpu_batch_size", "tbfm_gpu_batch_advance", "tbfm_gpu_batch_destroy",
            "tbfm_gpu_batch_last_error", "tbfm_gpu_device_memory", "tbfm_gpu_write_solution",
-           "tbfm_write_solution_from_ini", "tbfm_volume_file_name", "tbfm_initial_state_from_ini"]
+           "tbfm_write_solution_from_ini", "tbfm_volume_file_name", "tbfm_initial_state_from_ini",
+           "tbfm_gpu_get_turbo_history"]
 
 _lib = None
 
@@ -98,6 +99,7 @@ def lib():
         L.tbfm_gpu_batch_last_error.argtypes = [vp]
         L.tbfm_gpu_device_memory.argtypes = [C.POINTER(C.c_long), C.POINTER(C.c_long)]
         L.tbfm_gpu_write_solution.argtypes = [vp, C.c_char_p]
+        L.tbfm_gpu_get_turbo_history.argtypes = [vp, C.c_long, C.c_long, dp]
         L.tbfm_write_solution_from_ini.argtypes = [C.c_char_p, dp, dp, dp, dp, C.c_char_p]
         L.tbfm_volume_file_name.argtypes = [C.c_char_p, C.c_long, C.c_char_p, C.c_long]
         L.tbfm_initial_state_from_ini.argtypes = [C.c_char_p, dp, dp, dp]
@@ -300,6 +302,15 @@ class Solver:
         """The reference's volume solution file (Output::writeSolution) of the current
         state and side outputs."""
         self._check(lib().tbfm_gpu_write_solution(self.h, os.fsencode(path)))
+
+    def turbo_history(self, first: int = 1, n: Optional[int] = None) -> np.ndarray:
+        """Per-iteration rows (time after the iteration, mass flow, PRtt, TRtt, ETAtt) of
+        a case created with TURBO_OUTPUT = yes."""
+        if n is None:
+            n = self.iterations - first + 1
+        out = np.zeros((max(n, 0), 5))
+        self._check(lib().tbfm_gpu_get_turbo_history(self.h, first, n, _dp(out)))
+        return out
 
     def turbo(self) -> np.ndarray:
         out = np.zeros(10)

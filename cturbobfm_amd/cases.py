@@ -91,11 +91,13 @@ def rotor3d(ni: int = 193, nj: int = 49, nk: int = 129, n_iterations: int = 2000
             muscl: bool = False, limiter: str = "None", outlet_pressure: float = 95000.0,
             length: float = 0.8, blockage: bool = True, time_integration: int = 0,
             time_step: str = "local", outlet: str = "radial_equilibrium", off_design: bool = False,
-            stall: bool = False) -> Dict:
+            stall: bool = False, turbo_output: bool = False) -> Dict:
     """Annular rotor; the last k plane is a bitwise copy of k = 0 (full annulus).
     bfm_model Hall, Lift-Drag or Hall-Thollet (K_n = K_f = K_d = 1, nu = 1.48e-5 as the
     reference's GER4 cases; off_design adds the deviation-pivot term); stall switches on
-    STALLED_BFM_ACTIVE with a stall deviation of 0.12 rad."""
+    STALLED_BFM_ACTIVE with a stall deviation of 0.12 rad.  turbo_output: TURBO_OUTPUT (the
+    mass flow, PRtt, TRtt, ETAtt of every iteration; the figures of the current state are
+    available either way)."""
     p0, t0 = 101325.0, 288.15
     mach = math.sqrt(max(((p0 / outlet_pressure) ** ((GAMMA - 1.0) / GAMMA) - 1.0) * 2.0 / (GAMMA - 1.0), 1e-4))
     _, t_init = _static_from_total(p0, t0, mach)
@@ -156,7 +158,7 @@ def rotor3d(ni: int = 193, nj: int = 49, nk: int = 129, n_iterations: int = 2000
         "RESIDUALS_DROP_CONVERGENCE": "16", "SAVE_ITERATIONS_INTERVAL": str(max(n_iterations, 1)),
         "TIME_INTEGRATION_TYPE": str(time_integration), "TIME_STEP_METHOD": time_step,
         "CONVECTION_SCHEME": scheme, "MUSCL_RECONSTRUCTION": "yes" if muscl else "no",
-        "FLUX_LIMITER": limiter, "TURBO_OUTPUT": "yes", "TURBO_DIRECTION": "i",
+        "FLUX_LIMITER": limiter, "TURBO_OUTPUT": "yes" if turbo_output else "no", "TURBO_DIRECTION": "i",
         "SOLUTION_NAME": "results",
     }
     if time_step == "fixed":

@@ -90,7 +90,7 @@ struct Plan {
     CaseDev* dcases = nullptr;        // device case table
     size_t dcap = 0;
     bool ownsCases = false;
-    Table dt, mn, prof, bnd, per, norm, start, pre;
+    Table dt, mn, prof, bnd, per, norm, start, pre, turbo;
     std::vector<StageGroup> groups;
     int nstages = 0;
     int fluids = 0;                   // bit 0: an ideal-gas member, bit 1: a table-fluid member
@@ -619,6 +619,50 @@ void setup(tbfm_case* cs, const GridInput& grid) {
     c.partial_min = dalloc<double>(c.nblk_dt, own);
     cs->hist_cap = std::max<long>(cfg.has("N_ITERATIONS") ? cfg.getMaxIterations() : 0, 1);
     c.conv.hist = dalloc<double>(5 * cs->hist_cap, own);
+    // turbomachinery history of every iteration (TURBO_OUTPUT, solver.cpp:1340-1341)
+    c.turbo_on = cfg.getBool("TURBO_OUTPUT", false) ? 1 : 0;
+    if (c.turbo_on) {
+        const Geometry& gg = cs->geo;
+        const int td = cfg.getTurboDirection();
+        int bi[2] = {0, 1};
+        if (td == 1) { bi[0] = 1; bi[1] = 0; } else if (td == 2) { bi[0] = 2; bi[1] = 3; } else if (td == 3) { bi[0] = 3; bi[1] = 2; }
+        const bool jdir = (bi[0] >= 2);
+        if (jdir && cs->topo == Topology::OneD) throw std::runtime_error("TURBO_DIRECTION j needs a 2-D grid");
+        const long n1 = jdir ? NI : NJ, n2 = NK;
+        c.turbo_nf = (int)(n1 * n2);
+        c.turbo_nblk = (int)((n1 * n2 + 255) / 256);
+        if (c.turbo_nblk > 256) throw std::runtime_error("turbo boundary too large (more than 65536 faces)");
+        c.turbo_part = dalloc<double>(2 * (size_t)c.turbo_nblk * dev::TURBO_SUMS, own);
+        std::vector<int> tn(2 * n1 * n2);
+        std::vector<double> tS(6 * n1 * n2);
+        for (int s = 0; s < 2; ++s)
+            for (long p = 0; p < n1; ++p)
+                for (long q = 0; q < n2; ++q) {
+                    const long f = (long)s * n1 * n2 + p * n2 + q;
+                    const double* S;
+                    long i, j;
+                    // boundary surfaces as the mesh stores them (Mesh::getMeshBoundary)
+                    switch (bi[s]) {
+                        case 0: S = &gg.surfI[((0 * NJ + p) * NK + q) * 3]; i = 0; j = p; break;
+                        case 1: S = &gg.surfI[((NI * NJ + p) * NK + q) * 3]; i = NI - 1; j = p; break;
+                        case 2: S = &gg.surfJ[((p * (NJ + 1) + 0) * NK + q) * 3]; i = p; j = 0; break;
+                        default: S = &gg.surfJ[((p * (NJ + 1) + NJ) * NK + q) * 3]; i = p; j = NJ - 1; break;
+                    }
+                    tn[f] = (int)nodeDev(cs, i, j, q);
+                    for (int d = 0; d < 3; ++d) tS[3 * f + d] = S[d];
+                }
+        c.turbo_node = upload(tn, own);
+        c.turbo_S = upload(tS, own);
+        double perDeg = 0.0;
+        for (auto& pt : cs->patches)
+            if (pt.type == PatchType::Periodic && !pt.values.empty() && pt.values[0] == 0)
+                perDeg = (pt.values[2] == 360.0) ? 0.0 : pt.values[2];
+        c.turbo_mscale = 1.0;
+        if (cs->topo == Topology::Axisymmetric) c.turbo_mscale = 2.0 * M_PI / gg.wedgeAngle;
+        else if (perDeg != 0.0) c.turbo_mscale = 360.0 / perDeg;
+        c.turbo_hist = dalloc<double>(5 * cs->hist_cap, own);
+        c.turbo_time = dalloc<double>(1, own);
+    }
     c.conv.init = dalloc<double>(5, own);
     c.conv.conv_at = dalloc<long>(1, own);
     c.conv.diverged = dalloc<int>(1, own);
@@ -730,7 +774,7 @@ void freePlan(Plan& P) {
     if (P.stream) hipStreamSynchronize(P.stream);
     if (P.graph) hipGraphExecDestroy(P.graph);
     P.graph = nullptr;
-    for (Table* t : {&P.dt, &P.mn, &P.prof, &P.bnd, &P.per, &P.norm, &P.start, &P.pre}) freeTable(*t);
+    for (Table* t : {&P.dt, &P.mn, &P.prof, &P.bnd, &P.per, &P.norm, &P.start, &P.pre, &P.turbo}) freeTable(*t);
     for (auto& g : P.groups) freeTable(g.t);
     P.groups.clear();
     if (P.ownsCases && P.dcases) hipFree(P.dcases);
@@ -783,7 +827,7 @@ std::vector<dev::Work> xcdOrder(const std::vector<dev::Work>& seq) {
 
 // Builds the work tables of every kernel from the members' current tiling.
 void buildTables(Plan& P) {
-    std::vector<dev::Work> dt, mn, prof, bnd, per, norm, src;
+    std::vector<dev::Work> dt, mn, prof, bnd, per, norm, src, turbo;
     for (auto& g : P.groups) freeTable(g.t);
     P.groups.clear();
     P.fluids = 0;
@@ -799,7 +843,8 @@ void buildTables(Plan& P) {
         const CaseDev& c = cs->c;
         if (c.nstages != P.nstages) throw std::runtime_error("cases of one batch must use the same RK scheme");
         for (int b = 0; b < c.nblk_dt; ++b) dt.push_back({m, b, 0, 0});
-        if (c.dt_mode == 1) mn.push_back({m, 0, 0, 0});
+        if (c.dt_mode == 1 || c.turbo_on) mn.push_back({m, 0, 0, 0});
+        for (int b = 0; c.turbo_on && b < 2 * c.turbo_nblk; ++b) turbo.push_back({m, b, 0, 3});
         if (c.nprof > 0) prof.push_back({m, 0, 0, 1});
         const long nb = 2 * cs->nj * cs->nk + ((cs->topo != Topology::OneD) ? 2 * cs->ni * cs->nk : 0) +
                         ((cs->topo == Topology::ThreeD) ? 2 * cs->ni * cs->nj : 0);
@@ -846,6 +891,7 @@ void buildTables(Plan& P) {
         return v;
     };
     std::vector<dev::Work> start = padded(prof), pre = padded(bnd);
+    uploadTable(P.turbo, turbo);
     const std::vector<dev::Work> dto = xcdOrder(dt), srco = xcdOrder(src);
     start.insert(start.end(), dto.begin(), dto.end());
     pre.insert(pre.end(), srco.begin(), srco.end());
@@ -906,10 +952,15 @@ void launchStageGroups(const Plan& P, int in, int out, int stage, int last, int 
 
 // k_iterstart over a table of time-step and / or profile blocks, then the global-step
 // minimum for the cases that use it.
-void launchStart(const Plan& P, const Table& t, int in, bool minimum) {
+// iteration: the launch of an explicit iteration (also closes the turbomachinery rows)
+void launchStart(const Plan& P, const Table& t, int in, bool minimum, bool iteration = false) {
     if (t.n) hipLaunchKernelGGL(dev::k_iterstart, dim3((unsigned)t.n), dim3(256), 0, P.stream, launchArg(P, t, in, 0, 0, 0, 0));
+    if (iteration && P.turbo.n)
+        hipLaunchKernelGGL(dev::k_turbo_partial, dim3((unsigned)P.turbo.n), dim3(256), 0, P.stream,
+                           launchArg(P, P.turbo, in, 0, 0, 0, 0));
     if (minimum && P.mn.n)
-        hipLaunchKernelGGL(dev::k_min_reduce, dim3((unsigned)P.mn.n), dim3(256), 0, P.stream, launchArg(P, P.mn, in, 0, 0, 0, 0));
+        hipLaunchKernelGGL(dev::k_min_reduce, dim3((unsigned)P.mn.n), dim3(256), 0, P.stream,
+                           launchArg(P, P.mn, in, 0, 0, iteration ? 1 : 0, 0));
 }
 void launchTimestep(const Plan& P, int in) { launchStart(P, P.dt, in, true); }
 void launchProfiles(const Plan& P, int in) { launchStart(P, P.prof, in, false); }
@@ -952,7 +1003,7 @@ void launchStage(Plan& P, int s, int in, int out, bool last, bool rmode, tbfm_ca
 // One explicit iteration of every member (solver.cpp:1337-1381): time step, radial
 // profiles, then the RK stages A -> B -> C -> ... -> A with the state back in A.
 void launchIteration(Plan& P, tbfm_case* prof) {
-    launchStart(P, P.start, 0, true);
+    launchStart(P, P.start, 0, true, true);
     for (int s = 0; s < P.nstages; ++s)
         launchStage(P, s, dev::stage_in(s), dev::stage_out(s, P.nstages), s + 1 == P.nstages, false, prof);
     HIPCHK(hipGetLastError());
@@ -1079,10 +1130,15 @@ void growHistory(tbfm_case* c, long last) {
     size_t* saved = g_bytes;
     g_bytes = &c->dev_bytes;
     double* h = dalloc<double>(5 * cap, c->owned);
+    double* t = c->c.turbo_on ? dalloc<double>(5 * cap, c->owned) : nullptr;
     g_bytes = saved;
     HIPCHK(hipStreamSynchronize(c->solo.stream));
     HIPCHK(hipMemcpy(h, c->c.conv.hist, 5 * c->hist_cap * 8, hipMemcpyDeviceToDevice));
     c->c.conv.hist = h;
+    if (t) {
+        HIPCHK(hipMemcpy(t, c->c.turbo_hist, 5 * c->hist_cap * 8, hipMemcpyDeviceToDevice));
+        c->c.turbo_hist = t;
+    }
     c->c.conv.hist_len = cap;
     c->hist_cap = cap;
     pushCase(c);
@@ -1829,6 +1885,18 @@ int tbfm_initial_state_from_ini(const char* ini, double* U, double* finv, double
         std::copy(u.begin(), u.end(), U);
         if (finv) std::copy(fi.begin(), fi.end(), finv);
         if (fvisc) std::copy(fv.begin(), fv.end(), fvisc);
+        return TBFM_OK;
+    });
+}
+
+int tbfm_gpu_get_turbo_history(tbfm_case* c, long first, long n, double* out) {
+    if (!c || !out || first < 1 || n < 0) return fail(c, TBFM_ERR_ARGUMENT, "bad argument");
+    return guarded(c, [&]() {
+        if (!c->c.turbo_on) throw std::invalid_argument("the case was created without TURBO_OUTPUT");
+        if (first + n - 1 >= c->iter_host) throw std::invalid_argument("iterations beyond those executed");
+        DevLock lk(g_dev_mu);
+        HIPCHK(hipStreamSynchronize(c->solo.stream));
+        if (n > 0) HIPCHK(hipMemcpy(out, c->c.turbo_hist + 5 * (first - 1), 5 * n * 8, hipMemcpyDeviceToHost));
         return TBFM_OK;
     });
 }

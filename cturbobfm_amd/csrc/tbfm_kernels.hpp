@@ -134,7 +134,16 @@ struct CaseDev {
     double* rper;                      // [2][5][NI*NJ] residuals of the periodic planes
     double* partial;                   // [stage blocks + periodic blocks][5] norm partial sums
     int nparts_stage, nparts_per;
-    double* partial_min;               // [nblk_dt] per-block minima of the global time step
+    double* partial_min;               // [nblk_dt] per-block minima of the time step (global dt, turbo time)
+    // turbomachinery history (TURBO_OUTPUT): the inlet and outlet boundary of the turbo
+    // direction, turbo_nf faces each (device node index, boundary surface vector)
+    int turbo_on, turbo_nf, turbo_nblk;  // nblk: partial-sum blocks per boundary (256 faces each)
+    double* turbo_part;                // [2][nblk][5] partial sums
+    const int* turbo_node;             // [2][nf]
+    const double* turbo_S;             // [2][nf][3]
+    double turbo_mscale;               // 2 pi / wedge angle, 360 / periodicity angle, or 1
+    double* turbo_hist;                // [hist_len][5]: time after the iteration, mass flow, PRtt, TRtt, ETAtt
+    double* turbo_time;                // physical time accumulated over the iterations
     int nblk_dt;
 };
 

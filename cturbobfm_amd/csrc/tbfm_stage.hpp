@@ -205,7 +205,7 @@ __device__ inline void timestep_block(const CaseDev& c, const double* U, int bx)
         }
         c.dt[n] = dtv;
     }
-    if (c.dt_mode == 1) {
+    if (c.dt_mode == 1 || c.turbo_on) {
         smin[threadIdx.x] = dtv;
         __syncthreads();
         for (int s = 128; s > 0; s >>= 1) {
@@ -213,6 +213,129 @@ __device__ inline void timestep_block(const CaseDev& c, const double* U, int bx)
             __syncthreads();
         }
         if (threadIdx.x == 0) c.partial_min[bx] = smin[0];
+    }
+}
+
+// ------------------------------------------------------- turbomachinery history
+// Solver::updateMassFlows + updateTurboPerformance (src/solver.cpp:1633-1781) of the
+// state at the start of the iteration, and the physical time after it
+// (solver.cpp:1383-1385: += the minimum time step).  Partial sums over 256 boundary
+// faces run as blocks of k_turbo_partial; k_min_reduce adds them up (block tree reductions:
+// the reference sums face by face, the two differ in the last bits only) and writes row
+// *c.iter of turbo_hist.
+constexpr int TURBO_SUMS = 5;   // per boundary: mass flow, rho u.S times p_t, T_t, h_t, s
+
+// Block bx: boundary bx / turbo_nblk, faces [256 (bx % turbo_nblk), +256).
+template <int FL>
+__device__ inline void turbo_partial_block(const CaseDev& c, const double* U, int bx) {
+    __shared__ double red[TURBO_SUMS][256];
+    const int b = bx / c.turbo_nblk, f = (bx % c.turbo_nblk) * 256 + threadIdx.x;
+    double acc[TURBO_SUMS] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (f < c.turbo_nf) {
+        const Gas<FL> gs(c);
+        const int n = c.turbo_node[b * c.turbo_nf + f];
+        const double* S = c.turbo_S + 3 * ((long)b * c.turbo_nf + f);
+        double Un[5];
+        load5(U, (int)c.N, n, Un);
+        const double rho = Un[0], ux = Un[1] / rho, uy = Un[2] / rho, uz = Un[3] / rho, et = Un[4] / rho;
+        acc[0] = S[0] * Un[1] + S[1] * Un[2] + S[2] * Un[3];
+        double pt, Tt, ht = 0.0, sv = 0.0;
+        if constexpr (FL == 1) {
+            const double qq = ux * ux + uy * uy + uz * uz;
+            const double e = et - 0.5 * qq;
+            pt = lut_total_pressure(*gs.L, rho, qq, et);
+            Tt = lut_total_temperature(*gs.L, rho, qq, et);
+            ht = et + lut_at(*gs.L, LUT_P, rho, e) / rho;
+            sv = lut_at(*gs.L, LUT_S, rho, e);
+        } else {
+            const double gm = gs.g;
+            const double um = sqrt(ux * ux + uy * uy + uz * uz);
+            const double e = et - 0.5 * (um * um);
+            const double pr = (gm - 1) * rho * e;
+            const double a = sqrt(gm * pr / rho);
+            const double M = um / a;
+            pt = pr * pow(1.0 + (gm - 1.0) / 2.0 * M * M, gm / (gm - 1));
+            const double T = pr / (rho * c.rgas);
+            Tt = T * (1.0 + (gm - 1.0) / 2.0 * M * M);
+        }
+        acc[1] = S[0] * (rho * ux * pt) + S[1] * (rho * uy * pt) + S[2] * (rho * uz * pt);
+        acc[2] = S[0] * (rho * ux * Tt) + S[1] * (rho * uy * Tt) + S[2] * (rho * uz * Tt);
+        acc[3] = S[0] * (rho * ux * ht) + S[1] * (rho * uy * ht) + S[2] * (rho * uz * ht);
+        acc[4] = S[0] * (rho * ux * sv) + S[1] * (rho * uy * sv) + S[2] * (rho * uz * sv);
+    }
+#pragma unroll
+    for (int q = 0; q < TURBO_SUMS; ++q) red[q][threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st)
+#pragma unroll
+            for (int q = 0; q < TURBO_SUMS; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x < TURBO_SUMS) c.turbo_part[(long)bx * TURBO_SUMS + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// Work: one entry per 256 turbo-boundary faces of a case with TURBO_OUTPUT (a launch of
+// its own: the shared arrays would cost the time-step blocks occupancy).
+__global__ void __launch_bounds__(256) k_turbo_partial(Launch L) {
+    const Work w = L.work[blockIdx.x];
+    const CaseDev& c = L.cases[w.cs];
+    if (*c.stop) return;
+    const double* U = c.U[L.in];
+    if (c.real) turbo_partial_block<1>(c, U, w.x);
+    else turbo_partial_block<0>(c, U, w.x);
+}
+
+// The boundary totals, the performance figures and the row; one block (256 threads,
+// turbo_nblk <= 256 partials per boundary).
+__device__ inline void turbo_final(const CaseDev& c) {
+    __shared__ double red[2 * TURBO_SUMS][256];
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int q = 0; q < TURBO_SUMS; ++q)
+            red[b * TURBO_SUMS + q][threadIdx.x] =
+                (threadIdx.x < c.turbo_nblk) ? c.turbo_part[((long)b * c.turbo_nblk + threadIdx.x) * TURBO_SUMS + q] : 0.0;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st)
+            for (int q = 0; q < 2 * TURBO_SUMS; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const double mf0 = red[0][0], mf1 = red[TURBO_SUMS][0];
+    double m = 0.5 * (fabs(mf0) + fabs(mf1));
+    m *= c.turbo_mscale;
+    double ptm[2], ttm[2], htm[2], sm[2];
+    for (int b = 0; b < 2; ++b) {
+        double mfl = red[b * TURBO_SUMS][0];
+        if (fabs(mfl) < 1e-12) mfl = (mfl >= 0.0 ? 1e-12 : -1e-12);
+        ptm[b] = red[b * TURBO_SUMS + 1][0] / mfl;
+        ttm[b] = red[b * TURBO_SUMS + 2][0] / mfl;
+        htm[b] = red[b * TURBO_SUMS + 3][0] / mfl;
+        sm[b] = red[b * TURBO_SUMS + 4][0] / mfl;
+    }
+    const double PR = (fabs(ptm[0]) > 1e-12) ? ptm[1] / ptm[0] : 1.0;
+    const double TR = (fabs(ttm[0]) > 1e-12) ? ttm[1] / ttm[0] : 1.0;
+    double eta;
+    if (c.real) {
+        // enthalpy-based total-to-total efficiency with h(p_t,out, s_in)
+        const double dh = htm[1] - htm[0], dhs = lut_h_p_s(*c.lut, ptm[1], sm[0]) - htm[0];
+        if (fabs(dh) < 1e-12) eta = 1.0;
+        else if (dh >= 0.0) eta = dhs / dh;
+        else eta = (fabs(dhs) < 1e-12) ? 1.0 : dh / dhs;
+    } else {
+        const double gm = c.gamma;
+        const double dT = TR - 1.0, dTs = pow(PR, (gm - 1.0) / gm) - 1.0;
+        if (fabs(dT) < 1e-12) eta = 1.0;
+        else if (dT >= 0.0) eta = dTs / dT;
+        else eta = (fabs(dTs) < 1e-12) ? 1.0 : dT / dTs;
+    }
+    const double t = *c.turbo_time + *c.dtmin;
+    *c.turbo_time = t;
+    const long it = *c.iter;
+    if (it >= 1 && it <= c.conv.hist_len) {
+        double* row = c.turbo_hist + (it - 1) * 5;
+        row[0] = t; row[1] = m; row[2] = PR; row[3] = TR; row[4] = eta;
     }
 }
 
@@ -231,6 +354,11 @@ __global__ void __launch_bounds__(256) k_min_reduce(Launch L) {
         __syncthreads();
     }
     if (threadIdx.x == 0) c.dtmin[0] = smin[0];
+    // the iteration's launch (L.last) also closes the turbomachinery row
+    if (c.turbo_on && L.last) {
+        __syncthreads();
+        turbo_final(c);
+    }
 }
 
 // ------------------------------------------------------- radial equilibrium profile

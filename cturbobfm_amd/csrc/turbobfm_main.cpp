@@ -8,9 +8,15 @@
 //       iterations, at N_ITERATIONS and at convergence (solver.cpp:1400-1413,
 //       output.cpp:367-447);
 //   residuals.csv  "Rho,RhoU,RhoV,RhoW,RhoE" and one row of log residuals per iteration,
-//       six significant digits (Solver::writeLogResidualsToCsvFile, solver.cpp:2520-2556).
+//       six significant digits (Solver::writeLogResidualsToCsvFile, solver.cpp:2520-2556);
+//   turbo.csv  with TURBO_OUTPUT = yes: time [us], mass flow, PRtt, TRtt, ETAtt of every
+//       iteration, six significant digits (Solver::writeTurboPerformanceToCsvFile,
+//       solver.cpp:2558-2595), the rows paired with the time list the way the reference
+//       pairs them between its periodic flushes.
 #include <sys/stat.h>
 
+#include <algorithm>
+#include <cctype>
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -57,6 +63,48 @@ bool writeVolume(tbfm_case* c, const char* ini, long it) {
     }
     std::printf("\nSolution written to file: %s\n\n", name);
     return true;
+}
+
+bool iniBool(const char* ini, const char* key) {
+    std::string v = iniValue(ini, key);
+    std::transform(v.begin(), v.end(), v.begin(), [](unsigned char ch) { return (char)std::tolower(ch); });
+    return v == "yes" || v == "true" || v == "1";
+}
+
+// turbo.csv from the per-iteration rows (time after the iteration, mass flow, PRtt, TRtt,
+// ETAtt).  The reference keeps a time list that starts with 0 and grows by one entry per
+// iteration, and a list of turbo rows; at every flush (SAVE_ITERATIONS_INTERVAL,
+// N_ITERATIONS, HISTORY_BUFFER_SIZE residual rows, convergence) row q is written with
+// time entry q and both lists are emptied (solver.cpp:1383-1421, 2558-2595).
+void writeTurboCsv(const std::vector<double>& rows, long done, long conv, long nmax, long freq, long bufsize) {
+    FILE* f = std::fopen("turbo.csv", "w");
+    if (!f) return;
+    std::fprintf(f, "Time[\xce\xbcs],Massflow[kg/s],PRtt,TRtt,ETAtt\n");
+    std::vector<double> times{0.0};
+    std::vector<const double*> pending;
+    double cur = 0.0;
+    long buffered = 0;
+    auto flush = [&]() {
+        for (size_t q = 0; q < pending.size(); ++q) {
+            const double t = (q < times.size()) ? times[q] : cur;
+            const double* r = pending[q];
+            std::fprintf(f, "%g,%g,%g,%g,%g\n", t * 1E6, r[1], r[2], r[3], r[4]);
+        }
+        pending.clear();
+        times.clear();
+        buffered = 0;
+    };
+    for (long it = 1; it <= done; ++it) {
+        const double* r = &rows[5 * (it - 1)];
+        pending.push_back(r);
+        cur = r[0];
+        times.push_back(cur);
+        ++buffered;
+        if (conv == it) { flush(); break; }
+        if (it % freq == 0 || it == nmax || buffered >= bufsize) flush();
+    }
+    std::fclose(f);
+    std::printf("\nTurbo performance appended to turbo.csv\n\n");
 }
 
 }  // namespace
@@ -114,6 +162,16 @@ int main(int argc, char** argv) {
                          hist[5 * r + 4]);
         std::fclose(f);
         std::printf("\nLog residuals appended to residuals.csv\n\n");
+    }
+    if (status == 0 && done > 0 && iniBool(ini, "TURBO_OUTPUT")) {
+        std::vector<double> rows(5 * done);
+        if (tbfm_gpu_get_turbo_history(c, 1, done, rows.data()) == TBFM_OK) {
+            const std::string hb = iniValue(ini, "HISTORY_BUFFER_SIZE");
+            const long bufsize = hb.empty() ? interval : (long)std::atof(hb.c_str());
+            writeTurboCsv(rows, done, conv, nmax, interval, bufsize);
+        } else {
+            std::fprintf(stderr, "turbo history: %s\n", tbfm_gpu_last_error(c));
+        }
     }
     double turbo[10];
     if (status == 0 && tbfm_gpu_get_turbo(c, turbo) == TBFM_OK)

@@ -120,6 +120,15 @@ int tbfm_gpu_get_side_outputs(tbfm_case* c, double* finv, double* fvisc, double*
  * out[7] total-pressure ratio, out[8] total-temperature ratio, out[9] efficiency. */
 int tbfm_gpu_get_turbo(tbfm_case* c, double* out);
 
+/* Turbomachinery performance of every iteration, recorded on the device when the case
+ * was created with TURBO_OUTPUT = yes (Solver::updateMassFlows + updateTurboPerformance
+ * at the start of each iteration, solver.cpp:1340-1341, 1633-1781; the physical time
+ * accumulated as solver.cpp:1383-1385).  Rows of 5 values for iterations first ..
+ * first+n-1 (counted from 1): time after the iteration [s], mass flow [kg/s], PRtt,
+ * TRtt, ETAtt.  The boundary sums are tree reductions (the reference sums in face order:
+ * the values agree to the last bits, and exactly at the six digits turbo.csv keeps). */
+int tbfm_gpu_get_turbo_history(tbfm_case* c, long first, long n, double* out);
+
 /* Static geometry as uploaded (reference layout), for parity checks of the mesh:
  * which 0..2 surfaces I/J/K, 3 volumes, 4 blockage gradient. */
 int tbfm_gpu_get_geometry(tbfm_case* c, int which, double* out);

@@ -58,6 +58,14 @@ def test_cli_regression_case(name, tmp_path):
     assert lines[0] == "Rho,RhoU,RhoV,RhoW,RhoE"
     conv = int(r.stdout.split("Convergence reached at iteration")[1].split()[0])
     assert len(lines) == conv + 1
+    if RC.CONFIGS[name].get("TURBO_OUTPUT") == "yes":
+        with open(os.path.join(folder, "turbo.csv"), encoding="utf-8") as f:
+            tl = f.read().splitlines()
+        assert tl[0] == "Time[\u03bcs],Massflow[kg/s],PRtt,TRtt,ETAtt"
+        assert len(tl) == conv + 1
+        last = [float(v) for v in tl[-1].split(",")]
+        print("%s turbo last row %s" % (name, last), flush=True)
+        assert last[1] > 0 and 0.5 < last[2] < 3.0
 
 
 def test_write_solution_matches_host_writer(tmp_path):
@@ -97,3 +105,38 @@ def test_restart_state_and_continuation(tmp_path):
     Ug, Uo = s.state(), oc.state()
     for v in range(5):
         assert np.max(np.abs(Ug[v] - Uo[v])) <= 1e-11 * max(np.max(np.abs(Uo[v])), 1e-300)
+
+
+def test_turbo_history_matches_host_turbo():
+    """Per-iteration device rows against the host evaluation of the same states (face-order
+    sums) and the time against the running sum of the minimum time step."""
+    case = cases.rotor3d(17, 6, 9, n_iterations=50, turbo_output=True)
+    s = Solver.from_case(case)
+    t = 0.0
+    for it in range(1, 9):
+        host = s.turbo()
+        s.advance(1)
+        dt = s.side_outputs()[3]
+        t = t + dt.min()
+        row = s.turbo_history(it, 1)[0]
+        print(it, row, host[6:], flush=True)
+        assert row[0] == t
+        assert np.max(np.abs(row[1:4] - host[6:9]) / np.abs(host[6:9])) < 1e-12
+        # the efficiency divides by TR - 1, which amplifies last-bit differences
+        assert abs(row[4] - host[9]) <= 1e-8 * max(1.0, abs(host[9]))
+
+
+def test_turbo_history_alone_equals_batch():
+    from cturbobfm_amd import Batch
+    case = cases.rotor3d(17, 6, 9, n_iterations=50, turbo_output=True)
+    a = Solver.from_case(case)
+    a.advance(20)
+    b = Batch()
+    m = Solver.from_case(case)
+    other = Solver.from_case(cases.bump2d(33, 17))
+    b.admit(m)
+    b.admit(other)
+    b.advance(20)
+    m.status()
+    assert np.array_equal(a.turbo_history(1, 20), m.turbo_history(1, 20))
+    b.close()
