@@ -29,7 +29,7 @@ from __future__ import annotations
 
 import math
 import os
-from typing import Dict, List, Tuple
+from typing import Optional, Dict, List, Tuple
 
 import numpy as np
 
@@ -50,7 +50,8 @@ def _tanh_cluster(n: int, beta: float) -> np.ndarray:
 
 def bump2d(ni: int = 129, nj: int = 65, n_iterations: int = 2000, scheme: str = "JST",
            cfl: float = 2.0, muscl: bool = False, limiter: str = "None",
-           time_integration: int = 0, time_step: str = "local", mach_out: float = 0.5) -> Dict:
+           time_integration: int = 0, time_step: str = "local", mach_out: float = 0.5,
+           outflow: Optional[str] = None) -> Dict:
     p0, t0 = 101325.0, 288.15
     p_out, t_init = _static_from_total(p0, t0, mach_out)
     xs = np.linspace(0.0, 3.0, ni)
@@ -68,7 +69,7 @@ def bump2d(ni: int = 129, nj: int = 65, n_iterations: int = 2000, scheme: str = 
         "KIND_SOLVER": "Euler", "TOPOLOGY": "2D",
         "FLUID_GAMMA": "1.4", "FLUID_R_CONSTANT": "287.05", "FLUID_MODEL": "Ideal",
         "INFLOW": "inlet, 101325, 288.15, 1, 0, 0",
-        "OUTFLOW": "outlet, %.17g" % p_out,
+        "OUTFLOW": outflow if outflow else "outlet, %.17g" % p_out,
         "LOWER_WALL": "inviscid_wall", "UPPER_WALL": "inviscid_wall",
         "INIT_MACH_NUMBER": "%.17g" % mach_out, "INIT_TEMPERATURE": "%.17g" % t_init,
         "INIT_PRESSURE": "%.17g" % p_out, "INIT_DIRECTION": "1, 0, 0",
@@ -91,13 +92,14 @@ def rotor3d(ni: int = 193, nj: int = 49, nk: int = 129, n_iterations: int = 2000
             muscl: bool = False, limiter: str = "None", outlet_pressure: float = 95000.0,
             length: float = 0.8, blockage: bool = True, time_integration: int = 0,
             time_step: str = "local", outlet: str = "radial_equilibrium", off_design: bool = False,
-            stall: bool = False, turbo_output: bool = False) -> Dict:
+            stall: bool = False, turbo_output: bool = False, outflow: Optional[str] = None) -> Dict:
     """Annular rotor; the last k plane is a bitwise copy of k = 0 (full annulus).
     bfm_model Hall, Lift-Drag or Hall-Thollet (K_n = K_f = K_d = 1, nu = 1.48e-5 as the
     reference's GER4 cases; off_design adds the deviation-pivot term); stall switches on
     STALLED_BFM_ACTIVE with a stall deviation of 0.12 rad.  turbo_output: TURBO_OUTPUT (the
     mass flow, PRtt, TRtt, ETAtt of every iteration; the figures of the current state are
-    available either way)."""
+    available either way).  outflow: the OUTFLOW patch string instead of
+    "<outlet>, <outlet_pressure>" (e.g. "throttle, k_t" or "mass_flow, target[, p_init]")."""
     p0, t0 = 101325.0, 288.15
     mach = math.sqrt(max(((p0 / outlet_pressure) ** ((GAMMA - 1.0) / GAMMA) - 1.0) * 2.0 / (GAMMA - 1.0), 1e-4))
     _, t_init = _static_from_total(p0, t0, mach)
@@ -149,7 +151,7 @@ def rotor3d(ni: int = 193, nj: int = 49, nk: int = 129, n_iterations: int = 2000
         "BLOCKAGE_ACTIVE": "yes" if blockage else "no",
         "LEADING_EDGE_INDEX": str(le_idx), "TRAILING_EDGE_INDEX": str(te_idx),
         "INFLOW": "inlet, 101325, 288.15, 1, 0, 0",
-        "OUTFLOW": "%s, %.17g" % (outlet, outlet_pressure),
+        "OUTFLOW": outflow if outflow else "%s, %.17g" % (outlet, outlet_pressure),
         "HUB": "inviscid_wall", "SHROUD": "inviscid_wall",
         "PER1": "periodic, 0, 0, 360", "PER2": "periodic, 1, 0, 360",
         "INIT_MACH_NUMBER": "%.17g" % mach, "INIT_TEMPERATURE": "%.17g" % t_init,

@@ -111,6 +111,7 @@ struct tbfm_case {
     HostFluid hf;                     // the fluid on the host (tables of FLUID_MODEL = real)
     LutSet lut{};                     // host copy of the fluid tables (points into hf.values)
     GridInput grid;                   // grid points and the blockage / rpm fields, for output files
+    double profInit[dev::MAX_PROF] = {0.0, 0.0, 0.0, 0.0};   // > 0: fixed initial outlet profile value
     std::vector<double> rk;
     Geometry geo;
     std::vector<Patch> patches;
@@ -307,13 +308,33 @@ void setup(tbfm_case* cs, const GridInput& grid) {
                 pd.type = dev::PT_OUTLET; pd.v0 = p.values[0];
                 break;
             case PatchType::RadialEquilibrium:
-                if (p.i0 != p.i1) throw std::runtime_error("Radial equilibrium only supported on j-k patches");
-                if (p.i0 == 0) throw std::runtime_error("Radial equilibrium only supported to the last i-position");
-                if (p.j1 <= p.j0) throw std::runtime_error("Invalid boundary indices for radial equilibrium patch: " + p.name);
-                if (nprof >= dev::MAX_PROF) throw std::runtime_error("too many radial-equilibrium patches");
+            case PatchType::Throttle:
+            case PatchType::MassFlow: {
+                // outlet pressure profiles (Solver::buildBoundaryDataStructures,
+                // solver.cpp:296-340): radial equilibrium unless a mass-flow outlet opts out
+                bool radial = true;
+                if (p.type == PatchType::MassFlow) {
+                    const bool can = (p.i0 == p.i1 && p.i0 != 0 && p.j1 > p.j0);
+                    if (p.values.size() >= 4) radial = (p.values[3] > 0.5) && can;
+                    else radial = can && (cs->topo == Topology::Axisymmetric || bfm);
+                    if (p.values.empty() || p.values[0] <= 0.0)
+                        throw std::runtime_error("Target mass flow for boundary patch " + p.name +
+                                                 " must be specified and positive.");
+                }
+                if (p.type == PatchType::Throttle && p.values.empty())
+                    throw std::runtime_error("throttle patch " + p.name + " needs a throttle coefficient");
+                if (radial) {
+                    if (p.i0 != p.i1) throw std::runtime_error("Radial equilibrium only supported on j-k patches");
+                    if (p.i0 == 0) throw std::runtime_error("Radial equilibrium only supported to the last i-position");
+                    if (p.j1 <= p.j0) throw std::runtime_error("Invalid boundary indices for radial equilibrium patch: " + p.name);
+                }
+                if (nprof >= dev::MAX_PROF) throw std::runtime_error("too many outlet pressure profiles");
                 pd.type = dev::PT_RADIAL; pd.profile = nprof;
+                c.prof_kind[nprof] = (p.type == PatchType::Throttle) ? 1 : (p.type == PatchType::MassFlow ? 2 : 0);
+                c.prof_radial[nprof] = radial ? 1 : 0;
                 profOfPatch[q] = nprof++;
                 break;
+            }
             case PatchType::Wall: pd.type = dev::PT_WALL; break;
             default: pd.type = dev::PT_FAKE; break;
         }
@@ -402,7 +423,31 @@ void setup(tbfm_case* cs, const GridInput& grid) {
         const Patch& p = cs->patches[q];
         c.prof_i[pr] = (int)(p.i0 > 0 ? p.i0 - 1 : 0);
         c.prof_j0[pr] = (int)p.j0;
-        c.prof_n[pr] = (int)(p.j1 - p.j0);
+        c.prof_n[pr] = (int)((p.j1 > p.j0) ? p.j1 - p.j0 : 1);
+        if (c.prof_kind[pr] == 1) {
+            // hub = p0 of the first inlet + k_t m^2 (solver.cpp:2722-2732)
+            double pt_in = 0.0;
+            for (auto& b : cs->patches)
+                if (b.type == PatchType::Inlet) { pt_in = b.values[0]; break; }
+            c.prof_par[pr][0] = p.values[0];
+            c.prof_par[pr][1] = pt_in;
+            c.prof_need_mf = 1;
+        } else if (c.prof_kind[pr] == 2) {
+            // solver.cpp:2734-2770
+            c.prof_par[pr][0] = p.values[0];
+            c.prof_par[pr][1] = (p.values.size() >= 3 && p.values[2] > 0.0) ? p.values[2] : cfg.getFloat("MASS_FLOW_RELAXATION", 0.05);
+            c.prof_par[pr][2] = cfg.getFloat("MASS_FLOW_MAX_PRESSURE_CHANGE", 0.02);
+            const double ip = cfg.has("INIT_PRESSURE") ? cfg.getInitPressure() : 0.0;
+            c.prof_par[pr][3] = ip > 0.0 ? ip : 101325.0;
+            if (p.i0 != p.i1) throw std::runtime_error("mass-flow outlet " + p.name + " must lie on an i-boundary");
+            c.prof_mfb[pr] = (p.i0 == 0) ? 0 : 1;   // turbo boundaries I_START / I_END, remapped below
+            c.prof_need_mf = 1;
+            // initial profile (solver.cpp:334-337, 818-821): the patch's pressure when given;
+            // an empty j range keeps the set-up value
+            if (p.values.size() >= 2 && p.values[1] > 0.0) cs->profInit[pr] = p.values[1];
+            else if (p.j1 <= p.j0) cs->profInit[pr] = c.prof_par[pr][3];
+        }
+        if (!c.prof_radial[pr]) continue;
         for (long j = p.j0; j < p.j1; ++j) {
             prof_r[pr * NJ + (j - p.j0)] = g.radius[((p.i1 - 1) * NJ + j) * NK + 0];
             double th = g.theta[(c.prof_i[pr] * NJ + j) * NK + 0];
@@ -620,7 +665,8 @@ void setup(tbfm_case* cs, const GridInput& grid) {
     cs->hist_cap = std::max<long>(cfg.has("N_ITERATIONS") ? cfg.getMaxIterations() : 0, 1);
     c.conv.hist = dalloc<double>(5 * cs->hist_cap, own);
     // turbomachinery history of every iteration (TURBO_OUTPUT, solver.cpp:1340-1341)
-    c.turbo_on = cfg.getBool("TURBO_OUTPUT", false) ? 1 : 0;
+    // also needed by throttle and mass-flow outlets (solver.cpp:1312-1317)
+    c.turbo_on = (cfg.getBool("TURBO_OUTPUT", false) || c.prof_need_mf) ? 1 : 0;
     if (c.turbo_on) {
         const Geometry& gg = cs->geo;
         const int td = cfg.getTurboDirection();
@@ -628,6 +674,11 @@ void setup(tbfm_case* cs, const GridInput& grid) {
         if (td == 1) { bi[0] = 1; bi[1] = 0; } else if (td == 2) { bi[0] = 2; bi[1] = 3; } else if (td == 3) { bi[0] = 3; bi[1] = 2; }
         const bool jdir = (bi[0] >= 2);
         if (jdir && cs->topo == Topology::OneD) throw std::runtime_error("TURBO_DIRECTION j needs a 2-D grid");
+        if (jdir && c.prof_need_mf)
+            throw std::runtime_error("throttle and mass-flow outlets need TURBO_DIRECTION i or -i on the GPU path");
+        // mass-flow outlets: which of the two turbo boundaries is theirs
+        for (int q = 0; q < c.nprof; ++q)
+            if (c.prof_kind[q] == 2) c.prof_mfb[q] = (bi[0] == c.prof_mfb[q]) ? 0 : 1;
         const long n1 = jdir ? NI : NJ, n2 = NK;
         c.turbo_nf = (int)(n1 * n2);
         c.turbo_nblk = (int)((n1 * n2 + 255) / 256);
@@ -954,16 +1005,23 @@ void launchStageGroups(const Plan& P, int in, int out, int stage, int last, int 
 // minimum for the cases that use it.
 // iteration: the launch of an explicit iteration (also closes the turbomachinery rows)
 void launchStart(const Plan& P, const Table& t, int in, bool minimum, bool iteration = false) {
-    if (t.n) hipLaunchKernelGGL(dev::k_iterstart, dim3((unsigned)t.n), dim3(256), 0, P.stream, launchArg(P, t, in, 0, 0, 0, 0));
+    // boundary partial sums first: the outlet profiles of throttle and mass-flow outlets
+    // read them
     if (iteration && P.turbo.n)
         hipLaunchKernelGGL(dev::k_turbo_partial, dim3((unsigned)P.turbo.n), dim3(256), 0, P.stream,
                            launchArg(P, P.turbo, in, 0, 0, 0, 0));
+    if (t.n) hipLaunchKernelGGL(dev::k_iterstart, dim3((unsigned)t.n), dim3(256), 0, P.stream, launchArg(P, t, in, 0, 0, 0, 0));
     if (minimum && P.mn.n)
         hipLaunchKernelGGL(dev::k_min_reduce, dim3((unsigned)P.mn.n), dim3(256), 0, P.stream,
                            launchArg(P, P.mn, in, 0, 0, iteration ? 1 : 0, 0));
 }
 void launchTimestep(const Plan& P, int in) { launchStart(P, P.dt, in, true); }
-void launchProfiles(const Plan& P, int in) { launchStart(P, P.prof, in, false); }
+void launchProfiles(const Plan& P, int in) {
+    if (P.turbo.n)   // the mass flows of throttle and mass-flow outlets
+        hipLaunchKernelGGL(dev::k_turbo_partial, dim3((unsigned)P.turbo.n), dim3(256), 0, P.stream,
+                           launchArg(P, P.turbo, in, 0, 0, 0, 0));
+    launchStart(P, P.prof, in, false);
+}
 
 // k_prestage over a table of boundary and / or source blocks.
 void launchPre(const Plan& P, const Table& t, int in, int out, int stage, int last, int rmode) {
@@ -1250,6 +1308,10 @@ int tbfm_gpu_set_state(tbfm_case* c, const double* U) {
         const double g = c->c.gamma;
         for (int q = 0; q < c->c.nprof; ++q)
             for (int m = 0; m < c->c.prof_n[q]; ++m) {
+                if (c->profInit[q] > 0.0) {
+                    prof[q * c->nj + m] = c->profInit[q];
+                    continue;
+                }
                 long n = ((long)c->c.prof_i[q] * c->nj + c->c.prof_j0[q] + m) * c->nk + 0;
                 double rho = U[n], W1 = U[c->N + n] / rho, W2 = U[2 * c->N + n] / rho, W3 = U[3 * c->N + n] / rho,
                        W4 = U[4 * c->N + n] / rho;
@@ -1261,6 +1323,10 @@ int tbfm_gpu_set_state(tbfm_case* c, const double* U) {
                     prof[q * c->nj + m] = (g - 1) * rho * e;
                 }
             }
+        // a mass-flow outlet clamps the face's j into its profile (boundary_outlet_mass_flow.cpp:13-16)
+        for (int q = 0; q < c->c.nprof; ++q)
+            if (c->c.prof_kind[q] == 2)
+                for (long m = c->c.prof_n[q]; m < c->nj; ++m) prof[q * c->nj + m] = prof[q * c->nj + c->c.prof_n[q] - 1];
         HIPCHK(hipMemcpyAsync(c->c.prof_p, prof.data(), prof.size() * 8, hipMemcpyHostToDevice, st));
         HIPCHK(hipStreamSynchronize(st));
         return TBFM_OK;

@@ -274,6 +274,8 @@ PatchType patchTypeFromString(const std::string& s) {
     if (k == "inviscid_wall" || k == "no_slip_wall") return PatchType::Wall;
     if (k == "periodic") return PatchType::Periodic;
     if (k == "wedge") return PatchType::Wedge;
+    if (k == "throttle") return PatchType::Throttle;
+    if (k == "mass_flow") return PatchType::MassFlow;
     throw std::runtime_error("BoundaryType " + s + " is outside the GPU residual path");
 }
 

@@ -97,6 +97,12 @@ struct CaseDev {
     // radial equilibrium profiles
     int nprof;
     int prof_i[MAX_PROF], prof_j0[MAX_PROF], prof_n[MAX_PROF];
+    // hub pressure of each outlet profile: 0 radial equilibrium (kept), 1 throttle
+    // (par: k_t, inlet p0), 2 mass flow (par: target, relaxation, maximum change, initial
+    // pressure; mfb: which turbo boundary carries the patch); radial: integrated radially,
+    // else uniform.  need_mf: some profile reads the boundary mass flows.
+    int prof_kind[MAX_PROF], prof_radial[MAX_PROF], prof_mfb[MAX_PROF], prof_need_mf;
+    double prof_par[MAX_PROF][4];
     double* prof_p;                    // [MAX_PROF][NJ]
     const double* prof_r;              // [MAX_PROF][NJ]
     const double* prof_cs;             // [MAX_PROF][NJ] cos(theta) at (i, j, 0)

@@ -362,22 +362,68 @@ __global__ void __launch_bounds__(256) k_min_reduce(Launch L) {
 }
 
 // ------------------------------------------------------- radial equilibrium profile
-// Solver::updateRadialProfiles + integrateRadialEquilibrium, radial_equilibrium branch
-// (src/solver.cpp:2688-2787, src/math_utils.cpp:141-170).  The terms rho*ut*ut/r of the
-// outlet column are evaluated by all threads in parallel; the trapezoid recurrence
-// P[m] = P[m-1] + 0.5*(d[m-1] + d[m])*dr is then run by one thread, in order, exactly
-// as the reference accumulates it.  One block per case with profiles.
+// Solver::updateRadialProfiles + integrateRadialEquilibrium (src/solver.cpp:2688-2787,
+// src/math_utils.cpp:141-170).  The hub pressure per profile: radial equilibrium keeps
+// it; a throttle sets p0 of the first inlet + k_t m^2 with the turbomachinery mass flow;
+// a mass-flow outlet relaxes it towards the target (error clamped to the maximum
+// change, at least 100 Pa).  The boundary mass flows come from the partial sums of
+// k_turbo_partial (launched before), added as k_min_reduce adds them.  The terms
+// rho*ut*ut/r of the outlet column are evaluated by all threads in parallel; the
+// trapezoid recurrence P[m] = P[m-1] + 0.5*(d[m-1] + d[m])*dr is then run by one
+// thread, in order, exactly as the reference accumulates it.  One block per case with
+// profiles.
 __device__ inline void radial_profile_block(const CaseDev& c, const double* U) {
     __shared__ double dval[257];
+    __shared__ double mfs[2][256];
     const int tid = threadIdx.x;
+    double mb[2] = {0.0, 0.0}, mturbo = 0.0;
+    if (c.prof_need_mf) {
+        for (int b = 0; b < 2; ++b)
+            mfs[b][tid] = (tid < c.turbo_nblk) ? c.turbo_part[((long)b * c.turbo_nblk + tid) * TURBO_SUMS] : 0.0;
+        __syncthreads();
+        for (int st = 128; st > 0; st >>= 1) {
+            if (tid < st) {
+                mfs[0][tid] += mfs[0][tid + st];
+                mfs[1][tid] += mfs[1][tid + st];
+            }
+            __syncthreads();
+        }
+        mb[0] = mfs[0][0];
+        mb[1] = mfs[1][0];
+        mturbo = 0.5 * (fabs(mb[0]) + fabs(mb[1]));
+        mturbo *= c.turbo_mscale;
+    }
     for (int q = 0; q < c.nprof; ++q) {
         const int n = c.prof_n[q];
-        if (n <= 1) continue;   // single point: profile keeps the hub value
-        const int i = c.prof_i[q];
         double* P = c.prof_p + (long)q * c.nj;
+        double hub = P[0];
+        if (c.prof_kind[q] == 1) {
+            hub = c.prof_par[q][1] + c.prof_par[q][0] * mturbo * mturbo;
+        } else if (c.prof_kind[q] == 2) {
+            double cur = fabs(mb[c.prof_mfb[q]]);
+            cur *= c.turbo_mscale;
+            const double target = c.prof_par[q][0], relax = c.prof_par[q][1], maxc = c.prof_par[q][2];
+            const double err = (cur - target) / target;
+            double d = relax * err;
+            d = (d < -maxc) ? -maxc : ((maxc < d) ? maxc : d);
+            double cp = P[0];
+            if (cp <= 0.0) cp = c.prof_par[q][3];
+            hub = cp * (1.0 + d);
+            hub = (hub < 100.0) ? 100.0 : hub;
+        }
+        if (!c.prof_radial[q] || n <= 1) {
+            // uniform profile at the hub value (a mass-flow outlet clamps the face's j
+            // into the profile: every entry of the row holds the value)
+            if (tid == 0)
+                for (int m = 0; m < (c.prof_kind[q] == 2 ? c.nj : n); ++m) P[m] = hub;
+            __syncthreads();
+            continue;
+        }
+        const int i = c.prof_i[q];
         const double* r = c.prof_r + (long)q * c.nj;
         const double* cs = c.prof_cs + (long)q * c.nj;
         const double* sn = c.prof_sn + (long)q * c.nj;
+        if (tid == 0) P[0] = hub;
         for (int base = 0; base < n; base += 256) {
             // dval[0] carries d[base-1] of the previous chunk
             const int m = base + tid;
@@ -404,6 +450,8 @@ __device__ inline void radial_profile_block(const CaseDev& c, const double* U) {
             }
             __syncthreads();
         }
+        if (c.prof_kind[q] == 2 && tid == 0)
+            for (int m = n; m < c.nj; ++m) P[m] = P[n - 1];
     }
 }
 

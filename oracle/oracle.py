@@ -21,7 +21,7 @@ TOPO = {"2D": 0, "2d": 0, "3D": 1, "3d": 1, "axisymmetric": 2, "Axisymmetric": 2
         "AXISYMMETRIC": 2, "1D": 3, "1d": 3}
 LIMITER = {"None": 0, "van albada": 1, "van leer": 2, "min mod": 3}
 BC = {"inlet": 0, "outlet": 1, "radial_equilibrium": 2, "inviscid_wall": 3, "no_slip_wall": 3,
-      "periodic": 4, "wedge": 5}
+      "periodic": 4, "wedge": 5, "throttle": 6, "mass_flow": 7}
 BFM = {"None": 0, "Hall": 1, "Lift-Drag": 2, "Blockage": 3, "Hall-Thollet": 4}
 
 
@@ -37,7 +37,8 @@ class Params(C.Structure):
                 ("omega_scaling", C.c_double), ("inlet_cylindrical", C.c_int), ("turbo_dir", C.c_int),
                 ("residual_drop", C.c_long), ("steady", C.c_int), ("fluid_real", C.c_int),
                 ("kn", C.c_double), ("kf", C.c_double), ("kd", C.c_double), ("thollet_off_design", C.c_int),
-                ("stall_active", C.c_int), ("nu", C.c_double)]
+                ("stall_active", C.c_int), ("nu", C.c_double),
+                ("mass_flow_relaxation", C.c_double), ("mass_flow_max_change", C.c_double)]
 
 
 class Table(C.Structure):
@@ -141,6 +142,8 @@ def lib():
         L.tbo_num_nodes.argtypes = [C.c_void_p]
         L.tbo_init_state.argtypes = [C.c_void_p]
         L.tbo_set_state.argtypes = [C.c_void_p, dp, C.c_int]
+        L.tbo_get_profile.restype = C.c_long
+        L.tbo_get_profile.argtypes = [C.c_void_p, C.c_int, dp]
         L.tbo_get_state.argtypes = [C.c_void_p, dp]
         L.tbo_run.restype = C.c_long
         L.tbo_run.argtypes = [C.c_void_p, C.c_long, dp, dp, C.c_int]
@@ -232,6 +235,8 @@ def params_from_config(case: Dict) -> Params:
     td = cfg.get("TURBO_DIRECTION", "i")
     p.turbo_dir = {"i": 0, "+i": 0, "-i": 1, "j": 2, "+j": 2, "-j": 3}.get(td, 0)
     p.residual_drop = int(float(cfg.get("RESIDUALS_DROP_CONVERGENCE", "16")))
+    p.mass_flow_relaxation = float(cfg.get("MASS_FLOW_RELAXATION", "0.05"))
+    p.mass_flow_max_change = float(cfg.get("MASS_FLOW_MAX_PRESSURE_CHANGE", "0.02"))
     p.steady = int(_cfg_bool(cfg, "SIMULATION_IS_STEADY", True))
     if p.bfm_active and p.bfm_model == BFM["Hall-Thollet"]:
         # source_bfm_thollet.hpp:47-50 (config.hpp:295-299), config.cpp:441-450
@@ -356,6 +361,14 @@ class OracleCase:
         dt = np.zeros_like(dev)
         lib().tbo_side_outputs(self.h, _dp(finv), _dp(fvisc), _dp(dev), _dp(dt))
         return finv, fvisc, dev, dt
+
+    def profile(self, q: int = 0) -> np.ndarray:
+        """Outlet pressure profile q (radial equilibrium, throttle or mass flow)."""
+        n = lib().tbo_get_profile(self.h, q, None)
+        out = np.zeros(max(n, 0))
+        if n > 0:
+            lib().tbo_get_profile(self.h, q, _dp(out))
+        return out
 
     def turbo(self, U: Optional[np.ndarray] = None) -> np.ndarray:
         U = self.state() if U is None else np.ascontiguousarray(U, dtype=np.float64)

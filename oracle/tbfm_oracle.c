@@ -70,9 +70,11 @@ struct tbo_case {
     int periodic;
     double per_angle_deg, per_angle_rad;
     double wedge_angle;
-    /* radial equilibrium profiles (one per radial_equilibrium patch) */
+    /* outlet pressure profiles (one per radial_equilibrium, throttle or mass_flow patch);
+     * prof_radial: integrated radially (else uniform at the hub value) */
     int nprof;
     int prof_patch[8];
+    int prof_radial[8];
     double *prof_p[8], *prof_r[8];
     long prof_n[8];
     int patch_prof[64];
@@ -647,10 +649,19 @@ static st5 bc_flux(const tbo_case* c, int pidx, st5 Uint, v3 S, long fi, long fj
     switch (pa->type) {
         case TBO_BC_INLET: return inlet_flux(c, pa->values, Uint, S, fi, fj, fk);
         case TBO_BC_OUTLET: return outlet_flux(c, Uint, S, pa->values[0], it);
-        case TBO_BC_RADIAL_EQ: {
-            /* boundary_outlet_radial_equilibrium.cpp:5-16: profile[j of the face] */
+        case TBO_BC_RADIAL_EQ:
+        case TBO_BC_THROTTLE: {
+            /* boundary_outlet_radial_equilibrium.cpp:5-16, boundary_outlet_throttle.cpp:5-16:
+             * profile[j of the face] */
             const double* prof = c->prof_p[c->patch_prof[pidx]];
             return outlet_flux(c, Uint, S, prof[fj], it);
+        }
+        case TBO_BC_MASS_FLOW: {
+            /* boundary_outlet_mass_flow.cpp:4-19: j of the face, clamped to the profile */
+            int q = c->patch_prof[pidx];
+            long idx = fj;
+            if (idx >= c->prof_n[q]) idx = c->prof_n[q] - 1;
+            return outlet_flux(c, Uint, S, c->prof_p[q][idx], it);
         }
         case TBO_BC_WALL: return wall_flux(c, Uint, S);
         default: return z;
@@ -1309,8 +1320,16 @@ static void periodic_solution(tbo_case* c, double* U) {
         }
 }
 
-/* solver.cpp:2688-2787 (radial_equilibrium branch) */
+/* Solver::updateRadialProfiles (solver.cpp:2688-2787): the hub pressure of each outlet
+ * profile (radial equilibrium: kept; throttle: p0 of the first inlet + k_t m^2 with the
+ * turbomachinery mass flow; mass flow: relaxed towards the target mass flow of the
+ * patch's boundary), then the radial-equilibrium integration or a uniform profile. */
+void tbo_turbo(tbo_case* c, const double* U, double* out);
 static void update_radial_profiles(tbo_case* c, const double* U) {
+    double t[10];
+    int need_mf = 0;
+    for (int q = 0; q < c->nprof; ++q) need_mf |= (c->patches[c->prof_patch[q]].type != TBO_BC_RADIAL_EQ);
+    if (need_mf) tbo_turbo(c, U, t);   /* updateMassFlows + updateTurboPerformance */
     for (int q = 0; q < c->nprof; ++q) {
         const tbo_patch* pa = &c->patches[c->prof_patch[q]];
         long n = c->prof_n[q];
@@ -1328,7 +1347,29 @@ static void update_radial_profiles(tbo_case* c, const double* U) {
             k++;
         }
         double hub = c->prof_p[q][0];
-        if (n > 1) tbo_radial_equilibrium(rho, ut, c->prof_r[q], (int)n, hub, c->prof_p[q]);
+        if (pa->type == TBO_BC_THROTTLE) {
+            double kt = pa->values[0], mflow = t[6], pt_in = 0.0;
+            for (int b = 0; b < c->npatches; ++b)
+                if (c->patches[b].type == TBO_BC_INLET) { pt_in = c->patches[b].values[0]; break; }
+            hub = pt_in + kt * mflow * mflow;
+        } else if (pa->type == TBO_BC_MASS_FLOW) {
+            double target = pa->values[0];
+            /* BoundaryIndex of the patch: I_START (i = 0) or I_END (i = NI) */
+            double cur = fabs(t[(pa->i0 == 0) ? 0 : 1]);
+            if (c->p.topology == TBO_TOPO_AXI) cur *= 2.0 * M_PI / c->wedge_angle;
+            else if (c->per_angle_deg != 0.0) cur *= 360.0 / c->per_angle_deg;
+            double relax = c->p.mass_flow_relaxation;
+            if (pa->nvalues >= 3 && pa->values[2] > 0.0) relax = pa->values[2];
+            double maxc = c->p.mass_flow_max_change;
+            double err = (cur - target) / target;
+            double d = relax * err;
+            d = (d < -maxc) ? -maxc : ((maxc < d) ? maxc : d);   /* std::clamp */
+            double cp = c->prof_p[q][0];
+            if (cp <= 0.0) cp = c->p.init_pressure > 0.0 ? c->p.init_pressure : 101325.0;
+            hub = cp * (1.0 + d);
+            hub = (hub < 100.0) ? 100.0 : hub;   /* std::max(hub, 100) */
+        }
+        if (c->prof_radial[q] && n > 1) tbo_radial_equilibrium(rho, ut, c->prof_r[q], (int)n, hub, c->prof_p[q]);
         else for (long m = 0; m < n; ++m) c->prof_p[q][m] = hub;
         free(rho);
         free(ut);
@@ -1561,19 +1602,28 @@ tbo_case* tbo_create(const tbo_params* p, const double* x, const double* y, cons
         tbo_destroy(c);
         return NULL;
     }
-    /* radial equilibrium profiles: solver.cpp:296-331 */
+    /* outlet pressure profiles: Solver::buildBoundaryDataStructures (solver.cpp:296-340) */
     for (int q = 0; q < npatches; ++q) {
         c->patch_prof[q] = -1;
-        if (patches[q].type != TBO_BC_RADIAL_EQ) continue;
+        const tbo_patch* pa = &patches[q];
+        if (pa->type != TBO_BC_RADIAL_EQ && pa->type != TBO_BC_THROTTLE && pa->type != TBO_BC_MASS_FLOW) continue;
+        int radial = 1;
+        if (pa->type == TBO_BC_MASS_FLOW) {
+            int can = (pa->i0 == pa->i1 && pa->i0 != 0 && pa->j1 > pa->j0);
+            if (pa->nvalues >= 4) radial = (pa->values[3] > 0.5) && can;
+            else radial = can && (p->topology == TBO_TOPO_AXI || p->bfm_active);
+        }
         int np = c->nprof++;
         c->prof_patch[np] = q;
         c->patch_prof[q] = np;
-        long n = patches[q].j1 - patches[q].j0;
+        c->prof_radial[np] = radial;
+        long n = (pa->j1 > pa->j0) ? pa->j1 - pa->j0 : 1;
         c->prof_n[np] = n;
         c->prof_p[np] = dalloc(n);
         c->prof_r[np] = dalloc(n);
-        for (long j = patches[q].j0; j < patches[q].j1; ++j)
-            c->prof_r[np][j - patches[q].j0] = c->radius[IDX(c, patches[q].i1 - 1, j, 0)];
+        if (radial)
+            for (long j = pa->j0; j < pa->j1; ++j)
+                c->prof_r[np][j - pa->j0] = c->radius[IDX(c, pa->i1 - 1, j, 0)];
     }
     c->U0 = dalloc(5 * N); c->U = dalloc(5 * N); c->R = dalloc(5 * N); c->dt = dalloc(N);
     c->finv = dalloc(3 * N); c->fvisc = dalloc(3 * N); c->dev = dalloc(N);
@@ -1596,9 +1646,14 @@ void tbo_destroy(tbo_case* c) {
 long tbo_num_nodes(const tbo_case* c) { return c->N; }
 
 static void init_profiles(tbo_case* c) {
-    /* solver.cpp:818-834: profile = static pressure of the current solution at i_min-1, k = 0 */
+    /* solver.cpp:818-834: profile = static pressure of the current solution at i_min-1,
+     * k = 0; a mass-flow patch with an initial pressure (second value > 0) takes that */
     for (int q = 0; q < c->nprof; ++q) {
         const tbo_patch* pa = &c->patches[c->prof_patch[q]];
+        if (pa->type == TBO_BC_MASS_FLOW && pa->nvalues >= 2 && pa->values[1] > 0.0) {
+            for (long m = 0; m < c->prof_n[q]; ++m) c->prof_p[q][m] = pa->values[1];
+            continue;
+        }
         long i = pa->i0 > 0 ? pa->i0 - 1 : 0;
         long k = 0;
         for (long j = pa->j0; j < pa->j1 && k < c->prof_n[q]; j++) {
@@ -1648,6 +1703,13 @@ void tbo_set_state(tbo_case* c, const double* U, int reset_profiles) {
     if (reset_profiles) init_profiles(c);
 }
 void tbo_get_state(const tbo_case* c, double* U) { memcpy(U, c->U0, sizeof(double) * 5 * c->N); }
+
+/* Outlet pressure profile q (prof_n values); returns the count, -1 if q is out of range. */
+long tbo_get_profile(const tbo_case* c, int q, double* out) {
+    if (q < 0 || q >= c->nprof) return -1;
+    if (out) memcpy(out, c->prof_p[q], sizeof(double) * c->prof_n[q]);
+    return c->prof_n[q];
+}
 
 void tbo_side_outputs(const tbo_case* c, double* finv, double* fvisc, double* dev, double* dt) {
     if (finv) memcpy(finv, c->finv, 24 * c->N);

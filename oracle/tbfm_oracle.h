@@ -22,7 +22,7 @@ enum { TBO_LIM_NONE = 0, TBO_LIM_VAN_ALBADA = 1, TBO_LIM_VAN_LEER = 2, TBO_LIM_M
 enum { TBO_DT_LOCAL = 0, TBO_DT_GLOBAL = 1, TBO_DT_FIXED = 2 };
 enum { TBO_BFM_NONE = 0, TBO_BFM_HALL = 1, TBO_BFM_LIFT_DRAG = 2, TBO_BFM_BLOCKAGE = 3, TBO_BFM_THOLLET = 4 };
 enum { TBO_BC_INLET = 0, TBO_BC_OUTLET = 1, TBO_BC_RADIAL_EQ = 2, TBO_BC_WALL = 3,
-       TBO_BC_PERIODIC = 4, TBO_BC_WEDGE = 5 };
+       TBO_BC_PERIODIC = 4, TBO_BC_WEDGE = 5, TBO_BC_THROTTLE = 6, TBO_BC_MASS_FLOW = 7 };
 
 typedef struct {
     long ni, nj, nk;
@@ -47,6 +47,9 @@ typedef struct {
     int thollet_off_design;          /* HALL_THOLLET_OFF_DESIGN_ACTIVE */
     int stall_active;                /* STALLED_BFM_ACTIVE (all models' force direction) */
     double nu;                       /* FLUID_KINEMATIC_VISCOSITY */
+    /* mass-flow outlet (config.hpp:291-293) */
+    double mass_flow_relaxation;     /* MASS_FLOW_RELAXATION (0.05) */
+    double mass_flow_max_change;     /* MASS_FLOW_MAX_PRESSURE_CHANGE (0.02) */
 } tbo_params;
 
 /* A fluid look-up table (include/fluid_real.hpp:11-60 Table2D): nx*ny values, row-major
@@ -103,6 +106,7 @@ long tbo_num_nodes(const tbo_case* c);
 void tbo_init_state(tbo_case* c);          /* initializeSolutionFromScratch + radial profiles */
 void tbo_set_state(tbo_case* c, const double* U, int reset_profiles);
 void tbo_get_state(const tbo_case* c, double* U);
+long tbo_get_profile(const tbo_case* c, int q, double* out);
 
 /* Runs n explicit iterations of Solver::solve (solver.cpp:1337-1424).
  * logres: n*5 log10 residual norms; turbo: n*4 {massflow, PRtt, TRtt, eta} at the

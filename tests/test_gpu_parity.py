@@ -45,6 +45,10 @@ CASES = {
     "rotor3d_roe_muscl_thollet_od_stall": lambda: cases.rotor3d(25, 9, 17, n_iterations=200, scheme="ROE", muscl=True,
                                                                   limiter="van albada", bfm_model="Hall-Thollet",
                                                                   off_design=True, stall=True, cfl=1.2),
+    # outlets whose hub pressure follows the mass flow every iteration
+    "rotor3d_jst_throttle": lambda: cases.rotor3d(25, 9, 17, n_iterations=200, outflow="throttle, 0.01"),
+    "rotor3d_jst_massflow": lambda: cases.rotor3d(25, 9, 17, n_iterations=200, outflow="mass_flow, 15"),
+    "bump2d_jst_massflow_uniform": lambda: cases.bump2d(33, 17, n_iterations=200, outflow="mass_flow, 60, 90000"),
 }
 
 

@@ -1,0 +1,82 @@
+"""Throttle and mass-flow outlets in the CPU restatement (Solver::updateRadialProfiles,
+src/solver.cpp:2688-2787; boundary_outlet_throttle.cpp, boundary_outlet_mass_flow.cpp).
+
+The solve loop is in solver.cpp, which cannot be built here, so the hub-pressure rules
+are pinned by what they fix: a throttle outlet's hub is the first inlet's p0 plus
+k_t m^2 with the turbomachinery mass flow; a mass-flow outlet keeps its hub when the
+mass flow meets the target, moves it by relaxation x relative error, clamps the move at
+the maximum change in both directions, takes the patch's own relaxation when given, and
+never goes below 100 Pa; a mass-flow outlet without radial equilibrium is uniform; the
+patch's initial pressure (second value) starts its profile.
+"""
+import numpy as np
+import pytest
+
+from cturbobfm_amd import cases
+
+import oracle as O
+
+
+def update(case, U=None):
+    """Profile 0 before and after one profile update of the state (the residual call
+    updates the profiles first)."""
+    oc = O.OracleCase(case)
+    U = oc.state() if U is None else U
+    oc.set_state(U)
+    before = oc.profile(0)
+    oc.residual(U, it=1, update_profiles=True)
+    return oc, before, oc.profile(0)
+
+
+def perturbed(case, seed=5):
+    oc = O.OracleCase(case)
+    return cases.perturb_state(case, oc.state(), 0.03, seed)
+
+
+def test_throttle_hub():
+    kt = 7.5
+    case = cases.rotor3d(17, 6, 9, outflow="throttle, %r" % kt)
+    U = perturbed(case)
+    oc, before, after = update(case, U)
+    m = oc.turbo(U)[6]
+    assert after[0] == 101325.0 + kt * m * m
+    assert not np.array_equal(before, after)
+
+
+def cur_mass_flow(oc, U):
+    return abs(oc.turbo(U)[1])        # I_END boundary, 360-degree annulus: no scaling
+
+
+@pytest.mark.parametrize("ratio, relax, expect", [
+    (1.0, None, 1.0),            # on target: hub unchanged
+    (0.5, None, 1.02),           # error +1: 0.05 clamped to +0.02
+    (2.0, None, 0.98),           # error -0.5: -0.025 clamped to -0.02
+    (1.0 / 1.5, 0.01, 1.005),    # the patch's relaxation 0.01, error +0.5
+])
+def test_mass_flow_hub(ratio, relax, expect):
+    base = cases.rotor3d(17, 6, 9)
+    U = perturbed(base)
+    m = float(cur_mass_flow(O.OracleCase(base), U))
+    target = m * ratio
+    out = "mass_flow, %.17g" % target if relax is None else "mass_flow, %.17g, 0, %.17g" % (target, relax)
+    case = cases.rotor3d(17, 6, 9, outflow=out)
+    oc, before, after = update(case, U)
+    if ratio == 1.0:
+        assert after[0] == before[0]
+    else:
+        err = (m - target) / target
+        d = max(-0.02, min(0.02, (relax or 0.05) * err))
+        assert after[0] == before[0] * (1.0 + d)
+        assert after[0] == pytest.approx(before[0] * expect, rel=1e-15)
+
+
+def test_mass_flow_floor_and_uniform_profile():
+    # 2-D, no body force: no radial equilibrium, a uniform profile from the patch's
+    # initial pressure; a huge target drives the hub down, never under 100 Pa
+    case = cases.bump2d(17, 9, outflow="mass_flow, 1e9, 150")
+    oc, before, after = update(case)
+    assert np.all(before == 150.0)
+    assert np.all(after == after[0]) and after[0] == 150.0 * (1.0 - 0.02)
+    case = cases.bump2d(17, 9, outflow="mass_flow, 1e9, 101")
+    oc, before, after = update(case)
+    assert np.all(after == 100.0)
