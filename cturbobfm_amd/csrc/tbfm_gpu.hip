@@ -143,6 +143,10 @@ struct tbfm_batch {
 };
 
 namespace {
+void hostTurbo(tbfm_case* c, const std::vector<double>& U, double* out);
+}
+
+namespace {
 
 void drainProfile(tbfm_case* c);
 
@@ -454,6 +458,21 @@ void setup(tbfm_case* cs, const GridInput& grid) {
             prof_cs[pr * NJ + (j - p.j0)] = std::cos(th);
             prof_sn[pr * NJ + (j - p.j0)] = std::sin(th);
         }
+    }
+    // Greitzer plenum (solver.cpp:730-747, config.cpp:684-693, config.hpp:347-353)
+    c.greitzer = cfg.getBool("ENABLE_GREITZER_MODELING", false) ? 1 : 0;
+    if (c.greitzer) {
+        if (c.dt_mode != 2) throw std::logic_error("Greitzer modeling is only compatible with fixed time step method");
+        bool thr = false;
+        for (auto& p : cs->patches)
+            if (p.type == PatchType::Throttle) { thr = true; c.gz_kt = p.values[0]; }
+        if (!thr) throw std::runtime_error("Greitzer modeling enabled but no throttle boundary condition found!");
+        c.gz_vol = cfg.has("GREITZER_PLENUM_VOLUME") ? cfg.getFloat("GREITZER_PLENUM_VOLUME")
+                                                      : cfg.getFloat("GREITZER_PENUM_VOLUME");
+        c.gz_amb = cfg.getFloat("GREITZER_AMBIENT_PRESSURE", 101325.0);
+        c.gz_dt = cfg.getFixedTimeStep();
+        c.gz = dalloc<double>(3, own);
+        c.prof_need_mf = 1;
     }
     c.prof_p = dalloc<double>(dev::MAX_PROF * NJ, own);
     c.prof_r = upload(prof_r, own);
@@ -1328,6 +1347,29 @@ int tbfm_gpu_set_state(tbfm_case* c, const double* U) {
             if (c->c.prof_kind[q] == 2)
                 for (long m = c->c.prof_n[q]; m < c->nj; ++m) prof[q * c->nj + m] = prof[q * c->nj + c->c.prof_n[q] - 1];
         HIPCHK(hipMemcpyAsync(c->c.prof_p, prof.data(), prof.size() * 8, hipMemcpyHostToDevice, st));
+        if (c->c.greitzer) {
+            // GreitzerModel::initializeState (solver.cpp:802-816): outlet hub node (NI-1, 0,
+            // 0) static pressure and temperature, turbomachinery mass flow in and out
+            std::vector<double> Uv(U, U + 5 * c->N);
+            double t[10];
+            hostTurbo(c, Uv, t);
+            const long n = ((c->ni - 1) * c->nj + 0) * c->nk + 0;
+            const double rho = U[n], W1 = U[c->N + n] / rho, W2 = U[2 * c->N + n] / rho, W3 = U[3 * c->N + n] / rho,
+                         W4 = U[4 * c->N + n] / rho;
+            double gz[3];
+            if (c->c.real) {
+                const double e = W4 - 0.5 * (W1 * W1 + W2 * W2 + W3 * W3);
+                gz[0] = lut_at(c->lut, LUT_P, rho, e);
+                gz[2] = lut_at(c->lut, LUT_T, rho, e);
+            } else {
+                const double vm = std::sqrt(W1 * W1 + W2 * W2 + W3 * W3);
+                const double e = W4 - 0.5 * (vm * vm);
+                gz[0] = (g - 1) * rho * e;
+                gz[2] = gz[0] / (rho * c->c.rgas);
+            }
+            gz[1] = t[6];
+            HIPCHK(hipMemcpyAsync(c->c.gz, gz, sizeof gz, hipMemcpyHostToDevice, st));
+        }
         HIPCHK(hipStreamSynchronize(st));
         return TBFM_OK;
     });
@@ -1504,6 +1546,16 @@ int tbfm_gpu_get_turbo(tbfm_case* c, double* out) {
         std::vector<double> U(5 * c->N);
         int rc = tbfm_gpu_get_state(c, U.data());
         if (rc != TBFM_OK) return rc;
+        hostTurbo(c, U, out);
+        return TBFM_OK;
+    });
+}
+
+namespace {
+// Solver::updateMassFlows + updateTurboPerformance (solver.cpp:1633-1781) of a state in
+// the reference layout, face by face in the reference's order.
+void hostTurbo(tbfm_case* c, const std::vector<double>& U, double* out) {
+    {
         const long NI = c->ni, NJ = c->nj, NK = c->nk, N = c->N;
         const Geometry& g = c->geo;
         const double gm = c->c.gamma, Rg = c->c.rgas;
@@ -1619,9 +1671,9 @@ int tbfm_gpu_get_turbo(tbfm_case* c, double* out) {
             else eta = (std::abs(dTs) < 1e-12) ? 1.0 : dT / dTs;
         }
         out[9] = eta;
-        return TBFM_OK;
-    });
+    }
 }
+}  // namespace
 
 int tbfm_gpu_get_geometry(tbfm_case* c, int which, double* out) {
     if (!c || !out) return fail(c, TBFM_ERR_ARGUMENT, "null argument");

@@ -103,6 +103,11 @@ struct CaseDev {
     // else uniform.  need_mf: some profile reads the boundary mass flows.
     int prof_kind[MAX_PROF], prof_radial[MAX_PROF], prof_mfb[MAX_PROF], prof_need_mf;
     double prof_par[MAX_PROF][4];
+    // Greitzer plenum (ENABLE_GREITZER_MODELING): every profile's hub is the plenum
+    // pressure; gz = {plenum pressure, outlet mass flow, temperature}
+    int greitzer;
+    double gz_dt, gz_vol, gz_amb, gz_kt;
+    double* gz;
     double* prof_p;                    // [MAX_PROF][NJ]
     const double* prof_r;              // [MAX_PROF][NJ]
     const double* prof_cs;             // [MAX_PROF][NJ] cos(theta) at (i, j, 0)

@@ -393,11 +393,25 @@ __device__ inline void radial_profile_block(const CaseDev& c, const double* U) {
         mturbo = 0.5 * (fabs(mb[0]) + fabs(mb[1]));
         mturbo *= c.turbo_mscale;
     }
+    // GreitzerModel::computePlenumPressure (greitzer_model.cpp:36-56), one step per
+    // iteration; thread 0 alone uses the hub
+    double gz_hub = 0.0;
+    if (c.greitzer && tid == 0) {
+        double* g = c.gz;
+        const double a = c.real ? lut_at(*c.lut, LUT_A_PT, g[0], g[2]) : sqrt(c.gamma * c.rgas * g[2]);
+        const double np = g[0] + c.gz_dt * a * a / c.gz_vol * (mturbo - g[1]);
+        g[0] = np;
+        const double dp = np - c.gz_amb;
+        g[1] = (dp > 0.0) ? sqrt(dp / c.gz_kt) : 0.0;
+        gz_hub = np;
+    }
     for (int q = 0; q < c.nprof; ++q) {
         const int n = c.prof_n[q];
         double* P = c.prof_p + (long)q * c.nj;
         double hub = P[0];
-        if (c.prof_kind[q] == 1) {
+        if (c.greitzer) {
+            hub = gz_hub;
+        } else if (c.prof_kind[q] == 1) {
             hub = c.prof_par[q][1] + c.prof_par[q][0] * mturbo * mturbo;
         } else if (c.prof_kind[q] == 2) {
             double cur = fabs(mb[c.prof_mfb[q]]);

@@ -38,7 +38,8 @@ class Params(C.Structure):
                 ("residual_drop", C.c_long), ("steady", C.c_int), ("fluid_real", C.c_int),
                 ("kn", C.c_double), ("kf", C.c_double), ("kd", C.c_double), ("thollet_off_design", C.c_int),
                 ("stall_active", C.c_int), ("nu", C.c_double),
-                ("mass_flow_relaxation", C.c_double), ("mass_flow_max_change", C.c_double)]
+                ("mass_flow_relaxation", C.c_double), ("mass_flow_max_change", C.c_double),
+                ("greitzer", C.c_int), ("greitzer_volume", C.c_double), ("greitzer_ambient", C.c_double)]
 
 
 class Table(C.Structure):
@@ -237,6 +238,12 @@ def params_from_config(case: Dict) -> Params:
     p.residual_drop = int(float(cfg.get("RESIDUALS_DROP_CONVERGENCE", "16")))
     p.mass_flow_relaxation = float(cfg.get("MASS_FLOW_RELAXATION", "0.05"))
     p.mass_flow_max_change = float(cfg.get("MASS_FLOW_MAX_PRESSURE_CHANGE", "0.02"))
+    p.greitzer = int(_cfg_bool(cfg, "ENABLE_GREITZER_MODELING", False))
+    if p.greitzer:
+        # config.hpp:347-353
+        p.greitzer_volume = float(cfg["GREITZER_PLENUM_VOLUME"] if "GREITZER_PLENUM_VOLUME" in cfg
+                                  else cfg["GREITZER_PENUM_VOLUME"])
+        p.greitzer_ambient = float(cfg.get("GREITZER_AMBIENT_PRESSURE", "101325"))
     p.steady = int(_cfg_bool(cfg, "SIMULATION_IS_STEADY", True))
     if p.bfm_active and p.bfm_model == BFM["Hall-Thollet"]:
         # source_bfm_thollet.hpp:47-50 (config.hpp:295-299), config.cpp:441-450

@@ -75,6 +75,8 @@ struct tbo_case {
     int nprof;
     int prof_patch[8];
     int prof_radial[8];
+    /* Greitzer plenum state: pressure, outlet mass flow, temperature (greitzer_model.cpp) */
+    double gz_p, gz_mout, gz_T;
     double *prof_p[8], *prof_r[8];
     long prof_n[8];
     int patch_prof[64];
@@ -1325,11 +1327,39 @@ static void periodic_solution(tbo_case* c, double* U) {
  * turbomachinery mass flow; mass flow: relaxed towards the target mass flow of the
  * patch's boundary), then the radial-equilibrium integration or a uniform profile. */
 void tbo_turbo(tbo_case* c, const double* U, double* out);
+
+/* Throttle coefficient of the Greitzer model: the last throttle patch's (solver.cpp:733-741) */
+static double greitzer_kt(const tbo_case* c) {
+    double kt = 0.0;
+    for (int b = 0; b < c->npatches; ++b)
+        if (c->patches[b].type == TBO_BC_THROTTLE) kt = c->patches[b].values[0];
+    return kt;
+}
+/* FluidBase::computeSoundSpeed_p_T: ideal sqrt(gamma R T) (fluid_ideal.cpp), table a(p, T)
+ * (fluid_real.cpp:422-424) */
+static double f_a_p_T(const tbo_case* c, double p, double T) {
+    if (c->p.fluid_real) return lut_interp(c, TBO_LUT_A_PT, p, T);
+    return sqrt(c->p.gamma * c->p.r_gas * T);
+}
+/* GreitzerModel::computePlenumPressure (greitzer_model.cpp:36-56) */
+static double greitzer_plenum_pressure(tbo_case* c, double mflow) {
+    double min = mflow;
+    double a = f_a_p_T(c, c->gz_p, c->gz_T);
+    double np = c->gz_p + c->p.fixed_dt * a * a / c->p.greitzer_volume * (min - c->gz_mout);
+    c->gz_p = np;
+    double dp = c->gz_p - c->p.greitzer_ambient;
+    c->gz_mout = (dp > 0.0) ? sqrt(dp / greitzer_kt(c)) : 0.0;
+    return c->gz_p;
+}
+
 static void update_radial_profiles(tbo_case* c, const double* U) {
     double t[10];
     int need_mf = 0;
     for (int q = 0; q < c->nprof; ++q) need_mf |= (c->patches[c->prof_patch[q]].type != TBO_BC_RADIAL_EQ);
+    need_mf |= c->p.greitzer;
     if (need_mf) tbo_turbo(c, U, t);   /* updateMassFlows + updateTurboPerformance */
+    double gz_hub = 0.0;
+    if (c->p.greitzer) gz_hub = greitzer_plenum_pressure(c, t[6]);
     for (int q = 0; q < c->nprof; ++q) {
         const tbo_patch* pa = &c->patches[c->prof_patch[q]];
         long n = c->prof_n[q];
@@ -1347,7 +1377,9 @@ static void update_radial_profiles(tbo_case* c, const double* U) {
             k++;
         }
         double hub = c->prof_p[q][0];
-        if (pa->type == TBO_BC_THROTTLE) {
+        if (c->p.greitzer) {
+            hub = gz_hub;   /* every profile follows the plenum (solver.cpp:2695-2698, 2719-2721) */
+        } else if (pa->type == TBO_BC_THROTTLE) {
             double kt = pa->values[0], mflow = t[6], pt_in = 0.0;
             for (int b = 0; b < c->npatches; ++b)
                 if (c->patches[b].type == TBO_BC_INLET) { pt_in = c->patches[b].values[0]; break; }
@@ -1646,6 +1678,17 @@ void tbo_destroy(tbo_case* c) {
 long tbo_num_nodes(const tbo_case* c) { return c->N; }
 
 static void init_profiles(tbo_case* c) {
+    /* solver.cpp:802-816: the Greitzer plenum starts at the outlet hub node's static
+     * pressure and temperature with the turbomachinery mass flow in and out */
+    if (c->p.greitzer) {
+        double t[10];
+        tbo_turbo(c, c->U0, t);
+        st5 W = prim(ld(c, c->U0, IDX(c, c->p.ni - 1, 0, 0)));
+        v3 u = mk3(W.v[1], W.v[2], W.v[3]);
+        c->gz_p = f_p(c, W.v[0], u, W.v[4]);
+        c->gz_T = f_T(c, W.v[0], u, W.v[4]);
+        c->gz_mout = t[6];
+    }
     /* solver.cpp:818-834: profile = static pressure of the current solution at i_min-1,
      * k = 0; a mass-flow patch with an initial pressure (second value > 0) takes that */
     for (int q = 0; q < c->nprof; ++q) {

@@ -50,6 +50,10 @@ typedef struct {
     /* mass-flow outlet (config.hpp:291-293) */
     double mass_flow_relaxation;     /* MASS_FLOW_RELAXATION (0.05) */
     double mass_flow_max_change;     /* MASS_FLOW_MAX_PRESSURE_CHANGE (0.02) */
+    /* Greitzer plenum (greitzer_model.cpp, config.hpp:315-353) */
+    int greitzer;                    /* ENABLE_GREITZER_MODELING (fixed time step) */
+    double greitzer_volume;          /* GREITZER_PLENUM_VOLUME */
+    double greitzer_ambient;         /* GREITZER_AMBIENT_PRESSURE (101325) */
 } tbo_params;
 
 /* A fluid look-up table (include/fluid_real.hpp:11-60 Table2D): nx*ny values, row-major

@@ -49,7 +49,16 @@ CASES = {
     "rotor3d_jst_throttle": lambda: cases.rotor3d(25, 9, 17, n_iterations=200, outflow="throttle, 0.01"),
     "rotor3d_jst_massflow": lambda: cases.rotor3d(25, 9, 17, n_iterations=200, outflow="mass_flow, 15"),
     "bump2d_jst_massflow_uniform": lambda: cases.bump2d(33, 17, n_iterations=200, outflow="mass_flow, 60, 90000"),
+    "rotor3d_jst_throttle_greitzer": lambda: greitzer(cases.rotor3d(25, 9, 17, n_iterations=200,
+                                                                    outflow="throttle, 0.01")),
 }
+
+
+def greitzer(case):
+    """Greitzer plenum behind the throttle: fixed time step, plenum of 0.05 m^3."""
+    case["config"].update({"ENABLE_GREITZER_MODELING": "yes", "GREITZER_PLENUM_VOLUME": "0.05",
+                           "TIME_STEP_METHOD": "fixed", "FIXED_TIME_STEP": "2e-6"})
+    return case
 
 
 @pytest.mark.parametrize("name", list(CASES))

@@ -9,6 +9,8 @@ the maximum change in both directions, takes the patch's own relaxation when giv
 never goes below 100 Pa; a mass-flow outlet without radial equilibrium is uniform; the
 patch's initial pressure (second value) starts its profile.
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -80,3 +82,29 @@ def test_mass_flow_floor_and_uniform_profile():
     case = cases.bump2d(17, 9, outflow="mass_flow, 1e9, 101")
     oc, before, after = update(case)
     assert np.all(after == 100.0)
+
+
+def test_greitzer_plenum_steps():
+    """The plenum starts at the outlet hub node's pressure and temperature with the mass
+    flow in and out equal (no change on the first step); the next step integrates
+    dp/dt = a^2/V (m_in - m_out) with m_out = sqrt((p - p_amb)/k_t) of the first step."""
+    kt, vol, dt, amb = 0.01, 0.05, 2e-6, 101325.0
+    case = cases.rotor3d(17, 6, 9, outflow="throttle, %r" % kt)
+    case["config"].update({"ENABLE_GREITZER_MODELING": "yes", "GREITZER_PLENUM_VOLUME": "%r" % vol,
+                           "TIME_STEP_METHOD": "fixed", "FIXED_TIME_STEP": "%r" % dt,
+                           "GREITZER_AMBIENT_PRESSURE": "%r" % amb})
+    U = perturbed(case)
+    oc = O.OracleCase(case)
+    oc.set_state(U)
+    m = oc.turbo(U)[6]
+    rho = U[0, -1, 0, 0]
+    u = U[1:4, -1, 0, 0] / rho
+    vm = math.sqrt(float(u @ u))
+    p0 = (1.4 - 1) * rho * (U[4, -1, 0, 0] / rho - 0.5 * (vm * vm))
+    T0 = p0 / (rho * 287.05)
+    oc.residual(U, it=1, update_profiles=True)
+    assert oc.profile(0)[0] == p0
+    mout = math.sqrt((p0 - amb) / kt) if p0 > amb else 0.0
+    a = math.sqrt(1.4 * 287.05 * T0)
+    oc.residual(U, it=2, update_profiles=True)
+    assert oc.profile(0)[0] == p0 + dt * a * a / vol * (m - mout)
