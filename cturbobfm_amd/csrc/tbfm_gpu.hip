@@ -340,6 +340,7 @@ void setup(tbfm_case* cs, const GridInput& grid) {
                 break;
             }
             case PatchType::Wall: pd.type = dev::PT_WALL; break;
+            case PatchType::Transparent: pd.type = dev::PT_TRANSPARENT; break;
             default: pd.type = dev::PT_FAKE; break;
         }
         c.patch[q] = pd;

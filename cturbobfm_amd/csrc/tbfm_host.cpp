@@ -276,6 +276,7 @@ PatchType patchTypeFromString(const std::string& s) {
     if (k == "wedge") return PatchType::Wedge;
     if (k == "throttle") return PatchType::Throttle;
     if (k == "mass_flow") return PatchType::MassFlow;
+    if (k == "transparent") return PatchType::Transparent;
     throw std::runtime_error("BoundaryType " + s + " is outside the GPU residual path");
 }
 

@@ -23,7 +23,7 @@ namespace dev {
 
 constexpr int NT = 256;         // threads per stage-kernel block (tile columns)
 
-enum : int { PT_INLET = 0, PT_OUTLET = 1, PT_RADIAL = 2, PT_WALL = 3, PT_FAKE = 4 };
+enum : int { PT_INLET = 0, PT_OUTLET = 1, PT_RADIAL = 2, PT_WALL = 3, PT_FAKE = 4, PT_TRANSPARENT = 5 };
 enum : int { TOPO_2D = 0, TOPO_3D = 1, TOPO_AXI = 2, TOPO_1D = 3 };
 
 struct PatchDev {

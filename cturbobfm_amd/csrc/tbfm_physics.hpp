@@ -444,6 +444,23 @@ __device__ inline void outlet_flux(const CaseDev& c, const Gas<FL>& gs, const do
 }
 
 // Inviscid wall (src/boundary_inviscid_wall.cpp:5-30)
+// Transparent end (boundary_transparent.cpp:5-16): the scheme's own face flux with the
+// node's state on all four stencil positions, per unit area; S outward.
+template <int FL>
+__device__ inline void transparent_flux(const CaseDev& c, const Gas<FL>& gs, const double U[5], double sx, double sy,
+                                        double sz, double F[5]) {
+    const double sm = sqrt(sx * sx + sy * sy + sz * sz);
+    const double nx = sx / sm, ny = sy / sm, nz = sz / sm;
+    double W[5], p, rf;
+    derive<double, FL>(U, W, p, rf, gs);
+    if (c.scheme == 0) {
+        const double s = jst_sensor<double>(p, p, p);
+        jst_face<double, FL>(U, U, U, U, W, W, rf, rf, s, s, nx, ny, nz, 1.0, gs, F);
+    } else {
+        roe_face<double, FL>(W, W, W, W, c.muscl, c.limiter, nx, ny, nz, 1.0, gs, F);
+    }
+}
+
 template <int FL>
 __device__ inline void wall_flux(const Gas<FL>& gs, const double U[5], double sx, double sy, double sz,
                                  double F[5]) {

@@ -543,6 +543,8 @@ __device__ inline void boundary_block(const CaseDev& c, const double* U, int bx)
         outlet_flux<FL>(c, gs, Un, Sx, Sy, Sz, c.prof_p[(long)pa.profile * NJ + fj], it, F);
     } else if (pa.type == PT_WALL) {
         wall_flux<FL>(gs, Un, Sx, Sy, Sz, F);
+    } else if (pa.type == PT_TRANSPARENT) {
+        transparent_flux<FL>(c, gs, Un, Sx, Sy, Sz, F);
     }
     const double sm = sqrt(Sx * Sx + Sy * Sy + Sz * Sz);
     double* o = out + (long)side * 5 * nsz;

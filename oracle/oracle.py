@@ -21,7 +21,7 @@ TOPO = {"2D": 0, "2d": 0, "3D": 1, "3d": 1, "axisymmetric": 2, "Axisymmetric": 2
         "AXISYMMETRIC": 2, "1D": 3, "1d": 3}
 LIMITER = {"None": 0, "van albada": 1, "van leer": 2, "min mod": 3}
 BC = {"inlet": 0, "outlet": 1, "radial_equilibrium": 2, "inviscid_wall": 3, "no_slip_wall": 3,
-      "periodic": 4, "wedge": 5, "throttle": 6, "mass_flow": 7}
+      "periodic": 4, "wedge": 5, "throttle": 6, "mass_flow": 7, "transparent": 8}
 BFM = {"None": 0, "Hall": 1, "Lift-Drag": 2, "Blockage": 3, "Hall-Thollet": 4}
 
 

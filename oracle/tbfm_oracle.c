@@ -666,6 +666,9 @@ static st5 bc_flux(const tbo_case* c, int pidx, st5 Uint, v3 S, long fi, long fj
             return outlet_flux(c, Uint, S, c->prof_p[q][idx], it);
         }
         case TBO_BC_WALL: return wall_flux(c, Uint, S);
+        /* boundary_transparent.cpp:5-16: the scheme's flux with the internal state on all
+         * four positions */
+        case TBO_BC_TRANSPARENT: return face_flux(c, Uint, Uint, Uint, Uint, S);
         default: return z;
     }
 }

@@ -22,7 +22,8 @@ enum { TBO_LIM_NONE = 0, TBO_LIM_VAN_ALBADA = 1, TBO_LIM_VAN_LEER = 2, TBO_LIM_M
 enum { TBO_DT_LOCAL = 0, TBO_DT_GLOBAL = 1, TBO_DT_FIXED = 2 };
 enum { TBO_BFM_NONE = 0, TBO_BFM_HALL = 1, TBO_BFM_LIFT_DRAG = 2, TBO_BFM_BLOCKAGE = 3, TBO_BFM_THOLLET = 4 };
 enum { TBO_BC_INLET = 0, TBO_BC_OUTLET = 1, TBO_BC_RADIAL_EQ = 2, TBO_BC_WALL = 3,
-       TBO_BC_PERIODIC = 4, TBO_BC_WEDGE = 5, TBO_BC_THROTTLE = 6, TBO_BC_MASS_FLOW = 7 };
+       TBO_BC_PERIODIC = 4, TBO_BC_WEDGE = 5, TBO_BC_THROTTLE = 6, TBO_BC_MASS_FLOW = 7,
+       TBO_BC_TRANSPARENT = 8 };
 
 typedef struct {
     long ni, nj, nk;

@@ -49,12 +49,31 @@ GER4 = dict(COMMON, BFM_ACTIVE="yes", BLOCKAGE_ACTIVE="yes", BFM_MODEL="Hall-Tho
 CONFIGS["GER4_Axisymmetric_HallThollet"] = dict(GER4, TOPOLOGY="axisymmetric")
 CONFIGS["GER4_FullAnnulus"] = dict(GER4, TOPOLOGY="3D", PER1="periodic, 0, 0, 360.0",
                                    PER2="periodic, 1, 0, 360.0")
+# Sod shock tube, 1-D Roe + MUSCL van Leer, transparent ends, fixed time step, started
+# from the reference's restart file (2500 iterations; the 16-order drop is never reached)
+CONFIGS["Sod_Test_Roe2"] = dict(COMMON, TOPOLOGY="1D", FLUID_NAME="Air", LEFT_END="transparent",
+                                RIGHT_END="transparent", CFL="0.6", N_ITERATIONS="2500",
+                                RESIDUALS_DROP_CONVERGENCE="16", TIME_STEP_METHOD="fixed",
+                                FIXED_TIME_STEP="0.0001", CONVECTION_SCHEME="Roe", MUSCL_RECONSTRUCTION="yes",
+                                FLUX_LIMITER="van leer", RESTART_SOLUTION="yes",
+                                RESTART_SOLUTION_FILEPATH=os.path.join(HERE, "Sod_Test_Roe2",
+                                                                       "grid_250_RestartData.csv"))
+GRID_FILES = {"Sod_Test_Roe2": "grid_250_OnlyCoords.csv"}
+
+
+def max_iterations(name):
+    """Iteration limit of a case: N_ITERATIONS when the case runs a fixed count."""
+    return min(int(float(CONFIGS[name]["N_ITERATIONS"])), 40000)
+
+
+def converges(name):
+    return float(CONFIGS[name]["RESIDUALS_DROP_CONVERGENCE"]) < 16
 
 
 def load_case(name):
     folder = os.path.join(HERE, name)
     cfg = dict(CONFIGS[name])
-    grid = os.path.join(folder, "grid.csv")
+    grid = os.path.join(folder, GRID_FILES.get(name, "grid.csv"))
     ni = nj = nk = 0
     skip = 0
     with open(grid) as f:
@@ -75,7 +94,9 @@ def load_case(name):
     fields = {h: data[:, c].reshape(ni, nj, nk).copy() for c, h in enumerate(header) if c >= 3}
     patches = [("INFLOW", 0, 0, 0, nj, 0, nk), ("OUTFLOW", ni, ni, 0, nj, 0, nk),
                ("LOWER_WALL", 0, ni, 0, 0, 0, nk), ("UPPER_WALL", 0, ni, nj, nj, 0, nk)]
-    if nk > 1:
+    if cfg["TOPOLOGY"] == "1D":
+        patches = [("LEFT_END", 0, 0, 0, nj, 0, nk), ("RIGHT_END", ni, ni, 0, nj, 0, nk)]
+    elif nk > 1:
         patches += [("PER1", 0, ni, 0, nj, 0, 0), ("PER2", 0, ni, 0, nj, nk, nk)]
     return {"ni": ni, "nj": nj, "nk": nk, "x": X, "y": Y, "z": Z, "fields": fields, "config": cfg,
             "patches": patches, "name": name}
@@ -104,3 +125,13 @@ def rel_l2(ref, res):
 def case_files(name, folder):
     """input.ini + grid + boundaries for `turbobfm input.ini` runs."""
     return cases.write_case(load_case(name), folder)
+
+
+def initial_state(name, folder):
+    """The case's initial state: the restart file through the product's host reader
+    (pinned on its own in test_io.py) when RESTART_SOLUTION is set, else None (the
+    uniform INIT_* state every solver builds itself)."""
+    if CONFIGS[name].get("RESTART_SOLUTION") != "yes":
+        return None
+    import cturbobfm_amd as T
+    return T.initial_state_from_ini(case_files(name, folder))[0]

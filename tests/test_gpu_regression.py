@@ -21,28 +21,36 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("name", list(RC.CONFIGS))
-def test_gpu_regression_case(name):
+def test_gpu_regression_case(name, tmp_path):
     print("case", name, flush=True)
     case = RC.load_case(name)
     oc = O.OracleCase(case)
+    start = RC.initial_state(name, str(tmp_path / name))
+    if start is not None:
+        oc.set_state(start)
     U0 = oc.state()
-    lo, conv_o, _ = oc.run(40000, threads=8)
+    nmax = RC.max_iterations(name)
+    lo, conv_o, _ = oc.run(nmax, threads=8)
+    assert (conv_o > 0) == RC.converges(name)
+    nrun = conv_o if conv_o else nmax
     # the oracle's own sensitivity: same run from a state 1 ulp away
     op = O.OracleCase(case)
     op.set_state(np.nextafter(U0, np.inf))
-    lp, _, _ = op.run(conv_o, threads=8)
-    spread = float(np.max(np.abs(lp[:conv_o] - lo[:conv_o])))
+    lp, _, _ = op.run(nrun, threads=8)
+    spread = float(np.max(np.abs(lp[:nrun] - lo[:nrun])))
     hist_tol = min(max(1e-11, 4.0 * spread), 1e-9)
     s = Solver.from_case(case)
-    s.solve(40000)
+    if start is not None:
+        s.set_state(start)
+    s.solve(nmax)
     assert s.converged_at == conv_o
     lg = s.history
-    assert lg.shape == lo[:conv_o].shape
-    hist_err = float(np.max(np.abs(lg - lo[:conv_o])))
+    assert lg.shape == lo[:nrun].shape
+    hist_err = float(np.max(np.abs(lg - lo[:nrun])))
     Ug, Uo = s.state(), oc.state()
     f_err = max(float(np.max(np.abs(Ug[v] - Uo[v])) / np.max(np.abs(Uo[v]))) for v in range(5) if np.max(np.abs(Uo[v])) > 0)
-    print("converged at %d; history err %.3e (oracle 1-ulp spread %.3e); field err %.3e"
-          % (conv_o, hist_err, spread, f_err))
+    print("converged at %d (of %d); history err %.3e (oracle 1-ulp spread %.3e); field err %.3e"
+          % (conv_o, nmax, hist_err, spread, f_err))
     assert hist_err <= hist_tol and f_err <= 1e-11
     ref = RC.load_reference_solution(name)
     got = RC.derived_fields(Ug)
