@@ -137,7 +137,7 @@ def bench_serve(args):
                                lut_path=lut)
     arrivals = serve.poisson_arrivals(len(specs), rate=args.rate, seed=args.seed)
     progress = lambda m: print(m, file=sys.stderr, flush=True)
-    srv = serve.Server(drop=args.drop, max_iter=args.max_iter, chunk=100, keep_turbo=True, log=progress)
+    srv = serve.Server(drop=args.drop, max_iter=args.max_iter, chunk=args.chunk, keep_turbo=True, log=progress)
     t0 = time.perf_counter()
     res = srv.run(specs, arrivals)
     wall = time.perf_counter() - t0
@@ -170,7 +170,8 @@ def bench_serve(args):
         "vs_baseline": None, "dtype": "f64-state/f32-flux",
         "data": "synthetic bump and rotor cases (cturbobfm_amd/serve.py mixed_stream, seed %d)" % args.seed,
         "config": {"workload": "%d cases, Poisson arrivals %.1f/s, retire at %.1f orders of density drop "
-                               "(cap %d iterations)" % (len(specs), args.rate, args.drop, args.max_iter)},
+                               "(cap %d iterations)" % (len(specs), args.rate, args.drop, args.max_iter),
+                   "chunk_iterations": args.chunk},
         "served": served, "plain_fp64": plain,
         "table_fluid_cases": int(sum("table-fluid" in sp.name for sp in specs)),
         "converged_rel_diff_vs_plain": {"mass_flow_max": max(dm) if dm else None,
@@ -191,6 +192,7 @@ def main():
     ap.add_argument("--batch-cases", type=int, default=64)
     ap.add_argument("--serve-cases", type=int, default=40)
     ap.add_argument("--rate", type=float, default=2.0)
+    ap.add_argument("--chunk", type=int, default=100, help="serve: iterations per batch advance between admissions")
     ap.add_argument("--drop", type=float, default=3.0)
     ap.add_argument("--max-iter", type=int, default=20000)
     ap.add_argument("--steps", type=int, default=300)
