@@ -51,7 +51,7 @@ def traffic_from_profiles(kernel_prefix: str):
 
 def cpu_baseline(case, seconds: float = 15.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O  # the checker; timed here only as the reported CPU baseline
+    import oracle as O  # the test oracle; timed here only as the reported CPU baseline
     threads = int(os.environ.get("OMP_NUM_THREADS", str(os.cpu_count() or 1)))
     oc = O.OracleCase(case)
     stages = oc.params.n_stages
@@ -206,7 +206,7 @@ def main():
                     help="only run the plain copy kernel of this many bytes (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (requested --gpus %d)" % args.gpus,
+        print("bench.py measures one GPU: there is no multi-GPU path (requested --gpus %d)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
 

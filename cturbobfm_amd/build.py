@@ -2,7 +2,7 @@
 
   cturbobfm_amd/libtbfm_gpu.so   HIP kernels for gfx950 + C-ABI (include/tbfm_gpu.h)
   cturbobfm_amd/turbobfm         `turbobfm input.ini` command-line driver
-  oracle/liboracle.so            CPU restatement used as the checker (tests, bench baseline)
+  oracle/liboracle.so            CPU restatement, the test oracle (tests, bench CPU baseline)
   oracle/_ref/libtbfm_ref.so     reference translation units, only where /root/reference exists
 """
 from __future__ import annotations

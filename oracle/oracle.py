@@ -1,7 +1,7 @@
 """ctypes binding of the CPU restatement (liboracle.so) -- TEST INFRASTRUCTURE.
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module,
-always as the checker.  It turns a case dict (see cturbobfm_amd/cases.py) into the numeric
+always as the test oracle.  It turns a case dict (see cturbobfm_amd/cases.py) into the numeric
 parameters the C restatement takes, restating the reference's input.ini semantics
 (src/config.cpp, include/config.hpp; key names, defaults and accepted spellings).
 """

@@ -1,4 +1,4 @@
-// Checker-only driver around the reference's own path translation units.
+// Test-only driver around the reference's own path translation units.
 //
 // TEST INFRASTRUCTURE. This file is compiled together with the unchanged
 // reference sources (src/config.cpp, src/input.cpp, src/mesh.cpp,

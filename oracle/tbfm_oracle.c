@@ -1,7 +1,7 @@
 /* CPU restatement of CTurboBFM's explicit residual path -- TEST INFRASTRUCTURE.
  *
  * The oracle.  Loaded only by tests/, __graft_entry__.smoke() and the cpu_baseline
- * leg of bench.py, always as the checker.  Each function names the reference
+ * leg of bench.py, always as the test oracle.  Each function names the reference
  * file:line it restates; operation order follows the reference so that, with
  * -ffp-contract=off on both sides, results agree bit for bit where no libm
  * transcendental is involved.

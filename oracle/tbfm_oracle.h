@@ -1,7 +1,7 @@
 /* CPU restatement of CTurboBFM's explicit residual path -- TEST INFRASTRUCTURE.
  *
  * This is the oracle: only tests/, __graft_entry__.smoke() and bench.py's
- * cpu_baseline leg may load it, and only as the checker.  It is never part of the
+ * cpu_baseline leg may load it, and only as the test oracle.  It is never part of the
  * product path.  Plain C11, compiled with -ffp-contract=off; OpenMP over grid lines
  * only, with the per-node operation order of the reference kept unchanged.
  *

@@ -65,7 +65,7 @@ def fixture_cases():
 def main():
     R = O.ref_lib()
     if R is None:
-        raise SystemExit("build the reference checker first: make -C oracle ref")
+        raise SystemExit("build the reference library first: make -C oracle ref")
     rng = np.random.default_rng(20261203)
     tmp = os.path.join(ROOT, "cases_tmp", "golden")
     for name, case in fixture_cases().items():

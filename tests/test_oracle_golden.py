@@ -93,7 +93,7 @@ def test_radial_equilibrium():
     assert np.array_equal(p, r["p"])
 
 
-@pytest.mark.skipif(O.ref_lib() is None, reason="reference checker build absent (no /root/reference)")
+@pytest.mark.skipif(O.ref_lib() is None, reason="reference library build absent (no /root/reference)")
 def test_against_reference_build_fresh_inputs():
     """Where the reference build exists, also compare on fresh random inputs."""
     import ctypes as C
