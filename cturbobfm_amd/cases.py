@@ -180,6 +180,29 @@ def rotor3d(ni: int = 193, nj: int = 49, nk: int = 129, n_iterations: int = 2000
             "name": "rotor3d_%dx%dx%d" % (ni, nj, nk)}
 
 
+def viscous(case: Dict, model: str = "sutherland", mu: float = 1.8e-5,
+            walls: Optional[Dict[str, tuple]] = None) -> Dict:
+    """The case with VISCOSITY_ACTIVE = yes: air's Sutherland law (mu_ref 1.716e-5 Pa s at
+    273.15 K, S = 110.4 K), or a constant viscosity mu; c_p = 1006 J/(kg K), Pr = 0.708.
+    walls: patch name -> wall velocity (m/s) of the walls made no_slip_wall; by default
+    every inviscid wall becomes a wall at rest."""
+    out = dict(case)
+    cfg = dict(case["config"])
+    cfg.update({"VISCOSITY_ACTIVE": "yes", "FLUID_CP": "1006", "FLUID_PRANDTL_NUMBER": "0.708"})
+    if model == "constant":
+        cfg.update({"FLUID_VISCOSITY_MODEL": "constant", "FLUID_MU_CONSTANT": "%.17g" % mu})
+    else:
+        cfg.update({"FLUID_SUTHERLAND_MU_REF": "1.716E-5", "FLUID_SUTHERLAND_T_REF": "273.15",
+                    "FLUID_SUTHERLAND_S": "110.4"})
+    if walls is None:
+        walls = {k: (0.0, 0.0, 0.0) for k, v in cfg.items() if v == "inviscid_wall"}
+    for name, u in walls.items():
+        cfg[name] = "no_slip_wall, %.17g, %.17g, %.17g" % tuple(u)
+    out["config"] = cfg
+    out["name"] = case["name"] + "_viscous"
+    return out
+
+
 def perturb_state(case: Dict, U: np.ndarray, amplitude: float, seed: int) -> np.ndarray:
     """Adds a smooth seeded perturbation to a conserved state (5, ni, nj, nk)."""
     rng = np.random.default_rng(seed)

@@ -20,7 +20,7 @@ REF_LIB_PATH = os.path.join(HERE, "_ref", "libtbfm_ref.so")
 TOPO = {"2D": 0, "2d": 0, "3D": 1, "3d": 1, "axisymmetric": 2, "Axisymmetric": 2,
         "AXISYMMETRIC": 2, "1D": 3, "1d": 3}
 LIMITER = {"None": 0, "van albada": 1, "van leer": 2, "min mod": 3}
-BC = {"inlet": 0, "outlet": 1, "radial_equilibrium": 2, "inviscid_wall": 3, "no_slip_wall": 3,
+BC = {"inlet": 0, "outlet": 1, "radial_equilibrium": 2, "inviscid_wall": 3, "no_slip_wall": 9,
       "periodic": 4, "wedge": 5, "throttle": 6, "mass_flow": 7, "transparent": 8}
 BFM = {"None": 0, "Hall": 1, "Lift-Drag": 2, "Blockage": 3, "Hall-Thollet": 4}
 
@@ -39,7 +39,10 @@ class Params(C.Structure):
                 ("kn", C.c_double), ("kf", C.c_double), ("kd", C.c_double), ("thollet_off_design", C.c_int),
                 ("stall_active", C.c_int), ("nu", C.c_double),
                 ("mass_flow_relaxation", C.c_double), ("mass_flow_max_change", C.c_double),
-                ("greitzer", C.c_int), ("greitzer_volume", C.c_double), ("greitzer_ambient", C.c_double)]
+                ("greitzer", C.c_int), ("greitzer_volume", C.c_double), ("greitzer_ambient", C.c_double),
+                ("viscous", C.c_int), ("mu_constant_model", C.c_int), ("mu_constant", C.c_double),
+                ("mu_ref", C.c_double), ("mu_t_ref", C.c_double), ("mu_s", C.c_double),
+                ("cp_visc", C.c_double), ("prandtl", C.c_double)]
 
 
 class Table(C.Structure):
@@ -150,6 +153,8 @@ def lib():
         L.tbo_run.argtypes = [C.c_void_p, C.c_long, dp, dp, C.c_int]
         L.tbo_residual.argtypes = [C.c_void_p, dp, C.c_long, C.c_int, dp]
         L.tbo_timestep.argtypes = [C.c_void_p, dp, dp]
+        L.tbo_gradients.argtypes = [C.c_void_p, dp]
+        L.tbo_viscous_flux.argtypes = [C.c_void_p, dp, dp, dp, dp]
         L.tbo_geometry.restype = C.c_int
         L.tbo_geometry.argtypes = [C.c_void_p, C.c_int, dp]
         L.tbo_side_outputs.argtypes = [C.c_void_p, dp, dp, dp, dp]
@@ -255,6 +260,25 @@ def params_from_config(case: Dict) -> Params:
             raise ValueError("Hall-Thollet needs FLUID_KINEMATIC_VISCOSITY on the inviscid path")
         p.nu = float(cfg["FLUID_KINEMATIC_VISCOSITY"])
     p.stall_active = int(_cfg_bool(cfg, "STALLED_BFM_ACTIVE", False))
+    p.viscous = int(_cfg_bool(cfg, "VISCOSITY_ACTIVE", False))
+    if p.viscous:
+        # Config::getViscosityModel / getFluidMuConstant (config.cpp:575-597), the fluid's
+        # transport properties (fluid_ideal.cpp:209-222)
+        vm = cfg.get("FLUID_VISCOSITY_MODEL", "None")
+        if vm == "None":
+            p.mu_constant_model = int("FLUID_MU_CONSTANT" in cfg or "FLUID_CONSTANT_MU" in cfg)
+        elif vm.lower() in ("sutherland", "constant"):
+            p.mu_constant_model = int(vm.lower() == "constant")
+        else:
+            raise ValueError("unknown FLUID_VISCOSITY_MODEL " + vm)
+        if p.mu_constant_model:
+            p.mu_constant = float(cfg["FLUID_MU_CONSTANT"] if "FLUID_MU_CONSTANT" in cfg else cfg["FLUID_CONSTANT_MU"])
+        else:
+            p.mu_ref = float(cfg["FLUID_SUTHERLAND_MU_REF"])
+            p.mu_t_ref = float(cfg["FLUID_SUTHERLAND_T_REF"])
+            p.mu_s = float(cfg["FLUID_SUTHERLAND_S"])
+        p.cp_visc = float(cfg["FLUID_CP"])
+        p.prandtl = float(cfg["FLUID_PRANDTL_NUMBER"])
     fm = cfg.get("FLUID_MODEL", "ideal")
     if fm in ("real", "Real", "REAL"):
         p.fluid_real = 1
@@ -344,6 +368,12 @@ class OracleCase:
         R = np.zeros_like(U)
         lib().tbo_residual(self.h, _dp(U), it, int(update_profiles), _dp(R))
         return R
+
+    def gradients(self) -> np.ndarray:
+        """[ux, uy, uz, T] x [x, y, z] gradients of the last residual evaluation (viscous)."""
+        g = np.zeros((12, self.ni, self.nj, self.nk))
+        lib().tbo_gradients(self.h, _dp(g))
+        return g
 
     def timestep(self, U: np.ndarray) -> np.ndarray:
         U = np.ascontiguousarray(U, dtype=np.float64)

@@ -145,6 +145,43 @@ int ref_get_geometry(int which, double* out) {
     }
 }
 
+// computeGradientsGreenGauss4 (math_utils.cpp:237-392) of four node fields (N each,
+// k-fastest) over the loaded mesh; out: [field][x, y, z][N].
+void ref_gradients4(const double* f, double* out) {
+    const Mesh& m = *g_case.mesh;
+    const size_t ni = m.getNumberPointsI(), nj = m.getNumberPointsJ(), nk = m.getNumberPointsK(), N = ni * nj * nk;
+    Matrix3D<FloatType> F[4];
+    Matrix3D<Vector3D> G[4];
+    for (int q = 0; q < 4; ++q) {
+        F[q].resize(ni, nj, nk);
+        G[q].resize(ni, nj, nk);
+        for (size_t n = 0; n < N; ++n) F[q][n] = f[q * N + n];
+    }
+    computeGradientsGreenGauss4(m.getSurfacesI(), m.getSurfacesJ(), m.getSurfacesK(), m.getMidPointsI(),
+                                m.getMidPointsJ(), m.getMidPointsK(), m.getVertices(), m.getVolumes(), F[0], F[1],
+                                F[2], F[3], G[0], G[1], G[2], G[3]);
+    for (int q = 0; q < 4; ++q)
+        for (size_t n = 0; n < N; ++n) {
+            out[(3 * q) * N + n] = G[q][n].x();
+            out[(3 * q + 1) * N + n] = G[q][n].y();
+            out[(3 * q + 2) * N + n] = G[q][n].z();
+        }
+}
+
+// Molecular viscosity of the loaded case's fluid after setTransportProperties
+// (fluid_ideal.cpp:209-231), as the solver sets it up when VISCOSITY_ACTIVE (solver.cpp:72-74).
+double ref_mu(double T) {
+    g_case.fluid->setTransportProperties(*g_case.config);
+    return g_case.fluid->computeMolecularDynamicViscosity(T);
+}
+
+// computeViscousStressTensor (math_utils.cpp:593-608): out xx, yy, zz, xy, xz, yz.
+void ref_stress_tensor(double mu, double lam, const double* gu, const double* gv, const double* gw, double* out) {
+    ViscousStressTensor t = computeViscousStressTensor(mu, lam, Vector3D(gu[0], gu[1], gu[2]),
+                                                       Vector3D(gv[0], gv[1], gv[2]), Vector3D(gw[0], gw[1], gw[2]));
+    out[0] = t.xx; out[1] = t.yy; out[2] = t.zz; out[3] = t.xy; out[4] = t.xz; out[5] = t.yz;
+}
+
 // Face flux of the configured scheme (JST or Roe[+MUSCL]), per unit area.
 void ref_face_flux(const double* Ull, const double* Ul, const double* Ur, const double* Urr,
                    const double* S, double* out) {

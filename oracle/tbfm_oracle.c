@@ -84,6 +84,11 @@ struct tbo_case {
     double *U0, *U, *R, *dt;
     double *finv, *fvisc, *dev;
     double t_phys;
+    /* no-slip walls: node flag and wall velocity of the last no_slip_wall patch holding the
+     * node (solver.cpp:369-389); gradients [4 fields][3][N] of a viscous case */
+    int* nsw;
+    double* wallv;
+    double* grad;
     /* look-up-table fluid (FLUID_MODEL = real) */
     tbo_lut lut;
     double dx[TBO_LUT_COUNT], dy[TBO_LUT_COUNT];
@@ -632,8 +637,29 @@ static st5 wall_flux(const tbo_case* c, st5 Uint, v3 S) {
     st5 F = {{0.0, pw * n.x, pw * n.y, pw * n.z, 0.0}};
     return F;
 }
-/* src/boundary_inlet.cpp:4-43 (inviscid: the no-slip branch is never taken) */
+/* BoundaryBase::computeStagnationWallFlux (boundary_base.cpp:126-180), ideal gas: the
+ * wall's stagnation pressure on the face, no mass or energy flux */
+static st5 stagnation_wall_flux(const tbo_case* c, v3 S, double pt, double Tt, v3 uw) {
+    double um = mag3(uw), pb = pt;
+    if (um > 0.0) {
+        double g = c->p.gamma, Rg = c->p.r_gas;
+        double cp = g * Rg / (g - 1.0);
+        double Tb = Tt - 0.5 * um * um / cp;
+        if (Tb < 0.01 * Tt) Tb = 0.01 * Tt;
+        pb = pt * pow(Tb / Tt, g / (g - 1.0));
+    }
+    v3 n = div3(S, mag3(S));
+    st5 F = {{0.0, pb * n.x, pb * n.y, pb * n.z, 0.0}};
+    return F;
+}
+/* src/boundary_inlet.cpp:4-43.  The face indices name the node checked against the no-slip
+ * walls (boundary_base.hpp:44-56): a start face is its node, an end face lies outside the
+ * node range in its direction and never matches. */
 static st5 inlet_flux(const tbo_case* c, const double* vals, st5 Uint, v3 S, long fi, long fj, long fk) {
+    if (c->nsw && fi < c->p.ni && fj < c->p.nj && fk < c->p.nk) {
+        long w = IDX(c, fi, fj, fk);
+        if (c->nsw[w]) return stagnation_wall_flux(c, S, vals[0], vals[1], mk3(c->wallv[3 * w], c->wallv[3 * w + 1], c->wallv[3 * w + 2]));
+    }
     v3 dir = mk3(vals[2], vals[3], vals[4]);
     if (c->p.inlet_cylindrical) {
         long idx = IDX(c, fi, fj, fk);
@@ -665,7 +691,8 @@ static st5 bc_flux(const tbo_case* c, int pidx, st5 Uint, v3 S, long fi, long fj
             if (idx >= c->prof_n[q]) idx = c->prof_n[q] - 1;
             return outlet_flux(c, Uint, S, c->prof_p[q][idx], it);
         }
-        case TBO_BC_WALL: return wall_flux(c, Uint, S);
+        case TBO_BC_WALL:
+        case TBO_BC_NO_SLIP: return wall_flux(c, Uint, S);   /* solver.cpp:391-399 */
         /* boundary_transparent.cpp:5-16: the scheme's flux with the internal state on all
          * four positions */
         case TBO_BC_TRANSPARENT: return face_flux(c, Uint, Uint, Uint, Uint, S);
@@ -1411,16 +1438,182 @@ static void update_radial_profiles(tbo_case* c, const double* U) {
     }
 }
 
+/* ------------------------------------------------------------- viscous terms */
+/* FluidIdeal::computeMolecularDynamicViscosity (fluid_ideal.cpp:224-231) */
+static double f_mu(const tbo_case* c, double T) {
+    if (c->p.mu_constant_model) return c->p.mu_constant;
+    const double Tr = c->p.mu_t_ref, Sc = c->p.mu_s;
+    return c->p.mu_ref * pow(T / Tr, 1.5) * ((Tr + Sc) / (T + Sc));
+}
+
+/* Solver::setMomentumSolutionOnViscousWalls with the wall velocity (solver.cpp:3147-3185),
+ * as preprocessSolution applies it (solver.cpp:3189-3204) */
+static void wall_momentum(const tbo_case* c, double* U) {
+    const long N = c->N;
+    for (long n = 0; n < N; ++n) {
+        if (!c->nsw[n]) continue;
+        const double rho = U[n];
+        for (int e = 1; e <= 3; ++e) U[e * N + n] = rho * c->wallv[3 * n + e - 1];
+    }
+}
+
+/* Solver::computeSolutionGradient (solver.cpp:3033-3058): velocity and temperature of
+ * every node, then computeGradientsGreenGauss4 (math_utils.cpp:237-392): face values
+ * interpolated by the distances of the two nodes to the face centre, the node value on
+ * boundary faces, summed over the faces W, E, N, S, B, T and divided by the volume. */
+static void solution_gradient(tbo_case* c, const double* U) {
+    const long N = c->N, NI = c->p.ni, NJ = c->p.nj, NK = c->p.nk;
+    double* f = (double*)malloc(sizeof(double) * 4 * N);
+    const double k = (c->p.gamma - 1.0) / c->p.r_gas;
+    for (long n = 0; n < N; ++n) {
+        double rho = U[n];
+        double ux = U[N + n] / rho, uy = U[2 * N + n] / rho, uz = U[3 * N + n] / rho, et = U[4 * N + n] / rho;
+        f[n] = ux; f[N + n] = uy; f[2 * N + n] = uz;
+        f[3 * N + n] = (et - (ux * ux + uy * uy + uz * uz) * 0.5) * k;
+    }
+#pragma omp parallel for collapse(2) schedule(static)
+    for (long i = 0; i < NI; ++i)
+        for (long j = 0; j < NJ; ++j)
+            for (long kk = 0; kk < NK; ++kk) {
+                const long n = IDX(c, i, j, kk);
+                const v3 p0 = vtx(c, i, j, kk);
+                /* neighbour, face centre and outward face vector of W E N S B T */
+                long nb[6];
+                const double* mid[6];
+                v3 sf[6];
+                nb[0] = (i == 0) ? -1 : IDX(c, i - 1, j, kk);        mid[0] = &c->cI[FI(i, j, kk)];
+                nb[1] = (i == NI - 1) ? -1 : IDX(c, i + 1, j, kk);   mid[1] = &c->cI[FI(i + 1, j, kk)];
+                nb[2] = (j == NJ - 1) ? -1 : IDX(c, i, j + 1, kk);   mid[2] = &c->cJ[FJ(i, j + 1, kk)];
+                nb[3] = (j == 0) ? -1 : IDX(c, i, j - 1, kk);        mid[3] = &c->cJ[FJ(i, j, kk)];
+                nb[4] = (kk == 0) ? -1 : IDX(c, i, j, kk - 1);       mid[4] = &c->cK[FK(i, j, kk)];
+                nb[5] = (kk == NK - 1) ? -1 : IDX(c, i, j, kk + 1);  mid[5] = &c->cK[FK(i, j, kk + 1)];
+                sf[0] = neg3(P3(&c->sI[FI(i, j, kk)]));
+                sf[1] = P3(&c->sI[FI(i + 1, j, kk)]);
+                sf[2] = P3(&c->sJ[FJ(i, j + 1, kk)]);
+                sf[3] = neg3(P3(&c->sJ[FJ(i, j, kk)]));
+                sf[4] = neg3(P3(&c->sK[FK(i, j, kk)]));
+                sf[5] = P3(&c->sK[FK(i, j, kk + 1)]);
+                v3 g[4] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+                for (int q = 0; q < 6; ++q) {
+                    double w = 0.0;
+                    if (nb[q] >= 0) {
+                        const v3 p1 = mk3(c->x[nb[q]], c->y[nb[q]], c->z[nb[q]]);
+                        const v3 m = P3(mid[q]);
+                        const double d0 = mag3(sub3(m, p0)), d1 = mag3(sub3(m, p1));
+                        w = d0 / (d0 + d1);
+                    }
+                    for (int v = 0; v < 4; ++v) {
+                        const double fc = f[v * N + n];
+                        const double sv = (nb[q] < 0) ? fc : fc + (f[v * N + nb[q]] - fc) * w;
+                        g[v] = add3(g[v], mul3(sf[q], sv));
+                    }
+                }
+                const double iv = 1.0 / c->vol[n];
+                for (int v = 0; v < 4; ++v) {
+                    c->grad[(3 * v) * N + n] = g[v].x * iv;
+                    c->grad[(3 * v + 1) * N + n] = g[v].y * iv;
+                    c->grad[(3 * v + 2) * N + n] = g[v].z * iv;
+                }
+            }
+    free(f);
+}
+
+void tbo_gradients(const tbo_case* c, double* out) {
+    if (c->grad) memcpy(out, c->grad, sizeof(double) * 12 * c->N);
+}
+
+/* Solver::computeViscousFlux (solver.cpp:2173-2226) with no eddy viscosity
+ * (turbulence_model_none.hpp:49-51) and computeViscousStressTensor (math_utils.cpp:593-608):
+ * flux per unit area through the face direction S/|S| */
+static st5 viscous_flux(const tbo_case* c, st5 U, v3 gx, v3 gy, v3 gz, v3 gt, v3 S) {
+    st5 W = prim(U);
+    v3 vel = mk3(W.v[1], W.v[2], W.v[3]);
+    double T = f_T(c, W.v[0], vel, W.v[4]);
+    double mu = f_mu(c, T) + 0.0;
+    double kap = c->p.cp_visc * f_mu(c, T) / c->p.prandtl + 0.0;
+    double lam = -2.0 / 3.0 * mu;
+    double div = gx.x + gy.y + gz.z;
+    double txx = lam * div + 2.0 * mu * gx.x;
+    double tyy = lam * div + 2.0 * mu * gy.y;
+    double tzz = lam * div + 2.0 * mu * gz.z;
+    double txy = mu * (gx.y + gy.x), txz = mu * (gx.z + gz.x), tyz = mu * (gy.z + gz.y);
+    v3 tX = mk3(txx, txy, txz), tY = mk3(txy, tyy, tyz), tZ = mk3(txz, tyz, tzz);
+    double thx = dot3(tX, vel) + kap * gt.x;
+    double thy = dot3(tY, vel) + kap * gt.y;
+    double thz = dot3(tZ, vel) + kap * gt.z;
+    v3 n = div3(S, mag3(S));
+    st5 F = {{0.0, dot3(tX, n), dot3(tY, n), dot3(tZ, n), thx * n.x + thy * n.y + thz * n.z}};
+    return F;
+}
+
+void tbo_viscous_flux(const tbo_case* c, const double* U, const double* g, const double* S, double* out) {
+    st5 u;
+    memcpy(u.v, U, 40);
+    st5 F = viscous_flux(c, u, mk3(g[0], g[1], g[2]), mk3(g[3], g[4], g[5]), mk3(g[6], g[7], g[8]),
+                         mk3(g[9], g[10], g[11]), mk3(S[0], S[1], S[2]));
+    memcpy(out, F.v, 40);
+}
+
+/* Solver::computeViscousFluxResiduals (solver.cpp:2060-2170): internal faces only, state
+ * and gradients averaged over the two nodes, flux*|S|*(-1) added to the left node and
+ * subtracted from the right one; faces of a grid line in increasing order */
+static void viscous_residuals(tbo_case* c, int d, const double* U, double* R) {
+    const long NI = c->p.ni, NJ = c->p.nj, NK = c->p.nk, N = c->N;
+    const long A = (d == 0) ? NJ : NI, B = (d == 2) ? NJ : NK, L = (d == 0) ? NI : ((d == 1) ? NJ : NK);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (long a = 0; a < A; ++a)
+        for (long b = 0; b < B; ++b)
+            for (long f = 1; f < L; ++f) {
+                long il, jl, kl, ir, jr, kr;
+                v3 S;
+                if (d == 0) { ir = f; jr = a; kr = b; il = f - 1; jl = a; kl = b; S = P3(&c->sI[FI(ir, jr, kr)]); }
+                else if (d == 1) { ir = a; jr = f; kr = b; il = a; jl = f - 1; kl = b; S = P3(&c->sJ[FJ(ir, jr, kr)]); }
+                else { ir = a; jr = b; kr = f; il = a; jl = b; kl = f - 1; S = P3(&c->sK[FK(ir, jr, kr)]); }
+                const long l = IDX(c, il, jl, kl), r = IDX(c, ir, jr, kr);
+                st5 Ua;
+                for (int v = 0; v < 5; ++v) Ua.v[v] = (U[v * N + l] + U[v * N + r]) * 0.5;
+                v3 g[4];
+                for (int v = 0; v < 4; ++v) {
+                    const double* G = c->grad + (long)(3 * v) * N;
+                    g[v] = mul3(add3(mk3(G[l], G[N + l], G[2 * N + l]), mk3(G[r], G[N + r], G[2 * N + r])), 0.5);
+                }
+                st5 F = viscous_flux(c, Ua, g[0], g[1], g[2], g[3], S);
+                const double m = mag3(S);
+                for (int v = 0; v < 5; ++v) {
+                    const double fm = F.v[v] * m * (-1.0);
+                    R[v * N + l] += fm;
+                    R[v * N + r] -= fm;
+                }
+            }
+}
+
+/* Solver::enforceNoSlipWallsOnResiduals (solver.cpp:1855-1862): momentum residual of a
+ * wall node = its density residual times the zero wall effect */
+static void noslip_residuals(const tbo_case* c, double* R) {
+    const long N = c->N;
+    for (long n = 0; n < N; ++n)
+        if (c->nsw[n])
+            for (int e = 1; e <= 3; ++e) R[e * N + n] = R[n] * 0.0;
+}
+
+/* Solver::computeResiduals (solver.cpp:1782-1833) */
 static void compute_residuals(tbo_case* c, const double* U, double* R, long it) {
     memset(R, 0, sizeof(double) * 5 * c->N);
     advection_residuals(c, 0, U, R, it);
     if (c->p.topology != TBO_TOPO_1D) advection_residuals(c, 1, U, R, it);
     if (c->p.topology == TBO_TOPO_3D) advection_residuals(c, 2, U, R, it);
+    if (c->p.viscous) {
+        solution_gradient(c, U);
+        viscous_residuals(c, 0, U, R);
+        viscous_residuals(c, 1, U, R);
+        if (c->p.topology == TBO_TOPO_3D) viscous_residuals(c, 2, U, R);
+    }
     source_residuals(c, U, R);
     if (c->periodic) periodic_residuals(c, R);
+    if (c->p.viscous) noslip_residuals(c, R);
 }
 
-/* solver.cpp:1531-1630 (inviscid) */
+/* solver.cpp:1531-1630 */
 void tbo_timestep(tbo_case* c, const double* U, double* dt) {
     const long NI = c->p.ni, NJ = c->p.nj, NK = c->p.nk;
     const long DJ = NJ + 1, DK = NK + 1;
@@ -1447,8 +1640,11 @@ void tbo_timestep(tbo_case* c, const double* U, double* dt) {
                 double lam = (ui + a) / dsI;
                 if (c->p.topology != TBO_TOPO_1D) lam += (uj + a) / dsJ;
                 if (c->p.topology == TBO_TOPO_3D) lam += (uk + a) / dsK;
-                double nu = 0.0 / W.v[0], nut = 0.0 / W.v[0];
+                double mu = c->p.viscous ? f_mu(c, f_T(c, W.v[0], u, W.v[4])) : 0.0;
+                double nu = mu / W.v[0], nut = 0.0 / W.v[0];
                 double invH2 = 1.0 / (dsI * dsI);
+                if (c->p.topology != TBO_TOPO_1D) invH2 += 1.0 / (dsJ * dsJ);
+                if (c->p.topology == TBO_TOPO_3D) invH2 += 1.0 / (dsK * dsK);
                 double lv = 4.0 * (nu + nut) * invH2;
                 dt[idx] = c->p.cfl / (lam + lv);
             }
@@ -1662,6 +1858,33 @@ tbo_case* tbo_create(const tbo_params* p, const double* x, const double* y, cons
     }
     c->U0 = dalloc(5 * N); c->U = dalloc(5 * N); c->R = dalloc(5 * N); c->dt = dalloc(N);
     c->finv = dalloc(3 * N); c->fvisc = dalloc(3 * N); c->dev = dalloc(N);
+    /* no-slip wall nodes and velocities, patches in file order (solver.cpp:369-389,
+     * boundary_base.cpp:227-268); fewer than three values leave the wall at rest */
+    for (int q = 0; q < npatches; ++q) {
+        const tbo_patch* pa = &patches[q];
+        if (pa->type != TBO_BC_NO_SLIP) continue;
+        if (!c->nsw) { c->nsw = (int*)calloc((size_t)N, sizeof(int)); c->wallv = dalloc(3 * N); }
+        long r0[3] = {pa->i0, pa->j0, pa->k0}, r1[3] = {pa->i1, pa->j1, pa->k1}, nn[3] = {NI, NJ, NK};
+        for (int d = 0; d < 3; ++d)
+            if (r0[d] == r1[d]) {
+                if (r0[d] == 0) r1[d] = 1;
+                else { r0[d] = nn[d] - 1; r1[d] = nn[d]; }
+                break;
+            }
+        double wv[3] = {0.0, 0.0, 0.0};
+        if (pa->nvalues >= 3) { wv[0] = pa->values[0]; wv[1] = pa->values[1]; wv[2] = pa->values[2]; }
+        for (long i = r0[0]; i < r1[0]; ++i)
+            for (long j = r0[1]; j < r1[1]; ++j)
+                for (long k = r0[2]; k < r1[2]; ++k) {
+                    long n = IDX(c, i, j, k);
+                    c->nsw[n] = 1;
+                    c->wallv[3 * n] = wv[0]; c->wallv[3 * n + 1] = wv[1]; c->wallv[3 * n + 2] = wv[2];
+                }
+    }
+    if (p->viscous) {
+        if (!c->nsw) { c->nsw = (int*)calloc((size_t)N, sizeof(int)); c->wallv = dalloc(3 * N); }
+        c->grad = dalloc(12 * N);
+    }
     return c;
 }
 
@@ -1671,10 +1894,10 @@ void tbo_destroy(tbo_case* c) {
     double* ptrs[] = {c->x, c->y, c->z, c->dual, c->sI, c->sJ, c->sK, c->cI, c->cJ, c->cK, c->vol,
                       c->radius, c->theta, c->gradb, c->blockage, c->nax, c->nrad, c->ntan, c->rpm,
                       c->present, c->nblades, c->metal, c->slen, c->beta0, c->kp, c->U0, c->U, c->R,
-                      c->dt, c->finv, c->fvisc, c->dev, c->pivot, c->stall};
+                      c->dt, c->finv, c->fvisc, c->dev, c->pivot, c->stall, c->wallv, c->grad};
     for (size_t q = 0; q < sizeof(ptrs) / sizeof(ptrs[0]); ++q) free(ptrs[q]);
     for (int q = 0; q < c->nprof; ++q) { free(c->prof_p[q]); free(c->prof_r[q]); }
-    free(c->mapI); free(c->mapJ); free(c->mapK); free(c->patches);
+    free(c->mapI); free(c->mapJ); free(c->mapK); free(c->patches); free(c->nsw);
     free(c);
 }
 
@@ -1742,11 +1965,13 @@ void tbo_init_state(tbo_case* c) {
         c->U0[4 * c->N + n] = rho * et;
     }
     init_profiles(c);
+    if (c->p.viscous) wall_momentum(c, c->U0);   /* preprocessSolution before the loop, solver.cpp:1324 */
 }
 
 void tbo_set_state(tbo_case* c, const double* U, int reset_profiles) {
     memcpy(c->U0, U, sizeof(double) * 5 * c->N);
     if (reset_profiles) init_profiles(c);
+    if (c->p.viscous) wall_momentum(c, c->U0);   /* preprocessSolution before the loop, solver.cpp:1324 */
 }
 void tbo_get_state(const tbo_case* c, double* U) { memcpy(U, c->U0, sizeof(double) * 5 * c->N); }
 
@@ -1790,6 +2015,7 @@ long tbo_run(tbo_case* c, long n, double* logres, double* turbo, int nthreads) {
         tbo_timestep(c, c->U0, c->dt);
         memcpy(c->U, c->U0, sizeof(double) * 5 * N);
         update_radial_profiles(c, c->U);
+        if (c->p.viscous) wall_momentum(c, c->U);   /* preprocessSolution, solver.cpp:3189-3204 */
         for (int s = 0; s < c->p.n_stages; ++s) {
             compute_residuals(c, c->U, c->R, it);
             update_solution(c, c->U0, c->U, c->R, c->p.rk[s]);

@@ -23,7 +23,7 @@ enum { TBO_DT_LOCAL = 0, TBO_DT_GLOBAL = 1, TBO_DT_FIXED = 2 };
 enum { TBO_BFM_NONE = 0, TBO_BFM_HALL = 1, TBO_BFM_LIFT_DRAG = 2, TBO_BFM_BLOCKAGE = 3, TBO_BFM_THOLLET = 4 };
 enum { TBO_BC_INLET = 0, TBO_BC_OUTLET = 1, TBO_BC_RADIAL_EQ = 2, TBO_BC_WALL = 3,
        TBO_BC_PERIODIC = 4, TBO_BC_WEDGE = 5, TBO_BC_THROTTLE = 6, TBO_BC_MASS_FLOW = 7,
-       TBO_BC_TRANSPARENT = 8 };
+       TBO_BC_TRANSPARENT = 8, TBO_BC_NO_SLIP = 9 };
 
 typedef struct {
     long ni, nj, nk;
@@ -55,6 +55,13 @@ typedef struct {
     int greitzer;                    /* ENABLE_GREITZER_MODELING (fixed time step) */
     double greitzer_volume;          /* GREITZER_PLENUM_VOLUME */
     double greitzer_ambient;         /* GREITZER_AMBIENT_PRESSURE (101325) */
+    /* laminar viscous terms (VISCOSITY_ACTIVE, config.hpp:341-377, fluid_ideal.cpp:209-235) */
+    int viscous;                     /* VISCOSITY_ACTIVE */
+    int mu_constant_model;           /* FLUID_VISCOSITY_MODEL constant (else Sutherland) */
+    double mu_constant;              /* FLUID_MU_CONSTANT */
+    double mu_ref, mu_t_ref, mu_s;   /* FLUID_SUTHERLAND_MU_REF / _T_REF / _S */
+    double cp_visc;                  /* FLUID_CP (thermal conductivity only) */
+    double prandtl;                  /* FLUID_PRANDTL_NUMBER */
 } tbo_params;
 
 /* A fluid look-up table (include/fluid_real.hpp:11-60 Table2D): nx*ny values, row-major
@@ -127,6 +134,13 @@ void tbo_timestep(tbo_case* c, const double* U, double* dt);
 /* Geometry: which 0..2 surfaces I/J/K, 3..5 centres I/J/K, 6 volumes, 7 dual nodes,
  * 8 blockage gradient.  Same shapes as the reference Mesh arrays. */
 int tbo_geometry(const tbo_case* c, int which, double* out);
+
+/* Velocity and temperature gradients of the last residual evaluation of a viscous case:
+ * 12*N, [ux, uy, uz, T][x, y, z][N] (Solver::computeSolutionGradient, solver.cpp:3033-3058). */
+void tbo_gradients(const tbo_case* c, double* out);
+/* Solver::computeViscousFlux of a conserved state U (5), gradients g (ux, uy, uz, T as
+ * xyz triples, 12) and face vector S: the flux per unit area (5). */
+void tbo_viscous_flux(const tbo_case* c, const double* U, const double* g, const double* S, double* out);
 
 /* Side outputs of the last source evaluation: finv, fvisc (3*N xyz), deviation (N). */
 void tbo_side_outputs(const tbo_case* c, double* finv, double* fvisc, double* dev, double* dt);
