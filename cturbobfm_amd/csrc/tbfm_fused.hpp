@@ -489,6 +489,7 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
                 }
                 R[v] = r;
             }
+            if (c.visc) viscous_add(c, i, j, k, n, R);
             if (srcOn && i >= c.bi0 && i < c.bi1) {
                 // geometric, Gong, body force, subtracted in that order
                 const int sl = (k * NJ + j) * (c.bi1 - c.bi0) + (i - c.bi0);
@@ -498,6 +499,9 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
 #pragma unroll
                     for (int v = 0; v < 5; ++v) R[v] -= gld(c.srcbuf + (long)(5 * t + v) * ns, (unsigned)sl);
             }
+            // no-slip walls after the periodic closure (solver.cpp:1828-1831): here for
+            // the nodes outside the periodic planes, in k_periodic for theirs
+            if (c.visc && !parked && c.nsw[n]) R[1] = R[2] = R[3] = R[0] * 0.0;
             if (parked) {
                 double* rp = c.rper + (k == 0 ? 0 : 5) * NIJ;
 #pragma unroll
@@ -550,7 +554,7 @@ __global__ void __launch_bounds__(NT) k_stage(const CaseDev* __restrict__ cases,
     const CaseDev* __restrict__ cp = cases + w.cs;
     if (*cp->stop) return;
     StageOps st;
-    st.Uin = cp->U[La.in];
+    st.Uin = stage_input(*cp, La);
     st.U0 = cp->U[0];
     st.Uout = cp->U[La.out];
     st.alpha = cp->alpha[La.stage];
@@ -594,6 +598,10 @@ __global__ void __launch_bounds__(256) k_periodic(Launch La) {
 #pragma unroll
         for (int v = 0; v < 5; ++v) back[v] = avg[v];
         rotx(back, c.per_c, c.per_s);
+        if (c.visc) {
+            if (c.nsw[n0]) avg[1] = avg[2] = avg[3] = avg[0] * 0.0;
+            if (c.nsw[n1]) back[1] = back[2] = back[3] = back[0] * 0.0;
+        }
         if (st.rmode) {
 #pragma unroll
             for (int v = 0; v < 5; ++v) {

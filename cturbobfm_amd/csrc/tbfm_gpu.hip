@@ -90,7 +90,7 @@ struct Plan {
     CaseDev* dcases = nullptr;        // device case table
     size_t dcap = 0;
     bool ownsCases = false;
-    Table dt, mn, prof, bnd, per, norm, start, pre, turbo;
+    Table dt, mn, prof, bnd, per, norm, start, pre, turbo, visc, wpre;
     std::vector<StageGroup> groups;
     int nstages = 0;
     int fluids = 0;                   // bit 0: an ideal-gas member, bit 1: a table-fluid member
@@ -115,6 +115,8 @@ struct tbfm_case {
     std::vector<double> rk;
     Geometry geo;
     std::vector<Patch> patches;
+    std::vector<int> nswHost;         // no-slip wall nodes (device layout), empty when none
+    std::vector<double> wallvHost;    // [3][N] their wall velocities
     std::vector<void*> owned;
     size_t dev_bytes = 0;             // device memory held by the case
     CaseDev c{};
@@ -339,7 +341,8 @@ void setup(tbfm_case* cs, const GridInput& grid) {
                 profOfPatch[q] = nprof++;
                 break;
             }
-            case PatchType::Wall: pd.type = dev::PT_WALL; break;
+            case PatchType::Wall:
+            case PatchType::NoSlipWall: pd.type = dev::PT_WALL; break;   // solver.cpp:391-399
             case PatchType::Transparent: pd.type = dev::PT_TRANSPARENT; break;
             default: pd.type = dev::PT_FAKE; break;
         }
@@ -658,6 +661,99 @@ void setup(tbfm_case* cs, const GridInput& grid) {
     c.s_beta0 = upload(s_beta0, own); c.s_kp = upload(s_kp, own);
     c.s_stwl = upload(s_stwl, own); c.s_piv = upload(s_piv, own); c.s_stall = upload(s_stall, own);
     c.axi_gf = upload(axi_gf, own);
+
+    // ---- no-slip walls and laminar viscous terms
+    // wall nodes and velocities of the no_slip_wall patches in file order, the last patch
+    // holding a node setting its velocity (solver.cpp:369-389, boundary_base.cpp:227-268);
+    // fewer than three values leave the wall at rest
+    bool moving = false;
+    for (const Patch& p : cs->patches) {
+        if (p.type != PatchType::NoSlipWall) continue;
+        if (cs->nswHost.empty()) { cs->nswHost.assign(N, 0); cs->wallvHost.assign(3 * N, 0.0); }
+        long r0[3] = {p.i0, p.j0, p.k0}, r1[3] = {p.i1, p.j1, p.k1};
+        const long nn[3] = {NI, NJ, NK};
+        for (int d = 0; d < 3; ++d)
+            if (r0[d] == r1[d]) {
+                if (r0[d] == 0) r1[d] = 1;
+                else { r0[d] = nn[d] - 1; r1[d] = nn[d]; }
+                break;
+            }
+        double u[3] = {0.0, 0.0, 0.0};
+        if (p.values.size() >= 3) { u[0] = p.values[0]; u[1] = p.values[1]; u[2] = p.values[2]; }
+        for (long i = r0[0]; i < r1[0]; ++i)
+            for (long j = r0[1]; j < r1[1]; ++j)
+                for (long k = r0[2]; k < r1[2]; ++k) {
+                    const long n = (k * NJ + j) * NI + i;
+                    cs->nswHost[n] = 1;
+                    for (int e = 0; e < 3; ++e) cs->wallvHost[e * N + n] = u[e];
+                }
+        moving = moving || u[0] != 0.0 || u[1] != 0.0 || u[2] != 0.0;
+    }
+    if (!cs->nswHost.empty()) {
+        if (cs->fluid == FluidModel::Real)
+            throw std::runtime_error("no_slip_wall patches need FLUID_MODEL = ideal on the GPU path");
+        c.nsw = upload(cs->nswHost, own);
+        c.wallv = upload(cs->wallvHost, own);
+    }
+    c.visc = cfg.getBool("VISCOSITY_ACTIVE", false) ? 1 : 0;
+    if (cfg.getBool("TURBULENCE_ACTIVE", false))
+        throw std::runtime_error("TURBULENCE_ACTIVE is outside the GPU residual path");
+    if (c.visc) {
+        if (cs->fluid == FluidModel::Real)
+            throw std::runtime_error("VISCOSITY_ACTIVE needs FLUID_MODEL = ideal on the GPU path");
+        if (cs->topo == Topology::Axisymmetric || cs->topo == Topology::OneD)
+            throw std::runtime_error("VISCOSITY_ACTIVE is on the GPU path for 2D and 3D topologies");
+        // Config::getViscosityModel / getFluidMuConstant (config.cpp:575-597),
+        // FluidIdeal::setTransportProperties (fluid_ideal.cpp:209-222)
+        std::string vm = cfg.has("FLUID_VISCOSITY_MODEL") ? cfg.get("FLUID_VISCOSITY_MODEL") : std::string("None");
+        if (vm == "None") {
+            c.mu_const = (cfg.has("FLUID_MU_CONSTANT") || cfg.has("FLUID_CONSTANT_MU")) ? 1 : 0;
+        } else {
+            std::transform(vm.begin(), vm.end(), vm.begin(), [](unsigned char ch) { return (char)std::tolower(ch); });
+            if (vm != "sutherland" && vm != "constant") throw std::runtime_error("Unknown viscosity model: " + vm);
+            c.mu_const = (vm == "constant") ? 1 : 0;
+        }
+        if (c.mu_const) {
+            c.mu_c = cfg.has("FLUID_MU_CONSTANT") ? cfg.getFloat("FLUID_MU_CONSTANT") : cfg.getFloat("FLUID_CONSTANT_MU");
+        } else {
+            c.mu_ref = cfg.getFloat("FLUID_SUTHERLAND_MU_REF");
+            c.mu_tref = cfg.getFloat("FLUID_SUTHERLAND_T_REF");
+            c.mu_s = cfg.getFloat("FLUID_SUTHERLAND_S");
+        }
+        c.cp_v = cfg.getFloat("FLUID_CP");
+        c.pr = cfg.getFloat("FLUID_PRANDTL_NUMBER");
+        if (cs->nswHost.empty()) {   // the residual enforcement reads the wall flags
+            c.nsw = upload(std::vector<int>(N, 0), own);
+            c.wallv = upload(std::vector<double>(3 * N, 0.0), own);
+        }
+        c.wall_moving = moving ? 1 : 0;
+        // face interpolation weights d0 / (d0 + d1), d the distance of a node to the face
+        // centre (math_utils.cpp:270-360), for faces W E N S B T of every node
+        std::vector<double> gw(6 * N, 0.0);
+        const std::vector<double>&X = grid.x, &Y = grid.y, &Z = grid.z;
+        auto rid = [&](long i, long j, long k) { return (i * NJ + j) * NK + k; };
+        for (long i = 0; i < NI; ++i)
+            for (long j = 0; j < NJ; ++j)
+                for (long k = 0; k < NK; ++k) {
+                    const long r = rid(i, j, k), n = (k * NJ + j) * NI + i;
+                    auto weight = [&](const double* mid, long rn) {
+                        const double ax = mid[0] - X[r], ay = mid[1] - Y[r], az = mid[2] - Z[r];
+                        const double bx = mid[0] - X[rn], by = mid[1] - Y[rn], bz = mid[2] - Z[rn];
+                        const double d0 = std::sqrt(ax * ax + ay * ay + az * az);
+                        const double d1 = std::sqrt(bx * bx + by * by + bz * bz);
+                        return d0 / (d0 + d1);
+                    };
+                    if (i > 0) gw[0 * N + n] = weight(&g.centI[((i * NJ + j) * NK + k) * 3], rid(i - 1, j, k));
+                    if (i < NI - 1) gw[1 * N + n] = weight(&g.centI[(((i + 1) * NJ + j) * NK + k) * 3], rid(i + 1, j, k));
+                    if (j < NJ - 1) gw[2 * N + n] = weight(&g.centJ[((i * (NJ + 1) + j + 1) * NK + k) * 3], rid(i, j + 1, k));
+                    if (j > 0) gw[3 * N + n] = weight(&g.centJ[((i * (NJ + 1) + j) * NK + k) * 3], rid(i, j - 1, k));
+                    if (k > 0) gw[4 * N + n] = weight(&g.centK[((i * NJ + j) * (NK + 1) + k) * 3], rid(i, j, k - 1));
+                    if (k < NK - 1) gw[5 * N + n] = weight(&g.centK[((i * NJ + j) * (NK + 1) + k + 1) * 3], rid(i, j, k + 1));
+                }
+        c.gw = upload(gw, own);
+        c.grad = dalloc<double>(12 * N, own);
+        c.vf = dalloc<double>(12 * N, own);
+    }
     c.finv = dalloc<double>(3 * N, own);
     c.fvisc = dalloc<double>(3 * N, own);
     c.dev = dalloc<double>(N, own);
@@ -845,7 +941,8 @@ void freePlan(Plan& P) {
     if (P.stream) hipStreamSynchronize(P.stream);
     if (P.graph) hipGraphExecDestroy(P.graph);
     P.graph = nullptr;
-    for (Table* t : {&P.dt, &P.mn, &P.prof, &P.bnd, &P.per, &P.norm, &P.start, &P.pre, &P.turbo}) freeTable(*t);
+    for (Table* t : {&P.dt, &P.mn, &P.prof, &P.bnd, &P.per, &P.norm, &P.start, &P.pre, &P.turbo, &P.visc, &P.wpre})
+        freeTable(*t);
     for (auto& g : P.groups) freeTable(g.t);
     P.groups.clear();
     if (P.ownsCases && P.dcases) hipFree(P.dcases);
@@ -898,7 +995,7 @@ std::vector<dev::Work> xcdOrder(const std::vector<dev::Work>& seq) {
 
 // Builds the work tables of every kernel from the members' current tiling.
 void buildTables(Plan& P) {
-    std::vector<dev::Work> dt, mn, prof, bnd, per, norm, src, turbo;
+    std::vector<dev::Work> dt, mn, prof, bnd, per, norm, src, turbo, visc, wpre;
     for (auto& g : P.groups) freeTable(g.t);
     P.groups.clear();
     P.fluids = 0;
@@ -923,6 +1020,8 @@ void buildTables(Plan& P) {
         for (int b = 0; b < c.nparts_per; ++b) per.push_back({m, b, 0, 0});
         for (int b = 0; b < (c.ns + 255) / 256; ++b) src.push_back({m, b, 0, (cs->topo == Topology::ThreeD) ? 1 : 2});
         norm.push_back({m, 0, 0, 0});
+        for (long b = 0; c.visc && b < (cs->N + 255) / 256; ++b) visc.push_back({m, (int)b, 0, 0});
+        for (long b = 0; c.wall_moving && b < (cs->N + 255) / 256; ++b) wpre.push_back({m, (int)b, 0, 0});
         const int sch = (cs->scheme == Scheme::JST) ? 0 : 1;
         const bool three = (cs->topo == Topology::ThreeD), mixed = (cs->precision == Precision::Mixed);
         const bool real = (c.real != 0);
@@ -953,6 +1052,8 @@ void buildTables(Plan& P) {
     uploadTable(P.bnd, bnd);
     uploadTable(P.per, per);
     uploadTable(P.norm, norm);
+    uploadTable(P.visc, xcdOrder(visc));
+    uploadTable(P.wpre, xcdOrder(wpre));
     // combined launches: profiles then time-step blocks; boundary then source blocks.
     // Profile and boundary blocks go first, round robin over the XCDs (the latency-bound
     // inlet bisection spread out); the time-step and source blocks follow in XCD order
@@ -985,6 +1086,7 @@ dev::Launch launchArg(const Plan& P, const Table& t, int in, int out, int stage,
     L.cases = P.dcases;
     L.work = t.d;
     L.in = in; L.out = out; L.stage = stage; L.last = last; L.rmode = rmode;
+    L.pre = 0;
     return L;
 }
 
@@ -1009,10 +1111,11 @@ void launchStageS(const Plan& P, const StageGroup& g, const dev::Launch& L) {
     }
 }
 
-void launchStageGroups(const Plan& P, int in, int out, int stage, int last, int rmode) {
+void launchStageGroups(const Plan& P, int in, int out, int stage, int last, int rmode, int pre = 0) {
     for (const StageGroup& g : P.groups) {
         if (g.t.n == 0) continue;
-        const dev::Launch L = launchArg(P, g.t, in, out, stage, last, rmode);
+        dev::Launch L = launchArg(P, g.t, in, out, stage, last, rmode);
+        L.pre = pre;
         if (g.sch == 0) {
             if (g.three) launchStageS<0, true>(P, g, L); else launchStageS<0, false>(P, g, L);
         } else {
@@ -1044,9 +1147,10 @@ void launchProfiles(const Plan& P, int in) {
 }
 
 // k_prestage over a table of boundary and / or source blocks.
-void launchPre(const Plan& P, const Table& t, int in, int out, int stage, int last, int rmode) {
+void launchPre(const Plan& P, const Table& t, int in, int out, int stage, int last, int rmode, int pre = 0) {
     if (!t.n) return;
-    const dev::Launch L = launchArg(P, t, in, out, stage, last, rmode);
+    dev::Launch L = launchArg(P, t, in, out, stage, last, rmode);
+    L.pre = pre;
     if (P.fluids & 1) hipLaunchKernelGGL(dev::k_prestage<0>, dim3((unsigned)t.n), dim3(256), 0, P.stream, L);
     if (P.fluids & 2) hipLaunchKernelGGL(dev::k_prestage<1>, dim3((unsigned)t.n), dim3(256), 0, P.stream, L);
 }
@@ -1065,9 +1169,22 @@ void launchStage(Plan& P, int s, int in, int out, bool last, bool rmode, tbfm_ca
         }
         HIPCHK(hipEventRecord(prof->pool[2 * prof->pool_used], P.stream));
     }
+    // viscous cases: the first stage's wall momentum, then (after the boundary fluxes and
+    // slab source terms) the gradients and viscous face fluxes of the stage input
+    dev::Launch V = launchArg(P, P.visc, in, out, s, last ? 1 : 0, rmode ? 1 : 0);
+    V.pre = (s == 0 && !rmode) ? 1 : 0;
+    if (V.pre && P.wpre.n) {
+        dev::Launch Wl = V;
+        Wl.work = P.wpre.d;
+        hipLaunchKernelGGL(dev::k_wallpre, dim3((unsigned)P.wpre.n), dim3(256), 0, P.stream, Wl);
+    }
     // boundary fluxes and slab source terms, then the fused stage kernel (timed together)
-    launchPre(P, P.pre, in, out, s, last ? 1 : 0, rmode ? 1 : 0);
-    launchStageGroups(P, in, out, s, last ? 1 : 0, rmode ? 1 : 0);
+    launchPre(P, P.pre, in, out, s, last ? 1 : 0, rmode ? 1 : 0, V.pre);
+    if (P.visc.n) {
+        hipLaunchKernelGGL(dev::k_vgrad, dim3((unsigned)P.visc.n), dim3(256), 0, P.stream, V);
+        hipLaunchKernelGGL(dev::k_vface, dim3((unsigned)P.visc.n), dim3(256), 0, P.stream, V);
+    }
+    launchStageGroups(P, in, out, s, last ? 1 : 0, rmode ? 1 : 0, V.pre);
     if (prof) {
         HIPCHK(hipEventRecord(prof->pool[2 * prof->pool_used + 1], P.stream));
         prof->pool_used += 1;
@@ -1321,6 +1438,12 @@ int tbfm_gpu_set_state(tbfm_case* c, const double* U) {
             toDevice(c, U + v * c->N, tmp);
             std::copy(tmp.begin(), tmp.end(), dU.begin() + v * c->N);
         }
+        // a viscous case starts from the state with rho * wall velocity on the no-slip
+        // nodes (preprocessSolution before the time loop, solver.cpp:1324)
+        if (c->c.visc && !c->nswHost.empty())
+            for (long n = 0; n < c->N; ++n)
+                if (c->nswHost[n])
+                    for (int e = 1; e <= 3; ++e) dU[e * c->N + n] = dU[n] * c->wallvHost[(e - 1) * c->N + n];
         hipStream_t st = c->solo.stream;
         HIPCHK(hipMemcpyAsync(c->c.U[0], dU.data(), dU.size() * 8, hipMemcpyHostToDevice, st));
         // radial-equilibrium profiles from the new state (solver.cpp:818-834)

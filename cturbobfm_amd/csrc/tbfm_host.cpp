@@ -271,7 +271,8 @@ PatchType patchTypeFromString(const std::string& s) {
     if (k == "inlet") return PatchType::Inlet;
     if (k == "outlet") return PatchType::Outlet;
     if (k == "radial_equilibrium") return PatchType::RadialEquilibrium;
-    if (k == "inviscid_wall" || k == "no_slip_wall") return PatchType::Wall;
+    if (k == "inviscid_wall") return PatchType::Wall;
+    if (k == "no_slip_wall") return PatchType::NoSlipWall;
     if (k == "periodic") return PatchType::Periodic;
     if (k == "wedge") return PatchType::Wedge;
     if (k == "throttle") return PatchType::Throttle;

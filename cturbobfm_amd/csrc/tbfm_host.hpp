@@ -24,7 +24,7 @@ enum class Limiter { None = 0, VanAlbada = 1, VanLeer = 2, MinMod = 3 };
 enum class TimeStepMethod { Local = 0, Global = 1, Fixed = 2 };
 enum class BfmModel { None = 0, Hall = 1, LiftDrag = 2, Blockage = 3, HallThollet = 4 };
 enum class PatchType { Inlet = 0, Outlet = 1, RadialEquilibrium = 2, Wall = 3, Periodic = 4, Wedge = 5,
-                       Throttle = 6, MassFlow = 7, Transparent = 8 };
+                       Throttle = 6, MassFlow = 7, Transparent = 8, NoSlipWall = 9 };
 enum class Precision { FP64 = 0, Mixed = 1 };
 enum class FluidModel { Ideal = 0, Real = 1 };
 

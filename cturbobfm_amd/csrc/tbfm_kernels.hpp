@@ -137,6 +137,16 @@ struct CaseDev {
     ConvDev conv;
     // state buffers by role: 0 = A holds U^(0) at iteration start and receives the last
     // stage in place; stages alternate between B (1) and C (2) in between
+    // laminar viscous terms (VISCOSITY_ACTIVE) and no-slip walls
+    int visc, mu_const, wall_moving;   // wall_moving: a no-slip wall with a nonzero velocity
+    double mu_c, mu_ref, mu_tref, mu_s, cp_v, pr;
+    const int* nsw;                    // N: node on a no_slip_wall patch (null when there is none)
+    const double* wallv;               // [3][N] wall velocity of those nodes
+    const double* gw;                  // [6][N] face interpolation weights W E N S B T
+    double* grad;                      // [ux, uy, uz, T][x, y, z][N] stage-input gradients
+    double* vf;                        // [I, J, K][mom x, y, z, energy][N] viscous flux * |S| of
+                                       // the face before each node (index i, j or k)
+
     double* U[3];
     int nstages;
     double alpha[4];
@@ -171,7 +181,15 @@ struct Launch {
     int stage;             // RK stage index
     int last;              // last stage: norms and side outputs
     int rmode;             // write the residual instead of the update (residual entry point)
+    int pre;               // first stage of an iteration: moving-wall viscous cases read the
+                           // preprocessed input (U[2], wall momentum = rho * wall velocity)
 };
+
+// The state a stage reads: the iteration's first stage of a viscous case with a moving
+// no-slip wall reads the copy whose wall momentum was reset (solver.cpp:1365, 3189-3204).
+__device__ __forceinline__ const double* stage_input(const CaseDev& c, const Launch& La) {
+    return (La.pre && c.wall_moving) ? c.U[2] : c.U[La.in];
+}
 
 // RK stage s of ns reads role stage_in(s) and writes role stage_out(s, ns):
 // A -> B -> C -> B -> A (4 stages), A -> B -> C -> A (3 stages).

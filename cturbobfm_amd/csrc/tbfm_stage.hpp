@@ -2,6 +2,7 @@
 // fluxes, the fused residual + RK stage kernel, periodic closure and norm reductions.
 #pragma once
 #include "tbfm_physics.hpp"
+#include "tbfm_viscous.hpp"
 
 namespace tbfm {
 namespace dev {
@@ -142,7 +143,7 @@ __device__ inline void sources_block(const CaseDev& c, const Launch& La, int bx)
     const int SW = c.bi1 - c.bi0, NJ = c.nj;
     const int i = c.bi0 + s % SW, j = (s / SW) % NJ, k = s / (SW * NJ);
     const int n = nid(c, i, j, k);
-    const double* Uin = c.U[La.in];
+    const double* Uin = stage_input(c, La);
     const Gas<FL> gs(c);
     double U[5], W[5];
     load5(Uin, (int)c.N, n, U);
@@ -200,7 +201,13 @@ __device__ inline void timestep_block(const CaseDev& c, const double* U, int bx)
                 lam += (uk + a) / dsK;
             }
             double invH2 = 1.0 / (dsI * dsI);
-            double lv = 4.0 * (0.0 / W[0] + 0.0 / W[0]) * invH2;
+            double mu = 0.0;
+            if (c.visc) {
+                if (c.topo != TOPO_1D) invH2 += 1.0 / (dsJ * dsJ);
+                if (c.topo == TOPO_3D) invH2 += 1.0 / (dsK * dsK);
+                mu = visc_mu(c, p / (W[0] * c.rgas));
+            }
+            double lv = 4.0 * (mu / W[0] + 0.0 / W[0]) * invH2;
             dtv = c.cfl / (lam + lv);
         }
         c.dt[n] = dtv;
@@ -534,7 +541,12 @@ __device__ inline void boundary_block(const CaseDev& c, const double* U, int bx)
     double F[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     const long it = *c.iter;
     const Gas<FL> gs(c);
-    if (pa.type == PT_INLET) {
+    if (pa.type == PT_INLET && !side && c.nsw && c.nsw[nid(c, fi, fj, fk)]) {
+        // an inlet face on a no-slip wall node: the wall's stagnation pressure
+        // (boundary_inlet.cpp:16-19; end faces lie outside the node range and never match)
+        const long w = nid(c, fi, fj, fk);
+        stagnation_wall_flux(c, Sx, Sy, Sz, pa.v0, pa.v1, c.wallv[w], c.wallv[c.N + w], c.wallv[2 * c.N + w], F);
+    } else if (pa.type == PT_INLET) {
         const double* d = dirv + (long)side * 3 * nsz;
         inlet_flux<FL>(c, gs, pa.tab, Un, Sx, Sy, Sz, pa.v0, pa.v1, d[q], d[nsz + q], d[2 * nsz + q], F);
     } else if (pa.type == PT_OUTLET) {
@@ -587,7 +599,7 @@ __global__ void __launch_bounds__(256) k_prestage(Launch La) {
     if (w.cs < 0) return;   // padding entry of an XCD-ordered table
     const CaseDev& c = La.cases[w.cs];
     if (*c.stop || c.real != FL) return;
-    if (w.z == 0) boundary_block<FL>(c, c.U[La.in], w.x);
+    if (w.z == 0) boundary_block<FL>(c, stage_input(c, La), w.x);
     else if (w.z == 1) sources_block<true, FL>(c, La, w.x);
     else sources_block<false, FL>(c, La, w.x);
 }
