@@ -200,6 +200,8 @@ def main():
     ap.add_argument("--size", default="193x49x129")
     ap.add_argument("--scheme", default="JST")
     ap.add_argument("--precision", default="fp64")
+    ap.add_argument("--viscous", action="store_true",
+                    help="stage: the same rotor with VISCOSITY_ACTIVE (Sutherland air, walls unchanged)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", type=int, default=0,
@@ -227,6 +229,8 @@ def main():
 
     ni, nj, nk = (int(v) for v in args.size.split("x"))
     case = cases.rotor3d(ni, nj, nk, n_iterations=args.warmup + 3 * args.steps + 10, scheme=args.scheme)
+    if args.viscous:
+        case = cases.viscous(case, walls={})
     overrides = {"GPU_PRECISION": args.precision}
     s = Solver.from_case(case, overrides)
     nodes = ni * nj * nk
@@ -264,15 +268,17 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f64-state/f32-flux",
         "data": "synthetic annular rotor grid and blade fields (cturbobfm_amd/cases.py), free-stream start",
-        "config": {"workload": "annular rotor %dx%dx%d nodes, Hall body force, %s, RK41A, CFL 2, local dt, %s"
-                               % (ni, nj, nk, args.scheme, args.precision),
+        "config": {"workload": "annular rotor %dx%dx%d nodes, Hall body force, %s, RK41A, CFL 2, local dt, %s%s"
+                               % (ni, nj, nk, args.scheme, args.precision,
+                                  ", laminar viscous terms (Sutherland)" if args.viscous else ""),
                    "nodes": nodes, "stages_per_step": stages, "body_force_nodes": nbf,
                    "stage_tiling": s.tuning().split("->")[-1].strip(),
                    "autotune_ms": s.tuning().split("->")[0].strip(),
                    "ms_per_step_graph": (g1 - g0) / args.steps * 1e3},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_stage (fused residual + RK stage)", "avg_launch_ms": avg_ms,
+                     "kernel": ("k_prestage + k_vgrad + k_vface + k_stage (viscous stage)" if args.viscous
+                                else "k_stage (fused residual + RK stage)"), "avg_launch_ms": avg_ms,
                      "algorithmic_bytes_per_launch": alg_bytes, "launches": launches},
     }
     if not args.no_cpu:

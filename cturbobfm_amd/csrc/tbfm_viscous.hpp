@@ -124,8 +124,9 @@ __device__ inline void viscous_flux(const CaseDev& c, const double Ua[5], const 
     for (int v = 1; v < 5; ++v) W[v] = Ua[v] / Ua[0];
     const Gas<0> gs(c);
     const double T = pressure_w<double, 0>(gs, W) / (W[0] * c.rgas);
-    const double mu = visc_mu(c, T) + 0.0;
-    const double kap = c.cp_v * visc_mu(c, T) / c.pr + 0.0;
+    const double muL = visc_mu(c, T);          // one law evaluation for mu and kappa
+    const double mu = muL + 0.0;                // + eddy viscosity (none)
+    const double kap = c.cp_v * muL / c.pr + 0.0;
     const double lam = -2.0 / 3.0 * mu;
     const double div = g[0] + g[4] + g[8];
     const double txx = lam * div + 2.0 * mu * g[0];
