@@ -257,7 +257,7 @@ def params_from_config(case: Dict) -> Params:
         p.kd = float(cfg["HALL_THOLLET_COEFFICIENT_KD"])
         p.thollet_off_design = int(_cfg_bool(cfg, "HALL_THOLLET_OFF_DESIGN_ACTIVE", False))
         if "FLUID_KINEMATIC_VISCOSITY" not in cfg:
-            raise ValueError("Hall-Thollet needs FLUID_KINEMATIC_VISCOSITY on the inviscid path")
+            raise ValueError("Hall-Thollet needs FLUID_KINEMATIC_VISCOSITY (the viscosity-law route is not restated)")
         p.nu = float(cfg["FLUID_KINEMATIC_VISCOSITY"])
     p.stall_active = int(_cfg_bool(cfg, "STALLED_BFM_ACTIVE", False))
     p.viscous = int(_cfg_bool(cfg, "VISCOSITY_ACTIVE", False))
