@@ -159,9 +159,16 @@ __global__ void __launch_bounds__(256) k_vface(Launch La) {
     const long i = n % NI, j = (n / NI) % NJ, k = n / (NI * NJ);
     const double* __restrict__ U = stage_input(c, La);
     const int nd = (c.topo == TOPO_3D) ? 3 : 2;
-    for (int d = 0; d < nd; ++d) {
+    // the node's own state and gradients, loaded once for its two or three faces
+    double Un[5], Gn[12];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) Un[v] = U[v * N + n];
+#pragma unroll
+    for (int q = 0; q < 12; ++q) Gn[q] = c.grad[q * N + n];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
         const long idx = (d == 0) ? i : (d == 1 ? j : k);
-        if (idx == 0) continue;
+        if (d >= nd || idx == 0) continue;
         const long l = n - ((d == 0) ? 1 : (d == 1 ? NI : NI * NJ));
         double Sx, Sy, Sz;
         if (d == 0) { const long f = (k * NJ + j) * (NI + 1) + i; Sx = c.six[f]; Sy = c.siy[f]; Sz = c.siz[f]; }
@@ -169,9 +176,9 @@ __global__ void __launch_bounds__(256) k_vface(Launch La) {
         else { const long f = (k * NJ + j) * NI + i; Sx = c.skx[f]; Sy = c.sky[f]; Sz = c.skz[f]; }
         double Ua[5], g[12], F[5];
 #pragma unroll
-        for (int v = 0; v < 5; ++v) Ua[v] = (U[v * N + l] + U[v * N + n]) * 0.5;
+        for (int v = 0; v < 5; ++v) Ua[v] = (U[v * N + l] + Un[v]) * 0.5;
 #pragma unroll
-        for (int q = 0; q < 12; ++q) g[q] = (c.grad[q * N + l] + c.grad[q * N + n]) * 0.5;
+        for (int q = 0; q < 12; ++q) g[q] = (c.grad[q * N + l] + Gn[q]) * 0.5;
         viscous_flux(c, Ua, g, Sx, Sy, Sz, F);
         const double m = sqrt(Sx * Sx + Sy * Sy + Sz * Sz);
 #pragma unroll
