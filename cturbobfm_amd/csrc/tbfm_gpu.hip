@@ -701,8 +701,8 @@ void setup(tbfm_case* cs, const GridInput& grid) {
     if (c.visc) {
         if (cs->fluid == FluidModel::Real)
             throw std::runtime_error("VISCOSITY_ACTIVE needs FLUID_MODEL = ideal on the GPU path");
-        if (cs->topo == Topology::Axisymmetric || cs->topo == Topology::OneD)
-            throw std::runtime_error("VISCOSITY_ACTIVE is on the GPU path for 2D and 3D topologies");
+        if (cs->topo == Topology::OneD)
+            throw std::runtime_error("VISCOSITY_ACTIVE is on the GPU path for 2D, axisymmetric and 3D topologies");
         // Config::getViscosityModel / getFluidMuConstant (config.cpp:575-597),
         // FluidIdeal::setTransportProperties (fluid_ideal.cpp:209-222)
         std::string vm = cfg.has("FLUID_VISCOSITY_MODEL") ? cfg.get("FLUID_VISCOSITY_MODEL") : std::string("None");

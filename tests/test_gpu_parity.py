@@ -11,6 +11,7 @@ import pytest
 from cturbobfm_amd import Solver, cases
 
 import oracle as O
+import regression_cases as RC
 
 pytestmark = pytest.mark.gpu
 
@@ -64,6 +65,8 @@ CASES = {
         no_bfm(cases.rotor3d(25, 9, 17, n_iterations=200, scheme="ROE", muscl=True, limiter="van albada", cfl=1.2)),
         model="constant", mu=0.02, walls={"HUB": (0.0, 0.0, 0.0), "SHROUD": (0.0, 0.0, 30.0)}),
     "duct3d_jst_viscous_suth": lambda: cases.viscous(no_bfm(cases.rotor3d(25, 9, 17, n_iterations=200))),
+    # axisymmetric: the wedge faces enter the gradients (their face vectors do not cancel)
+    "ger4_axi_thollet_viscous_suth": lambda: cases.viscous(RC.load_case("GER4_Axisymmetric_HallThollet"), walls={}),
 }
 
 
