@@ -124,16 +124,16 @@ def bench_batch(args):
     }
 
 
-def bench_serve(args):
+def serve_leg(args, n_cases):
     """Served solves: a seeded Poisson stream (mean 2 cases/s) of 2-D bump and 3-D rotor
-    cases, admitted by device-memory footprint into one running batch (FP32 flux / FP64
+    cases, admitted by device-memory footprint into running batches (FP32 flux / FP64
     state) and retired on a density-residual drop; against the plain FP64 path serving
     the same stream one case at a time (its measured service times in a FIFO queue)."""
     import tempfile
     from cturbobfm_amd import serve
     tmp = tempfile.mkdtemp(prefix="tbfm_serve_")
     lut = serve.stream_table(tmp)
-    specs = serve.mixed_stream(args.serve_cases, seed=args.seed, precision="mixed", real_fraction=0.05,
+    specs = serve.mixed_stream(n_cases, seed=args.seed, precision="mixed", real_fraction=0.05,
                                lut_path=lut)
     arrivals = serve.poisson_arrivals(len(specs), rate=args.rate, seed=args.seed)
     progress = lambda m: print(m, file=sys.stderr, flush=True)
@@ -163,6 +163,11 @@ def bench_serve(args):
     plain = {"p50_s": float(np.percentile(ttc, 50)), "p99_s": float(np.percentile(ttc, 99)),
              "cases_per_hour": len(specs) / plain_wall * 3600.0, "wall_s": plain_wall,
              "converged": int(sum(r.converged for r in alone)), "iterations": int(sum(r.iterations for r in alone))}
+    return specs, served, plain, dm, dpr
+
+
+def bench_serve(args):
+    specs, served, plain, dm, dpr = serve_leg(args, args.serve_cases)
     return {
         "metric": "served cases/hour; p50/p99 time-to-converge vs plain FP64",
         "value": served["cases_per_hour"], "unit": "cases/hour", "n_gpus": 1, "steps": len(specs), "warmup": 0,
@@ -191,6 +196,8 @@ def main():
     ap.add_argument("--seed", type=int, default=11)
     ap.add_argument("--batch-cases", type=int, default=64)
     ap.add_argument("--serve-cases", type=int, default=40)
+    ap.add_argument("--serve-leg", type=int, default=12,
+                    help="stage: cases of the bounded served stream whose time-to-converge the line reports (0: none)")
     ap.add_argument("--rate", type=float, default=2.0)
     ap.add_argument("--chunk", type=int, default=100, help="serve: iterations per batch advance between admissions")
     ap.add_argument("--drop", type=float, default=3.0)
@@ -243,7 +250,14 @@ def main():
     s.sync()
     t1 = time.perf_counter()
     stage_ms, launches, alg_bytes = s.profile_get()
+    pre_ms, pre_bytes = s.profile_get_prestage()
     s.profile(False, True)
+    # the timed steps must have run: a stopped (converged or diverged) case skips its
+    # kernels and would report an inflated rate
+    its, conv, stopped, last = s.status()
+    if stopped or conv or not np.all(np.isfinite(last)):
+        raise SystemExit("bench case stopped or diverged inside the timed region (iterations %d, converged %d, "
+                         "last log residuals %s)" % (its, conv, last))
     # the same steps through the captured-graph path (no per-kernel events), for reference
     g0 = time.perf_counter()
     s.advance(args.steps)
@@ -254,6 +268,7 @@ def main():
     nbf = body_force_nodes(case)
     avg_ms = stage_ms / max(launches, 1)
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    avg_pre_ms = pre_ms / max(launches, 1)
     traffic = traffic_from_profiles("k_stage")
     out = {
         "metric": "cell-stage updates/s + HBM roofline %; p50/p99 time-to-converge vs plain FP64",
@@ -277,10 +292,29 @@ def main():
                    "ms_per_step_graph": (g1 - g0) / args.steps * 1e3},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": ("k_prestage + k_vgrad + k_vface + k_stage (viscous stage)" if args.viscous
-                                else "k_stage (fused residual + RK stage)"), "avg_launch_ms": avg_ms,
+                     "kernel": "k_stage (fused residual + RK stage, alone)", "avg_launch_ms": avg_ms,
                      "algorithmic_bytes_per_launch": alg_bytes, "launches": launches},
+        # the kernels launched before it in each stage (boundary fluxes and blade-row
+        # sources; viscous gradients and faces), timed by the same events
+        "prestage": {"kernels": ("k_prestage + k_vgrad + k_vface" if args.viscous else "k_prestage"),
+                     "avg_launch_ms": avg_pre_ms, "body_force_bytes_per_launch": pre_bytes,
+                     "stage_pair_achieved_GBps": (alg_bytes + pre_bytes) / ((avg_ms + avg_pre_ms) * 1e-3) / 1e9},
     }
+    s.close()
+    if args.serve_leg > 0:
+        # the metric's second half: time-to-converge of a bounded served stream against the
+        # plain FP64 path serving the same stream one case at a time
+        specs, served, plain, dm, dpr = serve_leg(args, args.serve_leg)
+        out["time_to_converge"] = {
+            "workload": "%d served cases (bump 65x33 / 129x65, rotor 97x25x65 / 129x33x97 / 193x49x129, "
+                        "5 %% table fluid), Poisson %.1f/s (seed %d), retired at a %.1f-order density drop"
+                        % (len(specs), args.rate, args.seed, args.drop),
+            "served_p50_s": served["p50_s"], "served_p99_s": served["p99_s"],
+            "served_cases_per_hour": served["cases_per_hour"], "served_converged": served["converged"],
+            "plain_fp64_p50_s": plain["p50_s"], "plain_fp64_p99_s": plain["p99_s"],
+            "plain_fp64_cases_per_hour": plain["cases_per_hour"],
+            "cases_per_hour_ratio": served["cases_per_hour"] / plain["cases_per_hour"],
+            "mass_flow_rel_diff_max": max(dm) if dm else None, "prtt_rel_diff_max": max(dpr) if dpr else None}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
     print(json.dumps(out), flush=True)

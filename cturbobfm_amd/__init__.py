@@ -42,7 +42,7 @@ EXPORTS = ["tbfm_gpu_case_create", "tbfm_gpu_case_create_from_ini", "tbfm_gpu_ca
            "tbfm_gpu_last_error", "tbfm_gpu_dims", "tbfm_gpu_set_state", "tbfm_gpu_get_state",
            "tbfm_gpu_advance", "tbfm_gpu_compute_residual", "tbfm_gpu_compute_timestep",
            "tbfm_gpu_get_side_outputs", "tbfm_gpu_get_turbo", "tbfm_gpu_get_geometry",
-           "tbfm_gpu_profile", "tbfm_gpu_profile_get", "tbfm_gpu_sync", "tbfm_gpu_copy_calibration",
+           "tbfm_gpu_profile", "tbfm_gpu_profile_get", "tbfm_gpu_profile_get_prestage", "tbfm_gpu_sync", "tbfm_gpu_copy_calibration",
            "tbfm_gpu_tuning", "tbfm_gpu_case_device_bytes", "tbfm_gpu_case_set_convergence",
            "tbfm_gpu_case_status", "tbfm_gpu_batch_create", "tbfm_gpu_batch_admit", "tbfm_gpu_batch_retire",
            "tbfm_gpu_batch_size", "tbfm_gpu_batch_advance", "tbfm_gpu_batch_destroy",
@@ -82,6 +82,7 @@ def lib():
         L.tbfm_gpu_get_geometry.argtypes = [vp, C.c_int, dp]
         L.tbfm_gpu_profile.argtypes = [vp, C.c_int, C.c_int]
         L.tbfm_gpu_profile_get.argtypes = [vp, dp, C.POINTER(C.c_long), dp]
+        L.tbfm_gpu_profile_get_prestage.argtypes = [vp, dp, dp]
         L.tbfm_gpu_sync.argtypes = [vp]
         L.tbfm_gpu_copy_calibration.argtypes = [C.c_long, C.c_int, dp]
         L.tbfm_gpu_tuning.restype = C.c_char_p
@@ -333,6 +334,13 @@ class Solver:
         ms, n, b = C.c_double(), C.c_long(), C.c_double()
         self._check(lib().tbfm_gpu_profile_get(self.h, C.byref(ms), C.byref(n), C.byref(b)))
         return ms.value, n.value, b.value
+
+    def profile_get_prestage(self):
+        """Pre-stage kernel milliseconds over the profiled stages and the body-force data
+        bytes one launch reads."""
+        ms, b = C.c_double(), C.c_double()
+        self._check(lib().tbfm_gpu_profile_get_prestage(self.h, C.byref(ms), C.byref(b)))
+        return ms.value, b.value
 
     def tuning(self) -> str:
         return lib().tbfm_gpu_tuning(self.h).decode()

@@ -130,9 +130,12 @@ struct tbfm_case {
     tbfm_batch* batch = nullptr;      // batch the case is admitted to
     // profiling of the stage-kernel launches (eager launches only)
     bool prof = false;
-    std::vector<hipEvent_t> pool;     // event pairs recorded around stage-kernel launches
-    size_t pool_used = 0;             // pairs recorded since the last read-out
-    double prof_ms = 0.0;
+    // event triples per stage: before the pre-stage kernel, between it and the stage
+    // kernel, after the stage kernel
+    std::vector<hipEvent_t> pool;
+    size_t pool_used = 0;             // triples recorded since the last read-out
+    double prof_ms = 0.0;             // stage kernel alone
+    double prof_pre_ms = 0.0;         // pre-stage (and viscous) kernels before it
     long prof_launches = 0;
     double bytes_per_node = 0.0;
     long body_force_nodes = 0;        // nodes whose stage also runs the blade sources
@@ -698,6 +701,11 @@ void setup(tbfm_case* cs, const GridInput& grid) {
     c.visc = cfg.getBool("VISCOSITY_ACTIVE", false) ? 1 : 0;
     if (cfg.getBool("TURBULENCE_ACTIVE", false))
         throw std::runtime_error("TURBULENCE_ACTIVE is outside the GPU residual path");
+    // the time-dependent body-force options (source_bfm_base.cpp:58, 215-243; config.cpp:695-718)
+    if (cfg.getBool("PERTURBATION_BODY_FORCE", false))
+        throw std::runtime_error("PERTURBATION_BODY_FORCE is not on the GPU residual path");
+    if (cfg.getBool("ACCELERATION_ACTIVE", false))
+        throw std::runtime_error("ACCELERATION_ACTIVE (time-dependent rotational speed) is not on the GPU residual path");
     if (c.visc) {
         if (cs->fluid == FluidModel::Real)
             throw std::runtime_error("VISCOSITY_ACTIVE needs FLUID_MODEL = ideal on the GPU path");
@@ -1160,14 +1168,12 @@ void launchBoundary(const Plan& P, int in) { launchPre(P, P.bnd, in, 0, 0, 0, 0)
 // closure, and after the last stage the norms and convergence test.
 void launchStage(Plan& P, int s, int in, int out, bool last, bool rmode, tbfm_case* prof) {
     if (prof) {
-        if (prof->pool_used * 2 + 2 > prof->pool.size()) {
-            hipEvent_t a, b;
+        while (prof->pool_used * 3 + 3 > prof->pool.size()) {
+            hipEvent_t a;
             HIPCHK(hipEventCreate(&a));
-            HIPCHK(hipEventCreate(&b));
             prof->pool.push_back(a);
-            prof->pool.push_back(b);
         }
-        HIPCHK(hipEventRecord(prof->pool[2 * prof->pool_used], P.stream));
+        HIPCHK(hipEventRecord(prof->pool[3 * prof->pool_used], P.stream));
     }
     // viscous cases: the first stage's wall momentum, then (after the boundary fluxes and
     // slab source terms) the gradients and viscous face fluxes of the stage input
@@ -1184,9 +1190,10 @@ void launchStage(Plan& P, int s, int in, int out, bool last, bool rmode, tbfm_ca
         hipLaunchKernelGGL(dev::k_vgrad, dim3((unsigned)P.visc.n), dim3(256), 0, P.stream, V);
         hipLaunchKernelGGL(dev::k_vface, dim3((unsigned)P.visc.n), dim3(256), 0, P.stream, V);
     }
+    if (prof) HIPCHK(hipEventRecord(prof->pool[3 * prof->pool_used + 1], P.stream));
     launchStageGroups(P, in, out, s, last ? 1 : 0, rmode ? 1 : 0, V.pre);
     if (prof) {
-        HIPCHK(hipEventRecord(prof->pool[2 * prof->pool_used + 1], P.stream));
+        HIPCHK(hipEventRecord(prof->pool[3 * prof->pool_used + 2], P.stream));
         prof->pool_used += 1;
     }
     if (P.per.n)
@@ -1821,9 +1828,11 @@ namespace {
 void drainProfile(tbfm_case* c) {
     HIPCHK(hipStreamSynchronize(c->solo.stream));
     for (size_t q = 0; q < c->pool_used; ++q) {
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, c->pool[2 * q], c->pool[2 * q + 1]));
+        float pre = 0.f, ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&pre, c->pool[3 * q], c->pool[3 * q + 1]));
+        HIPCHK(hipEventElapsedTime(&ms, c->pool[3 * q + 1], c->pool[3 * q + 2]));
         c->prof_ms += ms;
+        c->prof_pre_ms += pre;
         c->prof_launches += 1;
     }
     c->pool_used = 0;
@@ -1836,7 +1845,7 @@ int tbfm_gpu_profile(tbfm_case* c, int enable, int reset) {
         DevLock lk(g_dev_mu);
         drainProfile(c);
         c->prof = enable != 0;
-        if (reset) { c->prof_ms = 0.0; c->prof_launches = 0; }
+        if (reset) { c->prof_ms = 0.0; c->prof_pre_ms = 0.0; c->prof_launches = 0; }
         return TBFM_OK;
     });
 }
@@ -1848,9 +1857,22 @@ int tbfm_gpu_profile_get(tbfm_case* c, double* ms, long* launches, double* bytes
         drainProfile(c);
         if (ms) *ms = c->prof_ms;
         if (launches) *launches = c->prof_launches;
-        // 208 B (3-D FP64) per node plus 96 B per body-force node: n_ax, n_r, n_theta,
-        // N_b, rpm, bladePresent, b, grad b (3), r, theta
-        if (bytes) *bytes = c->bytes_per_node * (double)c->N + 96.0 * (double)c->body_force_nodes;
+        // the stage kernel's contract: 208 B per node in 3-D FP64 (read U^(k-1), U^(0),
+        // dt, V and one face vector per direction, write U^(k))
+        if (bytes) *bytes = c->bytes_per_node * (double)c->N;
+        return TBFM_OK;
+    });
+}
+
+int tbfm_gpu_profile_get_prestage(tbfm_case* c, double* ms, double* bytes) {
+    if (!c) return fail(c, TBFM_ERR_ARGUMENT, "null argument");
+    return guarded(c, [&]() {
+        DevLock lk(g_dev_mu);
+        drainProfile(c);
+        if (ms) *ms = c->prof_pre_ms;
+        // 96 B per body-force node: n_ax, n_r, n_theta, N_b, rpm, bladePresent, b,
+        // grad b (3), r, theta
+        if (bytes) *bytes = 96.0 * (double)c->body_force_nodes;
         return TBFM_OK;
     });
 }

@@ -133,12 +133,16 @@ int tbfm_gpu_get_turbo_history(tbfm_case* c, long first, long n, double* out);
  * which 0..2 surfaces I/J/K, 3 volumes, 4 blockage gradient. */
 int tbfm_gpu_get_geometry(tbfm_case* c, int which, double* out);
 
-/* Timing of the fused residual+RK stage kernel: when enabled, every stage-kernel launch
- * is bracketed by HIP events on the case stream.  get returns the accumulated kernel
- * milliseconds and launch count since the last reset, plus the algorithmic bytes of one
- * launch (per-node contract times nodes). */
+/* Timing of the fused residual+RK stage kernel: when enabled, every stage is bracketed
+ * by HIP events on the case stream, one before the pre-stage kernel (boundary fluxes,
+ * blade-row sources), one between it and the stage kernel, one after the stage kernel.
+ * get returns the accumulated stage-kernel milliseconds and launch count since the last
+ * reset, plus the algorithmic bytes of one stage-kernel launch (per-node contract times
+ * nodes); get_prestage the pre-stage milliseconds over the same launches and the
+ * body-force data bytes one pre-stage launch reads. */
 int tbfm_gpu_profile(tbfm_case* c, int enable, int reset);
 int tbfm_gpu_profile_get(tbfm_case* c, double* stage_ms, long* launches, double* bytes_per_launch);
+int tbfm_gpu_profile_get_prestage(tbfm_case* c, double* prestage_ms, double* bytes_per_launch);
 
 /* Measurement helper with no counterpart in the reference: runs `repeats` plain
  * device-to-device copies of nbytes (kernel k_copy_calibration) on the default stream

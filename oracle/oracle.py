@@ -152,6 +152,7 @@ def lib():
         L.tbo_run.restype = C.c_long
         L.tbo_run.argtypes = [C.c_void_p, C.c_long, dp, dp, C.c_int]
         L.tbo_residual.argtypes = [C.c_void_p, dp, C.c_long, C.c_int, dp]
+        L.tbo_source_residual.argtypes = [C.c_void_p, dp, dp]
         L.tbo_timestep.argtypes = [C.c_void_p, dp, dp]
         L.tbo_gradients.argtypes = [C.c_void_p, dp]
         L.tbo_viscous_flux.argtypes = [C.c_void_p, dp, dp, dp, dp]
@@ -237,6 +238,10 @@ def params_from_config(case: Dict) -> Params:
     p.le_index = int(float(cfg.get("LEADING_EDGE_INDEX", "0")))
     p.te_index = int(float(cfg.get("TRAILING_EDGE_INDEX", "0")))
     p.omega_scaling = _cfg_float(cfg, "ROTATIONAL_SPEED_SCALING_FACTOR", 1.0)
+    # time-dependent body-force options (source_bfm_base.cpp:58, 215-243; config.cpp:695-718)
+    for key in ("PERTURBATION_BODY_FORCE", "ACCELERATION_ACTIVE"):
+        if _cfg_bool(cfg, key, False):
+            raise ValueError("%s is not restated by the oracle" % key)
     p.inlet_cylindrical = int(cfg.get("INLET_REFERENCE_FRAME", "None").lower() == "cylindrical")
     td = cfg.get("TURBO_DIRECTION", "i")
     p.turbo_dir = {"i": 0, "+i": 0, "-i": 1, "j": 2, "+j": 2, "-j": 3}.get(td, 0)
@@ -367,6 +372,13 @@ class OracleCase:
         U = np.ascontiguousarray(U, dtype=np.float64)
         R = np.zeros_like(U)
         lib().tbo_residual(self.h, _dp(U), it, int(update_profiles), _dp(R))
+        return R
+
+    def source_residual(self, U: np.ndarray) -> np.ndarray:
+        """Minus the geometric, Gong and body-force source terms of U alone."""
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        R = np.zeros_like(U)
+        lib().tbo_source_residual(self.h, _dp(U), _dp(R))
         return R
 
     def gradients(self) -> np.ndarray:

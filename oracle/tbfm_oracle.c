@@ -1994,6 +1994,13 @@ void tbo_residual(tbo_case* c, const double* U, long it, int update_profiles, do
     compute_residuals(c, U, R, it);
 }
 
+/* solver.cpp:2790-2897 alone: R = minus the geometric, Gong and body-force sources of U
+ * (test hook for the closed-form checks of those terms) */
+void tbo_source_residual(tbo_case* c, const double* U, double* R) {
+    memset(R, 0, sizeof(double) * 5 * c->N);
+    source_residuals(c, U, R);
+}
+
 /* solver.cpp:1298-1424 (explicit RK branch) + 1426-1439 + 2985-3015 */
 long tbo_run(tbo_case* c, long n, double* logres, double* turbo, int nthreads) {
 #ifdef _OPENMP

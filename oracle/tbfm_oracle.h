@@ -129,6 +129,8 @@ long tbo_run(tbo_case* c, long n, double* logres, double* turbo, int nthreads);
 /* One computeResiduals(U) at iteration it with the current time-step array; R is
  * 5*N.  Refreshes the radial-equilibrium profiles first when update_profiles. */
 void tbo_residual(tbo_case* c, const double* U, long it, int update_profiles, double* R);
+/* The source terms alone (solver.cpp:2790-2897): R = -(geometric + Gong + body force). */
+void tbo_source_residual(tbo_case* c, const double* U, double* R);
 void tbo_timestep(tbo_case* c, const double* U, double* dt);
 
 /* Geometry: which 0..2 surfaces I/J/K, 3..5 centres I/J/K, 6 volumes, 7 dual nodes,

@@ -93,11 +93,10 @@ def test_radial_equilibrium():
     assert np.array_equal(p, r["p"])
 
 
-@pytest.mark.skipif(O.ref_lib() is None, reason="reference library build absent (no /root/reference)")
-def test_against_reference_build_fresh_inputs():
+def test_against_reference_build_fresh_inputs(ref_build):
     """Where the reference build exists, also compare on fresh random inputs."""
     import ctypes as C
-    R = O.ref_lib()
+    R = ref_build
     rng = np.random.default_rng(7)
     for case in (cases.bump2d(9, 6, scheme="ROE", muscl=True, limiter="van leer"),
                  cases.rotor3d(11, 5, 7, bfm_model="Hall")):

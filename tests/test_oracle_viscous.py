@@ -145,10 +145,9 @@ def test_inlet_face_on_a_no_slip_wall_carries_the_stagnation_pressure():
         assert np.any(Rv[:, 0, 0, 2] != 0.0)
 
 
-@pytest.mark.skipif(O.ref_lib() is None, reason="reference library build absent (no /root/reference)")
 @pytest.mark.parametrize("which", ["rotor", "bump"])
-def test_gradients_equal_reference_build(which, tmp_path):
-    R = O.ref_lib()
+def test_gradients_equal_reference_build(which, tmp_path, ref_build):
+    R = ref_build
     R.ref_gradients4.argtypes = [C.POINTER(C.c_double)] * 2
     case = cases.rotor3d(11, 5, 7) if which == "rotor" else cases.bump2d(9, 6)
     case = cases.viscous(case)
@@ -167,9 +166,8 @@ def test_gradients_equal_reference_build(which, tmp_path):
     assert np.array_equal(g, ref)
 
 
-@pytest.mark.skipif(O.ref_lib() is None, reason="reference library build absent (no /root/reference)")
-def test_viscosity_and_stress_tensor_equal_reference_build(tmp_path):
-    R = O.ref_lib()
+def test_viscosity_and_stress_tensor_equal_reference_build(tmp_path, ref_build):
+    R = ref_build
     R.ref_mu.restype = C.c_double
     R.ref_mu.argtypes = [C.c_double]
     R.ref_stress_tensor.argtypes = [C.c_double, C.c_double] + [C.POINTER(C.c_double)] * 4
