@@ -152,7 +152,11 @@ def lib():
         L.tbo_run.restype = C.c_long
         L.tbo_run.argtypes = [C.c_void_p, C.c_long, dp, dp, C.c_int]
         L.tbo_residual.argtypes = [C.c_void_p, dp, C.c_long, C.c_int, dp]
-        L.tbo_source_residual.argtypes = [C.c_void_p, dp, dp]
+        L.tbo_source_residual.argtypes = [C.c_void_p, dp, C.c_int, dp]
+        L.tbo_set_profile.restype = C.c_long
+        L.tbo_set_profile.argtypes = [C.c_void_p, C.c_int, dp]
+        L.tbo_patch_flux.argtypes = [C.c_void_p, C.c_int, dp, dp, C.POINTER(C.c_long), C.c_long, dp]
+        L.tbo_greitzer_steps.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, dp, C.c_int, dp, dp]
         L.tbo_timestep.argtypes = [C.c_void_p, dp, dp]
         L.tbo_gradients.argtypes = [C.c_void_p, dp]
         L.tbo_viscous_flux.argtypes = [C.c_void_p, dp, dp, dp, dp]
@@ -374,12 +378,33 @@ class OracleCase:
         lib().tbo_residual(self.h, _dp(U), it, int(update_profiles), _dp(R))
         return R
 
-    def source_residual(self, U: np.ndarray) -> np.ndarray:
-        """Minus the geometric, Gong and body-force source terms of U alone."""
+    def source_residual(self, U: np.ndarray, which: int = 7) -> np.ndarray:
+        """Minus the source terms of U alone: which = bit 1 geometric, 2 Gong, 4 body force."""
         U = np.ascontiguousarray(U, dtype=np.float64)
         R = np.zeros_like(U)
-        lib().tbo_source_residual(self.h, _dp(U), _dp(R))
+        lib().tbo_source_residual(self.h, _dp(U), which, _dp(R))
         return R
+
+    def set_profile(self, q: int, values) -> None:
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        if lib().tbo_set_profile(self.h, q, _dp(v)) != len(v):
+            raise ValueError("profile %d has another length" % q)
+
+    def patch_flux(self, pidx: int, Uint, S, face, it: int = 1) -> np.ndarray:
+        """Boundary flux of patch pidx (input order) at face (i, j, k), as the residual
+        computes it."""
+        out = np.zeros(5)
+        fc = (C.c_long * 3)(*[int(f) for f in face])
+        lib().tbo_patch_flux(self.h, pidx, _dp(np.ascontiguousarray(Uint, dtype=np.float64)),
+                             _dp(np.ascontiguousarray(S, dtype=np.float64)), fc, it, _dp(out))
+        return out
+
+    def greitzer_steps(self, p0: float, mout0: float, T0: float, mflow) -> tuple:
+        """The plenum model alone: plenum pressure and outlet mass flow after each step."""
+        m = np.ascontiguousarray(mflow, dtype=np.float64)
+        p, mo = np.zeros(len(m)), np.zeros(len(m))
+        lib().tbo_greitzer_steps(self.h, p0, mout0, T0, _dp(m), len(m), _dp(p), _dp(mo))
+        return p, mo
 
     def gradients(self) -> np.ndarray:
         """[ux, uy, uz, T] x [x, y, z] gradients of the last residual evaluation (viscous)."""

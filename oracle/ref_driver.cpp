@@ -4,7 +4,8 @@
 // reference sources (src/config.cpp, src/input.cpp, src/mesh.cpp,
 // src/math_utils.cpp, src/fluid_ideal.cpp, src/advection_{base,jst,roe}.cpp,
 // src/boundary_{base,inlet,outlet,outlet_radial_equilibrium,inviscid_wall,
-// fake}.cpp, src/source_bfm_{base,hall,lift_drag,thollet}.cpp, src/input_table.cpp,
+// fake,outlet_throttle,outlet_mass_flow}.cpp, src/greitzer_model.cpp,
+// src/source_bfm_{base,hall,lift_drag,thollet}.cpp, src/input_table.cpp,
 // src/fluid_real.cpp, src/output.cpp, src/turbulence_model_base.cpp)
 // into oracle/_ref/libtbfm_ref.so by oracle/Makefile.  Only tests/ and
 // tests/golden/gen_golden.py load it, to pin the CPU restatement in
@@ -16,6 +17,7 @@
 //   AdvectionBase::computeFlux          include/advection_base.hpp:21-25
 //   BoundaryBase::computeBoundaryFlux   include/boundary_base.hpp:26-32
 //   SourceBFMBase::computeTotalSource   include/source_bfm_base.hpp:20-31
+//   GreitzerModel                       include/greitzer_model.hpp:7-28
 //   FluidIdeal / FluidReal              include/fluid_ideal.hpp, include/fluid_real.hpp
 //   Output::writeSolution               include/output.hpp:30-34
 #include "types.hpp"
@@ -30,6 +32,9 @@
 #include "boundary_outlet.hpp"
 #include "boundary_outlet_radial_equilibrium.hpp"
 #include "boundary_inviscid_wall.hpp"
+#include "boundary_outlet_throttle.hpp"
+#include "boundary_outlet_mass_flow.hpp"
+#include "greitzer_model.hpp"
 #include "source_bfm_hall.hpp"
 #include "source_bfm_lift_drag.hpp"
 #include "source_bfm_thollet.hpp"
@@ -50,6 +55,10 @@ struct RefCase {
     std::unique_ptr<AdvectionBase> advection;
     std::unique_ptr<SourceBFMBase> bfm;
     std::vector<FloatType> radialProfile;
+    // side outputs the reference keeps as Solver members (solver.hpp:370-371), persisting
+    // across ref_bfm_sweep calls as they persist across stages
+    std::unique_ptr<Matrix3D<Vector3D>> sweepFinv, sweepFvisc;
+    std::unique_ptr<Matrix3D<FloatType>> sweepDev, sweepDt;
 };
 RefCase g_case;
 
@@ -191,7 +200,7 @@ void ref_face_flux(const double* Ull, const double* Ul, const double* Ur, const 
 }
 
 // Boundary flux.  kind: 0 inlet, 1 outlet, 2 radial equilibrium (profile given),
-// 3 inviscid wall.  values: the patch values of input.ini.  face = {i,j,k} of the
+// 3 inviscid wall, 4 throttle outlet (profile given), 5 mass-flow outlet (profile given).  values: the patch values of input.ini.  face = {i,j,k} of the
 // boundary face as the reference passes it.
 void ref_boundary_flux(int kind, const double* values, int nvalues, const double* Uint,
                        const double* S, const long* face, long iter, const double* profile,
@@ -213,11 +222,76 @@ void ref_boundary_flux(int kind, const double* values, int nvalues, const double
         BoundaryOutletRadialEquilibrium bc(*g_case.config, *g_case.mesh, *g_case.fluid,
                                            g_case.radialProfile);
         f = bc.computeBoundaryFlux(sv(Uint), surf, mid, idx, dummy, (size_t)iter);
+    } else if (kind == 4) {
+        g_case.radialProfile.assign(profile, profile + nprofile);
+        BoundaryOutletThrottle bc(*g_case.config, *g_case.mesh, *g_case.fluid, g_case.radialProfile);
+        f = bc.computeBoundaryFlux(sv(Uint), surf, mid, idx, dummy, (size_t)iter);
+    } else if (kind == 5) {
+        g_case.radialProfile.assign(profile, profile + nprofile);
+        BoundaryOutletMassFlow bc(*g_case.config, *g_case.mesh, *g_case.fluid, g_case.radialProfile);
+        f = bc.computeBoundaryFlux(sv(Uint), surf, mid, idx, dummy, (size_t)iter);
     } else {
         BoundaryInviscidWall bc(*g_case.config, *g_case.mesh, *g_case.fluid);
         f = bc.computeBoundaryFlux(sv(Uint), surf, mid, idx, dummy, (size_t)iter);
     }
     put(f, out);
+}
+
+// GreitzerModel (src/greitzer_model.cpp:3-56) of the loaded case's configuration and
+// fluid with throttle coefficient kt: initializeState(p0, m_in0, m_out0, T0), then one
+// computePlenumPressure per compressor mass flow; pout / mout receive the plenum
+// pressure and the plenum outlet mass flow after each step.
+void ref_greitzer_steps(double kt, double p0, double min0, double mout0, double T0, const double* mflow, int n,
+                        double* pout, double* mout) {
+    GreitzerModel gm(*g_case.config, *g_case.fluid, kt);
+    gm.initializeState(p0, min0, mout0, T0);
+    for (int q = 0; q < n; ++q) {
+        pout[q] = gm.computePlenumPressure(mflow[q]);
+        mout[q] = gm.getPlenumOutletMassflow((size_t)q + 1);
+    }
+}
+
+// The body-force part of Solver::computeSourceResiduals (src/solver.cpp:2814-2896) over
+// every node in the reference's loop order (i, then j, then k): at nodes whose blockage
+// gradient exceeds 1e-10 in magnitude, R -= computeTotalSource(primitives of U), with
+// one model object and the force, deviation and time-step arrays persisting across
+// calls (Solver members in the reference), as over the stages of a run.  U and R are
+// 5*N in the reference layout; finv / fvisc (3*N) and dev (N) receive the side outputs
+// after the sweep.
+void ref_bfm_sweep(const double* U, double time, double* R, double* finv, double* fvisc, double* dev) {
+    const Mesh& m = *g_case.mesh;
+    const size_t ni = m.getNumberPointsI(), nj = m.getNumberPointsJ(), nk = m.getNumberPointsK();
+    const size_t N = ni * nj * nk;
+    if (!g_case.sweepFinv) {
+        g_case.sweepFinv = std::make_unique<Matrix3D<Vector3D>>(ni, nj, nk);
+        g_case.sweepFvisc = std::make_unique<Matrix3D<Vector3D>>(ni, nj, nk);
+        g_case.sweepDev = std::make_unique<Matrix3D<FloatType>>(ni, nj, nk);
+        g_case.sweepDt = std::make_unique<Matrix3D<FloatType>>(ni, nj, nk);
+    }
+    FlowSolution sol(ni, nj, nk);
+    for (size_t i = 0; i < ni; ++i)
+        for (size_t j = 0; j < nj; ++j)
+            for (size_t k = 0; k < nk; ++k) {
+                const size_t n = (i * nj + j) * nk + k;
+                sol.set(i, j, k, StateVector({U[n], U[N + n], U[2 * N + n], U[3 * N + n], U[4 * N + n]}));
+            }
+    FloatType t = time, t2 = time;
+    for (size_t i = 0; i < ni; ++i)
+        for (size_t j = 0; j < nj; ++j)
+            for (size_t k = 0; k < nk; ++k) {
+                StateVector primitive = getPrimitiveVariablesFromConservative(sol.at(i, j, k));
+                Vector3D gb = m.getInputFieldsGradient(InputField::BLOCKAGE, i, j, k);
+                if (gb.magnitude() > 1E-10) {
+                    StateVector s = g_case.bfm->computeTotalSource(i, j, k, primitive, *g_case.sweepFinv,
+                                                                   *g_case.sweepFvisc, *g_case.sweepDev, t, sol,
+                                                                   *g_case.sweepDt, t2);
+                    const size_t n = (i * nj + j) * nk + k;
+                    for (int v = 0; v < 5; ++v) R[v * N + n] -= s[v];
+                }
+            }
+    dumpVec(*g_case.sweepFinv, finv);
+    dumpVec(*g_case.sweepFvisc, fvisc);
+    for (size_t idx = 0; idx < N; ++idx) dev[idx] = (*g_case.sweepDev)[idx];
 }
 
 // Body-force source of the configured BFM model at node (i,j,k) for the given

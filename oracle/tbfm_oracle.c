@@ -1175,8 +1175,11 @@ static st5 gong_source(const tbo_case* c, double omega, long i, long j, long k, 
     return scale5(s, V);
 }
 
-/* solver.cpp:2790-2897 */
-static void source_residuals(tbo_case* c, const double* U, double* R) {
+/* solver.cpp:2790-2897; which: bit 1 geometric, 2 Gong, 4 body force (7 in the solver,
+ * single terms for the tests that pin each) */
+static void source_terms(tbo_case* c, const double* U, double* R, int which);
+static void source_residuals(tbo_case* c, const double* U, double* R) { source_terms(c, U, R, 7); }
+static void source_terms(tbo_case* c, const double* U, double* R, int which) {
     const int topo = c->p.topology;
     if (topo == TBO_TOPO_3D && !c->p.bfm_active) return;
     const long NI = c->p.ni, NJ = c->p.nj, NK = c->p.nk, N = c->N;
@@ -1194,6 +1197,8 @@ static void source_residuals(tbo_case* c, const double* U, double* R) {
                 int geo = 0, gong = 0;
                 if (topo == TBO_TOPO_3D && c->p.bfm_active) geo = gong = (c->present[idx] > 0.0);
                 else if (topo == TBO_TOPO_AXI) geo = 1;
+                geo = geo && (which & 1);
+                gong = gong && (which & 2);
                 if (geo) {
                     double cs = cos(th), sn = sin(th);
                     v3 vcy = cyl_from_cart(u, cs, sn);
@@ -1221,7 +1226,7 @@ static void source_residuals(tbo_case* c, const double* U, double* R) {
                     st5 gs = gong_source(c, om, i, j, k, V, U);
                     for (int v = 0; v < 5; ++v) R[v * N + idx] -= gs.v[v];
                 }
-                if (c->p.bfm_active) {
+                if (c->p.bfm_active && (which & 4)) {
                     v3 gb = mk3(c->gradb[3 * idx], c->gradb[3 * idx + 1], c->gradb[3 * idx + 2]);
                     if (mag3(gb) > 1E-10) {
                         st5 bs = bfm_total_source(c, i, j, k, W, 1);
@@ -1982,6 +1987,21 @@ long tbo_get_profile(const tbo_case* c, int q, double* out) {
     return c->prof_n[q];
 }
 
+/* test hooks: overwrite outlet profile q; the boundary flux of patch pidx at face
+ * (fi, fj, fk) as the residual computes it (bc_flux) */
+long tbo_set_profile(tbo_case* c, int q, const double* in) {
+    if (q < 0 || q >= c->nprof) return -1;
+    memcpy(c->prof_p[q], in, sizeof(double) * c->prof_n[q]);
+    return c->prof_n[q];
+}
+void tbo_patch_flux(const tbo_case* c, int pidx, const double* Ui, const double* S, const long* face, long it,
+                    double* out) {
+    st5 U;
+    memcpy(U.v, Ui, 40);
+    st5 F = bc_flux(c, pidx, U, mk3(S[0], S[1], S[2]), face[0], face[1], face[2], it);
+    memcpy(out, F.v, 40);
+}
+
 void tbo_side_outputs(const tbo_case* c, double* finv, double* fvisc, double* dev, double* dt) {
     if (finv) memcpy(finv, c->finv, 24 * c->N);
     if (fvisc) memcpy(fvisc, c->fvisc, 24 * c->N);
@@ -1996,9 +2016,23 @@ void tbo_residual(tbo_case* c, const double* U, long it, int update_profiles, do
 
 /* solver.cpp:2790-2897 alone: R = minus the geometric, Gong and body-force sources of U
  * (test hook for the closed-form checks of those terms) */
-void tbo_source_residual(tbo_case* c, const double* U, double* R) {
+void tbo_source_residual(tbo_case* c, const double* U, int which, double* R) {
     memset(R, 0, sizeof(double) * 5 * c->N);
-    source_residuals(c, U, R);
+    source_terms(c, U, R, which);
+}
+
+/* GreitzerModel::initializeState + computePlenumPressure (greitzer_model.cpp:16-56) for a
+ * sequence of compressor mass flows; pout / mout receive the plenum pressure and outlet
+ * mass flow after each step (test hook of the plenum model alone) */
+void tbo_greitzer_steps(tbo_case* c, double p0, double mout0, double T0, const double* mflow, int n,
+                        double* pout, double* mout) {
+    c->gz_p = p0;
+    c->gz_mout = mout0;
+    c->gz_T = T0;
+    for (int q = 0; q < n; ++q) {
+        pout[q] = greitzer_plenum_pressure(c, mflow[q]);
+        mout[q] = c->gz_mout;
+    }
 }
 
 /* solver.cpp:1298-1424 (explicit RK branch) + 1426-1439 + 2985-3015 */

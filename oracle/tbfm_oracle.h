@@ -129,8 +129,18 @@ long tbo_run(tbo_case* c, long n, double* logres, double* turbo, int nthreads);
 /* One computeResiduals(U) at iteration it with the current time-step array; R is
  * 5*N.  Refreshes the radial-equilibrium profiles first when update_profiles. */
 void tbo_residual(tbo_case* c, const double* U, long it, int update_profiles, double* R);
-/* The source terms alone (solver.cpp:2790-2897): R = -(geometric + Gong + body force). */
-void tbo_source_residual(tbo_case* c, const double* U, double* R);
+/* The source terms alone (solver.cpp:2790-2897): R = minus the terms selected by which
+ * (bit 1 geometric, 2 Gong, 4 body force). */
+void tbo_source_residual(tbo_case* c, const double* U, int which, double* R);
+/* Test hooks: overwrite outlet profile q (prof_n values); the boundary flux of patch
+ * pidx at face {i, j, k} as the residual computes it. */
+long tbo_set_profile(tbo_case* c, int q, const double* in);
+void tbo_patch_flux(const tbo_case* c, int pidx, const double* Ui, const double* S, const long* face, long it,
+                    double* out);
+/* The Greitzer plenum alone (greitzer_model.cpp:16-56): state (p0, mout0, T0), then one
+ * computePlenumPressure per mass flow. */
+void tbo_greitzer_steps(tbo_case* c, double p0, double mout0, double T0, const double* mflow, int n,
+                        double* pout, double* mout);
 void tbo_timestep(tbo_case* c, const double* U, double* dt);
 
 /* Geometry: which 0..2 surfaces I/J/K, 3..5 centres I/J/K, 6 volumes, 7 dual nodes,

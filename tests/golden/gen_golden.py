@@ -18,8 +18,15 @@ Each fixture holds seeded inputs and the reference's outputs:
                           stored in the fixture): the 17 point functions, JST and
                           Roe-MUSCL face fluxes, inlet / outlet / wall fluxes, p a h_t T
                           p_t T_t s, and the initial state
+  outlets.npz             BoundaryOutletThrottle / BoundaryOutletMassFlow fluxes on a given
+                          pressure profile (mass flow: faces past the profile's end take
+                          its last value) and GreitzerModel plenum sequences
+  bfm_sweep_<model>.npz   the body-force part of the solver's source loop over a whole
+                          blade row in the reference's i, j, k order, three consecutive
+                          sweeps with one model object and persistent side-output arrays
 
-Run from the repository root:  python tests/golden/gen_golden.py
+Run from the repository root:  python tests/golden/gen_golden.py [group ...]
+(groups: base, outlets, sweep; default all)
 """
 import ctypes as C
 import os
@@ -62,12 +69,99 @@ def fixture_cases():
     return out
 
 
+def outlet_fixtures(R, tmp, n=64):
+    """Throttle and mass-flow outlet fluxes on a given profile, and Greitzer plenum
+    sequences (positive and negative plenum-to-ambient pressure differences)."""
+    rng = np.random.default_rng(20261207)
+    case = cases.rotor3d(11, 5, 7, outflow="throttle, 0.01")
+    case["config"].update({"ENABLE_GREITZER_MODELING": "yes", "GREITZER_PLENUM_VOLUME": "0.05",
+                           "TIME_STEP_METHOD": "fixed", "FIXED_TIME_STEP": "2e-6",
+                           "GREITZER_AMBIENT_PRESSURE": "101325"})
+    ini = cases.write_case(case, os.path.join(tmp, "outlets"))
+    assert R.ref_case_load(ini.encode()) == 0
+    nj = case["nj"]
+    R.ref_boundary_flux.argtypes = [C.c_int, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double), C.POINTER(C.c_long), C.c_long, C.POINTER(C.c_double),
+                                    C.c_int, C.POINTER(C.c_double)]
+    prof = rng.uniform(9.0e4, 1.0e5, nj)
+    recs = []
+    for q in range(n):
+        Ui, _ = random_states(rng, 1)
+        Ui = Ui[0]
+        Sv = rng.normal(size=3)
+        for kind in (4, 5):
+            j = int(rng.integers(0, nj + 3)) if kind == 5 else int(rng.integers(0, nj))
+            face = np.array([case["ni"], j, int(rng.integers(0, case["nk"]))], dtype=np.int64)
+            out = np.zeros(5)
+            v = np.zeros(8)
+            R.ref_boundary_flux(kind, dp(v), 1, dp(Ui), dp(Sv), face.ctypes.data_as(C.POINTER(C.c_long)), 3,
+                                dp(prof), nj, dp(out))
+            recs.append((kind, Ui, Sv, face.astype(np.float64), out))
+    R.ref_greitzer_steps.argtypes = [C.c_double] * 5 + [C.POINTER(C.c_double), C.c_int] + [C.POINTER(C.c_double)] * 2
+    gz = []
+    for p0, T0, base in ((1.02e5, 300.0, 20.0), (1.4e5, 340.0, 35.0), (1.0e5, 290.0, 15.0)):
+        m = base * (1.0 + 0.2 * rng.standard_normal(40))
+        po, mo = np.zeros(40), np.zeros(40)
+        mout0 = base * 0.9
+        R.ref_greitzer_steps(0.01, p0, base, mout0, T0, dp(m), 40, dp(po), dp(mo))
+        gz.append((p0, mout0, T0, m, po, mo))
+    np.savez_compressed(os.path.join(HERE, "outlets.npz"),
+                        kind=np.array([r[0] for r in recs]), U=np.array([r[1] for r in recs]),
+                        S=np.array([r[2] for r in recs]), face=np.array([r[3] for r in recs]),
+                        F=np.array([r[4] for r in recs]), profile=prof, iteration=3,
+                        gz_init=np.array([g[:3] for g in gz]), gz_mflow=np.array([g[3] for g in gz]),
+                        gz_p=np.array([g[4] for g in gz]), gz_mout=np.array([g[5] for g in gz]), kt=0.01)
+
+
+def sweep_cases():
+    """Blade rows whose body force the sweep fixtures cover."""
+    stall = cases.rotor3d(11, 5, 7, bfm_model="Hall-Thollet", off_design=True, stall=True)
+    f = stall["fields"]["deviationAngleStall"]
+    f[f > 0] = 0.6   # about half of the blade nodes stall, and some switch between sweeps
+    return {"hall": cases.rotor3d(11, 5, 7, bfm_model="Hall"), "thollet_od_stall": stall}
+
+
+def sweep_fixtures(R, tmp):
+    """Three consecutive body-force sweeps of perturbed states per case (one model object,
+    persistent side outputs), as the stages of a run see them."""
+    R.ref_bfm_sweep.argtypes = [C.POINTER(C.c_double), C.c_double] + [C.POINTER(C.c_double)] * 4
+    for name, case in sweep_cases().items():
+        ini = cases.write_case(case, os.path.join(tmp, "sweep_" + name))
+        assert R.ref_case_load(ini.encode()) == 0
+        oc = O.OracleCase(case)
+        ni, nj, nk = case["ni"], case["nj"], case["nk"]
+        Us, Rs, Fi, Fv, Dv = [], [], [], [], []
+        for q in range(3):
+            U = cases.perturb_state(case, oc.state(), 0.15, 100 + q)
+            # cross-flow momentum, so that the relative flow leaves the meridional plane
+            r2 = np.random.default_rng(200 + q)
+            U[2] += 0.1 * U[1] * r2.uniform(-1.0, 1.0, U[2].shape)
+            U[3] += 0.1 * U[1] * r2.uniform(-1.0, 1.0, U[3].shape)
+            Rr = np.zeros((5, ni, nj, nk))
+            fi, fv, dv = np.zeros((ni, nj, nk, 3)), np.zeros((ni, nj, nk, 3)), np.zeros((ni, nj, nk))
+            R.ref_bfm_sweep(dp(U), 0.0, dp(Rr), dp(fi), dp(fv), dp(dv))
+            Us.append(U); Rs.append(Rr); Fi.append(fi); Fv.append(fv); Dv.append(dv)
+        np.savez_compressed(os.path.join(HERE, "bfm_sweep_%s.npz" % name), U=np.array(Us), R=np.array(Rs),
+                            finv=np.array(Fi), fvisc=np.array(Fv), dev=np.array(Dv))
+
+
 def main():
     R = O.ref_lib()
     if R is None:
         raise SystemExit("build the reference library first: make -C oracle ref")
-    rng = np.random.default_rng(20261203)
+    groups = set(sys.argv[1:]) or {"base", "outlets", "sweep"}
     tmp = os.path.join(ROOT, "cases_tmp", "golden")
+    if "outlets" in groups:
+        outlet_fixtures(R, tmp)
+    if "sweep" in groups:
+        sweep_fixtures(R, tmp)
+    if "base" in groups:
+        base_fixtures(R, tmp)
+    print("golden fixtures written to", HERE)
+
+
+def base_fixtures(R, tmp):
+    rng = np.random.default_rng(20261203)
     for name, case in fixture_cases().items():
         ini = cases.write_case(case, os.path.join(tmp, name))
         assert R.ref_case_load(ini.encode()) == 0
@@ -164,7 +258,6 @@ def main():
     lut_fixtures(R, rng, tmp)
     thollet_fixtures(R, tmp)
     volume_fixtures(R, tmp)
-    print("golden fixtures written to", HERE)
 
 
 def volume_cases(lut_path):

@@ -10,6 +10,7 @@ never goes below 100 Pa; a mass-flow outlet without radial equilibrium is unifor
 patch's initial pressure (second value) starts its profile.
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -108,3 +109,69 @@ def test_greitzer_plenum_steps():
     a = math.sqrt(1.4 * 287.05 * T0)
     oc.residual(U, it=2, update_profiles=True)
     assert oc.profile(0)[0] == p0 + dt * a * a / vol * (m - mout)
+
+
+# ---- pins against the reference's own outlet classes and plenum model
+# (tests/golden/outlets.npz, written by BoundaryOutletThrottle, BoundaryOutletMassFlow and
+# GreitzerModel of the reference build: tests/golden/gen_golden.py outlets)
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def greitzer_case():
+    case = cases.rotor3d(11, 5, 7, outflow="throttle, 0.01")
+    case["config"].update({"ENABLE_GREITZER_MODELING": "yes", "GREITZER_PLENUM_VOLUME": "0.05",
+                           "TIME_STEP_METHOD": "fixed", "FIXED_TIME_STEP": "2e-6",
+                           "GREITZER_AMBIENT_PRESSURE": "101325"})
+    return case
+
+
+@pytest.mark.parametrize("kind, outflow", [(4, "throttle, 0.01"), (5, "mass_flow, 10")])
+def test_profile_outlet_fluxes_equal_reference(kind, outflow):
+    """The outlet flux on the patch's pressure profile at the face's j; the mass-flow
+    outlet takes the profile's last value for faces past its end."""
+    g = np.load(os.path.join(GOLD, "outlets.npz"), allow_pickle=False)
+    case = greitzer_case() if kind == 4 else cases.rotor3d(11, 5, 7, outflow=outflow)
+    oc = O.OracleCase(case)
+    oc.set_profile(0, g["profile"])
+    pidx = [p[0] for p in case["patches"]].index("OUTFLOW")
+    sel = np.nonzero(g["kind"] == kind)[0]
+    beyond = 0
+    for q in sel:
+        face = g["face"][q].astype(np.int64)
+        beyond += int(face[1] >= case["nj"])
+        out = oc.patch_flux(pidx, g["U"][q], g["S"][q], face, it=int(g["iteration"]))
+        assert np.array_equal(out, g["F"][q]), q
+    if kind == 5:
+        assert beyond > 0
+
+
+def test_greitzer_plenum_equals_reference():
+    """GreitzerModel::computePlenumPressure sequences (plenum above and below the ambient
+    pressure, so the outlet mass flow also takes its zero branch)."""
+    g = np.load(os.path.join(GOLD, "outlets.npz"), allow_pickle=False)
+    oc = O.OracleCase(greitzer_case())
+    below = 0
+    for q in range(len(g["gz_init"])):
+        p0, mout0, T0 = (float(v) for v in g["gz_init"][q])
+        p, mo = oc.greitzer_steps(p0, mout0, T0, g["gz_mflow"][q])
+        assert np.array_equal(p, g["gz_p"][q]) and np.array_equal(mo, g["gz_mout"][q]), q
+        below += int(np.any(mo == 0.0))
+    assert below > 0
+
+
+@pytest.mark.parametrize("current, expect", [(12.0, 101000.0), (8.0, 99000.0), (20.0, 102000.0)])
+def test_mass_flow_controller_known_answers(current, expect):
+    """The reference's controller known answers (test/unit_tests/test_outlet_mass_flow/
+    main.cpp:90-119): target 10, hub 100 kPa, relaxation 0.05, maximum change 0.02 -> a
+    mass flow of 12 gives 101 kPa, 8 gives 99 kPa, 20 is clamped to +2 %.  Here the
+    target is set to m * 10 / current for the state's own mass flow m, which gives the
+    same relative error."""
+    base = cases.bump2d(17, 9, outflow="mass_flow, 1, 100000")
+    oc0 = O.OracleCase(base)
+    U = oc0.state()
+    m = float(cur_mass_flow(oc0, U))
+    case = cases.bump2d(17, 9, outflow="mass_flow, %.17g, 100000" % (m * 10.0 / current))
+    oc, before, after = update(case, U)
+    assert np.all(before == 100000.0)
+    assert after[0] == pytest.approx(expect, rel=1e-12)
