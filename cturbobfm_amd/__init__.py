@@ -48,7 +48,7 @@ EXPORTS = ["tbfm_gpu_case_create", "tbfm_gpu_case_create_from_ini", "tbfm_gpu_ca
            "tbfm_gpu_batch_size", "tbfm_gpu_batch_advance", "tbfm_gpu_batch_destroy",
            "tbfm_gpu_batch_last_error", "tbfm_gpu_device_memory", "tbfm_gpu_write_solution",
            "tbfm_write_solution_from_ini", "tbfm_volume_file_name", "tbfm_initial_state_from_ini",
-           "tbfm_gpu_get_turbo_history"]
+           "tbfm_gpu_get_turbo_history", "tbfm_gpu_compute_timestep_noalloc"]
 
 _lib = None
 
@@ -77,6 +77,7 @@ def lib():
         L.tbfm_gpu_advance.argtypes = [vp, C.c_long, dp, C.POINTER(C.c_long)]
         L.tbfm_gpu_compute_residual.argtypes = [vp, dp, C.c_long, C.c_int, dp]
         L.tbfm_gpu_compute_timestep.argtypes = [vp, dp, dp]
+        L.tbfm_gpu_compute_timestep_noalloc.argtypes = [vp, vp, vp, vp]
         L.tbfm_gpu_get_side_outputs.argtypes = [vp, dp, dp, dp, dp]
         L.tbfm_gpu_get_turbo.argtypes = [vp, dp]
         L.tbfm_gpu_get_geometry.argtypes = [vp, C.c_int, dp]

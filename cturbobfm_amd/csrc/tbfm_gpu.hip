@@ -1364,6 +1364,12 @@ void tbfm_gpu_dims(const tbfm_case* c, long* ni, long* nj, long* nk) {
     *ni = c->ni; *nj = c->nj; *nk = c->nk;
 }
 
+extern "C++" {
+namespace {
+double* scratchOfCurrentDevice();
+}  // namespace
+}
+
 int tbfm_gpu_case_create(const tbfm_case_desc* d, tbfm_case** out) {
     if (!d || !out) return fail(nullptr, TBFM_ERR_ARGUMENT, "null argument");
     DevLock lk(g_dev_mu);
@@ -1389,6 +1395,7 @@ int tbfm_gpu_case_create(const tbfm_case_desc* d, tbfm_case** out) {
             cs->patches.push_back(p);
         }
         setup(cs.get(), g);
+        scratchOfCurrentDevice();   // set-up of the allocation-free time step on this GPU
         return TBFM_OK;
     });
     g_bytes = nullptr;
@@ -1411,6 +1418,7 @@ int tbfm_gpu_case_create_from_ini(const char* ini, tbfm_case** out) {
         GridInput g = readGridCsv(resolve(cs->cfg.getGridFile()));
         cs->patches = readBoundaryCsv(resolve(cs->cfg.getBoundaryFile()), cs->cfg);
         setup(cs.get(), g);
+        scratchOfCurrentDevice();   // set-up of the allocation-free time step on this GPU
         return TBFM_OK;
     });
     g_bytes = nullptr;
@@ -1919,6 +1927,89 @@ int tbfm_gpu_sync(tbfm_case* c) {
     return guarded(c, [&]() {
         DevLock lk(g_dev_mu);
         HIPCHK(hipStreamSynchronize(c->solo.stream));
+        return TBFM_OK;
+    });
+}
+
+// ------------------------------------------------------- allocation-free time step
+// Per-block minima of the global time step for tbfm_gpu_compute_timestep_noalloc, and in
+// the last slot the minimum itself.  One copy per GPU, held until the process ends;
+// every call on a GPU shares it (calls that overlap on one GPU race).  Each call writes
+// every slot it reads before reading it, so nothing has to start at zero.
+static __device__ __align__(256) double scratch[TBFM_NOALLOC_MAX_BLOCKS + 1];
+
+extern "C++" {
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};   // device address of `scratch` on each GPU
+
+// Device address of the scratch array on the current GPU, looked up once per device
+// (hipGetSymbolAddress allocates nothing).  The array's name is only a host-side
+// stand-in: kernels receive this address.
+double* scratchOfCurrentDevice() {
+    int d = 0;
+    HIPCHK(hipGetDevice(&d));
+    if (d < 0 || d >= kMaxDevices) throw std::runtime_error("device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[d]) {
+        void* ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) throw std::runtime_error("no device address for the time-step scratch array");
+        g_scratch[d] = static_cast<double*>(ptr);
+    }
+    return g_scratch[d];
+}
+
+template <int FL>
+__global__ void __launch_bounds__(256) k_timestep_noalloc(const dev::CaseDev c, const double* __restrict__ U,
+                                                          double* __restrict__ dt, double* __restrict__ part) {
+    dev::timestep_block<FL>(c, U, blockIdx.x, dt, part);
+}
+
+// minimum of the nb block minima into part[TBFM_NOALLOC_MAX_BLOCKS] (one block, the same
+// tree as k_min_reduce)
+__global__ void __launch_bounds__(256) k_min_noalloc(double* __restrict__ part, int nb) {
+    __shared__ double smin[256];
+    double m = 1.0e300;
+    for (int q = threadIdx.x; q < nb; q += 256) m = dev::dmind(m, part[q]);
+    smin[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) smin[threadIdx.x] = dev::dmind(smin[threadIdx.x], smin[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[TBFM_NOALLOC_MAX_BLOCKS] = smin[0];
+}
+
+// the global minimum at every node
+__global__ void __launch_bounds__(256) k_fill_noalloc(const double* __restrict__ part, double* __restrict__ dt,
+                                                      long N) {
+    const long n = (long)blockIdx.x * 256 + threadIdx.x;
+    if (n < N) dt[n] = part[TBFM_NOALLOC_MAX_BLOCKS];
+}
+}  // namespace
+}  // extern "C++"
+
+int tbfm_gpu_compute_timestep_noalloc(tbfm_case* c, const double* dU, double* ddt, void* stream) {
+    if (!c || !dU || !ddt) return fail(c, TBFM_ERR_ARGUMENT, "null argument");
+    if (c->N > (long)TBFM_NOALLOC_MAX_BLOCKS * 256)
+        return fail(c, TBFM_ERR_ARGUMENT,
+                    "grid of " + std::to_string(c->N) + " nodes exceeds the allocation-free time step's limit of " +
+                        std::to_string((long)TBFM_NOALLOC_MAX_BLOCKS * 256) + " nodes");
+    return guarded(c, [&]() {
+        double* part = scratchOfCurrentDevice();
+        hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->solo.stream;
+        const int nb = (int)((c->N + 255) / 256);
+        if (c->c.real)
+            hipLaunchKernelGGL(k_timestep_noalloc<1>, dim3((unsigned)nb), dim3(256), 0, st, c->c, dU, ddt, part);
+        else
+            hipLaunchKernelGGL(k_timestep_noalloc<0>, dim3((unsigned)nb), dim3(256), 0, st, c->c, dU, ddt, part);
+        if (c->c.dt_mode == 1) {
+            hipLaunchKernelGGL(k_min_noalloc, dim3(1), dim3(256), 0, st, part, nb);
+            hipLaunchKernelGGL(k_fill_noalloc, dim3((unsigned)nb), dim3(256), 0, st, (const double*)part, ddt, c->N);
+        }
+        HIPCHK(hipGetLastError());
         return TBFM_OK;
     });
 }

@@ -163,9 +163,12 @@ __device__ inline void sources_block(const CaseDev& c, const Launch& La, int bx)
 
 // --------------------------------------------------------------------- time step
 // Solver::computeTimestepArray, inviscid (src/solver.cpp:1531-1630)
-// One block of 256 nodes (bx = block index within the case).
+// One block of 256 nodes (bx = block index within the case); writes the nodes' time
+// steps to dt_out and, for the global time step (or the turbomachinery time), the
+// block's minimum to part_out[bx].
 template <int FL>
-__device__ inline void timestep_block(const CaseDev& c, const double* U, int bx) {
+__device__ inline void timestep_block(const CaseDev& c, const double* U, int bx, double* dt_out,
+                                      double* part_out) {
     __shared__ double smin[256];
     const long n = (long)bx * 256 + threadIdx.x;
     double dtv = 1.0e300;
@@ -210,7 +213,7 @@ __device__ inline void timestep_block(const CaseDev& c, const double* U, int bx)
             double lv = 4.0 * (mu / W[0] + 0.0 / W[0]) * invH2;
             dtv = c.cfl / (lam + lv);
         }
-        c.dt[n] = dtv;
+        dt_out[n] = dtv;
     }
     if (c.dt_mode == 1 || c.turbo_on) {
         smin[threadIdx.x] = dtv;
@@ -219,7 +222,7 @@ __device__ inline void timestep_block(const CaseDev& c, const double* U, int bx)
             if (threadIdx.x < s) smin[threadIdx.x] = dmind(smin[threadIdx.x], smin[threadIdx.x + s]);
             __syncthreads();
         }
-        if (threadIdx.x == 0) c.partial_min[bx] = smin[0];
+        if (threadIdx.x == 0) part_out[bx] = smin[0];
     }
 }
 
@@ -486,8 +489,8 @@ __global__ void __launch_bounds__(256) k_iterstart(Launch L) {
     if (*c.stop) return;
     const double* U = c.U[L.in];
     if (w.z == 1) radial_profile_block(c, U);
-    else if (c.real) timestep_block<1>(c, U, w.x);
-    else timestep_block<0>(c, U, w.x);
+    else if (c.real) timestep_block<1>(c, U, w.x, c.dt, c.partial_min);
+    else timestep_block<0>(c, U, w.x, c.dt, c.partial_min);
 }
 
 // ------------------------------------------------------------- boundary fluxes

@@ -111,6 +111,20 @@ int tbfm_gpu_compute_residual(tbfm_case* c, const double* U, long it, int update
 /* Solver::computeTimestepArray (solver.cpp:1531-1630) of the given state; dt is N. */
 int tbfm_gpu_compute_timestep(tbfm_case* c, const double* U, double* dt);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * the time step of tbfm_gpu_compute_timestep for a state already in device memory,
+ * computed without allocating.  dU: 5 planes of N doubles in the library's device
+ * layout (node (k*NJ + j)*NI + i, i fastest); ddt: N doubles, same layout (the global
+ * time step writes the domain minimum at every node).  Kernels are enqueued on `stream`
+ * (a hipStream_t; NULL: the case's stream) and the call returns without waiting.  The
+ * per-block minima of the global time step live in a file-scope device array of
+ * TBFM_NOALLOC_MAX_BLOCKS + 1 doubles (512 KiB per GPU, held until the process ends), so
+ * grids above TBFM_NOALLOC_MAX_BLOCKS * 256 nodes are refused; calls that overlap on one
+ * GPU share that array and race. */
+#define TBFM_NOALLOC_MAX_BLOCKS 65536
+int tbfm_gpu_compute_timestep_noalloc(tbfm_case* c, const double* dU, double* ddt, void* stream);
+
 /* Side outputs the reference keeps (solver.hpp:370-371): inviscid and viscous body
  * force (3*N, xyz per node), deviation angle (N) and time step (N); any may be NULL. */
 int tbfm_gpu_get_side_outputs(tbfm_case* c, double* finv, double* fvisc, double* dev, double* dt);
