@@ -22,13 +22,19 @@
 namespace tbfm {
 namespace dev {
 
+// A tile of TI x TJ (i, j) columns, one per thread of the block (NC <= 256: the threads
+// beyond the tile only stage halo positions and evaluate faces); the LDS plane adds a
+// 2-node halo on every side.
 template <int TI_, int TJ_>
 struct Tile {
     static constexpr int TI = TI_, TJ = TJ_;
+    static constexpr int NC = TI * TJ;
     static constexpr int HI = TI + 4, HJ = TJ + 4, NP = HI * HJ;
+    static constexpr int NH = NP - NC;   // halo positions
     static constexpr int NFI = 5 * TJ * (TI + 1), NFJ = 5 * (TJ + 1) * TI;
-    static_assert(TI * TJ == NT, "tiles have 256 columns");
-    static_assert(7 * NP >= NFI + NFJ, "face buffers must fit in the primitive region");
+    static_assert(NC <= NT && NC > NT - 64, "a tile fills all but part of one wavefront");
+    static_assert(NP >= NT, "the norm tree reuses one primitive plane");
+    static_assert(TJ * (TI + 1) + (TJ + 1) * TI <= 3 * NT, "at most three faces per thread");
 };
 
 template <class TL>
@@ -174,7 +180,58 @@ struct KQueue;
 
 template <int FL>
 struct KQueue<0, FL> {
-    // primitives of slots 1, 2 without the density, which is U[slot][0]
+    // primitives without the density (which is U[slot][0]) and spectral factor of slots
+    // 1, 2, 3 (Wv[s - 1], RF[s - 1]): each plane is derived once, when it enters slot 3
+    double U[4][5], P[4], Wv[3][4], RF[3];
+    __device__ __forceinline__ void put(const CaseDev& c, int s, const double* u) {
+        double w[5], rf;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) U[s][v] = u[v];
+        derive<double, FL>(U[s], w, P[s], rf, Gas<FL>(c));
+        if (s >= 1) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Wv[s - 1][v] = w[v + 1];
+            RF[s - 1] = rf;
+        }
+    }
+    __device__ __forceinline__ void init_end(const CaseDev&) {}
+    __device__ __forceinline__ void shift() {
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+#pragma unroll
+            for (int v = 0; v < 5; ++v) U[s][v] = U[s + 1][v];
+            P[s] = P[s + 1];
+        }
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Wv[s][v] = Wv[s + 1][v];
+            RF[s] = RF[s + 1];
+        }
+    }
+    __device__ __forceinline__ void after_shift(const CaseDev&) {}
+    __device__ __forceinline__ void Wc(double w[5]) const {
+        w[0] = U[1][0];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) w[v + 1] = Wv[0][v];
+    }
+    template <typename FT>
+    __device__ __forceinline__ void face(const CaseDev& c, const double nn[4], double out[5]) const {
+        const double sl = jst_sensor<double>(P[0], P[1], P[2]);
+        const double sr = jst_sensor<double>(P[1], P[2], P[3]);
+        const double Wl[5] = {U[1][0], Wv[0][0], Wv[0][1], Wv[0][2], Wv[0][3]};
+        const double Wr[5] = {U[2][0], Wv[1][0], Wv[1][1], Wv[1][2], Wv[1][3]};
+        jst_face<FT, FL>(U[0], U[1], U[2], U[3], Wl, Wr, RF[0], RF[1], sl, sr, nn[0], nn[1], nn[2], nn[3],
+                         Gas<FL>(c), out);
+    }
+};
+
+// The table fluid keeps only slots 1 and 2 derived (its derivations cost registers that
+// would take the kernel to one wavefront per SIMD): a plane is derived for its pressure
+// when it enters slot 3 and again for its primitives when it reaches slot 2.
+template <>
+struct KQueue<0, 1> {
+    static constexpr int FL = 1;
     double U[4][5], P[4], Wv[2][4], RF[2];
     __device__ __forceinline__ void put(const CaseDev& c, int s, const double* u) {
         double w[5], rf;
@@ -301,6 +358,12 @@ __device__ __forceinline__ int halo_pos(int h) {
     return row * HI + ((col < 2) ? col : TL::TI + col);
 }
 
+// Face s (0, 1, 2) of thread tid in the flat I/J face index of a tile: tid, tid + 256,
+// and for the faces beyond 512 the threads from the top down.
+__device__ __forceinline__ int face_of(int s, int tid) {
+    return s == 0 ? tid : (s == 1 ? tid + NT : 3 * NT - 1 - tid);
+}
+
 // The stage operands of one block: its case's buffers for this launch's roles.
 struct StageOps {
     const double* Uin;     // U^(k-1)
@@ -312,7 +375,7 @@ struct StageOps {
 
 template <class TL, int SCH, bool THREED, typename FT, int FL>
 __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, const Work w, const StageOps st) {
-    constexpr int TI = TL::TI, TJ = TL::TJ, HI = TL::HI, NP = TL::NP;
+    constexpr int TI = TL::TI, TJ = TL::TJ, HI = TL::HI, NH = TL::NH;
     constexpr int NFI = TJ * (TI + 1), NFJ = (TJ + 1) * TI;
     __shared__ PlaneLds<TL> L;
     const CaseDev& c = *cp;
@@ -321,7 +384,8 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
     const int i0 = w.x * TI, j0 = w.y * TJ;
     const int i = i0 + ti, j = j0 + tj;
     const int NI = c.ni, NJ = c.nj, NK = c.nk;
-    const bool act = (i < NI) && (j < NJ);
+    const bool col = tid < TL::NC;   // the thread owns a column of the tile
+    const bool act = col && (i < NI) && (j < NJ);
     // the block's k planes: piece w.z of sgz balanced pieces of [0, NK)
     const int kc0 = (w.z * NK) / c.sgz;
     const int kc1 = ((w.z + 1) * NK) / c.sgz;
@@ -362,7 +426,7 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
     // current plane's faces are evaluated.
     double pre[5];
     if (THREED && act && kc0 + 2 <= NK) load_k(c, Uin, i, j, kc0 + 2, pre);
-    const bool hown = tid < NP - NT;
+    const bool hown = tid < NH;
     const int hpos = hown ? halo_pos<TL>(tid) : 0;
     double hU[5];
     bool hok = false;
@@ -421,23 +485,32 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
             }
         }
         // tile positions outside the domain (possibly ghosts) and the halo by contiguous lanes
-        if (!act) fill_position<TL, SCH, FL>(c, Uin, L, own, i0, j0, k);
+        if (col && !act) fill_position<TL, SCH, FL>(c, Uin, L, own, i0, j0, k);
         if (hown) {
             commit_position<TL, SCH, FL>(c, L, hpos, hok, hU);
             if (k + 1 < kc1) hok = fetch_position<TL>(c, Uin, hpos, i0, j0, k + 1, hU);
         }
-        for (int h = tid + NT; h < NP - NT; h += NT) fill_position<TL, SCH, FL>(c, Uin, L, halo_pos<TL>(h), i0, j0, k);
+        for (int h = tid + NT; h < NH; h += NT) fill_position<TL, SCH, FL>(c, Uin, L, halo_pos<TL>(h), i0, j0, k);
         __syncthreads();
         // ---- every I and J face of the tile once: fq < NFI are I faces (row r, face m),
-        // then J faces (row r of faces, column m); one code path, no divergence.  The unit
-        // normal and area of the thread's next face are requested before the current face
-        // is evaluated (those of its first face before the plane was staged).
-        for (int fq = tid; fq < nfaces; fq += NT) {
+        // then J faces (row r of faces, column m); one code path, no divergence.  Each
+        // thread takes faces tid and tid + 256; the faces beyond 512 go to the last
+        // threads (the halo slots went to the first ones), so that no wavefront does both
+        // a halo position and a third face.  The unit normal and area of the thread's
+        // next face are requested before the current face is evaluated (those of its
+        // first face before the plane was staged).
+#pragma unroll 1
+        for (int s = 0; s < 3; ++s) {
+            const int fq = face_of(s, tid);
+            if (fq >= nfaces) continue;
             const FaceIdx<TL> f(fq, i0, j0, NI, NJ);
             double nn[4] = {nnx[0], nnx[1], nnx[2], nnx[3]};
-            if (fq + NT < nfaces) {
-                const FaceIdx<TL> g(fq + NT, i0, j0, NI, NJ);
-                if (g.valid && !g.bnd) face_normal(c, g, k, nnx);
+            if (s < 2) {
+                const int fn = face_of(s + 1, tid);
+                if (fn < nfaces) {
+                    const FaceIdx<TL> g(fn, i0, j0, NI, NJ);
+                    if (g.valid && !g.bnd) face_normal(c, g, k, nnx);
+                }
             }
             if (!f.valid) continue;
             double F[5];

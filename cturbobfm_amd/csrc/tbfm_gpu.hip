@@ -123,7 +123,7 @@ struct tbfm_case {
     CaseDev* cdev = nullptr;          // device copy: the case table of the case's own plan
     long hist_cap = 0;
     long iter_host = 1;               // next iteration number
-    int tile = 0;                     // 0: 32x8, 1: 16x16, 2: 64x4 (i x j columns)
+    int tile = 0;                     // 0: 32x8, 1: 16x16, 2: 64x4, 3: 28x9 (i x j columns)
     bool tuned = false;
     std::string tuning;               // autotuner report
     Plan solo;                        // the case advancing alone
@@ -184,12 +184,16 @@ void faceSoA(const std::vector<double>& src, long ai, long aj, long ak, std::vec
             }
 }
 
-int tileI(int t) { return t == 0 ? 32 : (t == 1 ? 16 : 64); }
+// Stage-kernel tile shapes (i x j columns): 0: 32x8, 1: 16x16, 2: 64x4, 3: 28x9 (252
+// columns: fewer idle columns on grids such as 193 x 49)
+constexpr int kTiles = 4;
+int tileI(int t) { static const int v[kTiles] = {32, 16, 64, 28}; return v[t]; }
+int tileJ(int t) { static const int v[kTiles] = {8, 16, 4, 9}; return v[t]; }
 
 // Tile shape and the number of k pieces: block z of a column of tiles marches planes
 // [z*NK/pieces, (z+1)*NK/pieces).
 void setTiling(tbfm_case* cs, int tile, int pieces) {
-    const int ti = tileI(tile), tj = 256 / ti;
+    const int ti = tileI(tile), tj = tileJ(tile);
     CaseDev& c = cs->c;
     cs->tile = tile;
     c.sgz = std::max(1, std::min(pieces, (int)cs->nk));
@@ -779,8 +783,8 @@ void setup(tbfm_case* cs, const GridInput& grid) {
     c.nparts_per = (periodic && cs->topo == Topology::ThreeD) ? (int)((NI * NJ + 255) / 256) : 0;
     // norm partials sized for the finest tiling the autotuner may pick
     long maxParts = 0;
-    for (int t = 0; t < 3; ++t) {
-        const int ti = tileI(t), tj = 256 / ti;
+    for (int t = 0; t < kTiles; ++t) {
+        const int ti = tileI(t), tj = tileJ(t);
         maxParts = std::max(maxParts, ((NI + ti - 1) / ti) * ((NJ + tj - 1) / tj) * NK);
     }
     c.partial = dalloc<double>(5 * (maxParts + c.nparts_per), own);
@@ -1115,6 +1119,7 @@ void launchStageS(const Plan& P, const StageGroup& g, const dev::Launch& L) {
     switch (g.tile) {
         case 1: launchStageT<dev::Tile<16, 16>, SCH, THREED>(P, g, L); break;
         case 2: launchStageT<dev::Tile<64, 4>, SCH, THREED>(P, g, L); break;
+        case 3: launchStageT<dev::Tile<28, 9>, SCH, THREED>(P, g, L); break;
         default: launchStageT<dev::Tile<32, 8>, SCH, THREED>(P, g, L); break;
     }
 }
@@ -1254,7 +1259,7 @@ void autotune(tbfm_case* cs) {
         auto it = g_tuned.find(key);
         if (it != g_tuned.end()) {
             setTiling(cs, it->second.first, it->second.second);
-            cs->tuning = "cached -> " + std::to_string(tileI(cs->tile)) + "x" + std::to_string(256 / tileI(cs->tile)) +
+            cs->tuning = "cached -> " + std::to_string(tileI(cs->tile)) + "x" + std::to_string(tileJ(cs->tile)) +
                          "/z" + std::to_string(cs->c.sgz);
             pushCase(cs);
             return;
@@ -1274,8 +1279,8 @@ void autotune(tbfm_case* cs) {
     // (m * 256 blocks: one or two resident blocks per CU of 256) and the plain planes-per-
     // piece choices 2, 4, 8, 16
     std::vector<std::pair<int, int>> cand;
-    for (int t = 0; t < 3; ++t) {
-        const long tiles = ((cs->ni + tileI(t) - 1) / tileI(t)) * ((cs->nj + 256 / tileI(t) - 1) / (256 / tileI(t)));
+    for (int t = 0; t < kTiles; ++t) {
+        const long tiles = ((cs->ni + tileI(t) - 1) / tileI(t)) * ((cs->nj + tileJ(t) - 1) / tileJ(t));
         std::vector<int> zs;
         if (cs->topo == Topology::ThreeD) {
             for (int m = 1; m <= 8; ++m) zs.push_back((int)(m * 256 / tiles));
@@ -1308,7 +1313,7 @@ void autotune(tbfm_case* cs) {
             }
         }
     for (size_t q = 0; q < cand.size(); ++q) {
-        std::snprintf(buf, sizeof buf, "%dx%d/z%d:%.4fms ", tileI(cand[q].first), 256 / tileI(cand[q].first),
+        std::snprintf(buf, sizeof buf, "%dx%d/z%d:%.4fms ", tileI(cand[q].first), tileJ(cand[q].first),
                       cand[q].second, tmin[q]);
         cs->tuning += buf;
         if (tmin[q] < best) { best = tmin[q]; bt = cand[q].first; bk = cand[q].second; }
@@ -1318,7 +1323,7 @@ void autotune(tbfm_case* cs) {
     setTiling(cs, bt, bk);
     pushCase(cs);
     buildTables(P);
-    std::snprintf(buf, sizeof buf, "-> %dx%d/z%d", tileI(bt), 256 / tileI(bt), bk);
+    std::snprintf(buf, sizeof buf, "-> %dx%d/z%d", tileI(bt), tileJ(bt), bk);
     cs->tuning += buf;
     std::lock_guard<std::mutex> lk(g_tuned_mu);
     g_tuned[key] = {bt, bk};
