@@ -94,6 +94,57 @@ __device__ __forceinline__ bool fetch_position(const CaseDev& c, const double* _
     return true;
 }
 
+// Where LDS position `pos` of every plane comes from, resolved once per block: kind 0 a
+// position no face reads, 1 a real node at in-plane offset a, 2 / 3 a ghost node built
+// from boundary node a and its inner neighbour b (2: before the domain start, 3: after
+// its end), exactly as fetch_position does plane by plane.
+struct HaloSrc {
+    int kind, a, b;
+};
+template <class TL>
+__device__ __forceinline__ HaloSrc halo_src(const CaseDev& c, int pos, int i0, int j0) {
+    const int hi = pos % TL::HI, hj = pos / TL::HI;
+    const int gi = i0 - 2 + hi, gj = j0 - 2 + hj;
+    const int NI = c.ni, NJ = c.nj;
+    const bool iin = (gi >= 0 && gi < NI), jin = (gj >= 0 && gj < NJ);
+    HaloSrc h{0, 0, 0};
+    if (iin && jin) {
+        h.kind = 1;
+        h.a = gj * NI + gi;
+        return h;
+    }
+    const bool ighost = jin && (gi == -1 || gi == NI);
+    const bool jghost = iin && (gj == -1 || gj == NJ) && c.topo != TOPO_1D;
+    if (!ighost && !jghost) return h;
+    bool lo;
+    if (ighost) {
+        lo = gi < 0;
+        h.a = gj * NI + (lo ? gi + 1 : gi - 1);
+        h.b = gj * NI + (lo ? gi + 2 : gi - 2);
+    } else {
+        lo = gj < 0;
+        h.a = (lo ? gj + 1 : gj - 1) * NI + gi;
+        h.b = (lo ? gj + 2 : gj - 2) * NI + gi;
+    }
+    h.kind = lo ? 2 : 3;
+    return h;
+}
+// The conserved state of a resolved position on the plane starting at node `plane`.
+__device__ __forceinline__ bool fetch_src(const double* __restrict__ U, int N, const HaloSrc& h, int plane,
+                                          double Un[5]) {
+    if (h.kind == 0) return false;
+    if (h.kind == 1) {
+        load5(U, N, plane + h.a, Un);
+        return true;
+    }
+    double A[5], B[5];
+    load5(U, N, plane + h.a, A);
+    load5(U, N, plane + h.b, B);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) Un[v] = (h.kind == 2) ? A[v] - (B[v] - A[v]) : A[v] + (A[v] - B[v]);
+    return true;
+}
+
 // Stores what the faces read at LDS position `pos` (JST: state, primitives, pressure,
 // spectral factor; Roe: primitives); unused positions get harmless constants.
 template <class TL, int SCH, int FL>
@@ -327,21 +378,26 @@ struct FaceIdx {
 };
 
 // Unit normal and area of K face f (index (k*NJ + j)*NI + i: the face below plane k).
+// Face record `rec` for the stage kernel of flux type FT: the FP64 records, or for the
+// FP32-flux kernels the FP32 records (their face arithmetic takes the float value of
+// every component, so the results are the same and the loads half as wide).
+template <typename FT>
+__device__ __forceinline__ void face_record(const CaseDev& c, unsigned rec, double nn[4]) {
+    if constexpr (sizeof(FT) == 4) gld_face32(c.nrm32, rec, nn);
+    else gld_face(c.nrm, rec, nn);
+}
+
+template <typename FT>
 __device__ __forceinline__ void kface_normal(const CaseDev& c, int f, double nn[4]) {
-    const double* nk = c.nrm + c.noffK;
-    const long nf = c.nfK;
-#pragma unroll
-    for (int v = 0; v < 4; ++v) nn[v] = gld(nk + v * nf, (unsigned)f);
+    face_record<FT>(c, (unsigned)(c.noffK + f), nn);
 }
 
 // Unit normal and area of an interior I or J face on plane k.
-template <class TL>
+template <class TL, typename FT>
 __device__ __forceinline__ void face_normal(const CaseDev& c, const FaceIdx<TL>& f, int k, double nn[4]) {
     const int NI = c.ni, NJ = c.nj;
-    const int fst = f.isI ? c.nfI : c.nfJ;
     const int fo = f.isI ? (k * NJ + f.gj) * (NI + 1) + f.gi : c.noffJ + (k * (NJ + 1) + f.gj) * NI + f.gi;
-#pragma unroll
-    for (int v = 0; v < 4; ++v) nn[v] = gld(c.nrm + (long)v * fst, (unsigned)fo);
+    face_record<FT>(c, (unsigned)fo, nn);
 }
 
 // LDS position of halo slot h (0 .. NP-NT-1): two full rows above, two below, and the two
@@ -416,7 +472,7 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
                 for (int v = 0; v < 5; ++v) FKl[v] = gld(c.bnd + c.boffK + (long)v * NIJ, (unsigned)(j * NI + i));
             } else {
                 double nk[4];
-                kface_normal(c, (kc0 * NJ + j) * NI + i, nk);
+                kface_normal<FT>(c, (kc0 * NJ + j) * NI + i, nk);
                 q.template face<FT>(c, nk, FKl);
             }
         }
@@ -428,9 +484,10 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
     if (THREED && act && kc0 + 2 <= NK) load_k(c, Uin, i, j, kc0 + 2, pre);
     const bool hown = tid < NH;
     const int hpos = hown ? halo_pos<TL>(tid) : 0;
+    const HaloSrc hsrc = hown ? halo_src<TL>(c, hpos, i0, j0) : HaloSrc{0, 0, 0};
     double hU[5];
     bool hok = false;
-    if (hown) hok = fetch_position<TL>(c, Uin, hpos, i0, j0, kc0, hU);
+    if (hown) hok = fetch_src(Uin, N, hsrc, kc0 * NIJ, hU);
 
 #pragma unroll 1
     for (int k = kc0; k < kc1; ++k) {
@@ -439,11 +496,11 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
         // unit normal and area of the K face above the node and of the thread's first I/J
         // face of this plane, requested before the plane is staged
         double nnk[4] = {1.0, 0.0, 0.0, 1.0};
-        if (THREED && act && !bladecol && k + 1 < NK) kface_normal(c, ((k + 1) * NJ + j) * NI + i, nnk);
+        if (THREED && act && !bladecol && k + 1 < NK) kface_normal<FT>(c, ((k + 1) * NJ + j) * NI + i, nnk);
         double nnx[4] = {1.0, 0.0, 0.0, 1.0};
         {
             const FaceIdx<TL> f(tid, i0, j0, NI, NJ);
-            if (tid < nfaces && f.valid && !f.bnd) face_normal(c, f, k, nnx);
+            if (tid < nfaces && f.valid && !f.bnd) face_normal<TL, FT>(c, f, k, nnx);
         }
         // update operands, read early and consumed at the update
         double u0v[5], dtv = 0.0, volv = 1.0;
@@ -488,7 +545,7 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
         if (col && !act) fill_position<TL, SCH, FL>(c, Uin, L, own, i0, j0, k);
         if (hown) {
             commit_position<TL, SCH, FL>(c, L, hpos, hok, hU);
-            if (k + 1 < kc1) hok = fetch_position<TL>(c, Uin, hpos, i0, j0, k + 1, hU);
+            if (k + 1 < kc1) hok = fetch_src(Uin, N, hsrc, (k + 1) * NIJ, hU);
         }
         for (int h = tid + NT; h < NH; h += NT) fill_position<TL, SCH, FL>(c, Uin, L, halo_pos<TL>(h), i0, j0, k);
         __syncthreads();
@@ -509,7 +566,7 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
                 const int fn = face_of(s + 1, tid);
                 if (fn < nfaces) {
                     const FaceIdx<TL> g(fn, i0, j0, NI, NJ);
-                    if (g.valid && !g.bnd) face_normal(c, g, k, nnx);
+                    if (g.valid && !g.bnd) face_normal<TL, FT>(c, g, k, nnx);
                 }
             }
             if (!f.valid) continue;

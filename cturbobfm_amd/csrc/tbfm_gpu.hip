@@ -498,12 +498,13 @@ void setup(tbfm_case* cs, const GridInput& grid) {
     c.sjx = upload(x, own); c.sjy = upload(y, own); c.sjz = upload(z, own);
     faceSoA(g.surfK, NI, NJ, NK + 1, x, y, z);
     c.skx = upload(x, own); c.sky = upload(y, own); c.skz = upload(z, own);
-    // unit normals and areas: [I: nx ny nz |S|][J: ...][K: ...] in one array
+    // unit normals and areas, one 32-byte record (nx ny nz |S|) per face: I faces, then
+    // J faces from record noffJ, then K faces from record noffK
     c.nfI = (int)((NI + 1) * NJ * NK);
     c.nfJ = (int)(NI * (NJ + 1) * NK);
     c.nfK = (int)(NI * NJ * (NK + 1));
-    c.noffJ = 4 * c.nfI;
-    c.noffK = 4 * (c.nfI + c.nfJ);
+    c.noffJ = c.nfI;
+    c.noffK = c.nfI + c.nfJ;
     {
         std::vector<double> nrm(4L * (c.nfI + c.nfJ + c.nfK));
         auto normals = [&](const std::vector<double>& S, long ai, long aj, long ak, long off) {
@@ -513,16 +514,26 @@ void setup(tbfm_case* cs, const GridInput& grid) {
             for (long q = 0; q < nf; ++q) {
                 // computeAdvectionFluxFromPrimitive: normal = surface / surface.magnitude()
                 const double m = std::sqrt(bx[q] * bx[q] + by[q] * by[q] + bz[q] * bz[q]);
-                nrm[off + q] = bx[q] / m;
-                nrm[off + nf + q] = by[q] / m;
-                nrm[off + 2 * nf + q] = bz[q] / m;
-                nrm[off + 3 * nf + q] = m;
+                double* r = &nrm[4 * (off + q)];
+                r[0] = bx[q] / m;
+                r[1] = by[q] / m;
+                r[2] = bz[q] / m;
+                r[3] = m;
             }
         };
         normals(g.surfI, NI + 1, NJ, NK, 0);
         normals(g.surfJ, NI, NJ + 1, NK, c.noffJ);
         normals(g.surfK, NI, NJ, NK + 1, c.noffK);
-        c.nrm = upload(nrm, own);
+        c.fp32 = (cs->precision == Precision::Mixed) ? 1 : 0;
+        if (cs->precision == Precision::Mixed) {
+            // the FP32-flux kernels read 16-byte FP32 records (the values they compute
+            // with: the FP64 values rounded to float)
+            std::vector<float> n32(nrm.size());
+            for (size_t q = 0; q < nrm.size(); ++q) n32[q] = (float)nrm[q];
+            c.nrm32 = upload(n32, own);
+        } else {
+            c.nrm = upload(nrm, own);
+        }
     }
     faceSoA(g.dual, NI + 1, NJ + 1, NK + 1, x, y, z);
     c.dux = upload(x, own); c.duy = upload(y, own); c.duz = upload(z, own);
@@ -1253,7 +1264,8 @@ void autotune(tbfm_case* cs) {
     Plan& P = cs->solo;
     const std::string key = std::to_string(cs->ni) + "x" + std::to_string(cs->nj) + "x" + std::to_string(cs->nk) + "/" +
                             std::to_string((int)cs->scheme) + std::to_string(cs->c.muscl) + "/" +
-                            std::to_string((int)cs->topo) + "/" + std::to_string((int)cs->precision);
+                            std::to_string((int)cs->topo) + "/" + std::to_string((int)cs->precision) + "/" +
+                            std::to_string(cs->c.real);
     {
         std::lock_guard<std::mutex> lk(g_tuned_mu);
         auto it = g_tuned.find(key);

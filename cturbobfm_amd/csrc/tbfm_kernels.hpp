@@ -79,10 +79,12 @@ struct CaseDev {
     // geometry
     const double *six, *siy, *siz, *sjx, *sjy, *sjz, *skx, *sky, *skz;
     // unit normals S/|S| and areas |S| of every face, computed on the host with the
-    // reference's own operations (Vector3D::magnitude, operator/), in one allocation:
-    // per direction four planes (nx, ny, nz, |S|) of nfI / nfJ / nfK faces (layout as S),
-    // the I block at 0, the J block at noffJ, the K block at noffK
+    // reference's own operations (Vector3D::magnitude, operator/), in one allocation of
+    // 32-byte records (nx, ny, nz, |S|), faces in the order of S per direction: the I
+    // faces from record 0, the J faces from record noffJ, the K faces from record noffK
+    int fp32;             // GPU_PRECISION = mixed: FP32 flux, dissipation and source arithmetic
     const double* nrm;
+    const float* nrm32;   // FP32-flux cases: the same records rounded to float (16 bytes each)
     int nfI, nfJ, nfK, noffJ, noffK;
     const double *vol;
     const double *dux, *duy, *duz;     // dual nodes (NI+1)(NJ+1)(NK+1)
@@ -212,6 +214,19 @@ __device__ __forceinline__ void gst(T* base, unsigned idx, T v) {
     *(TBFM_GLOBAL T*)((TBFM_GLOBAL char*)base + idx * (unsigned)sizeof(T)) = v;
 }
 
+// The 32-byte record of face `rec` (unit normal and area) as two 16-byte loads.
+typedef double dpair __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void gld_face(const double* base, unsigned rec, double out[4]) {
+    const TBFM_GLOBAL dpair* p = (const TBFM_GLOBAL dpair*)((const TBFM_GLOBAL char*)base + rec * 32u);
+    const dpair a = p[0], b = p[1];
+    out[0] = a.x; out[1] = a.y; out[2] = b.x; out[3] = b.y;
+}
+
+typedef float fquad __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void gld_face32(const float* base, unsigned rec, double out[4]) {
+    const fquad a = *(const TBFM_GLOBAL fquad*)((const TBFM_GLOBAL char*)base + rec * 16u);
+    out[0] = a.x; out[1] = a.y; out[2] = a.z; out[3] = a.w;
+}
 __device__ __forceinline__ double dmaxd(double a, double b) { return (a < b) ? b : a; }
 __device__ __forceinline__ double dmind(double a, double b) { return (b < a) ? b : a; }
 

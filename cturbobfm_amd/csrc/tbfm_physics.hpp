@@ -54,6 +54,13 @@ __device__ __forceinline__ double sound_speed_w(const Gas<FL>& gs, const double 
     else return gs.at(LUT_A, W[0], W[4] - 0.5 * (W[1] * W[1] + W[2] * W[2] + W[3] * W[3]));
 }
 
+// The same in the source type T of the body force (T = float: the ideal gas only).
+template <int FL, typename T>
+__device__ __forceinline__ T sound_speed_wt(const Gas<FL>& gs, const T W[5], T p) {
+    if constexpr (sizeof(T) == 8) return sound_speed_w<FL>(gs, W, p);
+    else return sqrt(T(gs.g) * p / W[0]);
+}
+
 // Speed of sound from p and rho (computeSoundSpeed_p_rho): ideal sqrt(g p / rho); real
 // a(rho, e(p, rho)) with the bisection for e (fluid_real.cpp:254-257).
 template <int FL>
@@ -476,143 +483,166 @@ __device__ inline void wall_flux(const Gas<FL>& gs, const double U[5], double sx
 }
 
 // ------------------------------------------------------------ body-force sources
-struct V3 {
-    double x, y, z;
+// Vector3D arithmetic of the reference (include/types.hpp) in the source type T: double
+// on the plain path, float for the FP32-source arithmetic of the mixed-precision path.
+template <typename T>
+struct V3T {
+    T x, y, z;
 };
-__device__ __forceinline__ V3 v3(double x, double y, double z) { return V3{x, y, z}; }
-__device__ __forceinline__ double vdot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ double vmag(V3 a) { return sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
-__device__ __forceinline__ V3 vsub(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
-__device__ __forceinline__ V3 vmul(V3 a, double s) { return v3(a.x * s, a.y * s, a.z * s); }
-__device__ __forceinline__ V3 vdiv(V3 a, double s) { return v3(a.x / s, a.y / s, a.z / s); }
-__device__ __forceinline__ V3 vnrm(V3 a) { double m = vmag(a); return (m == 0) ? v3(0, 0, 0) : vdiv(a, m); }
-__device__ __forceinline__ V3 cyl_of(V3 v, double cs, double sn) { return v3(v.x, v.y * cs + v.z * sn, -v.y * sn + v.z * cs); }
-__device__ __forceinline__ V3 cart_of(V3 v, double cs, double sn) { return v3(v.x, v.y * cs - v.z * sn, v.y * sn + v.z * cs); }
+using V3 = V3T<double>;
+template <typename T>
+__device__ __forceinline__ V3T<T> v3(T x, T y, T z) { return V3T<T>{x, y, z}; }
+template <typename T>
+__device__ __forceinline__ T vdot(V3T<T> a, V3T<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <typename T>
+__device__ __forceinline__ T vmag(V3T<T> a) { return sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+template <typename T>
+__device__ __forceinline__ V3T<T> vsub(V3T<T> a, V3T<T> b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+template <typename T>
+__device__ __forceinline__ V3T<T> vmul(V3T<T> a, T s) { return v3(a.x * s, a.y * s, a.z * s); }
+template <typename T>
+__device__ __forceinline__ V3T<T> vdiv(V3T<T> a, T s) { return v3(a.x / s, a.y / s, a.z / s); }
+template <typename T>
+__device__ __forceinline__ V3T<T> vnrm(V3T<T> a) {
+    T m = vmag(a);
+    return (m == T(0)) ? v3(T(0), T(0), T(0)) : vdiv(a, m);
+}
+template <typename T>
+__device__ __forceinline__ V3T<T> cyl_of(V3T<T> v, T cs, T sn) {
+    return v3(v.x, v.y * cs + v.z * sn, -v.y * sn + v.z * cs);
+}
+template <typename T>
+__device__ __forceinline__ V3T<T> cart_of(V3T<T> v, T cs, T sn) {
+    return v3(v.x, v.y * cs - v.z * sn, v.y * sn + v.z * cs);
+}
 
 // Compressibility correction of the Hall-Thollet model
 // (SourceBFMThollet::computeCompressibilityCorrection, source_bfm_thollet.cpp:50-71):
 // relative Mach number m, 1/sqrt(1-m^2) subsonic, 2/(pi sqrt(m^2-1)) supersonic,
 // m within 1e-3 of 1 taken as 0.999, capped at 3.
-__device__ inline double thollet_kmach(double wrel, double a) {
-    double m = wrel / a;
-    if (fabs(m - 1.0) < 1E-03) m = 0.999;
-    double kp;
-    if (m < 1.0) kp = 1.0 / sqrt(1.0 - m * m);
-    else kp = 2.0 / M_PI / sqrt(m * m - 1.0);
-    return (kp < 3.0) ? kp : 3.0;
+template <typename T>
+__device__ inline T thollet_kmach(T wrel, T a) {
+    T m = wrel / a;
+    if (fabs(m - T(1.0)) < T(1E-03)) m = T(0.999);
+    T kp;
+    if (m < T(1.0)) kp = T(1.0) / sqrt(T(1.0) - m * m);
+    else kp = T(2.0) / T(M_PI) / sqrt(m * m - T(1.0));
+    return (kp < T(3.0)) ? kp : T(3.0);
 }
 
 // Blockage + Hall / lift-drag / Hall-Thollet source of one node (source_bfm_base.cpp:25-199,
-// source_bfm_hall.cpp:3-34, source_bfm_lift_drag.cpp:3-88, source_bfm_thollet.cpp:3-138).
+// source_bfm_hall.cpp:3-34, source_bfm_lift_drag.cpp:3-88, source_bfm_thollet.cpp:3-138),
+// in the source type T (every input rounded to T first, the results widened back).
 // s = slab index.  keepv: the node is stalled under Hall-Thollet, which leaves the
 // viscous-force side output at its previous value (source_bfm_thollet.cpp:79-84).
-template <int FL, typename CT>
-__device__ inline void bfm_source(const CT& c, const Gas<FL>& gs, long s, const double W[5], double V, double S[5],
+template <int FL, typename T, typename CT>
+__device__ inline void bfm_source(const CT& c, const Gas<FL>& gs, long s, const double Wd[5], double Vd, double S[5],
                                   double fi[3], double fv[3], double& dev, bool& keepv) {
-    double blk[5] = {0, 0, 0, 0, 0};
+    const T W[5] = {T(Wd[0]), T(Wd[1]), T(Wd[2]), T(Wd[3]), T(Wd[4])};
+    const T V = T(Vd);
+    T blk[5] = {0, 0, 0, 0, 0};
     fi[0] = fi[1] = fi[2] = 0.0;
     fv[0] = fv[1] = fv[2] = 0.0;
     dev = 0.0;
     keepv = false;
     if (c.blockage_active) {
-        double p = pressure_w<double, FL>(gs, W);
-        double ht = W[4] + p / W[0];
-        double b = c.s_b[s];
-        double dd = W[1] * c.s_gbx[s] + W[2] * c.s_gby[s] + W[3] * c.s_gbz[s];
-        double common = -1.0 / b * W[0] * dd;
+        T p = pressure_w<T, FL>(gs, W);
+        T ht = W[4] + p / W[0];
+        T b = T(c.s_b[s]);
+        T dd = W[1] * T(c.s_gbx[s]) + W[2] * T(c.s_gby[s]) + W[3] * T(c.s_gbz[s]);
+        T common = T(-1.0) / b * W[0] * dd;
         blk[0] = common * V;
         blk[1] = common * W[1] * V;
         blk[2] = common * W[2] * V;
         blk[3] = common * W[3] * V;
         blk[4] = common * ht * V;
     }
-    double body[5] = {0, 0, 0, 0, 0};
+    T body[5] = {0, 0, 0, 0, 0};
     if ((c.sflag[s] & 4) && (c.bfm_model == 1 || c.bfm_model == 2 || c.bfm_model == 4)) {
         // computeFlowState (source_bfm_base.cpp:104-149)
-        const double cs = c.s_cs[s], sn = c.s_sn[s], r = c.s_r[s], om = c.s_om[s];
-        V3 vcy = cyl_of(v3(W[1], W[2], W[3]), cs, sn);
-        V3 rel = vsub(vcy, v3(0, 0, om * r));
-        V3 nc = v3(c.s_nax[s], c.s_nrad[s], c.s_ntan[s]);
+        const T cs = T(c.s_cs[s]), sn = T(c.s_sn[s]), r = T(c.s_r[s]), om = T(c.s_om[s]);
+        V3T<T> vcy = cyl_of(v3(W[1], W[2], W[3]), cs, sn);
+        V3T<T> rel = vsub(vcy, v3(T(0), T(0), om * r));
+        V3T<T> nc = v3(T(c.s_nax[s]), T(c.s_nrad[s]), T(c.s_ntan[s]));
         // deviation angle (:152-166)
-        V3 n = vdiv(nc, vmag(nc));
-        V3 sp = v3(-rel.y, rel.x, 0.0);
+        V3T<T> n = vdiv(nc, vmag(nc));
+        V3T<T> sp = v3(-rel.y, rel.x, T(0.0));
         sp = vdiv(sp, vmag(sp));
-        V3 vin = vsub(rel, vmul(sp, vdot(rel, sp)));
-        double dv = -asin(vdot(vin, n) / vmag(vin));
+        V3T<T> vin = vsub(rel, vmul(sp, vdot(rel, sp)));
+        T dv = -asin(vdot(vin, n) / vmag(vin));
         // inviscid force direction (:168-199): the camber normal itself once the
         // deviation exceeds the stall deviation (STALLED_BFM_ACTIVE, :178-184)
-        const bool stalled = c.stall && dv > c.s_stall[s];
-        V3 rd = vnrm(rel), nd = vnrm(nc);
-        V3 ver = stalled ? nd : vnrm(vsub(nd, vmul(rd, vdot(nd, rd))));
-        V3 sd = vnrm(v3(rel.y, -rel.x, 0.0));
-        V3 idir = vnrm(vsub(ver, vmul(sd, vdot(ver, sd))));
+        const bool stalled = c.stall && dv > T(c.s_stall[s]);
+        V3T<T> rd = vnrm(rel), nd = vnrm(nc);
+        V3T<T> ver = stalled ? nd : vnrm(vsub(nd, vmul(rd, vdot(nd, rd))));
+        V3T<T> sd = vnrm(v3(rel.y, -rel.x, T(0.0)));
+        V3T<T> idir = vnrm(vsub(ver, vmul(sd, vdot(ver, sd))));
         dev = dv;
         if (c.bfm_model == 1) {
-            double fm = vdot(rel, rel) * M_PI * dv / c.s_pitch[s] / fabs(c.s_ntan[s]);
-            V3 fcy = vmul(idir, fm);
-            V3 fca = cart_of(fcy, cs, sn);
+            T fm = vdot(rel, rel) * T(M_PI) * dv / T(c.s_pitch[s]) / fabs(T(c.s_ntan[s]));
+            V3T<T> fcy = vmul(idir, fm);
+            V3T<T> fca = cart_of(fcy, cs, sn);
             fi[0] = fca.x; fi[1] = fca.y; fi[2] = fca.z;
-            double sv[5] = {0, fca.x, fca.y, fca.z, fcy.z * om * r};
+            T sv[5] = {T(0), fca.x, fca.y, fca.z, fcy.z * om * r};
 #pragma unroll
             for (int v = 0; v < 5; ++v) body[v] = sv[v] * V * W[0];
         } else if (c.bfm_model == 4) {
             // inviscid component (source_bfm_thollet.cpp:23-48)
-            const double kmach = thollet_kmach(vmag(rel), sound_speed_w<FL>(gs, W, pressure_w<double, FL>(gs, W)));
-            const double rr = vdot(rel, rel);
-            const double pitch = c.s_pitch[s], b = c.s_b[s], antan = fabs(c.s_ntan[s]);
-            double fm = c.th_kn * kmach * rr * M_PI * dv / pitch / antan / b;
-            V3 fcy = vmul(idir, fm);
-            V3 fca = cart_of(fcy, cs, sn);
+            const T kmach = thollet_kmach(vmag(rel), sound_speed_wt<FL, T>(gs, W, pressure_w<T, FL>(gs, W)));
+            const T rr = vdot(rel, rel);
+            const T pitch = T(c.s_pitch[s]), b = T(c.s_b[s]), antan = fabs(T(c.s_ntan[s]));
+            T fm = T(c.th_kn) * kmach * rr * T(M_PI) * dv / pitch / antan / b;
+            V3T<T> fcy = vmul(idir, fm);
+            V3T<T> fca = cart_of(fcy, cs, sn);
             fi[0] = fca.x; fi[1] = fca.y; fi[2] = fca.z;
-            double si[5] = {0, fca.x, fca.y, fca.z, fcy.z * om * r};
-            double sv[5] = {0, 0, 0, 0, 0};
+            T si[5] = {T(0), fca.x, fca.y, fca.z, fcy.z * om * r};
+            T sv[5] = {0, 0, 0, 0, 0};
             if (stalled) {
                 keepv = true;
             } else {
                 // viscous component (:73-138): flat-plate skin friction on the streamwise
                 // length (leading-edge fallback applied on the host), plus the off-design
                 // loss about the pivot deviation
-                double rex = vmag(rel) * c.s_stwl[s] / c.th_nu;
-                double cf = 0.0592 * pow(rex, -0.2);
-                double coef;
-                if (!c.th_off_design) coef = c.th_kf * cf;
-                else coef = c.th_kf * cf + M_PI * kmach * pow(fabs(dv - c.s_piv[s]), 2.0 * c.th_kd);
-                double fm2 = rr / (pitch * b * antan) * coef;
-                V3 vdir = v3(-vnrm(rel).x, -vnrm(rel).y, -vnrm(rel).z);
-                V3 vcyl = vmul(vdir, fm2);
-                V3 vcar = cart_of(vcyl, cs, sn);
+                T rex = vmag(rel) * T(c.s_stwl[s]) / T(c.th_nu);
+                T cf = T(0.0592) * pow(rex, T(-0.2));
+                T coef;
+                if (!c.th_off_design) coef = T(c.th_kf) * cf;
+                else coef = T(c.th_kf) * cf + T(M_PI) * kmach * pow(fabs(dv - T(c.s_piv[s])), T(2.0) * T(c.th_kd));
+                T fm2 = rr / (pitch * b * antan) * coef;
+                V3T<T> vdir = v3(-vnrm(rel).x, -vnrm(rel).y, -vnrm(rel).z);
+                V3T<T> vcyl = vmul(vdir, fm2);
+                V3T<T> vcar = cart_of(vcyl, cs, sn);
                 fv[0] = vcar.x; fv[1] = vcar.y; fv[2] = vcar.z;
                 sv[1] = vcar.x; sv[2] = vcar.y; sv[3] = vcar.z; sv[4] = vcyl.z * om * r;
             }
 #pragma unroll
             for (int v = 0; v < 5; ++v) body[v] = si[v] * V * W[0] + sv[v] * V * W[0];
         } else {
-            double vmer = sqrt(vcy.x * vcy.x + vcy.y * vcy.y);
-            double flow_angle = atan2(rel.z, vmer);
-            V3 relc = cart_of(rel, cs, sn);
-            double w = vmag(relc);
-            double beta0 = c.s_beta0[s];
-            double sign = (c.s_ntan[s] >= 0.0) ? 1.0 : -1.0;
-            double fm = 0.5 * w * w * 2.0 * M_PI / c.s_spitch[s] * sign * (-flow_angle + beta0);
-            V3 fcy = vmul(idir, fm);
-            V3 fca = cart_of(fcy, cs, sn);
+            T vmer = sqrt(vcy.x * vcy.x + vcy.y * vcy.y);
+            T flow_angle = atan2(rel.z, vmer);
+            V3T<T> relc = cart_of(rel, cs, sn);
+            T w = vmag(relc);
+            T beta0 = T(c.s_beta0[s]);
+            T sign = (c.s_ntan[s] >= 0.0) ? T(1.0) : T(-1.0);
+            T fm = T(0.5) * w * w * T(2.0) * T(M_PI) / T(c.s_spitch[s]) * sign * (-flow_angle + beta0);
+            V3T<T> fcy = vmul(idir, fm);
+            V3T<T> fca = cart_of(fcy, cs, sn);
             fi[0] = fca.x; fi[1] = fca.y; fi[2] = fca.z;
-            double si[5] = {0, fca.x, fca.y, fca.z, fcy.z * om * r};
-            double dd = flow_angle - beta0;
-            double Kp = c.s_kp[s] + 2.0 * M_PI * c.s_sol[s] * (dd * dd);
-            double fm2 = Kp * w * w / c.s_spitch[s];
-            V3 vdir = vmul(vnrm(rel), -1.0);
-            vdir = v3(-vnrm(rel).x, -vnrm(rel).y, -vnrm(rel).z);
-            V3 vcyl = vmul(vdir, fm2);
-            V3 vcar = cart_of(vcyl, cs, sn);
+            T si[5] = {T(0), fca.x, fca.y, fca.z, fcy.z * om * r};
+            T dd = flow_angle - beta0;
+            T Kp = T(c.s_kp[s]) + T(2.0) * T(M_PI) * T(c.s_sol[s]) * (dd * dd);
+            T fm2 = Kp * w * w / T(c.s_spitch[s]);
+            V3T<T> vdir = v3(-vnrm(rel).x, -vnrm(rel).y, -vnrm(rel).z);
+            V3T<T> vcyl = vmul(vdir, fm2);
+            V3T<T> vcar = cart_of(vcyl, cs, sn);
             fv[0] = vcar.x; fv[1] = vcar.y; fv[2] = vcar.z;
-            double sv[5] = {0, vcar.x, vcar.y, vcar.z, vcyl.z * om * r};
+            T sv[5] = {T(0), vcar.x, vcar.y, vcar.z, vcyl.z * om * r};
 #pragma unroll
             for (int v = 0; v < 5; ++v) body[v] = si[v] * V * W[0] + sv[v] * V * W[0];
         }
     }
 #pragma unroll
-    for (int v = 0; v < 5; ++v) S[v] = blk[v] + body[v] * (1.0 + 0.0);
+    for (int v = 0; v < 5; ++v) S[v] = double(blk[v] + body[v] * (T(1.0) + T(0.0)));
 }
 
 }  // namespace dev

@@ -60,6 +60,30 @@ __device__ inline void load_k(const CaseDev& c, const double* U, int i, int j, i
 // x - (+0) == x for every x, so the order and the values are the reference's.
 // Also writes the body-force side outputs (inviscid / viscous force, deviation angle)
 // on the last stage.
+// Geometric source of one node (axisymmetric: (rho u_theta^2 + p, -rho u_r u_theta) A_K
+// dtheta; 3-D blade rows: the same over r times V), in the source type T.  col = j*NI + i.
+template <int FL, typename T>
+__device__ inline void geometric_source(const CaseDev& c, const Gas<FL>& gs, const double* Wd, long s, long col,
+                                        double Vd, double g[5]) {
+    const T Wc[5] = {T(Wd[0]), T(Wd[1]), T(Wd[2]), T(Wd[3]), T(Wd[4])};
+    const T V = T(Vd);
+    T p = pressure_w<T, FL>(gs, Wc);
+    const T cs = T(c.s_cs[s]), sn = T(c.s_sn[s]);
+    V3T<T> vcy = cyl_of(v3(Wc[1], Wc[2], Wc[3]), cs, sn);
+    V3T<T> scy;
+    if (c.topo == TOPO_AXI) {
+        const T gf = T(c.axi_gf[col]);
+        scy = v3(T(0.0), (+Wc[0] * vcy.z * vcy.z + p) * gf, -Wc[0] * vcy.y * vcy.z * gf);
+        V3T<T> sca = cart_of(scy, cs, sn);
+        g[0] = 0.0; g[1] = sca.x; g[2] = sca.y; g[3] = sca.z; g[4] = 0.0;
+    } else {
+        const T r = T(c.s_r[s]);
+        scy = v3(T(0.0), (+Wc[0] * vcy.z * vcy.z + p) / r, -Wc[0] * vcy.y * vcy.z / r);
+        V3T<T> sca = cart_of(scy, cs, sn);
+        g[0] = T(0.0) * V; g[1] = sca.x * V; g[2] = sca.y * V; g[3] = sca.z * V; g[4] = T(0.0) * V;
+    }
+}
+
 template <bool THREED, int FL>
 __device__ inline void source_terms(const CaseDev& c, const Gas<FL>& gs, const double* __restrict__ Uin, int last, int i, int j,
                                     int k, int n, long s, const double* Wc, double g[5], double gong[5],
@@ -73,20 +97,12 @@ __device__ inline void source_terms(const CaseDev& c, const Gas<FL>& gs, const d
     const bool present = (fl & 1) != 0;
     const bool geo = (c.topo == TOPO_AXI) || (THREED && present);
     if (geo) {
-        double p = pressure_w<double, FL>(gs, Wc);
-        const double cs = c.s_cs[s], sn = c.s_sn[s];
-        V3 vcy = cyl_of(v3(Wc[1], Wc[2], Wc[3]), cs, sn);
-        V3 scy;
-        if (c.topo == TOPO_AXI) {
-            const double gf = c.axi_gf[(long)j * NI + i];
-            scy = v3(0.0, (+Wc[0] * vcy.z * vcy.z + p) * gf, -Wc[0] * vcy.y * vcy.z * gf);
-            V3 sca = cart_of(scy, cs, sn);
-            g[0] = 0.0; g[1] = sca.x; g[2] = sca.y; g[3] = sca.z; g[4] = 0.0;
+        // mixed precision: in FP32 (ideal gas; the table fluid is FP64 only)
+        if constexpr (FL == 0) {
+            if (c.fp32) geometric_source<FL, float>(c, gs, Wc, s, (long)j * NI + i, V, g);
+            else geometric_source<FL, double>(c, gs, Wc, s, (long)j * NI + i, V, g);
         } else {
-            const double r = c.s_r[s];
-            scy = v3(0.0, (+Wc[0] * vcy.z * vcy.z + p) / r, -Wc[0] * vcy.y * vcy.z / r);
-            V3 sca = cart_of(scy, cs, sn);
-            g[0] = 0.0 * V; g[1] = sca.x * V; g[2] = sca.y * V; g[3] = sca.z * V; g[4] = 0.0 * V;
+            geometric_source<FL, double>(c, gs, Wc, s, (long)j * NI + i, V, g);
         }
     }
     if (THREED && present) {
@@ -117,7 +133,13 @@ __device__ inline void source_terms(const CaseDev& c, const Gas<FL>& gs, const d
     if (c.bfm_active && (fl & 2)) {
         double fi[3], fv[3], dv;
         bool keepv;
-        bfm_source<FL>(c, gs, s, Wc, V, bf, fi, fv, dv, keepv);
+        // mixed precision: the body force in FP32 (ideal gas; the table fluid is FP64 only)
+        if constexpr (FL == 0) {
+            if (c.fp32) bfm_source<FL, float>(c, gs, s, Wc, V, bf, fi, fv, dv, keepv);
+            else bfm_source<FL, double>(c, gs, s, Wc, V, bf, fi, fv, dv, keepv);
+        } else {
+            bfm_source<FL, double>(c, gs, s, Wc, V, bf, fi, fv, dv, keepv);
+        }
         if (last && (fl & 4)) {
             c.finv[3 * n] = fi[0]; c.finv[3 * n + 1] = fi[1]; c.finv[3 * n + 2] = fi[2];
             c.dev[n] = dv;
