@@ -611,6 +611,31 @@ void setup(tbfm_case* cs, const GridInput& grid) {
     const std::vector<double>* kp = (c.bfm_model == 2) ? field("liftDragKpEtamax", true) : nullptr;
     const long le = cfg.getLeadingEdgeIndex(), te = cfg.getTrailingEdgeIndex();
     const double scaling = cfg.getRotationalSpeedScalingFactor();
+    // time-dependent rotational speed (config.hpp:355-367, config.cpp:695-718): the slab
+    // keeps rpm 2 pi / 60 and the device scales it by the factor of the iteration's time
+    c.accel = cfg.getBool("ACCELERATION_ACTIVE", false) ? 1 : 0;
+    c.om_scaling = scaling;
+    if (c.accel) {
+        if (c.dt_mode != 2)
+            throw std::runtime_error("Rotational speed scaling factor is only active for fixed time step method");
+        c.acc_t0 = cfg.getFloat("ACCELERATION_INITIAL_TIME");
+        c.acc_t1 = cfg.getFloat("ACCELERATION_FINAL_TIME");
+        c.acc_s0 = cfg.getFloat("ROTATIONAL_SPEED_SCALING_FACTOR_INITIAL");
+        c.acc_s1 = cfg.getFloat("ROTATIONAL_SPEED_SCALING_FACTOR_FINAL");
+    }
+    // Gaussian body-force perturbation (source_bfm_base.cpp:8-19, 215-243)
+    c.pert = (bfm && cfg.getBool("PERTURBATION_BODY_FORCE", false)) ? 1 : 0;
+    double pcen[3] = {0.0, 0.0, 0.0}, prad = 1.0;
+    if (c.pert) {
+        auto cen = cfg.getList("PERTURBATION_CENTER");
+        if (cen.size() < 3) throw std::runtime_error("PERTURBATION_CENTER needs three coordinates");
+        for (int d = 0; d < 3; ++d) pcen[d] = std::stod(cen[d]);
+        prad = cfg.getFloat("PERTURBATION_RADIAL_EXTENSION");
+        c.pert_scale = cfg.getFloat("PERTURBATION_SCALING_FACTOR");
+        c.pert_start = cfg.getFloat("PERTURBATION_TIME_START");
+        c.pert_dur = cfg.getFloat("PERTURBATION_TIME_DURATION");
+    }
+    std::vector<double> s_psp(c.pert ? NS : 1, -1.0);
     for (long i = bi0; i < bi1; ++i)
         for (long j = 0; j < NJ; ++j)
             for (long k = 0; k < NK; ++k) {
@@ -627,8 +652,13 @@ void setup(tbfm_case* cs, const GridInput& grid) {
                 s_r[s] = g.radius[n];
                 if (rpm) {
                     double om = (*rpm)[n] * 2 * M_PI / 60;
-                    om *= scaling;
+                    if (!c.accel) om *= scaling;
                     s_om[s] = om;
+                }
+                if (c.pert) {
+                    const double dx = grid.x[n] - pcen[0], dy = grid.y[n] - pcen[1], dz = grid.z[n] - pcen[2];
+                    const double dist = std::sqrt(dx * dx + dy * dy + dz * dz);
+                    if (!(dist > 3.0 * prad)) s_psp[s] = std::exp(-(dist * dist) / (prad * prad));
                 }
                 if (bfm) {
                     s_b[s] = (*blockage)[n];
@@ -678,6 +708,7 @@ void setup(tbfm_case* cs, const GridInput& grid) {
     c.s_pitch = upload(s_pitch, own); c.s_spitch = upload(s_spitch, own); c.s_sol = upload(s_sol, own);
     c.s_beta0 = upload(s_beta0, own); c.s_kp = upload(s_kp, own);
     c.s_stwl = upload(s_stwl, own); c.s_piv = upload(s_piv, own); c.s_stall = upload(s_stall, own);
+    c.s_psp = upload(s_psp, own);
     c.axi_gf = upload(axi_gf, own);
 
     // ---- no-slip walls and laminar viscous terms
@@ -716,11 +747,6 @@ void setup(tbfm_case* cs, const GridInput& grid) {
     c.visc = cfg.getBool("VISCOSITY_ACTIVE", false) ? 1 : 0;
     if (cfg.getBool("TURBULENCE_ACTIVE", false))
         throw std::runtime_error("TURBULENCE_ACTIVE is outside the GPU residual path");
-    // the time-dependent body-force options (source_bfm_base.cpp:58, 215-243; config.cpp:695-718)
-    if (cfg.getBool("PERTURBATION_BODY_FORCE", false))
-        throw std::runtime_error("PERTURBATION_BODY_FORCE is not on the GPU residual path");
-    if (cfg.getBool("ACCELERATION_ACTIVE", false))
-        throw std::runtime_error("ACCELERATION_ACTIVE (time-dependent rotational speed) is not on the GPU residual path");
     if (c.visc) {
         if (cs->fluid == FluidModel::Real)
             throw std::runtime_error("VISCOSITY_ACTIVE needs FLUID_MODEL = ideal on the GPU path");
@@ -805,7 +831,10 @@ void setup(tbfm_case* cs, const GridInput& grid) {
     c.conv.hist = dalloc<double>(5 * cs->hist_cap, own);
     // turbomachinery history of every iteration (TURBO_OUTPUT, solver.cpp:1340-1341)
     // also needed by throttle and mass-flow outlets (solver.cpp:1312-1317)
-    c.turbo_on = (cfg.getBool("TURBO_OUTPUT", false) || c.prof_need_mf) ? 1 : 0;
+    // ... and by the time-dependent body force, which needs the physical time (the
+    // minimum time step closes each row)
+    c.turbo_on = (cfg.getBool("TURBO_OUTPUT", false) || c.prof_need_mf || c.accel || c.pert) ? 1 : 0;
+    c.t_now = dalloc<double>(1, own);
     if (c.turbo_on) {
         const Geometry& gg = cs->geo;
         const int td = cfg.getTurboDirection();

@@ -125,6 +125,15 @@ struct CaseDev {
     // Hall-Thollet: streamwise length with the reference's leading-edge fallback,
     // deviation pivot (off-design) and stall deviation (STALLED_BFM_ACTIVE), per slab node
     const double *s_stwl, *s_piv, *s_stall;
+    // PERTURBATION_BODY_FORCE: per slab node the spatial Gaussian exp(-d^2 / R^2) about
+    // the perturbation centre (host, glibc), -1 beyond three radii
+    const double* s_psp;
+    int pert;
+    double pert_scale, pert_start, pert_dur;
+    // ACCELERATION_ACTIVE: s_om holds rpm 2 pi / 60 unscaled and the device applies the
+    // scaling factor of the iteration's physical time
+    int accel;
+    double acc_t0, acc_t1, acc_s0, acc_s1, om_scaling;
     double th_kn, th_kf, th_kd, th_nu;
     int th_off_design, stall;
     const double* axi_gf;              // axisymmetric: |S_K(i,j,0)| * wedge angle, (j*NI + i)
@@ -167,6 +176,7 @@ struct CaseDev {
     double turbo_mscale;               // 2 pi / wedge angle, 360 / periodicity angle, or 1
     double* turbo_hist;                // [hist_len][5]: time after the iteration, mass flow, PRtt, TRtt, ETAtt
     double* turbo_time;                // physical time accumulated over the iterations
+    double* t_now;                     // physical time at the start of the current iteration
     int nblk_dt;
 };
 

@@ -535,9 +535,11 @@ __device__ inline T thollet_kmach(T wrel, T a) {
 // in the source type T (every input rounded to T first, the results widened back).
 // s = slab index.  keepv: the node is stalled under Hall-Thollet, which leaves the
 // viscous-force side output at its previous value (source_bfm_thollet.cpp:79-84).
+// omd: the node's rotational speed; pf: the perturbation factor (the body force is
+// multiplied by 1 + pf, source_bfm_base.cpp:58).
 template <int FL, typename T, typename CT>
-__device__ inline void bfm_source(const CT& c, const Gas<FL>& gs, long s, const double Wd[5], double Vd, double S[5],
-                                  double fi[3], double fv[3], double& dev, bool& keepv) {
+__device__ inline void bfm_source(const CT& c, const Gas<FL>& gs, long s, double omd, double pf, const double Wd[5],
+                                  double Vd, double S[5], double fi[3], double fv[3], double& dev, bool& keepv) {
     const T W[5] = {T(Wd[0]), T(Wd[1]), T(Wd[2]), T(Wd[3]), T(Wd[4])};
     const T V = T(Vd);
     T blk[5] = {0, 0, 0, 0, 0};
@@ -560,7 +562,7 @@ __device__ inline void bfm_source(const CT& c, const Gas<FL>& gs, long s, const 
     T body[5] = {0, 0, 0, 0, 0};
     if ((c.sflag[s] & 4) && (c.bfm_model == 1 || c.bfm_model == 2 || c.bfm_model == 4)) {
         // computeFlowState (source_bfm_base.cpp:104-149)
-        const T cs = T(c.s_cs[s]), sn = T(c.s_sn[s]), r = T(c.s_r[s]), om = T(c.s_om[s]);
+        const T cs = T(c.s_cs[s]), sn = T(c.s_sn[s]), r = T(c.s_r[s]), om = T(omd);
         V3T<T> vcy = cyl_of(v3(W[1], W[2], W[3]), cs, sn);
         V3T<T> rel = vsub(vcy, v3(T(0), T(0), om * r));
         V3T<T> nc = v3(T(c.s_nax[s]), T(c.s_nrad[s]), T(c.s_ntan[s]));
@@ -642,7 +644,7 @@ __device__ inline void bfm_source(const CT& c, const Gas<FL>& gs, long s, const 
         }
     }
 #pragma unroll
-    for (int v = 0; v < 5; ++v) S[v] = double(blk[v] + body[v] * (T(1.0) + T(0.0)));
+    for (int v = 0; v < 5; ++v) S[v] = double(blk[v] + body[v] * (T(1.0) + T(pf)));
 }
 
 }  // namespace dev

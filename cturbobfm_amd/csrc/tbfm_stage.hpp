@@ -60,6 +60,36 @@ __device__ inline void load_k(const CaseDev& c, const double* U, int i, int j, i
 // x - (+0) == x for every x, so the order and the values are the reference's.
 // Also writes the body-force side outputs (inviscid / viscous force, deviation angle)
 // on the last stage.
+// Rotational speed of slab node s: rpm 2 pi / 60 times ROTATIONAL_SPEED_SCALING_FACTOR, or
+// with ACCELERATION_ACTIVE times the factor of the iteration's physical time: the linear
+// ramp between ACCELERATION_INITIAL_TIME and _FINAL_TIME, the constant factor outside
+// (Config::getRotationalSpeedScalingFactor(t), config.cpp:695-718).
+__device__ __forceinline__ double slab_omega(const CaseDev& c, long s) {
+    if (!c.accel) return c.s_om[s];
+    const double t = *c.t_now;
+    double f;
+    if (t < c.acc_t0 || t > c.acc_t1) f = c.om_scaling;
+    else f = c.acc_s0 + (c.acc_s1 - c.acc_s0) * (t - c.acc_t0) / (c.acc_t1 - c.acc_t0);
+    double om = c.s_om[s];
+    om *= f;
+    return om;
+}
+
+// SourceBFMBase::computePerturbationFactor (source_bfm_base.cpp:215-243) of slab node s at
+// the iteration's physical time: the spatial Gaussian (host) times the temporal one about
+// the middle of the window (sigma a third of its duration), times the scaling factor.
+__device__ __forceinline__ double perturbation_factor(const CaseDev& c, long s) {
+    if (!c.pert) return 0.0;
+    const double t = *c.t_now;
+    if (t < c.pert_start || t > c.pert_start + c.pert_dur) return 0.0;
+    const double space = c.s_psp[s];
+    if (space < 0.0) return 0.0;
+    const double peak = c.pert_start + c.pert_dur / 2.0;
+    const double sigma = c.pert_dur / 3.0;
+    const double time = exp(-(pow(t - peak, 2.0)) / pow(sigma, 2.0));
+    return c.pert_scale * space * time;
+}
+
 // Geometric source of one node (axisymmetric: (rho u_theta^2 + p, -rho u_r u_theta) A_K
 // dtheta; 3-D blade rows: the same over r times V), in the source type T.  col = j*NI + i.
 template <int FL, typename T>
@@ -105,8 +135,8 @@ __device__ inline void source_terms(const CaseDev& c, const Gas<FL>& gs, const d
             geometric_source<FL, double>(c, gs, Wc, s, (long)j * NI + i, V, g);
         }
     }
+    const double om = slab_omega(c, s);
     if (THREED && present) {
-        const double om = c.s_om[s];
         if (!(fabs(om) < 1E-3)) {
             double U0g[5], U1g[5], U2g[5];
             int k0g = k, k1, k2;
@@ -134,11 +164,12 @@ __device__ inline void source_terms(const CaseDev& c, const Gas<FL>& gs, const d
         double fi[3], fv[3], dv;
         bool keepv;
         // mixed precision: the body force in FP32 (ideal gas; the table fluid is FP64 only)
+        const double pf = perturbation_factor(c, s);
         if constexpr (FL == 0) {
-            if (c.fp32) bfm_source<FL, float>(c, gs, s, Wc, V, bf, fi, fv, dv, keepv);
-            else bfm_source<FL, double>(c, gs, s, Wc, V, bf, fi, fv, dv, keepv);
+            if (c.fp32) bfm_source<FL, float>(c, gs, s, om, pf, Wc, V, bf, fi, fv, dv, keepv);
+            else bfm_source<FL, double>(c, gs, s, om, pf, Wc, V, bf, fi, fv, dv, keepv);
         } else {
-            bfm_source<FL, double>(c, gs, s, Wc, V, bf, fi, fv, dv, keepv);
+            bfm_source<FL, double>(c, gs, s, om, pf, Wc, V, bf, fi, fv, dv, keepv);
         }
         if (last && (fl & 4)) {
             c.finv[3 * n] = fi[0]; c.finv[3 * n + 1] = fi[1]; c.finv[3 * n + 2] = fi[2];
@@ -362,6 +393,7 @@ __device__ inline void turbo_final(const CaseDev& c) {
         else if (dT >= 0.0) eta = dTs / dT;
         else eta = (fabs(dTs) < 1e-12) ? 1.0 : dT / dTs;
     }
+    *c.t_now = *c.turbo_time;   // the time the body force of this iteration sees
     const double t = *c.turbo_time + *c.dtmin;
     *c.turbo_time = t;
     const long it = *c.iter;

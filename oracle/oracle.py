@@ -42,7 +42,11 @@ class Params(C.Structure):
                 ("greitzer", C.c_int), ("greitzer_volume", C.c_double), ("greitzer_ambient", C.c_double),
                 ("viscous", C.c_int), ("mu_constant_model", C.c_int), ("mu_constant", C.c_double),
                 ("mu_ref", C.c_double), ("mu_t_ref", C.c_double), ("mu_s", C.c_double),
-                ("cp_visc", C.c_double), ("prandtl", C.c_double)]
+                ("cp_visc", C.c_double), ("prandtl", C.c_double),
+                ("accel_active", C.c_int), ("acc_t0", C.c_double), ("acc_t1", C.c_double),
+                ("acc_s0", C.c_double), ("acc_s1", C.c_double),
+                ("pert_active", C.c_int), ("pert_center", C.c_double * 3), ("pert_radius", C.c_double),
+                ("pert_scale", C.c_double), ("pert_start", C.c_double), ("pert_duration", C.c_double)]
 
 
 class Table(C.Structure):
@@ -153,6 +157,9 @@ def lib():
         L.tbo_run.argtypes = [C.c_void_p, C.c_long, dp, dp, C.c_int]
         L.tbo_residual.argtypes = [C.c_void_p, dp, C.c_long, C.c_int, dp]
         L.tbo_source_residual.argtypes = [C.c_void_p, dp, C.c_int, dp]
+        L.tbo_set_time.argtypes = [C.c_void_p, C.c_double]
+        L.tbo_get_time.restype = C.c_double
+        L.tbo_get_time.argtypes = [C.c_void_p]
         L.tbo_set_profile.restype = C.c_long
         L.tbo_set_profile.argtypes = [C.c_void_p, C.c_int, dp]
         L.tbo_patch_flux.argtypes = [C.c_void_p, C.c_int, dp, dp, C.POINTER(C.c_long), C.c_long, dp]
@@ -242,10 +249,25 @@ def params_from_config(case: Dict) -> Params:
     p.le_index = int(float(cfg.get("LEADING_EDGE_INDEX", "0")))
     p.te_index = int(float(cfg.get("TRAILING_EDGE_INDEX", "0")))
     p.omega_scaling = _cfg_float(cfg, "ROTATIONAL_SPEED_SCALING_FACTOR", 1.0)
-    # time-dependent body-force options (source_bfm_base.cpp:58, 215-243; config.cpp:695-718)
-    for key in ("PERTURBATION_BODY_FORCE", "ACCELERATION_ACTIVE"):
-        if _cfg_bool(cfg, key, False):
-            raise ValueError("%s is not restated by the oracle" % key)
+    # time-dependent rotational speed (config.hpp:355-367, config.cpp:695-718)
+    p.accel_active = int(_cfg_bool(cfg, "ACCELERATION_ACTIVE", False))
+    if p.accel_active:
+        if cfg["TIME_STEP_METHOD"] != "fixed":
+            raise ValueError("Rotational speed scaling factor is only active for fixed time step method")
+        p.acc_t0 = float(cfg["ACCELERATION_INITIAL_TIME"])
+        p.acc_t1 = float(cfg["ACCELERATION_FINAL_TIME"])
+        p.acc_s0 = float(cfg["ROTATIONAL_SPEED_SCALING_FACTOR_INITIAL"])
+        p.acc_s1 = float(cfg["ROTATIONAL_SPEED_SCALING_FACTOR_FINAL"])
+    # Gaussian body-force perturbation (config.hpp:309-335, source_bfm_base.cpp:8-19)
+    p.pert_active = int(_cfg_bool(cfg, "PERTURBATION_BODY_FORCE", False))
+    if p.pert_active:
+        cen = [float(v) for v in cfg["PERTURBATION_CENTER"].split(",") if v.strip()]
+        for q in range(3):
+            p.pert_center[q] = cen[q]
+        p.pert_radius = float(cfg["PERTURBATION_RADIAL_EXTENSION"])
+        p.pert_scale = float(cfg["PERTURBATION_SCALING_FACTOR"])
+        p.pert_start = float(cfg["PERTURBATION_TIME_START"])
+        p.pert_duration = float(cfg["PERTURBATION_TIME_DURATION"])
     p.inlet_cylindrical = int(cfg.get("INLET_REFERENCE_FRAME", "None").lower() == "cylindrical")
     td = cfg.get("TURBO_DIRECTION", "i")
     p.turbo_dir = {"i": 0, "+i": 0, "-i": 1, "j": 2, "+j": 2, "-j": 3}.get(td, 0)
@@ -384,6 +406,13 @@ class OracleCase:
         R = np.zeros_like(U)
         lib().tbo_source_residual(self.h, _dp(U), which, _dp(R))
         return R
+
+    def set_time(self, t: float) -> None:
+        """Physical time of the next iteration (the time the body force sees)."""
+        lib().tbo_set_time(self.h, t)
+
+    def time(self) -> float:
+        return lib().tbo_get_time(self.h)
 
     def set_profile(self, q: int, values) -> None:
         v = np.ascontiguousarray(values, dtype=np.float64)

@@ -976,6 +976,31 @@ static v3 inviscid_dir(v3 rel, v3 nc, int stalled) {
     v3 sd = nrm3(mk3(vm.y, -vm.x, 0.0));
     return nrm3(sub3(ver, mul3(sd, dot3(ver, sd))));
 }
+/* Config::getRotationalSpeedScalingFactor(currentTime) (config.cpp:695-718): the constant
+ * factor, or during [t0, t1] with ACCELERATION_ACTIVE the linear ramp s0 -> s1 */
+static double omega_scale(const tbo_case* c, double t) {
+    if (!c->p.accel_active) return c->p.omega_scaling;
+    if (t < c->p.acc_t0 || t > c->p.acc_t1) return c->p.omega_scaling;
+    return c->p.acc_s0 + (c->p.acc_s1 - c->p.acc_s0) * (t - c->p.acc_t0) / (c->p.acc_t1 - c->p.acc_t0);
+}
+
+/* SourceBFMBase::computePerturbationFactor (source_bfm_base.cpp:215-243): a Gaussian in
+ * space about the centre (zero beyond three radii) and in time about the middle of the
+ * perturbation window (sigma a third of its duration) */
+static double perturbation_factor(const tbo_case* c, long idx, double t) {
+    if (!c->p.pert_active) return 0.0;
+    if (t < c->p.pert_start || t > c->p.pert_start + c->p.pert_duration) return 0.0;
+    double vmax = c->p.pert_scale;
+    v3 d = sub3(mk3(c->x[idx], c->y[idx], c->z[idx]), mk3(c->p.pert_center[0], c->p.pert_center[1], c->p.pert_center[2]));
+    double distance = mag3(d);
+    if (distance > 3.0 * c->p.pert_radius) return 0.0;
+    double space = exp(-(distance * distance) / (c->p.pert_radius * c->p.pert_radius));
+    double peak = c->p.pert_start + c->p.pert_duration / 2.0;
+    double sigma = c->p.pert_duration / 3.0;
+    double time = exp(-(pow(t - peak, 2)) / pow(sigma, 2));
+    return vmax * space * time;
+}
+
 /* src/source_bfm_base.cpp:104-149 */
 static flow_state compute_flow_state(const tbo_case* c, long idx, st5 W) {
     flow_state s;
@@ -985,7 +1010,7 @@ static flow_state compute_flow_state(const tbo_case* c, long idx, st5 W) {
     v3 vc = mk3(W.v[1], W.v[2], W.v[3]);
     v3 vcy = cyl_from_cart(vc, s.cs, s.sn);
     s.omega = c->rpm[idx] * 2 * M_PI / 60;
-    s.omega *= c->p.omega_scaling;
+    s.omega *= omega_scale(c, c->t_phys);
     v3 drag = mk3(0, 0, s.omega * s.radius);
     s.rel_cyl = sub3(vcy, drag);
     s.rel_cart = cart_from_cyl(s.rel_cyl, s.cs, s.sn);
@@ -1128,7 +1153,8 @@ static st5 bfm_total_source(tbo_case* c, long i, long j, long k, st5 W, int writ
         c->fvisc[3 * idx] = fvisc.x; c->fvisc[3 * idx + 1] = fvisc.y; c->fvisc[3 * idx + 2] = fvisc.z;
         c->dev[idx] = dev;
     }
-    for (int v = 0; v < 5; ++v) src.v[v] = blk.v[v] + body.v[v] * (1.0 + 0.0);
+    double pf = perturbation_factor(c, idx, c->t_phys);
+    for (int v = 0; v < 5; ++v) src.v[v] = blk.v[v] + body.v[v] * (1.0 + pf);
     return src;
 }
 
@@ -1222,7 +1248,7 @@ static void source_terms(tbo_case* c, const double* U, double* R, int which) {
                 }
                 if (gong) {
                     double om = c->rpm[idx] * 2 * M_PI / 60;
-                    om *= c->p.omega_scaling;
+                    om *= omega_scale(c, c->t_phys);
                     st5 gs = gong_source(c, om, i, j, k, V, U);
                     for (int v = 0; v < 5; ++v) R[v * N + idx] -= gs.v[v];
                 }
@@ -2021,6 +2047,9 @@ void tbo_source_residual(tbo_case* c, const double* U, int which, double* R) {
     source_terms(c, U, R, which);
 }
 
+void tbo_set_time(tbo_case* c, double t) { c->t_phys = t; }
+double tbo_get_time(const tbo_case* c) { return c->t_phys; }
+
 /* GreitzerModel::initializeState + computePlenumPressure (greitzer_model.cpp:16-56) for a
  * sequence of compressor mass flows; pout / mout receive the plenum pressure and outlet
  * mass flow after each step (test hook of the plenum model alone) */
@@ -2063,6 +2092,10 @@ long tbo_run(tbo_case* c, long n, double* logres, double* turbo, int nthreads) {
             if (c->periodic) periodic_solution(c, c->U);
         }
         memcpy(c->U0, c->U, sizeof(double) * 5 * N);
+        /* solver.cpp:1389: the physical time advances by the minimum time step */
+        double dtm = c->dt[0];
+        for (long q = 1; q < N; ++q) dtm = (c->dt[q] < dtm) ? c->dt[q] : dtm;
+        c->t_phys += dtm;
         double lr[5];
         int st = log_norms(c, c->R, lr);
         if (st < 0) return st;

@@ -62,6 +62,16 @@ typedef struct {
     double mu_ref, mu_t_ref, mu_s;   /* FLUID_SUTHERLAND_MU_REF / _T_REF / _S */
     double cp_visc;                  /* FLUID_CP (thermal conductivity only) */
     double prandtl;                  /* FLUID_PRANDTL_NUMBER */
+    /* time-dependent rotational speed (ACCELERATION_ACTIVE, fixed time step; config.cpp:695-718) */
+    int accel_active;
+    double acc_t0, acc_t1;           /* ACCELERATION_INITIAL_TIME / _FINAL_TIME */
+    double acc_s0, acc_s1;           /* ROTATIONAL_SPEED_SCALING_FACTOR_INITIAL / _FINAL */
+    /* Gaussian body-force perturbation (PERTURBATION_BODY_FORCE, source_bfm_base.cpp:215-243) */
+    int pert_active;
+    double pert_center[3];           /* PERTURBATION_CENTER */
+    double pert_radius;              /* PERTURBATION_RADIAL_EXTENSION */
+    double pert_scale;               /* PERTURBATION_SCALING_FACTOR */
+    double pert_start, pert_duration; /* PERTURBATION_TIME_START / _TIME_DURATION */
 } tbo_params;
 
 /* A fluid look-up table (include/fluid_real.hpp:11-60 Table2D): nx*ny values, row-major
@@ -137,6 +147,10 @@ void tbo_source_residual(tbo_case* c, const double* U, int which, double* R);
 long tbo_set_profile(tbo_case* c, int q, const double* in);
 void tbo_patch_flux(const tbo_case* c, int pidx, const double* Ui, const double* S, const long* face, long it,
                     double* out);
+/* Physical time of the next iteration (Solver::_currentTime, solver.cpp:41, 1389): the
+ * sum of the previous iterations' minimum time steps. */
+void tbo_set_time(tbo_case* c, double t);
+double tbo_get_time(const tbo_case* c);
 /* The Greitzer plenum alone (greitzer_model.cpp:16-56): state (p0, mout0, T0), then one
  * computePlenumPressure per mass flow. */
 void tbo_greitzer_steps(tbo_case* c, double p0, double mout0, double T0, const double* mflow, int n,

@@ -21,12 +21,15 @@ Each fixture holds seeded inputs and the reference's outputs:
   outlets.npz             BoundaryOutletThrottle / BoundaryOutletMassFlow fluxes on a given
                           pressure profile (mass flow: faces past the profile's end take
                           its last value) and GreitzerModel plenum sequences
+  bfm_tdep_<model>.npz    computeTotalSource with ACCELERATION_ACTIVE (rotational speed
+                          ramp) and PERTURBATION_BODY_FORCE (Gaussian in space and time) at
+                          times before, inside and after their windows
   bfm_sweep_<model>.npz   the body-force part of the solver's source loop over a whole
                           blade row in the reference's i, j, k order, three consecutive
                           sweeps with one model object and persistent side-output arrays
 
 Run from the repository root:  python tests/golden/gen_golden.py [group ...]
-(groups: base, outlets, sweep; default all)
+(groups: base, outlets, sweep, tdep; default all)
 """
 import ctypes as C
 import os
@@ -113,6 +116,46 @@ def outlet_fixtures(R, tmp, n=64):
                         gz_p=np.array([g[4] for g in gz]), gz_mout=np.array([g[5] for g in gz]), kt=0.01)
 
 
+def tdep_case(model):
+    """A blade row with the time-dependent options on: the rotational speed ramping from
+    0.5 to 1.0 of its value between 1 and 4 microseconds (ACCELERATION_ACTIVE, fixed time
+    step), and a Gaussian body-force perturbation of 40 % about a blade-row point, radius
+    4 cm, between 1 and 4 microseconds (PERTURBATION_BODY_FORCE)."""
+    case = cases.rotor3d(11, 5, 7, bfm_model=model, time_step="fixed")
+    case["config"].update({
+        "ACCELERATION_ACTIVE": "yes", "ACCELERATION_INITIAL_TIME": "1e-6", "ACCELERATION_FINAL_TIME": "4e-6",
+        "ROTATIONAL_SPEED_SCALING_FACTOR_INITIAL": "0.5", "ROTATIONAL_SPEED_SCALING_FACTOR_FINAL": "1.0",
+        "PERTURBATION_BODY_FORCE": "yes", "PERTURBATION_CENTER": "0.4, 0.25, 0.0",
+        "PERTURBATION_RADIAL_EXTENSION": "0.04", "PERTURBATION_SCALING_FACTOR": "0.4",
+        "PERTURBATION_TIME_START": "1e-6", "PERTURBATION_TIME_DURATION": "3e-6"})
+    return case
+
+
+def tdep_fixtures(R, tmp, n=96):
+    """SourceBFMBase::computeTotalSource with the time-dependent options at times before,
+    inside and after the ramp and the perturbation window (time 0 and the window edges
+    included)."""
+    rng = np.random.default_rng(20261208)
+    for model, tag in (("Hall", "hall"), ("Hall-Thollet", "thollet")):
+        case = tdep_case(model)
+        ini = cases.write_case(case, os.path.join(tmp, "tdep_" + tag))
+        assert R.ref_case_load(ini.encode()) == 0
+        cand = np.argwhere(case["fields"]["bladePresent"] > 0)
+        times = np.concatenate([[0.0, 1e-6, 2.5e-6, 4e-6, 5e-6], rng.uniform(0.0, 5e-6, n - 5)])
+        idx, Ws, Ts, outs, fis, fvs, devs = [], [], [], [], [], [], []
+        for q in range(n):
+            i, j, k = cand[rng.integers(0, len(cand))]
+            _, W = random_states(rng, 1)
+            W = W[0]
+            out, fi, fv, dv = np.zeros(5), np.zeros(3), np.zeros(3), C.c_double()
+            R.ref_bfm_source(int(i), int(j), int(k), dp(W), float(times[q]), dp(out), dp(fi), dp(fv), C.byref(dv))
+            idx.append((i, j, k)); Ws.append(W); Ts.append(times[q]); outs.append(out); fis.append(fi)
+            fvs.append(fv); devs.append(dv.value)
+        np.savez_compressed(os.path.join(HERE, "bfm_tdep_%s.npz" % tag), idx=np.array(idx), W=np.array(Ws),
+                            time=np.array(Ts), S=np.array(outs), finv=np.array(fis), fvisc=np.array(fvs),
+                            dev=np.array(devs))
+
+
 def sweep_cases():
     """Blade rows whose body force the sweep fixtures cover."""
     stall = cases.rotor3d(11, 5, 7, bfm_model="Hall-Thollet", off_design=True, stall=True)
@@ -149,12 +192,14 @@ def main():
     R = O.ref_lib()
     if R is None:
         raise SystemExit("build the reference library first: make -C oracle ref")
-    groups = set(sys.argv[1:]) or {"base", "outlets", "sweep"}
+    groups = set(sys.argv[1:]) or {"base", "outlets", "sweep", "tdep"}
     tmp = os.path.join(ROOT, "cases_tmp", "golden")
     if "outlets" in groups:
         outlet_fixtures(R, tmp)
     if "sweep" in groups:
         sweep_fixtures(R, tmp)
+    if "tdep" in groups:
+        tdep_fixtures(R, tmp)
     if "base" in groups:
         base_fixtures(R, tmp)
     print("golden fixtures written to", HERE)

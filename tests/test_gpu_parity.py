@@ -192,3 +192,50 @@ def test_thollet_missing_coefficient():
     from cturbobfm_amd import SolverError
     with pytest.raises(SolverError, match="HALL_THOLLET_COEFFICIENT_KF"):
         Solver.from_case(case)
+
+
+def time_dependent(case):
+    """ACCELERATION_ACTIVE (rotational speed ramping from 0.5 to 1.0 of its value between
+    1 and 4 microseconds) and PERTURBATION_BODY_FORCE (40 % Gaussian about a blade-row
+    point, radius 4 cm, between 1 and 4 microseconds) on a fixed 1e-7 s time step."""
+    case["config"].update({
+        "TIME_STEP_METHOD": "fixed", "FIXED_TIME_STEP": "1e-7",
+        "ACCELERATION_ACTIVE": "yes", "ACCELERATION_INITIAL_TIME": "1e-6", "ACCELERATION_FINAL_TIME": "4e-6",
+        "ROTATIONAL_SPEED_SCALING_FACTOR_INITIAL": "0.5", "ROTATIONAL_SPEED_SCALING_FACTOR_FINAL": "1.0",
+        "PERTURBATION_BODY_FORCE": "yes", "PERTURBATION_CENTER": "0.4, 0.25, 0.0",
+        "PERTURBATION_RADIAL_EXTENSION": "0.04", "PERTURBATION_SCALING_FACTOR": "0.4",
+        "PERTURBATION_TIME_START": "1e-6", "PERTURBATION_TIME_DURATION": "3e-6"})
+    return case
+
+
+@pytest.mark.parametrize("model", ["Hall", "Hall-Thollet"])
+def test_time_dependent_body_force(model):
+    """The rotational-speed ramp and the body-force perturbation over 60 iterations (6
+    microseconds: before, inside and after both windows), GPU against the oracle at the
+    run bar; the physical time both sides see is the sum of the fixed steps."""
+    case = time_dependent(cases.rotor3d(25, 9, 17, n_iterations=100, bfm_model=model))
+    print("case", case["name"], model, flush=True)
+    s = Solver.from_case(case)
+    oc = O.OracleCase(case)
+    lo, _, _ = oc.run(60, threads=8)
+    lg = s.advance(60)
+    hist = np.max(np.abs(lg - lo))
+    f_err = rel_err(s.state(), oc.state())
+    print("history abs err %.3e, field rel err %.3e" % (hist, f_err), flush=True)
+    assert hist <= RUN_TOL and f_err <= RUN_TOL
+    fi_g, _, dv_g, _ = s.side_outputs()
+    fi_o, _, dv_o, _ = oc.side_outputs()
+    assert np.max(np.abs(fi_g - fi_o)) / np.max(np.abs(fi_o)) <= RUN_TOL
+    # the options matter: the same case without them ends elsewhere
+    plain = cases.rotor3d(25, 9, 17, n_iterations=100, bfm_model=model, time_step="fixed")
+    p = Solver.from_case(plain)
+    p.advance(60)
+    assert rel_err(p.state(), s.state()) > 1e-8
+
+
+def test_acceleration_needs_fixed_time_step():
+    from cturbobfm_amd import SolverError
+    case = time_dependent(cases.rotor3d(17, 5, 9))
+    case["config"]["TIME_STEP_METHOD"] = "local"
+    with pytest.raises(SolverError, match="fixed time step"):
+        Solver.from_case(case)

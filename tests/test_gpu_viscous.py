@@ -55,16 +55,6 @@ def test_turbulence_refused():
         Solver.from_case(case)
 
 
-@pytest.mark.parametrize("key", ["PERTURBATION_BODY_FORCE", "ACCELERATION_ACTIVE"])
-def test_time_dependent_body_force_options_refused(key):
-    """Options the GPU path does not implement stop set-up with a message instead of
-    running without them."""
-    case = cases.rotor3d(17, 5, 9)
-    case["config"][key] = "yes"
-    with pytest.raises(SolverError, match=key):
-        Solver.from_case(case)
-
-
 def test_volume_file_of_viscous_case_refused(tmp_path):
     s = Solver.from_case(cases.viscous(cases.bump2d(17, 9)))
     s.advance(2)
