@@ -36,6 +36,10 @@ CASES = {
     "bump2d_jst": lambda: cases.bump2d(33, 17, n_iterations=200),
     "bump2d_roe_muscl_va": lambda: cases.bump2d(33, 17, n_iterations=200, scheme="ROE", muscl=True,
                                                   limiter="van albada", cfl=1.0),
+    "bump2d_roe_muscl_minmod": lambda: cases.bump2d(33, 17, n_iterations=200, scheme="ROE", muscl=True,
+                                                      limiter="min mod", cfl=1.0),
+    "rotor3d_roe_muscl_minmod_hall": lambda: cases.rotor3d(25, 9, 17, n_iterations=200, scheme="ROE", muscl=True,
+                                                            limiter="min mod", cfl=1.2),
     "bump2d_roe_1st_rk3_global": lambda: cases.bump2d(33, 17, n_iterations=200, scheme="ROE", cfl=0.8,
                                                         time_integration=1, time_step="global"),
     "rotor3d_jst_hall": lambda: cases.rotor3d(25, 9, 17, n_iterations=200),
@@ -67,6 +71,12 @@ CASES = {
     "duct3d_jst_viscous_suth": lambda: cases.viscous(no_bfm(cases.rotor3d(25, 9, 17, n_iterations=200))),
     # axisymmetric: the wedge faces enter the gradients (their face vectors do not cancel)
     "ger4_axi_thollet_viscous_suth": lambda: cases.viscous(RC.load_case("GER4_Axisymmetric_HallThollet"), walls={}),
+    # bladeless axisymmetric channel with no-slip hub (at rest) and shroud (moving): the
+    # wall paths (first-stage wall momentum, wall inlet faces, residual zeroing) on the
+    # wedge topology
+    "axi_channel_viscous_walls_moving": lambda: cases.viscous(
+        RC.load_case("AxisymmetricSwirlingChannel"), model="constant", mu=2e-3,
+        walls={"LOWER_WALL": (0.0, 0.0, 0.0), "UPPER_WALL": (0.0, 0.0, 25.0)}),
 }
 
 
