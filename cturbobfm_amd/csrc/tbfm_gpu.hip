@@ -45,12 +45,17 @@ thread_local size_t* g_bytes = nullptr;   // device-byte counter of the case bei
 std::recursive_mutex g_dev_mu;
 using DevLock = std::lock_guard<std::recursive_mutex>;
 
+// Device allocation, zero-filled before it returns.  The fill runs on the null stream,
+// which does not order itself with the non-blocking streams that later kernels use
+// (a table kernel launched right after the allocation could otherwise run before the
+// fill and have its results zeroed), so the null stream is drained here.
 template <typename T>
 T* dalloc(size_t n, std::vector<void*>& owned) {
     void* p = nullptr;
     if (n == 0) n = 1;
     HIPCHK(hipMalloc(&p, n * sizeof(T)));
-    HIPCHK(hipMemset(p, 0, n * sizeof(T)));
+    HIPCHK(hipMemsetAsync(p, 0, n * sizeof(T), nullptr));
+    HIPCHK(hipStreamSynchronize(nullptr));
     owned.push_back(p);
     if (g_bytes) *g_bytes += n * sizeof(T);
     return (T*)p;
