@@ -19,8 +19,12 @@ Workloads (sizes are node counts):
                  the lower wall; JST, RK41A, CFL 2, local dt; inlet p0 = 101325 Pa,
                  T0 = 288.15 K; outlet static pressure for M = 0.5.
   * ``rotor3d``  full annulus, hub r = 0.20 m, tip r = 0.30 m, x in [0, 0.8] m; blade
-                 row x in [0.3, 0.5] m with 20 blades at 6000 rpm, metal angle linear
-                 from -45 deg (leading edge) to 0 deg (trailing edge), blockage ramp
+                 row x in [0.3, 0.5] m with 20 blades at 6000 rpm, camber growing
+                 linearly 0 -> 45 deg from the leading edge: metal angle -45 deg at the
+                 leading edge (the relative inflow angle at mid-span, atan(-omega r /
+                 u_x), is about -46 deg in the body force's sign convention) turning to
+                 0 deg at the trailing edge (metal angles 0 -> +45 deg meet the inflow
+                 at 90 deg incidence and the solution diverges), blockage ramp
                  1 -> 0.98 across the row; inlet p0 = 101325 Pa, T0 = 288.15 K, axial;
                  radial-equilibrium outlet with INIT_PRESSURE = 95 kPa (the hub pressure
                  the reference freezes, solver.cpp:818-834); periodic k planes (360 deg).
