@@ -17,6 +17,8 @@
 // U = U0 - R * (alpha dt / V) (solver.cpp:2230-2249) is applied.  Nodes on the periodic
 // planes k = 0 and k = NK-1 park their residual for the periodic-closure kernel.
 #pragma once
+#include <type_traits>
+
 #include "tbfm_stage.hpp"
 
 namespace tbfm {
@@ -33,23 +35,37 @@ struct Tile {
     static constexpr int NH = NP - NC;   // halo positions
     static constexpr int NFI = 5 * TJ * (TI + 1), NFJ = 5 * (TJ + 1) * TI;
     static_assert(NC <= NT && NC > NT - 64, "a tile fills all but part of one wavefront");
-    static_assert(NP >= NT, "the norm tree reuses one primitive plane");
     static_assert(TJ * (TI + 1) + (TJ + 1) * TI <= 3 * NT, "at most three faces per thread");
 };
 
-template <class TL>
+// The staged plane in the storage type ST of the kernel instance (double; float for the
+// FP32 JST kernel) and the face fluxes of the tile, always double (boundary fluxes come
+// from the FP64 boundary buffer).  After the last plane the per-block norm tree runs in
+// the plane's storage.
+template <class TL, typename ST>
 struct PlaneLds {
-    double U[5][TL::NP];
+    ST U[5][TL::NP];
     struct {
-        double W[5][TL::NP];
-        double P[TL::NP];
-        double RF[TL::NP];
+        ST W[5][TL::NP];
+        ST P[TL::NP];
+        ST RF[TL::NP];
     } d;
     struct {
         double FI[TL::NFI];   // [v][tj][m]  m = 0..TI, I face of node i0+m (its left face)
         double FJ[TL::NFJ];   // [v][r][ti] r = 0..TJ
     } f;
+    static_assert(sizeof(U) + sizeof(d) >= 5 * NT * sizeof(double), "the norm tree fits in the plane");
+    __device__ __forceinline__ double (*red())[NT] { return reinterpret_cast<double (*)[NT]>(&U[0][0]); }
 };
+
+// Storage type of the staged states of a stage-kernel instance: float for the JST scheme
+// on the ideal gas with FP32 flux arithmetic (its sensors and dissipation differences
+// then run in FP32 too), double otherwise.
+template <int SCH, typename FT, int FL>
+using StageST = typename std::conditional<(SCH == 0 && FL == 0 && sizeof(FT) == 4), float, double>::type;
+// Face-record type: the FP32 records of the FP32-flux kernels, the FP64 ones otherwise.
+template <typename FT>
+using RecT = typename std::conditional<(sizeof(FT) == 4), float, double>::type;
 
 template <class TL>
 __device__ __forceinline__ int fiI(int v, int tj, int m) { return (v * TL::TJ + tj) * (TL::TI + 1) + m; }
@@ -147,56 +163,57 @@ __device__ __forceinline__ bool fetch_src(const double* __restrict__ U, int N, c
 
 // Stores what the faces read at LDS position `pos` (JST: state, primitives, pressure,
 // spectral factor; Roe: primitives); unused positions get harmless constants.
-template <class TL, int SCH, int FL>
-__device__ __forceinline__ void commit_position(const CaseDev& c, PlaneLds<TL>& L, int pos, bool ok,
+template <class TL, int SCH, int FL, typename ST>
+__device__ __forceinline__ void commit_position(const CaseDev& c, PlaneLds<TL, ST>& L, int pos, bool ok,
                                                 const double Un[5]) {
     if (!ok) {
 #pragma unroll
-        for (int v = 0; v < 5; ++v) L.d.W[v][pos] = 1.0;
+        for (int v = 0; v < 5; ++v) L.d.W[v][pos] = ST(1.0);
         if constexpr (SCH == 0) {
 #pragma unroll
-            for (int v = 0; v < 5; ++v) L.U[v][pos] = 1.0;
-            L.d.P[pos] = 1.0;
-            L.d.RF[pos] = 0.0;
+            for (int v = 0; v < 5; ++v) L.U[v][pos] = ST(1.0);
+            L.d.P[pos] = ST(1.0);
+            L.d.RF[pos] = ST(0.0);
         }
         return;
     }
-    double W[5], p, rf;
-    derive<double, FL>(Un, W, p, rf, Gas<FL>(c));
+    const ST Us[5] = {ST(Un[0]), ST(Un[1]), ST(Un[2]), ST(Un[3]), ST(Un[4])};
+    ST W[5], p, rf;
+    derive<ST, FL>(Us, W, p, rf, Gas<FL>(c));
 #pragma unroll
     for (int v = 0; v < 5; ++v) L.d.W[v][pos] = W[v];
     if constexpr (SCH == 0) {
 #pragma unroll
-        for (int v = 0; v < 5; ++v) L.U[v][pos] = Un[v];
+        for (int v = 0; v < 5; ++v) L.U[v][pos] = Us[v];
         L.d.P[pos] = p;
         L.d.RF[pos] = rf;
     }
 }
 
-template <class TL, int SCH, int FL>
-__device__ inline void fill_position(const CaseDev& c, const double* __restrict__ U, PlaneLds<TL>& L, int pos,
+template <class TL, int SCH, int FL, typename ST>
+__device__ inline void fill_position(const CaseDev& c, const double* __restrict__ U, PlaneLds<TL, ST>& L, int pos,
                                      int i0, int j0, int k) {
     double Un[5];
     const bool ok = fetch_position<TL>(c, U, pos, i0, j0, k, Un);
-    commit_position<TL, SCH, FL>(c, L, pos, ok, Un);
+    commit_position<TL, SCH, FL, ST>(c, L, pos, ok, Un);
 }
 
 // Face flux*|S| between LDS positions l and r (with ll = l - st, rr = r + st) along i
 // or j; n and area are the face's unit normal and |S|.
-template <class TL, int SCH, typename FT, int FL>
-__device__ __forceinline__ void lds_face(const CaseDev& c, const PlaneLds<TL>& L, int l, int st, double nx, double ny,
-                                         double nz, double area, double out[5]) {
+template <class TL, int SCH, typename FT, int FL, typename ST, typename RT>
+__device__ __forceinline__ void lds_face(const CaseDev& c, const PlaneLds<TL, ST>& L, int l, int st, RT nx, RT ny,
+                                         RT nz, RT area, double out[5]) {
     const int r = l + st, ll = l - st, rr = r + st;
-    if (SCH == 0) {
+    if constexpr (SCH == 0) {
         // JST sensors of the left and right nodes from the staged pressures
-        const double pll = L.d.P[ll], pl = L.d.P[l], pr = L.d.P[r], prr = L.d.P[rr];
-        const double sl = jst_sensor<double>(pll, pl, pr);
-        const double sr = jst_sensor<double>(pl, pr, prr);
+        const ST pll = L.d.P[ll], pl = L.d.P[l], pr = L.d.P[r], prr = L.d.P[rr];
+        const ST sl = jst_sensor<ST>(pll, pl, pr);
+        const ST sr = jst_sensor<ST>(pl, pr, prr);
         FT Wavg[5];
 #pragma unroll
         for (int v = 0; v < 5; ++v) Wavg[v] = (FT(L.d.W[v][l]) + FT(L.d.W[v][r])) * FT(0.5);
-        FT rr_ = FT(dmaxd(L.d.RF[l], L.d.RF[r]));
-        FT s = FT(dmaxd(sl, sr));
+        FT rr_ = FT(smax(L.d.RF[l], L.d.RF[r]));
+        FT s = FT(smax(sl, sr));
         FT psi2 = FT(0.5) * s * rr_;
         FT t4 = FT(0.02) * rr_ - FT(2.0) * psi2;
         FT psi4 = (FT(0.0) < t4) ? t4 : FT(0.0);
@@ -205,13 +222,14 @@ __device__ __forceinline__ void lds_face(const CaseDev& c, const PlaneLds<TL>& L
         const FT sm = FT(area);
 #pragma unroll
         for (int v = 0; v < 5; ++v) {
-            const double ul = L.U[v][l], ur = L.U[v][r];
-            double d1 = ur - ul;
-            double d3 = ((L.U[v][rr] - ur) - d1 * 2) + (ul - L.U[v][ll]);
+            const ST ul = L.U[v][l], ur = L.U[v][r];
+            ST d1 = ur - ul;
+            ST d3 = ((L.U[v][rr] - ur) - d1 * 2) + (ul - L.U[v][ll]);
             FT dis = FT(d1) * psi2 - FT(d3) * psi4;
             out[v] = double((F[v] - dis) * sm);
         }
     } else {
+        static_assert(sizeof(ST) == 8, "the Roe faces read FP64 primitives");
         double Wll[5], Wl[5], Wr[5], Wrr[5];
 #pragma unroll
         for (int v = 0; v < 5; ++v) {
@@ -226,19 +244,20 @@ __device__ __forceinline__ void lds_face(const CaseDev& c, const PlaneLds<TL>& L
 // spectral factor of slots 1 and 2 only (those of a new plane are derived when it
 // reaches slot 2); Roe keeps the primitives of all four.  Unused slot values (planes
 // beyond -1 .. NK) are never read by a face.
-template <int SCH, int FL>
+template <int SCH, int FL, typename ST>
 struct KQueue;
 
-template <int FL>
-struct KQueue<0, FL> {
+template <int FL, typename ST>
+struct KQueue<0, FL, ST> {
     // primitives without the density (which is U[slot][0]) and spectral factor of slots
-    // 1, 2, 3 (Wv[s - 1], RF[s - 1]): each plane is derived once, when it enters slot 3
-    double U[4][5], P[4], Wv[3][4], RF[3];
+    // 1, 2, 3 (Wv[s - 1], RF[s - 1]): each plane is derived once, when it enters slot 3;
+    // all in the storage type ST
+    ST U[4][5], P[4], Wv[3][4], RF[3];
     __device__ __forceinline__ void put(const CaseDev& c, int s, const double* u) {
-        double w[5], rf;
+        ST w[5], rf;
 #pragma unroll
-        for (int v = 0; v < 5; ++v) U[s][v] = u[v];
-        derive<double, FL>(U[s], w, P[s], rf, Gas<FL>(c));
+        for (int v = 0; v < 5; ++v) U[s][v] = ST(u[v]);
+        derive<ST, FL>(U[s], w, P[s], rf, Gas<FL>(c));
         if (s >= 1) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) Wv[s - 1][v] = w[v + 1];
@@ -261,17 +280,17 @@ struct KQueue<0, FL> {
         }
     }
     __device__ __forceinline__ void after_shift(const CaseDev&) {}
-    __device__ __forceinline__ void Wc(double w[5]) const {
+    __device__ __forceinline__ void Wc(ST w[5]) const {
         w[0] = U[1][0];
 #pragma unroll
         for (int v = 0; v < 4; ++v) w[v + 1] = Wv[0][v];
     }
-    template <typename FT>
-    __device__ __forceinline__ void face(const CaseDev& c, const double nn[4], double out[5]) const {
-        const double sl = jst_sensor<double>(P[0], P[1], P[2]);
-        const double sr = jst_sensor<double>(P[1], P[2], P[3]);
-        const double Wl[5] = {U[1][0], Wv[0][0], Wv[0][1], Wv[0][2], Wv[0][3]};
-        const double Wr[5] = {U[2][0], Wv[1][0], Wv[1][1], Wv[1][2], Wv[1][3]};
+    template <typename FT, typename RT>
+    __device__ __forceinline__ void face(const CaseDev& c, const RT nn[4], double out[5]) const {
+        const ST sl = jst_sensor<ST>(P[0], P[1], P[2]);
+        const ST sr = jst_sensor<ST>(P[1], P[2], P[3]);
+        const ST Wl[5] = {U[1][0], Wv[0][0], Wv[0][1], Wv[0][2], Wv[0][3]};
+        const ST Wr[5] = {U[2][0], Wv[1][0], Wv[1][1], Wv[1][2], Wv[1][3]};
         jst_face<FT, FL>(U[0], U[1], U[2], U[3], Wl, Wr, RF[0], RF[1], sl, sr, nn[0], nn[1], nn[2], nn[3],
                          Gas<FL>(c), out);
     }
@@ -281,7 +300,7 @@ struct KQueue<0, FL> {
 // would take the kernel to one wavefront per SIMD): a plane is derived for its pressure
 // when it enters slot 3 and again for its primitives when it reaches slot 2.
 template <>
-struct KQueue<0, 1> {
+struct KQueue<0, 1, double> {
     static constexpr int FL = 1;
     double U[4][5], P[4], Wv[2][4], RF[2];
     __device__ __forceinline__ void put(const CaseDev& c, int s, const double* u) {
@@ -317,8 +336,8 @@ struct KQueue<0, 1> {
 #pragma unroll
         for (int v = 0; v < 4; ++v) w[v + 1] = Wv[0][v];
     }
-    template <typename FT>
-    __device__ __forceinline__ void face(const CaseDev& c, const double nn[4], double out[5]) const {
+    template <typename FT, typename RT>
+    __device__ __forceinline__ void face(const CaseDev& c, const RT nn[4], double out[5]) const {
         const double sl = jst_sensor<double>(P[0], P[1], P[2]);
         const double sr = jst_sensor<double>(P[1], P[2], P[3]);
         const double Wl[5] = {U[1][0], Wv[0][0], Wv[0][1], Wv[0][2], Wv[0][3]};
@@ -328,8 +347,9 @@ struct KQueue<0, 1> {
     }
 };
 
-template <int FL>
-struct KQueue<1, FL> {
+template <int FL, typename ST>
+struct KQueue<1, FL, ST> {
+    static_assert(sizeof(ST) == 8, "the Roe column queue holds FP64 primitives");
     double W[4][5];
     __device__ __forceinline__ void put(const CaseDev& c, int s, const double* u) {
         double uu[5], p, rf;
@@ -349,8 +369,8 @@ struct KQueue<1, FL> {
 #pragma unroll
         for (int v = 0; v < 5; ++v) w[v] = W[1][v];
     }
-    template <typename FT>
-    __device__ __forceinline__ void face(const CaseDev& c, const double nn[4], double out[5]) const {
+    template <typename FT, typename RT>
+    __device__ __forceinline__ void face(const CaseDev& c, const RT nn[4], double out[5]) const {
         roe_face<FT, FL>(W[0], W[1], W[2], W[3], c.muscl, c.limiter, nn[0], nn[1], nn[2], nn[3], Gas<FL>(c), out);
     }
 };
@@ -382,19 +402,19 @@ struct FaceIdx {
 // FP32-flux kernels the FP32 records (their face arithmetic takes the float value of
 // every component, so the results are the same and the loads half as wide).
 template <typename FT>
-__device__ __forceinline__ void face_record(const CaseDev& c, unsigned rec, double nn[4]) {
+__device__ __forceinline__ void face_record(const CaseDev& c, unsigned rec, RecT<FT> nn[4]) {
     if constexpr (sizeof(FT) == 4) gld_face32(c.nrm32, rec, nn);
     else gld_face(c.nrm, rec, nn);
 }
 
 template <typename FT>
-__device__ __forceinline__ void kface_normal(const CaseDev& c, int f, double nn[4]) {
+__device__ __forceinline__ void kface_normal(const CaseDev& c, int f, RecT<FT> nn[4]) {
     face_record<FT>(c, (unsigned)(c.noffK + f), nn);
 }
 
 // Unit normal and area of an interior I or J face on plane k.
 template <class TL, typename FT>
-__device__ __forceinline__ void face_normal(const CaseDev& c, const FaceIdx<TL>& f, int k, double nn[4]) {
+__device__ __forceinline__ void face_normal(const CaseDev& c, const FaceIdx<TL>& f, int k, RecT<FT> nn[4]) {
     const int NI = c.ni, NJ = c.nj;
     const int fo = f.isI ? (k * NJ + f.gj) * (NI + 1) + f.gi : c.noffJ + (k * (NJ + 1) + f.gj) * NI + f.gi;
     face_record<FT>(c, (unsigned)fo, nn);
@@ -433,7 +453,9 @@ template <class TL, int SCH, bool THREED, typename FT, int FL>
 __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, const Work w, const StageOps st) {
     constexpr int TI = TL::TI, TJ = TL::TJ, HI = TL::HI, NH = TL::NH;
     constexpr int NFI = TJ * (TI + 1), NFJ = (TJ + 1) * TI;
-    __shared__ PlaneLds<TL> L;
+    using ST = StageST<SCH, FT, FL>;
+    using RT = RecT<FT>;
+    __shared__ PlaneLds<TL, ST> L;
     const CaseDev& c = *cp;
     const int tid = threadIdx.x;
     const int ti = tid % TI, tj = tid / TI;
@@ -455,7 +477,7 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
     const double* __restrict__ Uin = st.Uin;
 
     // K queue of the column (planes k-1 .. k+2 at the top of step k)
-    KQueue<SCH, FL> q;
+    KQueue<SCH, FL, ST> q;
     double FKl[5] = {0, 0, 0, 0, 0};
     if (THREED && act) {
 #pragma unroll
@@ -471,7 +493,7 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
 #pragma unroll
                 for (int v = 0; v < 5; ++v) FKl[v] = gld(c.bnd + c.boffK + (long)v * NIJ, (unsigned)(j * NI + i));
             } else {
-                double nk[4];
+                RT nk[4];
                 kface_normal<FT>(c, (kc0 * NJ + j) * NI + i, nk);
                 q.template face<FT>(c, nk, FKl);
             }
@@ -495,9 +517,9 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
         const bool parked = THREED && c.periodic && (k == 0 || k == NK - 1);
         // unit normal and area of the K face above the node and of the thread's first I/J
         // face of this plane, requested before the plane is staged
-        double nnk[4] = {1.0, 0.0, 0.0, 1.0};
+        RT nnk[4] = {RT(1.0), RT(0.0), RT(0.0), RT(1.0)};
         if (THREED && act && !bladecol && k + 1 < NK) kface_normal<FT>(c, ((k + 1) * NJ + j) * NI + i, nnk);
-        double nnx[4] = {1.0, 0.0, 0.0, 1.0};
+        RT nnx[4] = {RT(1.0), RT(0.0), RT(0.0), RT(1.0)};
         {
             const FaceIdx<TL> f(tid, i0, j0, NI, NJ);
             if (tid < nfaces && f.valid && !f.bnd) face_normal<TL, FT>(c, f, k, nnx);
@@ -509,7 +531,7 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
             dtv = (c.dt_mode == 1) ? gld(c.dtmin, 0u) : gld(c.dt, (unsigned)n);
             volv = gld(c.vol, (unsigned)n);
         }
-        double Wc[5];
+        ST Wc[5];
         if (THREED) {
             if (act) {
                 q.shift();
@@ -529,9 +551,11 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
                 }
             }
         } else if (act) {
-            double Uc[5], Pc, RFc;
-            load5(Uin, N, nid(c, i, j, k), Uc);
-            derive<double, FL>(Uc, Wc, Pc, RFc, Gas<FL>(c));
+            double Ud[5];
+            load5(Uin, N, nid(c, i, j, k), Ud);
+            const ST Uc[5] = {ST(Ud[0]), ST(Ud[1]), ST(Ud[2]), ST(Ud[3]), ST(Ud[4])};
+            ST Pc, RFc;
+            derive<ST, FL>(Uc, Wc, Pc, RFc, Gas<FL>(c));
 #pragma unroll
             for (int v = 0; v < 5; ++v) L.d.W[v][own] = Wc[v];
             if constexpr (SCH == 0) {
@@ -542,12 +566,12 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
             }
         }
         // tile positions outside the domain (possibly ghosts) and the halo by contiguous lanes
-        if (col && !act) fill_position<TL, SCH, FL>(c, Uin, L, own, i0, j0, k);
+        if (col && !act) fill_position<TL, SCH, FL, ST>(c, Uin, L, own, i0, j0, k);
         if (hown) {
-            commit_position<TL, SCH, FL>(c, L, hpos, hok, hU);
+            commit_position<TL, SCH, FL, ST>(c, L, hpos, hok, hU);
             if (k + 1 < kc1) hok = fetch_src(Uin, N, hsrc, (k + 1) * NIJ, hU);
         }
-        for (int h = tid + NT; h < NH; h += NT) fill_position<TL, SCH, FL>(c, Uin, L, halo_pos<TL>(h), i0, j0, k);
+        for (int h = tid + NT; h < NH; h += NT) fill_position<TL, SCH, FL, ST>(c, Uin, L, halo_pos<TL>(h), i0, j0, k);
         __syncthreads();
         // ---- every I and J face of the tile once: fq < NFI are I faces (row r, face m),
         // then J faces (row r of faces, column m); one code path, no divergence.  Each
@@ -561,7 +585,7 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
             const int fq = face_of(s, tid);
             if (fq >= nfaces) continue;
             const FaceIdx<TL> f(fq, i0, j0, NI, NJ);
-            double nn[4] = {nnx[0], nnx[1], nnx[2], nnx[3]};
+            const RT nn[4] = {nnx[0], nnx[1], nnx[2], nnx[3]};
             if (s < 2) {
                 const int fn = face_of(s + 1, tid);
                 if (fn < nfaces) {
@@ -579,7 +603,7 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
                 for (int v = 0; v < 5; ++v) F[v] = gld(c.bnd, (unsigned)(off + v * nsz));
             } else {
                 const int l = f.isI ? (f.r + 2) * HI + (f.m + 1) : (f.r + 1) * HI + (f.m + 2);
-                lds_face<TL, SCH, FT, FL>(c, L, l, f.isI ? 1 : HI, nn[0], nn[1], nn[2], nn[3], F);
+                lds_face<TL, SCH, FT, FL, ST, RT>(c, L, l, f.isI ? 1 : HI, nn[0], nn[1], nn[2], nn[3], F);
             }
             double* out = f.isI ? L.f.FI : L.f.FJ;
             const int base = f.isI ? f.r * (TI + 1) + f.m : f.r * TI + f.m;
@@ -658,7 +682,7 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
     // ---- per-block norm partial sums (deterministic tree), in the primitive buffer: its
     // last readers finished before the last face barrier
     if (st.last) {
-        double (*red)[TL::NP] = L.d.W;
+        double (*red)[NT] = L.red();
 #pragma unroll
         for (int v = 0; v < 5; ++v) red[v][tid] = nsq[v];
         __syncthreads();
@@ -676,9 +700,15 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
 // Work: one entry per (i, j) tile and k chunk of a case.  The case and work tables are
 // also passed as restrict kernel parameters (read-only for the launch), which lets the
 // compiler keep case fields in scalar registers or reload them freely.
+// Waves per SIMD the register allocation must allow: three for the FP32 JST kernels
+// (their LDS plane, 41.6 KB for 28x9 tiles, fits three blocks per CU), otherwise no bound
+// (the FP64 kernels need about 240 registers: two waves).
+template <int SCH, typename FT, int FL>
+constexpr int kStageWaves = std::is_same<StageST<SCH, FT, FL>, float>::value ? 3 : 1;
+
 template <class TL, int SCH, bool THREED, typename FT, int FL>
-__global__ void __launch_bounds__(NT) k_stage(const CaseDev* __restrict__ cases, const Work* __restrict__ work,
-                                              Launch La) {
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kStageWaves<SCH, FT, FL>)))
+k_stage(const CaseDev* __restrict__ cases, const Work* __restrict__ work, Launch La) {
     const Work w = work[blockIdx.x];
     if (w.cs < 0) return;   // padding entry of the XCD-interleaved table
     const CaseDev* __restrict__ cp = cases + w.cs;

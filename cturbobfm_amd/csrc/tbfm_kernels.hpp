@@ -233,11 +233,15 @@ __device__ __forceinline__ void gld_face(const double* base, unsigned rec, doubl
 }
 
 typedef float fquad __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ void gld_face32(const float* base, unsigned rec, double out[4]) {
+template <typename RT>
+__device__ __forceinline__ void gld_face32(const float* base, unsigned rec, RT out[4]) {
     const fquad a = *(const TBFM_GLOBAL fquad*)((const TBFM_GLOBAL char*)base + rec * 16u);
     out[0] = a.x; out[1] = a.y; out[2] = a.z; out[3] = a.w;
 }
 __device__ __forceinline__ double dmaxd(double a, double b) { return (a < b) ? b : a; }
+// std::max of either precision (the first argument unless it is less than the second)
+template <typename T>
+__device__ __forceinline__ T smax(T a, T b) { return (a < b) ? b : a; }
 __device__ __forceinline__ double dmind(double a, double b) { return (b < a) ? b : a; }
 
 }  // namespace dev

@@ -132,18 +132,17 @@ __device__ __forceinline__ T jst_sensor(T p1, T p2, T p3) {
 }
 
 // JST face flux times |S| (advection_jst.cpp:4-38; k2 = 0.5, k4 = 0.02, c4 = 2,
-// advection_jst.hpp:23).  Conserved differences are formed in double; the flux type FT
-// carries the rest.
-template <typename FT, int FL>
-__device__ __forceinline__ void jst_face(const double Ull[5], const double Ul[5], const double Ur[5],
-                                         const double Urr[5], const double Wl[5], const double Wr[5],
-                                         double rfl, double rfr, double sl, double sr, double nx, double ny,
-                                         double nz, double area, const Gas<FL>& gs, double out[5]) {
+// advection_jst.hpp:23).  Conserved differences are formed in the storage type ST of the
+// states (double, or float in the FP32 stage kernel); the flux type FT carries the rest.
+template <typename FT, int FL, typename ST, typename RT>
+__device__ __forceinline__ void jst_face(const ST Ull[5], const ST Ul[5], const ST Ur[5], const ST Urr[5],
+                                         const ST Wl[5], const ST Wr[5], ST rfl, ST rfr, ST sl, ST sr, RT nx,
+                                         RT ny, RT nz, RT area, const Gas<FL>& gs, double out[5]) {
     FT Wavg[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) Wavg[v] = (FT(Wl[v]) + FT(Wr[v])) * FT(0.5);
-    FT r = FT(dmaxd(rfl, rfr));
-    FT s = FT(dmaxd(sl, sr));
+    FT r = FT(smax(rfl, rfr));
+    FT s = FT(smax(sl, sr));
     FT psi2 = FT(0.5) * s * r;
     FT t4 = FT(0.02) * r - FT(2.0) * psi2;
     FT psi4 = (FT(0.0) < t4) ? t4 : FT(0.0);
@@ -152,8 +151,8 @@ __device__ __forceinline__ void jst_face(const double Ull[5], const double Ul[5]
     const FT sm = FT(area);
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
-        double d1 = Ur[v] - Ul[v];
-        double d3 = ((Urr[v] - Ur[v]) - d1 * 2) + (Ul[v] - Ull[v]);
+        ST d1 = Ur[v] - Ul[v];
+        ST d3 = ((Urr[v] - Ur[v]) - d1 * 2) + (Ul[v] - Ull[v]);
         FT dis = FT(d1) * psi2 - FT(d3) * psi4;
         F[v] -= dis;
         out[v] = double(F[v] * sm);
