@@ -86,9 +86,13 @@ struct StageGroup {
     Table t;
 };
 
+// Iterations captured in the plan's long graph: an advance of n iterations replays it
+// n / kGraphIters times and the one-iteration graph for the rest.
+constexpr int kGraphIters = 8;
+
 // Launch plan of a set of cases that advance together (one case, or the members of a
 // batch): the device case table, the work tables of every kernel and the captured
-// graph of one iteration.
+// graphs of one and of kGraphIters iterations.
 struct Plan {
     hipStream_t stream = nullptr;
     std::vector<tbfm_case*> members;
@@ -99,7 +103,8 @@ struct Plan {
     std::vector<StageGroup> groups;
     int nstages = 0;
     int fluids = 0;                   // bit 0: an ideal-gas member, bit 1: a table-fluid member
-    hipGraphExec_t graph = nullptr;
+    hipGraphExec_t graph = nullptr;   // one iteration
+    hipGraphExec_t graphN = nullptr;  // kGraphIters iterations
     bool graphValid = false;
     bool tablesValid = false;
 };
@@ -997,7 +1002,8 @@ void uploadTable(Table& t, const std::vector<dev::Work>& w) {
 void freePlan(Plan& P) {
     if (P.stream) hipStreamSynchronize(P.stream);
     if (P.graph) hipGraphExecDestroy(P.graph);
-    P.graph = nullptr;
+    if (P.graphN) hipGraphExecDestroy(P.graphN);
+    P.graph = P.graphN = nullptr;
     for (Table* t : {&P.dt, &P.mn, &P.prof, &P.bnd, &P.per, &P.norm, &P.start, &P.pre, &P.turbo, &P.visc, &P.wpre})
         freeTable(*t);
     for (auto& g : P.groups) freeTable(g.t);
@@ -1261,27 +1267,44 @@ void launchIteration(Plan& P, tbfm_case* prof) {
     HIPCHK(hipGetLastError());
 }
 
-// Captures one iteration of the plan into a graph (no memory operations inside).
-void capture(Plan& P) {
-    if (P.graphValid) return;
-    if (P.graph) {
-        HIPCHK(hipGraphExecDestroy(P.graph));
-        P.graph = nullptr;
-    }
+// Captures `iters` iterations of the plan into one graph (kernels only: no memory
+// operations inside).  An iteration after the device stopped the case is a no-op, so a
+// replay may run past convergence.
+hipGraphExec_t captureIterations(Plan& P, int iters) {
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(P.stream, hipStreamCaptureModeThreadLocal));
     try {
-        launchIteration(P, nullptr);
+        for (int q = 0; q < iters; ++q) launchIteration(P, nullptr);
     } catch (...) {
         hipStreamEndCapture(P.stream, &g);
         if (g) hipGraphDestroy(g);
         throw;
     }
     HIPCHK(hipStreamEndCapture(P.stream, &g));
-    hipError_t e = hipGraphInstantiate(&P.graph, g, nullptr, nullptr, 0);
+    hipGraphExec_t x = nullptr;
+    hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
     hipGraphDestroy(g);
     HIPCHK(e);
+    return x;
+}
+
+void capture(Plan& P) {
+    if (P.graphValid) return;
+    for (hipGraphExec_t* x : {&P.graph, &P.graphN})
+        if (*x) {
+            HIPCHK(hipGraphExecDestroy(*x));
+            *x = nullptr;
+        }
+    P.graph = captureIterations(P, 1);
+    P.graphN = captureIterations(P, kGraphIters);
     P.graphValid = true;
+}
+
+// Replays n iterations of a captured plan on its stream.
+void replay(Plan& P, long n) {
+    long q = 0;
+    for (; q + kGraphIters <= n; q += kGraphIters) HIPCHK(hipGraphLaunch(P.graphN, P.stream));
+    for (; q < n; ++q) HIPCHK(hipGraphLaunch(P.graph, P.stream));
 }
 
 // Autotuner of the stage kernel's tile shape and k planes per block: times launches of
@@ -1622,8 +1645,7 @@ int tbfm_gpu_advance(tbfm_case* c, long n, double* logres, long* converged_at) {
                 for (long q = 0; q < n; ++q) launchIteration(P, c);
         }
         // graph replays and the wait run outside the lock
-        if (!stopped && !c->prof)
-            for (long q = 0; q < n; ++q) HIPCHK(hipGraphLaunch(P.graph, P.stream));
+        if (!stopped && !c->prof) replay(P, n);
         HIPCHK(hipStreamSynchronize(P.stream));
         DevLock lk(g_dev_mu);
         if (c->prof) drainProfile(c);
@@ -2189,7 +2211,7 @@ int tbfm_gpu_batch_advance(tbfm_batch* b, long n) {
             }
             capture(P);
         }
-        for (long q = 0; q < n; ++q) HIPCHK(hipGraphLaunch(P.graph, P.stream));
+        replay(P, n);
         HIPCHK(hipStreamSynchronize(P.stream));
         DevLock lk(g_dev_mu);
         for (tbfm_case* m : P.members) settle(m);
