@@ -219,23 +219,32 @@ template <typename T>
 __device__ __forceinline__ T gld(const T* base, unsigned idx) {
     return *(const TBFM_GLOBAL T*)((const TBFM_GLOBAL char*)base + idx * (unsigned)sizeof(T));
 }
+// A load with the non-temporal hint, for data each kernel reads once (face records, dual
+// grid points): it is marked for early eviction, which keeps the rest of the working
+// set (state buffers, source buffer, blade-row data) in the L2 and the Infinity Cache
+// between kernels.
+template <typename T>
+__device__ __forceinline__ T gldnt(const T* base, unsigned idx) {
+    return __builtin_nontemporal_load((const TBFM_GLOBAL T*)((const TBFM_GLOBAL char*)base + idx * (unsigned)sizeof(T)));
+}
 template <typename T>
 __device__ __forceinline__ void gst(T* base, unsigned idx, T v) {
     *(TBFM_GLOBAL T*)((TBFM_GLOBAL char*)base + idx * (unsigned)sizeof(T)) = v;
 }
 
-// The 32-byte record of face `rec` (unit normal and area) as two 16-byte loads.
+// The 32-byte record of face `rec` (unit normal and area) as two 16-byte loads, with the
+// non-temporal hint (the records are read once per stage).
 typedef double dpair __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ void gld_face(const double* base, unsigned rec, double out[4]) {
     const TBFM_GLOBAL dpair* p = (const TBFM_GLOBAL dpair*)((const TBFM_GLOBAL char*)base + rec * 32u);
-    const dpair a = p[0], b = p[1];
+    const dpair a = __builtin_nontemporal_load(p), b = __builtin_nontemporal_load(p + 1);
     out[0] = a.x; out[1] = a.y; out[2] = b.x; out[3] = b.y;
 }
 
 typedef float fquad __attribute__((ext_vector_type(4)));
 template <typename RT>
 __device__ __forceinline__ void gld_face32(const float* base, unsigned rec, RT out[4]) {
-    const fquad a = *(const TBFM_GLOBAL fquad*)((const TBFM_GLOBAL char*)base + rec * 16u);
+    const fquad a = __builtin_nontemporal_load((const TBFM_GLOBAL fquad*)((const TBFM_GLOBAL char*)base + rec * 16u));
     out[0] = a.x; out[1] = a.y; out[2] = a.z; out[3] = a.w;
 }
 __device__ __forceinline__ double dmaxd(double a, double b) { return (a < b) ? b : a; }

@@ -235,9 +235,11 @@ __device__ inline void timestep_block(const CaseDev& c, const double* U, int bx,
             const long DI = c.ni + 1, DJ = c.nj + 1;
             const long d0 = ((long)k * DJ + j) * DI + i;
             const long dI = d0 + 1, dJ = d0 + DI, dK = d0 + DI * DJ;
-            double eIx = c.dux[dI] - c.dux[d0], eIy = c.duy[dI] - c.duy[d0], eIz = c.duz[dI] - c.duz[d0];
-            double eJx = c.dux[dJ] - c.dux[d0], eJy = c.duy[dJ] - c.duy[d0], eJz = c.duz[dJ] - c.duz[d0];
-            double eKx = c.dux[dK] - c.dux[d0], eKy = c.duy[dK] - c.duy[d0], eKz = c.duz[dK] - c.duz[d0];
+            const double *X = c.dux, *Y = c.duy, *Z = c.duz;
+            const unsigned u0 = (unsigned)d0, uI = (unsigned)dI, uJ = (unsigned)dJ, uK = (unsigned)dK;
+            double eIx = gldnt(X, uI) - gldnt(X, u0), eIy = gldnt(Y, uI) - gldnt(Y, u0), eIz = gldnt(Z, uI) - gldnt(Z, u0);
+            double eJx = gldnt(X, uJ) - gldnt(X, u0), eJy = gldnt(Y, uJ) - gldnt(Y, u0), eJz = gldnt(Z, uJ) - gldnt(Z, u0);
+            double eKx = gldnt(X, uK) - gldnt(X, u0), eKy = gldnt(Y, uK) - gldnt(Y, u0), eKz = gldnt(Z, uK) - gldnt(Z, u0);
             double dsI = sqrt(eIx * eIx + eIy * eIy + eIz * eIz);
             double dsJ = sqrt(eJx * eJx + eJy * eJy + eJz * eJz);
             double dsK = sqrt(eKx * eKx + eKy * eKy + eKz * eKz);
