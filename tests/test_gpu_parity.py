@@ -50,6 +50,15 @@ CASES = {
     "rotor3d_roe_muscl_thollet_od_stall": lambda: cases.rotor3d(25, 9, 17, n_iterations=200, scheme="ROE", muscl=True,
                                                                   limiter="van albada", bfm_model="Hall-Thollet",
                                                                   off_design=True, stall=True, cfl=1.2),
+    # grids smaller than one tile (every stage-kernel tile shape is 16 to 64 nodes wide),
+    # and one node beyond a whole number of 28 x 9 tiles in i and j; k pieces of one or two
+    # planes
+    "rotor3d_jst_tiny_9x4x5": lambda: cases.rotor3d(9, 4, 5, n_iterations=200),
+    "rotor3d_roe_muscl_ragged_29x10x5": lambda: cases.rotor3d(29, 10, 5, n_iterations=200, scheme="ROE", muscl=True,
+                                                                limiter="van albada", cfl=1.0),
+    "bump2d_jst_tiny_5x3": lambda: cases.bump2d(5, 3, n_iterations=200),
+    "bump2d_roe_muscl_ragged_30x10": lambda: cases.bump2d(30, 10, n_iterations=200, scheme="ROE", muscl=True,
+                                                            limiter="van albada", cfl=1.0),
     # outlets whose hub pressure follows the mass flow every iteration
     "rotor3d_jst_throttle": lambda: cases.rotor3d(25, 9, 17, n_iterations=200, outflow="throttle, 0.01"),
     "rotor3d_jst_massflow": lambda: cases.rotor3d(25, 9, 17, n_iterations=200, outflow="mass_flow, 15"),
