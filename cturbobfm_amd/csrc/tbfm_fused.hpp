@@ -729,7 +729,7 @@ k_stage(const CaseDev* __restrict__ cases, const Work* __restrict__ work, Launch
 // With st.rmode the averaged residuals themselves are written (residual entry point).
 // Work: one entry per 256 node-mate pairs of a periodic 3-D case.
 __global__ void __launch_bounds__(256) k_periodic(Launch La) {
-    __shared__ double red[256];
+    __shared__ double red[5][256];
     const Work w = La.work[blockIdx.x];
     const CaseDev& c = La.cases[w.cs];
     if (*c.stop) return;
@@ -796,16 +796,17 @@ __global__ void __launch_bounds__(256) k_periodic(Launch La) {
         }
     }
     if (st.last) {
-        for (int v = 0; v < 5; ++v) {
-            red[threadIdx.x] = nrm[v];
-            __syncthreads();
-            for (int s = 128; s > 0; s >>= 1) {
-                if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-                __syncthreads();
-            }
-            if (threadIdx.x == 0) c.partial[((long)c.nparts_stage + w.x) * 5 + v] = red[0];
+        // the five equations' trees side by side (per equation the same pairings)
+#pragma unroll
+        for (int v = 0; v < 5; ++v) red[v][threadIdx.x] = nrm[v];
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if (threadIdx.x < s)
+#pragma unroll
+                for (int v = 0; v < 5; ++v) red[v][threadIdx.x] += red[v][threadIdx.x + s];
             __syncthreads();
         }
+        if (threadIdx.x < 5) c.partial[((long)c.nparts_stage + w.x) * 5 + threadIdx.x] = red[threadIdx.x][0];
     }
 }
 
@@ -814,7 +815,7 @@ __global__ void __launch_bounds__(256) k_periodic(Launch La) {
 // (solver.cpp:2985-3015), all on the device.  Sets *stop once converged or diverged and
 // advances the iteration counter.  Work: one entry per case.
 __global__ void __launch_bounds__(256) k_norms(Launch La) {
-    __shared__ double red[256];
+    __shared__ double red[5][256];
     __shared__ double lr[5];
     const Work w = La.work[blockIdx.x];
     const CaseDev& c = La.cases[w.cs];
@@ -828,30 +829,32 @@ __global__ void __launch_bounds__(256) k_norms(Launch La) {
     for (int q = threadIdx.x; q < nparts; q += 256)
 #pragma unroll
         for (int v = 0; v < 5; ++v) acc[v] += partial[(long)q * 5 + v];
-    for (int v = 0; v < 5; ++v) {
-        red[threadIdx.x] = acc[v];
-        __syncthreads();
-        for (int h = 128; h > 0; h >>= 1) {
-            if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) {
-            double nrm = sqrt(red[0]);
-            double l;
-            if (isnan(nrm) || isinf(nrm)) {
-                l = nrm;
-                if (*cv.diverged == 0) *cv.diverged = 1 + v;
-            } else if (nrm >= 2.2250738585072014e-308) {
-                l = log10(nrm / (double)((long)c.ni * c.nj * c.nk));
-            } else {
-                l = -16.0;
-            }
-            if (l < -16.0) l = -16.0;
-            lr[v] = l;
-        }
+    // the five equations' trees side by side: per equation the pairing order of five
+    // separate trees, with one barrier per level
+#pragma unroll
+    for (int v = 0; v < 5; ++v) red[v][threadIdx.x] = acc[v];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h)
+#pragma unroll
+            for (int v = 0; v < 5; ++v) red[v][threadIdx.x] += red[v][threadIdx.x + h];
         __syncthreads();
     }
+    if (threadIdx.x < 5) {
+        const int v = threadIdx.x;
+        const double nrm = sqrt(red[v][0]);
+        double l;
+        if (isnan(nrm) || isinf(nrm)) l = nrm;
+        else if (nrm >= 2.2250738585072014e-308) l = log10(nrm / (double)((long)c.ni * c.nj * c.nk));
+        else l = -16.0;
+        if (l < -16.0) l = -16.0;
+        lr[v] = l;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
+        // the first equation whose norm is not finite (solver.cpp:1494-1496)
+        for (int v = 0; v < 5; ++v)
+            if ((isnan(lr[v]) || isinf(lr[v])) && *cv.diverged == 0) *cv.diverged = 1 + v;
         const long it = *c.iter;
         if (it >= 1 && it <= cv.hist_len)
             for (int v = 0; v < 5; ++v) cv.hist[(it - 1) * 5 + v] = lr[v];
