@@ -8,9 +8,11 @@
 // ghost nodes at the domain ends are built exactly as the reference extrapolates them.
 // Every I and J face of the tile is then evaluated once in a single divergence-free
 // pass over a flat face index, with the face's unit normal and area read from
-// host-precomputed arrays (the same IEEE operations the reference applies per call).
-// Face results go straight to LDS arrays and the residual norms accumulate in LDS,
-// which keeps the register budget at two waves per SIMD.
+// host-precomputed records (the same IEEE operations the reference applies per call).
+// Face results go straight to LDS arrays; the residual norms accumulate in registers
+// and are reduced per block after the last plane.  The FP64 instances need about 240
+// registers (two waves per SIMD); for JST on the ideal gas with FP32 flux the staged
+// plane and the column queue are float (StageST), which gives three.
 // The residual of each node is assembled in the reference's order
 //   (I left, I right, J left, J right, K left, K right, geometric, Gong, BFM)
 // (solver.cpp:1794-1821, 1997-2006, 2853-2891) and the RK update

@@ -14,7 +14,9 @@ from typing import Dict, Optional
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# TBFM_ORACLE_LIB selects another build of the same source (the build-to-build spread test
+# compiles one with the reference Makefile's flags, FMA contraction on)
+LIB_PATH = os.environ.get("TBFM_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 REF_LIB_PATH = os.path.join(HERE, "_ref", "libtbfm_ref.so")
 
 TOPO = {"2D": 0, "2d": 0, "3D": 1, "3d": 1, "axisymmetric": 2, "Axisymmetric": 2,
