@@ -493,7 +493,7 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
         if (!bladecol) {
             if (kc0 == 0) {
 #pragma unroll
-                for (int v = 0; v < 5; ++v) FKl[v] = gld(c.bnd + c.boffK + (long)v * NIJ, (unsigned)(j * NI + i));
+                for (int v = 0; v < 5; ++v) FKl[v] = gld(c.bnd + c.boffK, (unsigned)((j * NI + i) * 6 + v));
             } else {
                 RT nk[4];
                 kface_normal<FT>(c, (kc0 * NJ + j) * NI + i, nk);
@@ -600,9 +600,9 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
             if (f.bnd) {
                 const int nsz = f.isI ? NJ * NK : NI * NK;
                 const int bq = f.isI ? k * NJ + f.gj : k * NI + f.gi;
-                const int off = (f.isI ? 0 : c.boffJ) + (f.lo ? 0 : 5 * nsz) + bq;
+                const int off = (f.isI ? 0 : c.boffJ) + ((f.lo ? 0 : nsz) + bq) * 6;
 #pragma unroll
-                for (int v = 0; v < 5; ++v) F[v] = gld(c.bnd, (unsigned)(off + v * nsz));
+                for (int v = 0; v < 5; ++v) F[v] = gld(c.bnd, (unsigned)(off + v));
             } else {
                 const int l = f.isI ? (f.r + 2) * HI + (f.m + 1) : (f.r + 1) * HI + (f.m + 2);
                 lds_face<TL, SCH, FT, FL, ST, RT>(c, L, l, f.isI ? 1 : HI, nn[0], nn[1], nn[2], nn[3], F);
@@ -619,7 +619,7 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
             if (k + 1 == NK) {
 #pragma unroll
                 for (int v = 0; v < 5; ++v)
-                    FKr[v] = gld(c.bnd + c.boffK + (long)(5 + v) * NIJ, (unsigned)(j * NI + i));
+                    FKr[v] = gld(c.bnd + c.boffK, (unsigned)((NIJ + j * NI + i) * 6 + v));
             } else {
                 q.template face<FT>(c, nnk, FKr);
             }
@@ -699,17 +699,549 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
     }
 }
 
+// ------------------------------------------------------------------ JST stage, 3-D
+// The 3-D JST instances run the same per-node arithmetic as stage_block with a static
+// load schedule.  Vector-memory loads return in issue order and one counter tracks them,
+// so a wait for any load is a wait for every load issued before it; the counter's
+// value can only be known where every path through the loop issues the same loads.
+// Every load of the k loop is therefore issued by every lane (lanes without a use read a
+// safe address), in one order per plane:
+//   top:     the column's plane k+3 and the halo slot's plane k+1 (used one plane later)
+//   faces:   the K record and the three face-slot records of plane k+1
+//   end:     U^(0), dt and V of plane k+1
+// so that each value arrives a plane before it is used and no wait covers a younger
+// load.  The boundary fluxes of domain-boundary I/J faces are read through the face
+// slots (48-byte records behind the face records), and ghost positions of the staged
+// plane are extrapolated from the staged nodes in LDS (blocks at a domain end only, one
+// more barrier) instead of being loaded.  Loads whose results are used at once (the
+// virtual plane NK of a non-periodic case, the end-plane K boundary fluxes, blade-row
+// sources, viscous terms) stay where they are used.
+// A per-plane copy of a value that the compiler may not treat as loop-invariant: what is
+// derived from it (LDS and record offsets) is recomputed each plane instead of being
+// hoisted out of the k loop into registers that the plane's arithmetic needs.
+template <typename T>
+__device__ __forceinline__ T per_plane(T x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+// 40 bytes from the record base at a face slot's offset: a face record (32 or 16 bytes,
+// and what follows it) or a boundary-flux record (5 doubles).
+struct SlotData {
+    u32x4 a, b;
+    u32x2 e;
+};
+__device__ __forceinline__ SlotData slot_load(const char* fb, unsigned off) {
+    const TBFM_GLOBAL char* p = (const TBFM_GLOBAL char*)fb + off;
+    SlotData d;
+    d.a = __builtin_nontemporal_load((const TBFM_GLOBAL u32x4*)p);
+    d.b = __builtin_nontemporal_load((const TBFM_GLOBAL u32x4*)(p + 16));
+    d.e = __builtin_nontemporal_load((const TBFM_GLOBAL u32x2*)(p + 32));
+    return d;
+}
+template <typename RT>
+__device__ __forceinline__ void slot_record(const SlotData& d, RT nn[4]) {
+    if constexpr (sizeof(RT) == 8) {
+        const dpair x = __builtin_bit_cast(dpair, d.a), y = __builtin_bit_cast(dpair, d.b);
+        nn[0] = x.x; nn[1] = x.y; nn[2] = y.x; nn[3] = y.y;
+    } else {
+        const fquad x = __builtin_bit_cast(fquad, d.a);
+        nn[0] = x.x; nn[1] = x.y; nn[2] = x.z; nn[3] = x.w;
+    }
+}
+__device__ __forceinline__ void slot_flux(const SlotData& d, double F[5]) {
+    const dpair x = __builtin_bit_cast(dpair, d.a), y = __builtin_bit_cast(dpair, d.b);
+    F[0] = x.x; F[1] = x.y; F[2] = y.x; F[3] = y.y; F[4] = __builtin_bit_cast(double, d.e);
+}
+// The K record at byte offset off (FP64: two 16-byte loads; FP32: one).
+template <typename RT>
+__device__ __forceinline__ void krec_load(const char* fb, unsigned off, RT nn[4]) {
+    if constexpr (sizeof(RT) == 8) gld_face((const double*)fb, off / 32u, nn);
+    else gld_face32((const float*)fb, off / 16u, nn);
+}
+
+// Face s (0..2) of thread tid in a tile, resolved once per block: bit 0 valid, bit 1 a
+// domain-boundary face, bit 2 an I face, bits 3-12 the LDS position of its left node,
+// bits 13-23 its index in the face buffers; b0 the byte offset of its plane-0 data from
+// the record base (face record, or boundary-flux record), step the bytes per plane.
+struct FaceSlot {
+    unsigned b0;
+    int info;
+    __device__ __forceinline__ bool valid() const { return info & 1; }
+    __device__ __forceinline__ bool bnd() const { return info & 2; }
+    __device__ __forceinline__ bool isI() const { return info & 4; }
+    __device__ __forceinline__ int l() const { return (info >> 3) & 1023; }
+    __device__ __forceinline__ int obase() const { return info >> 13; }
+};
+template <class TL, unsigned RS>
+__device__ __forceinline__ FaceSlot face_slot(const CaseDev& c, int s, int tid, int i0, int j0) {
+    constexpr int TI = TL::TI, HI = TL::HI, NFI = TL::TJ * (TI + 1), NFJ = (TL::TJ + 1) * TI;
+    static_assert(TL::NP <= 1024 && NFI + NFJ <= 2048, "face-slot fields fit their bits");
+    const int NI = c.ni, NJ = c.nj, NK = c.nk;
+    FaceSlot d{0u, 0};
+    const int fq = face_of(s, tid);
+    if (fq >= NFI + NFJ) return d;
+    const FaceIdx<TL> f(fq, i0, j0, NI, NJ);
+    if (!f.valid) return d;
+    const int l = f.isI ? (f.r + 2) * HI + (f.m + 1) : (f.r + 1) * HI + (f.m + 2);
+    const int ob = f.isI ? f.r * (TI + 1) + f.m : f.r * TI + f.m;
+    d.info = 1 | (f.bnd ? 2 : 0) | (f.isI ? 4 : 0) | (l << 3) | (ob << 13);
+    if (f.bnd) {
+        // boundary-flux record (side, k*NJ + j) of the I block, (side, k*NI + i) of the J block
+        const int side = f.lo ? 0 : 1;
+        const unsigned q = f.isI ? (unsigned)(side * NJ * NK + f.gj) : (unsigned)(2 * NJ * NK + side * NI * NK + f.gi);
+        d.b0 = c.bnd_off + 48u * q;
+    } else {
+        d.b0 = RS * (unsigned)(f.isI ? f.gj * (NI + 1) + f.gi : c.noffJ + f.gj * NI + f.gi);
+    }
+    return d;
+}
+// Byte offset of a face slot's data on plane k (record 0 for a slot without a face).
+template <unsigned RS>
+__device__ __forceinline__ unsigned slot_off(const CaseDev& c, const FaceSlot& d, int k) {
+    const unsigned NI = c.ni, NJ = c.nj;
+    const unsigned step = d.bnd() ? 48u * (d.isI() ? NJ : NI) : RS * (d.isI() ? NJ * (NI + 1) : (NJ + 1) * NI);
+    return d.valid() ? d.b0 + (unsigned)k * step : 0u;
+}
+
+// LDS of the 3-D JST kernel: the staged plane (conserved state, velocity and specific
+// total energy, pressure, spectral factor; the density is U[0]), the I/J face fluxes of
+// the tile, the K flux of the face below each column's next node and the column's
+// residual-norm accumulators (which also hold the per-block norm tree).
+template <class TL>
+struct PlaneLdsJ {
+    double U[5][TL::NP];
+    double W[4][TL::NP];   // W[v - 1]: primitive v = 1..4
+    double P[TL::NP];
+    double RF[TL::NP];
+    double FI[5 * TL::TJ * (TL::TI + 1)];
+    double FJ[5 * (TL::TJ + 1) * TL::TI];
+    double FK[5][NT];
+    double nsq[5][NT];
+};
+
+// Stores what the faces read at LDS position pos (state, primitives, pressure, spectral
+// factor); unused positions get harmless constants (commit_position for this layout).
+template <class TL, int FL>
+__device__ __forceinline__ void commit_j(const CaseDev& c, PlaneLdsJ<TL>& L, int pos, bool ok, const double Un[5]) {
+    if (!ok) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) L.U[v][pos] = 1.0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) L.W[v][pos] = 1.0;
+        L.P[pos] = 1.0;
+        L.RF[pos] = 0.0;
+        return;
+    }
+    double W[5], p, rf;
+    derive<double, FL>(Un, W, p, rf, Gas<FL>(c));
+#pragma unroll
+    for (int v = 0; v < 5; ++v) L.U[v][pos] = Un[v];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) L.W[v][pos] = W[v + 1];
+    L.P[pos] = p;
+    L.RF[pos] = rf;
+}
+
+// JST face flux*|S| between LDS positions l and r = l + st (lds_face for this layout).
+template <class TL, int FL>
+__device__ __forceinline__ void lds_face_j(const CaseDev& c, const PlaneLdsJ<TL>& L, int l, int st, double nx,
+                                           double ny, double nz, double area, double out[5]) {
+    const int r = l + st, ll = l - st, rr = r + st;
+    const double pll = L.P[ll], pl = L.P[l], pr = L.P[r], prr = L.P[rr];
+    const double sl = jst_sensor<double>(pll, pl, pr);
+    const double sr = jst_sensor<double>(pl, pr, prr);
+    double Wavg[5];
+    Wavg[0] = (L.U[0][l] + L.U[0][r]) * 0.5;
+#pragma unroll
+    for (int v = 1; v < 5; ++v) Wavg[v] = (L.W[v - 1][l] + L.W[v - 1][r]) * 0.5;
+    const double rr_ = smax(L.RF[l], L.RF[r]);
+    const double s = smax(sl, sr);
+    const double psi2 = 0.5 * s * rr_;
+    const double t4 = 0.02 * rr_ - 2.0 * psi2;
+    const double psi4 = (0.0 < t4) ? t4 : 0.0;
+    double F[5];
+    phys_flux_n<double, FL>(Wavg, nx, ny, nz, Gas<FL>(c), F);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        const double ul = L.U[v][l], ur = L.U[v][r];
+        const double d1 = ur - ul;
+        const double d3 = ((L.U[v][rr] - ur) - d1 * 2) + (ul - L.U[v][ll]);
+        const double dis = d1 * psi2 - d3 * psi4;
+        out[v] = (F[v] - dis) * area;
+    }
+}
+
+// A ghost position of the staged plane (one step outside the domain in i or j), built
+// from the staged boundary node and its inner neighbour exactly as fetch_src builds it
+// from global memory (solver.cpp:1980-1993); other positions are left alone.
+template <class TL, int FL>
+__device__ __forceinline__ void ghost_from_lds(const CaseDev& c, PlaneLdsJ<TL>& L, int p, int i0, int j0) {
+    const int hi = p % TL::HI, hj = p / TL::HI;
+    const int gi = i0 - 2 + hi, gj = j0 - 2 + hj;
+    const int NI = c.ni, NJ = c.nj;
+    const bool iin = (gi >= 0 && gi < NI), jin = (gj >= 0 && gj < NJ);
+    const bool ighost = jin && (gi == -1 || gi == NI);
+    const bool jghost = iin && (gj == -1 || gj == NJ);
+    if (!ighost && !jghost) return;
+    const bool lo = ighost ? (gi < 0) : (gj < 0);
+    const int sd = ighost ? 1 : TL::HI;
+    const int a = lo ? p + sd : p - sd, b = lo ? p + 2 * sd : p - 2 * sd;
+    double Un[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        const double A = L.U[v][a], B = L.U[v][b];
+        Un[v] = lo ? A - (B - A) : A + (A - B);
+    }
+    commit_j<TL, FL>(c, L, p, true, Un);
+}
+
+// Column queue of the 3-D JST kernel along k.  At the K face of plane k it holds plane k-1
+// (state and pressure), k+1 and k+2 (state, pressure, velocity and specific total energy,
+// spectral factor) in registers; plane k itself is the column's own node of the staged
+// plane in LDS.  The ideal gas derives a plane when it enters at k+2; the table fluid
+// derives its pressure then and its primitives when it reaches k+1.
+template <int FL>
+struct KColJ {
+    double A[5], Ap;
+    double C[5], Cp, Cw[4], Crf;
+    double D[5], Dp, Dw[4], Drf;
+    __device__ __forceinline__ void enter(const CaseDev& c, const double u[5]) {
+        double w[5], rf;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) D[v] = u[v];
+        derive<double, FL>(D, w, Dp, rf, Gas<FL>(c));
+        if constexpr (FL == 0) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Dw[v] = w[v + 1];
+            Drf = rf;
+        }
+    }
+    // plane k+2 moves to k+1
+    __device__ __forceinline__ void d_to_c(const CaseDev& c) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) C[v] = D[v];
+        Cp = Dp;
+        if constexpr (FL == 0) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Cw[v] = Dw[v];
+            Crf = Drf;
+        } else {
+            double w[5], p;
+            derive<double, FL>(C, w, p, Crf, Gas<FL>(c));
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Cw[v] = w[v + 1];
+        }
+    }
+    // plane k-1 from the own node of the staged plane (before it is overwritten)
+    template <class TL>
+    __device__ __forceinline__ void own_to_a(const PlaneLdsJ<TL>& L, int own) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) A[v] = L.U[v][own];
+        Ap = L.P[own];
+    }
+    // plane k+1 becomes the own node of the staged plane
+    template <class TL>
+    __device__ __forceinline__ void c_to_own(PlaneLdsJ<TL>& L, int own) const {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) L.U[v][own] = C[v];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) L.W[v][own] = Cw[v];
+        L.P[own] = Cp;
+        L.RF[own] = Crf;
+    }
+    // K face between plane k (own LDS node) and k+1 (KQueue::face with slot 1 from LDS)
+    template <class TL>
+    __device__ __forceinline__ void face(const CaseDev& c, const PlaneLdsJ<TL>& L, int own, const double nn[4],
+                                         double out[5]) const {
+        double B[5], Bw[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) B[v] = L.U[v][own];
+        Bw[0] = B[0];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Bw[v + 1] = L.W[v][own];
+        const double Bp = L.P[own], Brf = L.RF[own];
+        const double sl = jst_sensor<double>(Ap, Bp, Cp);
+        const double sr = jst_sensor<double>(Bp, Cp, Dp);
+        const double Wr[5] = {C[0], Cw[0], Cw[1], Cw[2], Cw[3]};
+        jst_face<double, FL>(A, B, C, D, Bw, Wr, Brf, Crf, sl, sr, nn[0], nn[1], nn[2], nn[3], Gas<FL>(c), out);
+    }
+};
+
+template <class TL, int FL>
+__device__ __forceinline__ void stage_block_jst3d(const CaseDev* __restrict__ cp, const Work w, const StageOps st) {
+    constexpr int SCH = 0;
+    using FT = double;
+    constexpr int TI = TL::TI, TJ = TL::TJ, HI = TL::HI, NH = TL::NH, NP = TL::NP;
+    constexpr int NFI = TJ * (TI + 1), NFJ = (TJ + 1) * TI;
+    constexpr unsigned RS = 32u;   // FP64 face-record bytes
+    static_assert(sizeof(PlaneLdsJ<TL>) <= 80 * 1024, "two blocks per CU");
+    static_assert(NH <= NT, "one halo slot per thread");
+    __shared__ PlaneLdsJ<TL> L;
+    const CaseDev& c = *cp;
+    const int tid = threadIdx.x;
+    const int ti = tid % TI, tj = tid / TI;
+    const int i0 = w.x * TI, j0 = w.y * TJ;
+    const int i = i0 + ti, j = j0 + tj;
+    const int NI = c.ni, NJ = c.nj, NK = c.nk;
+    const bool col = tid < TL::NC;
+    const bool act = col && (i < NI) && (j < NJ);
+    const int kc0 = (w.z * NK) / c.sgz;
+    const int kc1 = ((w.z + 1) * NK) / c.sgz;
+    const int N = (int)c.N;
+    const int NIJ = NI * NJ;
+    const int own = (tj + 2) * HI + (ti + 2);
+    // the column, clamped into the grid: the address lanes without a column read
+    const int colc = min(j, NJ - 1) * NI + min(i, NI - 1);
+    const bool bladecol = c.bfm_active && act && gld(c.blade2d, (unsigned)colc);
+    const bool kOn = act && !bladecol;   // the column evaluates its K faces
+    const bool srcOn = (c.topo == TOPO_AXI) || c.bfm_active;
+    const double* __restrict__ Uin = st.Uin;
+    const char* fb = c.fbase;
+    // block-uniform: the staged region (tile and 2-node halo) holds ghost positions
+    const bool ghosts = (i0 == 0) || (NI <= i0 + TI + 1) || (j0 == 0) || (NJ <= j0 + TJ + 1);
+    // halo slot of the thread (threads beyond NH and slots that are not real nodes read
+    // plane offset h.a, a real node or 0)
+    const bool hown = tid < NH;
+    const int hpos = hown ? halo_pos<TL>(tid) : 0;
+    const HaloSrc hsrc = hown ? halo_src<TL>(c, hpos, i0, j0) : HaloSrc{0, 0, 0};
+    FaceSlot fs[3];
+#pragma unroll
+    for (int s = 0; s < 3; ++s) fs[s] = face_slot<TL, RS>(c, s, tid, i0, j0);
+    // plane read for the column queue's slot 3 at plane gk: the virtual plane NK of a
+    // periodic case is plane 1 (rotated when used); other planes beyond the grid read
+    // plane NK-1 (the virtual plane of other cases is built when used)
+    auto qplane = [&](int gk) { return gk < NK ? gk : ((c.periodic && gk == NK) ? 1 : NK - 1); };
+    // K record of plane k (the face above node k), record 0 where the column has none
+    auto krec_off = [&](int k) -> unsigned {
+        return (kOn && k + 1 < NK) ? RS * (unsigned)(c.noffK + (k + 1) * NIJ + colc) : 0u;
+    };
+    const double* dtp = (c.dt_mode == 1) ? c.dtmin : c.dt;
+
+    // ---- column queue warm-up (planes kc0-2 .. kc0+1; plane kc0-1 as the own node of the
+    // staged plane) and the K face below plane kc0
+    KColJ<FL> q;
+    {
+        double FKl[5] = {0, 0, 0, 0, 0};
+        if (act) {
+            double u[5] = {1.0, 1.0, 1.0, 1.0, 1.0};
+            if (kc0 - 2 >= -1) load_k(c, Uin, i, j, kc0 - 2, u);
+            {
+                double w[5], rf;
+#pragma unroll
+                for (int v = 0; v < 5; ++v) q.A[v] = u[v];
+                derive<double, FL>(q.A, w, q.Ap, rf, Gas<FL>(c));
+            }
+#pragma unroll
+            for (int v = 0; v < 5; ++v) u[v] = 1.0;
+            load_k(c, Uin, i, j, kc0 - 1, u);   // kc0 - 1 >= -1
+            commit_j<TL, FL>(c, L, own, true, u);
+            load_k(c, Uin, i, j, kc0, u);
+            q.enter(c, u);
+            q.d_to_c(c);
+#pragma unroll
+            for (int v = 0; v < 5; ++v) u[v] = 1.0;
+            if (kc0 + 1 <= NK) load_k(c, Uin, i, j, kc0 + 1, u);
+            q.enter(c, u);
+            if (!bladecol) {
+                if (kc0 == 0) {
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) FKl[v] = gld(c.bnd + c.boffK, (unsigned)((j * NI + i) * 6 + v));
+                } else {
+                    double nk[4];
+                    kface_normal<FT>(c, (kc0 * NJ + j) * NI + i, nk);
+                    q.face(c, L, own, nk, FKl);
+                }
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+            L.FK[v][tid] = FKl[v];
+            L.nsq[v][tid] = 0.0;
+        }
+    }
+    // ---- loads ahead of plane kc0, in the order the loop issues them
+    double pre[5], hU[5];
+    load5(Uin, N, qplane(kc0 + 2) * NIJ + colc, pre);
+    load5(Uin, N, kc0 * NIJ + hsrc.a, hU);
+    double nnk[4];
+    krec_load<double>(fb, krec_off(kc0), nnk);
+    SlotData sd[2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) sd[s] = slot_load(fb, slot_off<RS>(c, fs[s], kc0));
+    double u0v[5], dtv, volv;
+    {
+        const int nn = kc0 * NIJ + colc;
+        load5(st.U0, N, nn, u0v);
+        dtv = gld(dtp, (c.dt_mode == 1) ? 0u : (unsigned)nn);
+        volv = gld(c.vol, (unsigned)nn);
+    }
+
+#pragma unroll 1
+    for (int k = kc0; k < kc1; ++k) {
+        const int n = (k * NJ + j) * NI + i;
+        const bool parked = c.periodic && (k == 0 || k == NK - 1);
+        // ---- column queue (pre holds plane k+2) and the own node of the staged plane
+        if (act) {
+            q.own_to_a(L, own);
+            q.c_to_own(L, own);
+            q.d_to_c(c);
+            if (k + 2 <= NK) {
+                if (k + 2 == NK) {
+                    if (c.periodic) rotx(pre, c.per_c, c.per_s);
+                    else load_k(c, Uin, i, j, NK, pre);
+                }
+                q.enter(c, pre);
+            }
+        } else if (col) {
+            commit_j<TL, FL>(c, L, own, false, pre);
+        }
+        if (hown) commit_j<TL, FL>(c, L, hpos, hsrc.kind == 1, hU);
+        // ---- loads for the next plane: the column's plane k+3, the halo slot's plane k+1
+        load5(Uin, N, qplane(k + 3) * NIJ + colc, pre);
+        load5(Uin, N, min(k + 1, NK - 1) * NIJ + hsrc.a, hU);
+        __syncthreads();
+        if (ghosts) {
+            ghost_from_lds<TL, FL>(c, L, tid, i0, j0);
+            if (tid + NT < NP) ghost_from_lds<TL, FL>(c, L, tid + NT, i0, j0);
+            __syncthreads();
+        }
+        // ---- every I and J face of the tile once (face slots of the thread); the few
+        // threads with a third face (the last wavefront) read its data during the first
+        SlotData s2d;
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+            const FaceSlot f{per_plane(fs[s].b0), per_plane(fs[s].info)};
+            if (s == 1 && fs[2].valid()) s2d = slot_load(fb, slot_off<RS>(c, FaceSlot{per_plane(fs[2].b0), per_plane(fs[2].info)}, k));
+            if (!f.valid()) continue;
+            const SlotData& d = (s == 2) ? s2d : sd[s];
+            double F[5];
+            if (f.bnd()) {
+                slot_flux(d, F);
+            } else {
+                double nn[4];
+                slot_record<double>(d, nn);
+                lds_face_j<TL, FL>(c, L, f.l(), f.isI() ? 1 : HI, nn[0], nn[1], nn[2], nn[3], F);
+            }
+            double* out = f.isI() ? L.FI : L.FJ;
+            const int vstride = f.isI() ? NFI : NFJ;
+#pragma unroll
+            for (int v = 0; v < 5; ++v) out[v * vstride + f.obase()] = F[v];
+        }
+        // ---- K face on the right of the node (face index k+1) from the column queue
+        double FKr[5] = {0, 0, 0, 0, 0};
+        if (kOn) {
+            if (k + 1 == NK) {
+#pragma unroll
+                for (int v = 0; v < 5; ++v)
+                    FKr[v] = gld(c.bnd + c.boffK, (unsigned)((NIJ + j * NI + i) * 6 + v));
+            } else {
+                q.face(c, L, own, nnk, FKr);
+            }
+        }
+        // ---- the blade-row sources of the node (geometric, Gong, body force), read here
+        // so that they arrive during the barrier; then the loads for the next plane: K
+        // record and face slots
+        const bool slab = act && srcOn && i >= c.bi0 && i < c.bi1;
+        double S[15];
+        if (slab) {
+            const int sl = (k * NJ + j) * (c.bi1 - c.bi0) + (i - c.bi0);
+#pragma unroll
+            for (int t = 0; t < 15; ++t) S[t] = gld(c.srcbuf + (long)t * c.ns, (unsigned)sl);
+        }
+        {
+            const int kn = min(k + 1, NK - 1);
+            krec_load<double>(fb, krec_off(k + 1), nnk);
+#pragma unroll
+            for (int s = 0; s < 2; ++s) sd[s] = slot_load(fb, slot_off<RS>(c, FaceSlot{per_plane(fs[s].b0), per_plane(fs[s].info)}, kn));
+        }
+        __syncthreads();
+        // ---- residual assembly, sources, RK update
+        if (act) {
+            double R[5];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+                double r = 0.0;
+                const double fl = L.FI[fiI<TL>(v, tj, ti)], fr = L.FI[fiI<TL>(v, tj, ti + 1)];
+                r = (i == 0) ? r + fl : r - fl;
+                r = r + fr;
+                const double gl = L.FJ[fiJ<TL>(v, tj, ti)], gr = L.FJ[fiJ<TL>(v, tj + 1, ti)];
+                r = (j == 0) ? r + gl : r - gl;
+                r = r + gr;
+                if (!bladecol) {
+                    const double fkl = L.FK[v][tid];
+                    r = (k == 0) ? r + fkl : r - fkl;
+                    r = r + FKr[v];
+                }
+                R[v] = r;
+            }
+            if (c.visc) viscous_add(c, i, j, k, n, R);
+            if (slab) {
+                // geometric, Gong, body force, subtracted in that order
+#pragma unroll
+                for (int t = 0; t < 3; ++t)
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) R[v] -= S[5 * t + v];
+            }
+            // no-slip walls after the periodic closure (solver.cpp:1828-1831): here for
+            // the nodes outside the periodic planes, in k_periodic for theirs
+            if (c.visc && !parked && c.nsw[n]) R[1] = R[2] = R[3] = R[0] * 0.0;
+            if (parked) {
+                double* rp = c.rper + (k == 0 ? 0 : 5) * NIJ;
+#pragma unroll
+                for (int v = 0; v < 5; ++v) gst(rp + (long)v * NIJ, (unsigned)(j * NI + i), R[v]);
+            } else if (st.rmode) {
+#pragma unroll
+                for (int v = 0; v < 5; ++v) gst(st.Uout + (long)v * N, (unsigned)n, R[v]);
+            } else {
+                const double factor = st.alpha * dtv / volv;
+#pragma unroll
+                for (int v = 0; v < 5; ++v) gst(st.Uout + (long)v * N, (unsigned)n, u0v[v] - R[v] * factor);
+                if (st.last)
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) L.nsq[v][tid] += R[v] * R[v];
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 5; ++v) L.FK[v][tid] = FKr[v];
+        // ---- loads for the next plane: U^(0), dt and V
+        {
+            const int nn = min(k + 1, NK - 1) * NIJ + colc;
+            load5(st.U0, N, nn, u0v);
+            dtv = gld(dtp, (c.dt_mode == 1) ? 0u : (unsigned)nn);
+            volv = gld(c.vol, (unsigned)nn);
+        }
+    }
+    // ---- per-block norm partial sums: a deterministic tree over the accumulators
+    if (st.last) {
+        __syncthreads();
+        const long b = ((long)w.z * c.sgy + w.y) * c.sgx + w.x;
+        for (int s = NT / 2; s > 0; s >>= 1) {
+            if (tid < s)
+#pragma unroll
+                for (int v = 0; v < 5; ++v) L.nsq[v][tid] += L.nsq[v][tid + s];
+            __syncthreads();
+        }
+        if (tid < 5) gst(c.partial, (unsigned)(b * 5 + tid), L.nsq[tid][0]);
+    }
+}
+
 // Work: one entry per (i, j) tile and k chunk of a case.  The case and work tables are
 // also passed as restrict kernel parameters (read-only for the launch), which lets the
 // compiler keep case fields in scalar registers or reload them freely.
 // Waves per SIMD the register allocation must allow: three for the FP32 JST kernels
-// (their LDS plane, 41.6 KB for 28x9 tiles, fits three blocks per CU), otherwise no bound
-// (the FP64 kernels need about 240 registers: two waves).
-template <int SCH, typename FT, int FL>
-constexpr int kStageWaves = std::is_same<StageST<SCH, FT, FL>, float>::value ? 3 : 1;
+// (their LDS plane, 41.6 KB for 28x9 tiles, fits three blocks per CU), two for the FP64
+// 3-D JST kernel of the ideal gas (its LDS plane allows two blocks per CU), otherwise no
+// bound (the other FP64 kernels need about 240 registers: two waves).
+template <class TL, int SCH, bool THREED, typename FT, int FL>
+constexpr int kStageWaves = std::is_same<StageST<SCH, FT, FL>, float>::value ? 3
+                            : ((SCH == 0 && THREED && FL == 0 && sizeof(FT) == 8 &&
+                                sizeof(PlaneLdsJ<TL>) <= 80 * 1024) ? 2 : 1);
 
 template <class TL, int SCH, bool THREED, typename FT, int FL>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kStageWaves<SCH, FT, FL>)))
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kStageWaves<TL, SCH, THREED, FT, FL>)))
 k_stage(const CaseDev* __restrict__ cases, const Work* __restrict__ work, Launch La) {
     const Work w = work[blockIdx.x];
     if (w.cs < 0) return;   // padding entry of the XCD-interleaved table
@@ -722,7 +1254,10 @@ k_stage(const CaseDev* __restrict__ cases, const Work* __restrict__ work, Launch
     st.alpha = cp->alpha[La.stage];
     st.last = La.last;
     st.rmode = La.rmode;
-    stage_block<TL, SCH, THREED, FT, FL>(cp, w, st);
+    // the FP64 3-D JST instances whose LDS fits two blocks per CU use the static schedule
+    if constexpr (SCH == 0 && THREED && sizeof(FT) == 8 && sizeof(PlaneLdsJ<TL>) <= 80 * 1024)
+        stage_block_jst3d<TL, FL>(cp, w, st);
+    else stage_block<TL, SCH, THREED, FT, FL>(cp, w, st);
 }
 
 // Periodic closure of the k = 0 / k = NK-1 node-mates: residual averaging

@@ -435,9 +435,10 @@ void setup(tbfm_case* cs, const GridInput& grid) {
     c.dirI = upload(inletDirs(0, pI, NJ, NK), own);
     c.dirJ = upload(inletDirs(1, pJ, NI, NK), own);
     c.dirK = upload(inletDirs(2, pK, NI, NJ), own);
-    c.boffJ = (int)(10 * NJ * NK);
-    c.boffK = (int)(10 * NJ * NK + 10 * NI * NK);
-    c.bnd = dalloc<double>(10 * (NJ * NK + NI * NK + NI * NJ), own);
+    // boundary-face flux records (6 doubles each): allocated behind the face records below
+    c.boffJ = (int)(12 * NJ * NK);
+    c.boffK = (int)(12 * NJ * NK + 12 * NI * NK);
+    const long nbrec = 2 * (NJ * NK + NI * NK + NI * NJ);
 
     // ---- radial equilibrium profiles (solver.cpp:296-331)
     c.nprof = nprof;
@@ -535,15 +536,27 @@ void setup(tbfm_case* cs, const GridInput& grid) {
         normals(g.surfJ, NI, NJ + 1, NK, c.noffJ);
         normals(g.surfK, NI, NJ, NK + 1, c.noffK);
         c.fp32 = (cs->precision == Precision::Mixed) ? 1 : 0;
+        // The boundary-face flux records (48 bytes: 5 doubles and a pad) follow the face
+        // records in the same allocation, so that the stage kernel reads a face's record
+        // or its boundary flux from one base address.
+        const size_t nrec = nrm.size();
         if (cs->precision == Precision::Mixed) {
             // the FP32-flux kernels read 16-byte FP32 records (the values they compute
             // with: the FP64 values rounded to float)
-            std::vector<float> n32(nrm.size());
-            for (size_t q = 0; q < nrm.size(); ++q) n32[q] = (float)nrm[q];
+            std::vector<float> n32(nrec + 12 * (size_t)nbrec, 0.0f);
+            for (size_t q = 0; q < nrec; ++q) n32[q] = (float)nrm[q];
             c.nrm32 = upload(n32, own);
+            c.fbase = (const char*)c.nrm32;
+            c.bnd_off = (unsigned)(nrec * sizeof(float));
         } else {
+            nrm.resize(nrec + 6 * (size_t)nbrec, 0.0);
             c.nrm = upload(nrm, own);
+            c.fbase = (const char*)c.nrm;
+            c.bnd_off = (unsigned)(nrec * sizeof(double));
         }
+        if ((size_t)c.bnd_off + 48 * (size_t)nbrec >= (size_t)1 << 32)
+            throw std::runtime_error("grid too large: face records exceed 4 GiB");
+        c.bnd = (double*)(c.fbase + c.bnd_off);
     }
     faceSoA(g.dual, NI + 1, NJ + 1, NK + 1, x, y, z);
     c.dux = upload(x, own); c.duy = upload(y, own); c.duz = upload(z, own);

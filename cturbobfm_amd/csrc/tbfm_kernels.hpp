@@ -3,8 +3,8 @@
 // Device layout (i fastest): node n = (k*NJ + j)*NI + i; conserved state is five SoA
 // planes of N doubles.  Face vectors are SoA per component:
 //   S_I: (k*NJ + j)*(NI+1) + i     S_J: (k*(NJ+1) + j)*NI + i     S_K: (k*NJ + j)*NI + i
-// Boundary-face fluxes (already multiplied by |S|) live in per-side buffers written by
-// the boundary kernel before each stage.
+// Boundary-face fluxes (already multiplied by |S|) live in per-side 48-byte records
+// written by the boundary kernel before each stage, behind the face records.
 //
 // Arithmetic follows the reference operation by operation (the oracle header lists
 // the file:line of each formula); the library is compiled with -ffp-contract=off so
@@ -91,10 +91,15 @@ struct CaseDev {
     // boundary faces: patch id and inlet direction per face; flux buffers
     const int *pI, *pJ, *pK;           // [2][...] patch index
     const double *dirI, *dirJ, *dirK;  // [2][3][...]
-    // boundary-face fluxes * |S|, one allocation: per direction [2 sides][5][faces of a
-    // side] (bszI = NJ*NK, bszJ = NI*NK, bszK = NI*NJ), the I block at 0, J at boffJ, K at boffK
+    // boundary-face fluxes * |S|: per direction [2 sides][faces of a side][6] (5 doubles and
+    // a pad, 48-byte records; bszI = NJ*NK, bszJ = NI*NK, bszK = NI*NJ), the I block at 0,
+    // J at boffJ, K at boffK (doubles).  They live in the face-record allocation, bnd_off
+    // bytes from its start (fbase: nrm or nrm32), so that the stage kernel reads a face's
+    // record or its boundary flux from one base address.
     double* bnd;
     int boffJ, boffK;
+    const char* fbase;
+    unsigned bnd_off;
     PatchDev patch[MAX_PATCH];
     // radial equilibrium profiles
     int nprof;

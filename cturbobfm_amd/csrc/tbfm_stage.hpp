@@ -618,9 +618,9 @@ __device__ inline void boundary_block(const CaseDev& c, const double* U, int bx)
         transparent_flux<FL>(c, gs, Un, Sx, Sy, Sz, F);
     }
     const double sm = sqrt(Sx * Sx + Sy * Sy + Sz * Sz);
-    double* o = out + (long)side * 5 * nsz;
+    double* o = out + ((long)side * nsz + q) * 6;
 #pragma unroll
-    for (int v = 0; v < 5; ++v) o[(long)v * nsz + q] = F[v] * sm;
+    for (int v = 0; v < 5; ++v) o[v] = F[v] * sm;
 }
 
 // Bisection table of an ideal-gas inlet patch (p0, T0): node q of the first INLET_LEVELS
