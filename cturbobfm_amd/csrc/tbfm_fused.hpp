@@ -775,14 +775,26 @@ struct FaceSlot {
     __device__ __forceinline__ int l() const { return (info >> 3) & 1023; }
     __device__ __forceinline__ int obase() const { return info >> 13; }
 };
+// Slot 0 of thread tid is I face tid, slot 1 J face tid (so that both have a compile-time
+// direction), slot 2 one of the remaining I faces (index 256 on) or J faces, taken by the
+// threads from the top down.
 template <class TL, unsigned RS>
 __device__ __forceinline__ FaceSlot face_slot(const CaseDev& c, int s, int tid, int i0, int j0) {
     constexpr int TI = TL::TI, HI = TL::HI, NFI = TL::TJ * (TI + 1), NFJ = (TL::TJ + 1) * TI;
     static_assert(TL::NP <= 1024 && NFI + NFJ <= 2048, "face-slot fields fit their bits");
+    static_assert(NFI >= NT && NFJ >= NT && NFI + NFJ <= 3 * NT, "one I and one J face per thread, a third for some");
     const int NI = c.ni, NJ = c.nj, NK = c.nk;
     FaceSlot d{0u, 0};
-    const int fq = face_of(s, tid);
-    if (fq >= NFI + NFJ) return d;
+    int fq;
+    if (s == 0) {
+        fq = tid;
+    } else if (s == 1) {
+        fq = NFI + tid;
+    } else {
+        const int e = NT - 1 - tid;
+        fq = (e < NFI - NT) ? NT + e : NFI + NT + (e - (NFI - NT));
+        if (e >= NFI + NFJ - 2 * NT) return d;
+    }
     const FaceIdx<TL> f(fq, i0, j0, NI, NJ);
     if (!f.valid) return d;
     const int l = f.isI ? (f.r + 2) * HI + (f.m + 1) : (f.r + 1) * HI + (f.m + 2);
@@ -845,10 +857,13 @@ __device__ __forceinline__ void commit_j(const CaseDev& c, PlaneLdsJ<TL>& L, int
     L.RF[pos] = rf;
 }
 
-// JST face flux*|S| between LDS positions l and r = l + st (lds_face for this layout).
-template <class TL, int FL>
-__device__ __forceinline__ void lds_face_j(const CaseDev& c, const PlaneLdsJ<TL>& L, int l, int st, double nx,
-                                           double ny, double nz, double area, double out[5]) {
+// JST face flux*|S| between LDS positions l and r = l + st (lds_face for this layout; st is
+// the direction's LDS stride, 1 for I faces and the halo row length for J faces: SD when
+// the caller's direction is fixed at compile time, else the argument sdr).
+template <class TL, int FL, int SD>
+__device__ __forceinline__ void lds_face_j(const CaseDev& c, const PlaneLdsJ<TL>& L, int l, double nx,
+                                           double ny, double nz, double area, double out[5], int sdr = 0) {
+    const int st = SD ? SD : sdr;
     const int r = l + st, ll = l - st, rr = r + st;
     const double pll = L.P[ll], pl = L.P[l], pr = L.P[r], prr = L.P[rr];
     const double sl = jst_sensor<double>(pll, pl, pr);
@@ -905,7 +920,8 @@ __device__ __forceinline__ void ghost_from_lds(const CaseDev& c, PlaneLdsJ<TL>& 
 // derives its pressure then and its primitives when it reaches k+1.
 template <int FL>
 struct KColJ {
-    double A[5], Ap;
+    double A[5];
+    double sl;   // pressure sensor of plane k (the right sensor of the previous K face)
     double C[5], Cp, Cw[4], Crf;
     double D[5], Dp, Dw[4], Drf;
     __device__ __forceinline__ void enter(const CaseDev& c, const double u[5]) {
@@ -940,7 +956,6 @@ struct KColJ {
     __device__ __forceinline__ void own_to_a(const PlaneLdsJ<TL>& L, int own) {
 #pragma unroll
         for (int v = 0; v < 5; ++v) A[v] = L.U[v][own];
-        Ap = L.P[own];
     }
     // plane k+1 becomes the own node of the staged plane
     template <class TL>
@@ -952,10 +967,11 @@ struct KColJ {
         L.P[own] = Cp;
         L.RF[own] = Crf;
     }
-    // K face between plane k (own LDS node) and k+1 (KQueue::face with slot 1 from LDS)
+    // K face between plane k (own LDS node) and k+1 (KQueue::face with slot 1 from LDS),
+    // with the left sensor sl0; returns the right sensor (the next plane's left one)
     template <class TL>
-    __device__ __forceinline__ void face(const CaseDev& c, const PlaneLdsJ<TL>& L, int own, const double nn[4],
-                                         double out[5]) const {
+    __device__ __forceinline__ double face(const CaseDev& c, const PlaneLdsJ<TL>& L, int own, double sl0,
+                                           const double nn[4], double out[5]) const {
         double B[5], Bw[5];
 #pragma unroll
         for (int v = 0; v < 5; ++v) B[v] = L.U[v][own];
@@ -963,10 +979,10 @@ struct KColJ {
 #pragma unroll
         for (int v = 0; v < 4; ++v) Bw[v + 1] = L.W[v][own];
         const double Bp = L.P[own], Brf = L.RF[own];
-        const double sl = jst_sensor<double>(Ap, Bp, Cp);
         const double sr = jst_sensor<double>(Bp, Cp, Dp);
         const double Wr[5] = {C[0], Cw[0], Cw[1], Cw[2], Cw[3]};
-        jst_face<double, FL>(A, B, C, D, Bw, Wr, Brf, Crf, sl, sr, nn[0], nn[1], nn[2], nn[3], Gas<FL>(c), out);
+        jst_face<double, FL>(A, B, C, D, Bw, Wr, Brf, Crf, sl0, sr, nn[0], nn[1], nn[2], nn[3], Gas<FL>(c), out);
+        return sr;
     }
 };
 
@@ -1028,11 +1044,12 @@ __device__ __forceinline__ void stage_block_jst3d(const CaseDev* __restrict__ cp
         if (act) {
             double u[5] = {1.0, 1.0, 1.0, 1.0, 1.0};
             if (kc0 - 2 >= -1) load_k(c, Uin, i, j, kc0 - 2, u);
+            double Ap;   // pressure of plane kc0-2 (left sensor of the K face below plane kc0)
             {
                 double w[5], rf;
 #pragma unroll
                 for (int v = 0; v < 5; ++v) q.A[v] = u[v];
-                derive<double, FL>(q.A, w, q.Ap, rf, Gas<FL>(c));
+                derive<double, FL>(q.A, w, Ap, rf, Gas<FL>(c));
             }
 #pragma unroll
             for (int v = 0; v < 5; ++v) u[v] = 1.0;
@@ -1045,6 +1062,8 @@ __device__ __forceinline__ void stage_block_jst3d(const CaseDev* __restrict__ cp
             for (int v = 0; v < 5; ++v) u[v] = 1.0;
             if (kc0 + 1 <= NK) load_k(c, Uin, i, j, kc0 + 1, u);
             q.enter(c, u);
+            // left sensor of the first K face of the loop: plane kc0 (planes kc0-1 .. kc0+1)
+            q.sl = jst_sensor<double>(L.P[own], q.Cp, q.Dp);
             if (!bladecol) {
                 if (kc0 == 0) {
 #pragma unroll
@@ -1052,7 +1071,7 @@ __device__ __forceinline__ void stage_block_jst3d(const CaseDev* __restrict__ cp
                 } else {
                     double nk[4];
                     kface_normal<FT>(c, (kc0 * NJ + j) * NI + i, nk);
-                    q.face(c, L, own, nk, FKl);
+                    q.face(c, L, own, jst_sensor<double>(Ap, L.P[own], q.Cp), nk, FKl);
                 }
             }
         }
@@ -1123,10 +1142,13 @@ __device__ __forceinline__ void stage_block_jst3d(const CaseDev* __restrict__ cp
             } else {
                 double nn[4];
                 slot_record<double>(d, nn);
-                lds_face_j<TL, FL>(c, L, f.l(), f.isI() ? 1 : HI, nn[0], nn[1], nn[2], nn[3], F);
+                if (s == 0) lds_face_j<TL, FL, 1>(c, L, f.l(), nn[0], nn[1], nn[2], nn[3], F);
+                else if (s == 1) lds_face_j<TL, FL, HI>(c, L, f.l(), nn[0], nn[1], nn[2], nn[3], F);
+                else lds_face_j<TL, FL, 0>(c, L, f.l(), nn[0], nn[1], nn[2], nn[3], F, f.isI() ? 1 : HI);
             }
-            double* out = f.isI() ? L.FI : L.FJ;
-            const int vstride = f.isI() ? NFI : NFJ;
+            const bool fI = (s == 0) || (s == 2 && f.isI());
+            double* out = fI ? L.FI : L.FJ;
+            const int vstride = fI ? NFI : NFJ;
 #pragma unroll
             for (int v = 0; v < 5; ++v) out[v * vstride + f.obase()] = F[v];
         }
@@ -1138,7 +1160,7 @@ __device__ __forceinline__ void stage_block_jst3d(const CaseDev* __restrict__ cp
                 for (int v = 0; v < 5; ++v)
                     FKr[v] = gld(c.bnd + c.boffK, (unsigned)((NIJ + j * NI + i) * 6 + v));
             } else {
-                q.face(c, L, own, nnk, FKr);
+                q.sl = q.face(c, L, own, q.sl, nnk, FKr);
             }
         }
         // ---- the blade-row sources of the node (geometric, Gong, body force), read here
