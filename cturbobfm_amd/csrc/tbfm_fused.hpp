@@ -647,13 +647,16 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
             }
             if (c.visc) viscous_add(c, i, j, k, n, R);
             if (srcOn && i >= c.bi0 && i < c.bi1) {
-                // geometric, Gong, body force, subtracted in that order
+                // geometric, Gong, body force, subtracted in that order (the geometric
+                // source's components 0, 1 and 4 are +0: subtracting them changes nothing)
                 const int sl = (k * NJ + j) * (c.bi1 - c.bi0) + (i - c.bi0);
                 const int ns = c.ns;
+                R[2] -= gld(c.srcbuf, (unsigned)sl);
+                R[3] -= gld(c.srcbuf + ns, (unsigned)sl);
 #pragma unroll
-                for (int t = 0; t < 3; ++t)
+                for (int t = 0; t < 2; ++t)
 #pragma unroll
-                    for (int v = 0; v < 5; ++v) R[v] -= gld(c.srcbuf + (long)(5 * t + v) * ns, (unsigned)sl);
+                    for (int v = 0; v < 5; ++v) R[v] -= gld(c.srcbuf + (long)(2 + 5 * t + v) * ns, (unsigned)sl);
             }
             // no-slip walls after the periodic closure (solver.cpp:1828-1831): here for
             // the nodes outside the periodic planes, in k_periodic for theirs
@@ -1118,9 +1121,12 @@ __device__ __forceinline__ void stage_block_jst3d(const CaseDev* __restrict__ cp
             commit_j<TL, FL>(c, L, own, false, pre);
         }
         if (hown) commit_j<TL, FL>(c, L, hpos, hsrc.kind == 1, hU);
-        // ---- loads for the next plane: the column's plane k+3, the halo slot's plane k+1
-        load5(Uin, N, qplane(k + 3) * NIJ + colc, pre);
-        load5(Uin, N, min(k + 1, NK - 1) * NIJ + hsrc.a, hU);
+        // ---- loads for the next plane: the column's plane k+3, the halo slot's plane k+1.
+        // The last plane of the piece has no next one: its loads read what this plane read
+        // (cache-resident; the same loads in the same order keep every wait exact)
+        const int kq = (k + 1 < kc1) ? k + 1 : k;
+        load5(Uin, N, qplane(kq + 2) * NIJ + colc, pre);
+        load5(Uin, N, min(kq, NK - 1) * NIJ + hsrc.a, hU);
         __syncthreads();
         if (ghosts) {
             ghost_from_lds<TL, FL>(c, L, tid, i0, j0);
@@ -1167,15 +1173,15 @@ __device__ __forceinline__ void stage_block_jst3d(const CaseDev* __restrict__ cp
         // so that they arrive during the barrier; then the loads for the next plane: K
         // record and face slots
         const bool slab = act && srcOn && i >= c.bi0 && i < c.bi1;
-        double S[15];
+        double S[12];
         if (slab) {
             const int sl = (k * NJ + j) * (c.bi1 - c.bi0) + (i - c.bi0);
 #pragma unroll
-            for (int t = 0; t < 15; ++t) S[t] = gld(c.srcbuf + (long)t * c.ns, (unsigned)sl);
+            for (int t = 0; t < 12; ++t) S[t] = gld(c.srcbuf + (long)t * c.ns, (unsigned)sl);
         }
         {
-            const int kn = min(k + 1, NK - 1);
-            krec_load<double>(fb, krec_off(k + 1), nnk);
+            const int kn = min(kq, NK - 1);
+            krec_load<double>(fb, krec_off(kq), nnk);
 #pragma unroll
             for (int s = 0; s < 2; ++s) sd[s] = slot_load(fb, slot_off<RS>(c, FaceSlot{per_plane(fs[s].b0), per_plane(fs[s].info)}, kn));
         }
@@ -1201,11 +1207,14 @@ __device__ __forceinline__ void stage_block_jst3d(const CaseDev* __restrict__ cp
             }
             if (c.visc) viscous_add(c, i, j, k, n, R);
             if (slab) {
-                // geometric, Gong, body force, subtracted in that order
+                // geometric (y, z components; the others are +0), Gong, body force,
+                // subtracted in that order
+                R[2] -= S[0];
+                R[3] -= S[1];
 #pragma unroll
-                for (int t = 0; t < 3; ++t)
+                for (int t = 0; t < 2; ++t)
 #pragma unroll
-                    for (int v = 0; v < 5; ++v) R[v] -= S[5 * t + v];
+                    for (int v = 0; v < 5; ++v) R[v] -= S[2 + 5 * t + v];
             }
             // no-slip walls after the periodic closure (solver.cpp:1828-1831): here for
             // the nodes outside the periodic planes, in k_periodic for theirs
@@ -1230,7 +1239,7 @@ __device__ __forceinline__ void stage_block_jst3d(const CaseDev* __restrict__ cp
         for (int v = 0; v < 5; ++v) L.FK[v][tid] = FKr[v];
         // ---- loads for the next plane: U^(0), dt and V
         {
-            const int nn = min(k + 1, NK - 1) * NIJ + colc;
+            const int nn = min(kq, NK - 1) * NIJ + colc;
             load5(st.U0, N, nn, u0v);
             dtv = gld(dtp, (c.dt_mode == 1) ? 0u : (unsigned)nn);
             volv = gld(c.vol, (unsigned)nn);

@@ -600,7 +600,7 @@ void setup(tbfm_case* cs, const GridInput& grid) {
     // exist for axisymmetric cases and 3-D blade rows, body forces with BFM_ACTIVE)
     const bool srcOn = (cs->topo == Topology::Axisymmetric) || bfm;
     c.ns = srcOn ? (int)((bi1 - bi0) * NJ * NK) : 0;
-    c.srcbuf = c.ns > 0 ? dalloc<double>(15L * c.ns, own) : nullptr;
+    c.srcbuf = c.ns > 0 ? dalloc<double>(12L * c.ns, own) : nullptr;
     const long SW = bi1 - bi0, NS = std::max<long>(SW * NJ * NK, 1);
     std::vector<unsigned char> sflag(NS, 0);
     std::vector<double> s_cs(NS, 1.0), s_sn(NS, 0.0), s_r(NS, 0.0), s_om(NS, 0.0), s_b(NS, 1.0), s_gbx(NS, 0.0),

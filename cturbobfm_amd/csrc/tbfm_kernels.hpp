@@ -120,7 +120,8 @@ struct CaseDev {
     const double* prof_cs;             // [MAX_PROF][NJ] cos(theta) at (i, j, 0)
     const double* prof_sn;
     // body force slab: nodes with i in [bi0, bi1); ns = slab nodes; srcbuf holds the
-    // geometric, Gong and body-force source vectors of the current stage, [3][5][ns]
+    // source terms of the current stage, [12][ns]: the geometric source's y and z
+    // components (its other three are +0), the Gong source, the body force
     int bi0, bi1, ns;
     double* srcbuf;
     const unsigned char* blade2d;      // (j*NI + i): bladePresent(i,j,0) > 0

@@ -360,23 +360,56 @@ __device__ inline void inlet_flux(const CaseDev& c, const Gas<FL>& gs, const dou
         const double ex = (g - 1.0) / g;
         // tree node of the midpoint while inside the patch table, and that of pB
         int qn = 0, qB = -1;
-        for (int it = 0; it < 30; ++it) {
-            double pMid = 0.5 * (pLow + pHigh);
-            const bool inTab = tab && it < INLET_LEVELS;
-            double VMid;
-            if (inTab) {
-                VMid = gld(tab + INLET_NODES, (unsigned)qn);
-            } else {
-                double TMid = Tt * pow(pMid / pt, ex);
-                double hMid = cp * TMid;
-                VMid = sqrt(2.0 * dmaxd(0.0, htIn - hMid));
+        int it = 0;
+        bool done = false;
+        if (tab) {
+            // the tabulated levels, four per round: the speeds of the node and of its
+            // descendants 1-3 levels down (contiguous runs of 1, 2, 4, 8 in heap order)
+            // are requested together, then the four bisection steps run from registers
+            static_assert(INLET_LEVELS % 4 == 0, "whole rounds of four levels");
+            const double* Vt = tab + INLET_NODES;
+            for (int round = 0; round < INLET_LEVELS / 4 && !done; ++round) {
+                double v0 = gld(Vt, (unsigned)qn);
+                double v1[2], v2[4], v3[8];
+#pragma unroll
+                for (int q = 0; q < 2; ++q) v1[q] = gld(Vt, (unsigned)(2 * qn + 1 + q));
+#pragma unroll
+                for (int q = 0; q < 4; ++q) v2[q] = gld(Vt, (unsigned)(4 * qn + 3 + q));
+#pragma unroll
+                for (int q = 0; q < 8; ++q) v3[q] = gld(Vt, (unsigned)(8 * qn + 7 + q));
+                int pos = 0;   // position of the current node within its level of the subtree
+#pragma unroll
+                for (int d = 0; d < 4; ++d) {
+                    double VMid = v0;
+                    if (d == 1) VMid = pos ? v1[1] : v1[0];
+                    if (d == 2) VMid = (pos & 2) ? ((pos & 1) ? v2[3] : v2[2]) : ((pos & 1) ? v2[1] : v2[0]);
+                    if (d == 3) {
+                        const double a = (pos & 2) ? ((pos & 1) ? v3[3] : v3[2]) : ((pos & 1) ? v3[1] : v3[0]);
+                        const double b = (pos & 2) ? ((pos & 1) ? v3[7] : v3[6]) : ((pos & 1) ? v3[5] : v3[4]);
+                        VMid = (pos & 4) ? b : a;
+                    }
+                    const int qd = (qn << d) + ((1 << d) - 1) + pos;   // heap index of the node
+                    double pMid = 0.5 * (pLow + pHigh);
+                    double res = pMid + Z * Cg * VMid - Rout;
+                    ++it;
+                    pB = pMid;
+                    qB = qd;
+                    if (fabs(res) < 1e-4 * pt) { done = true; break; }
+                    if (res < 0.0) { pLow = pMid; pos = 2 * pos; } else { pHigh = pMid; pos = 2 * pos + 1; }
+                }
+                if (!done) qn = (qn << 4) + 15 + pos;
             }
+        }
+        for (; it < 30 && !done; ++it) {
+            double pMid = 0.5 * (pLow + pHigh);
+            double TMid = Tt * pow(pMid / pt, ex);
+            double hMid = cp * TMid;
+            double VMid = sqrt(2.0 * dmaxd(0.0, htIn - hMid));
             double res = pMid + Z * Cg * VMid - Rout;
-            if (fabs(res) < 1e-4 * pt) { pB = pMid; qB = inTab ? qn : -1; break; }
             pB = pMid;
-            qB = inTab ? qn : -1;
-            if (res < 0.0) { pLow = pMid; qn = 2 * qn + 1; } else { pHigh = pMid; qn = 2 * qn + 2; }
-            if (!inTab) qn = 0;
+            qB = -1;
+            if (fabs(res) < 1e-4 * pt) break;
+            if (res < 0.0) pLow = pMid; else pHigh = pMid;
         }
         double TB, VB;
         if (qB >= 0) {

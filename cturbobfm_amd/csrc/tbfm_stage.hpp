@@ -205,12 +205,15 @@ __device__ inline void sources_block(const CaseDev& c, const Launch& La, int bx)
     for (int v = 1; v < 5; ++v) W[v] = U[v] / U[0];
     double g[5], go[5], bf[5];
     source_terms<THREED, FL>(c, gs, Uin, La.last, i, j, k, n, s, W, g, go, bf);
+    // the geometric source's density, x-momentum and energy components are +0 (its
+    // cylindrical source has no axial part): only its y and z components are kept
     const int ns = c.ns;
+    c.srcbuf[s] = g[2];
+    c.srcbuf[ns + s] = g[3];
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
-        c.srcbuf[v * ns + s] = g[v];
-        c.srcbuf[(5 + v) * ns + s] = go[v];
-        c.srcbuf[(10 + v) * ns + s] = bf[v];
+        c.srcbuf[(2 + v) * ns + s] = go[v];
+        c.srcbuf[(7 + v) * ns + s] = bf[v];
     }
 }
 
