@@ -6,17 +6,37 @@
  * -ffp-contract=off on both sides, results agree bit for bit where no libm
  * transcendental is involved.
  *
- * Pinning (what checks each part; see tests/):
- *   geometry (dual grid, faces, centres, volumes, GG gradient)  vs reference build + closed forms
- *   fluid ideal functions, primitive conversion, physical flux  vs reference build
- *   JST / Roe / MUSCL face fluxes                               vs reference build + consistency
- *   boundary fluxes (inlet, outlet, radial eq., wall)           vs reference build
- *   BFM sources (blockage, Hall, lift-drag) + side outputs      vs reference build
- *   radial-equilibrium integration                              vs reference build
- *   face loop / residual assembly / periodic closure / sources  vs reference regression
- *     solutions (2-3 %), conservation (telescoping), free-stream preservation
+ * Pinning (what checks each part; the tests are under tests/):
+ *   geometry (dual grid, faces, volumes, blockage gradient)     reference build (golden geometry_*),
+ *                                                                dual-volume closed form
+ *   fluid relations, ideal and table fluid, primitive state     reference build (fluid, lut)
+ *   JST / Roe / MUSCL (4 limiters) face fluxes                  reference build (flux_*), Roe
+ *                                                                consistency and rotation
+ *   boundary fluxes: inlet, outlet + ramp, radial equilibrium,  reference build (boundary,
+ *     throttle / mass-flow outlets, inviscid wall               outlets, lut)
+ *   transparent end                                             Sod regression solution (0.002 %)
+ *   stagnation flux of inlet faces on no-slip walls             isentropic closed form
+ *   radial-equilibrium integration                              reference build + free vortex
+ *   outlet hub rules (throttle, mass flow), Greitzer plenum     closed forms / known answers;
+ *                                                                plenum steps vs reference build
+ *   body force: blockage, Hall, lift-drag, Hall-Thollet         reference build per node and over
+ *     (off-design, stall), side outputs, time dependence        three serial sweeps (bfm_*)
+ *   Gong and 3-D / axisymmetric geometric sources               closed forms (quadratic fields,
+ *                                                                wrap planes, solid-body swirl)
+ *   viscous gradients, stress tensor, viscosity                 reference build; Poiseuille /
+ *   viscous fluxes, no-slip walls, viscous time step            Couette closed forms
  *   time step                                                   closed form on Cartesian grids
- *   RK stage update, norms, convergence test                    closed forms + regression runs
+ *   face loop, residual assembly, periodic closure              free-stream preservation, closed-
+ *                                                                box conservation, node-mate and
+ *                                                                sector-rotation properties,
+ *                                                                reference regression solutions
+ *   RK stage update, norms, convergence test                    recursion closed forms, semantics,
+ *                                                                regression runs
+ *   turbomachinery figures (mass flow, PRtt, TRtt, ETAtt)       closed forms on uniform planes
+ * The reference regression cases match reference.csv within 0.002-0.6 % relative L2,
+ * except GER4_FullAnnulus at 1.3-2.7 %, where both codes converge to a non-axisymmetric
+ * seam state from axisymmetric inputs; off the hub row and the seam planes it agrees
+ * within 0.5-1.1 %.
  */
 #define _GNU_SOURCE
 #include "tbfm_oracle.h"
