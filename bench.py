@@ -8,9 +8,9 @@ profile, and four fused residual + RK stages over all nodes.
 Prints one JSON line:
   value       cell-stage updates/s = nodes * stages * steps / wall time of the timed steps
   roofline    the fused stage kernel: algorithmic bytes per launch (208 B per node for 3-D
-              FP64, +96 B per body-force node) / its average launch time from HIP events on
-              the case stream, against 8 TB/s; traffic from the committed rocprofv3
-              FETCH_SIZE/WRITE_SIZE summary when present
+              FP64, 172 B with the FP32 face records of --precision mixed) / its average
+              launch time from HIP events on the case stream, against 8 TB/s; traffic
+              from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary when present
   cpu_baseline the CPU restatement (oracle/liboracle.so, OpenMP) timed on a bounded sample
               of the same workload on this host
 """
@@ -73,7 +73,7 @@ def bench_batch(args):
     drawn uniformly from 90-110 kPa, seeded), FP32 flux / FP64 state, all advanced in one
     batch (one launch per kernel, captured graph per iteration).  Compared with the plain
     path: each case alone in FP64, one after another, for the same iterations."""
-    from cturbobfm_amd import Batch, Solver, cases
+    from cturbobfm_amd import Batch, Solver, SolverError, cases
     rng = np.random.default_rng(args.seed)
     sizes = [(97, 25, 65), (129, 33, 97), (193, 49, 129), (257, 65, 193)]
     per = args.batch_cases // len(sizes)
@@ -93,7 +93,17 @@ def bench_batch(args):
     t0 = time.perf_counter()
     b.advance(args.steps)
     wall = time.perf_counter() - t0
-    stopped = sum(1 for s in members if s.status()[2])
+    # a member that stopped (converged or diverged) skips its work from then on, so the
+    # rate counts every member's nodes only when none stopped in the run
+    stopped = diverged = 0
+    for s in members:
+        try:
+            stopped += bool(s.status()[2])
+        except SolverError:
+            diverged += 1
+    if stopped or diverged:
+        print("warning: %d members stopped and %d diverged; the rate below overstates the work done"
+              % (stopped, diverged), file=sys.stderr, flush=True)
     tiling = sorted(set(s.tuning().split("->")[-1].strip() for s in members))
     b.close()
     for s in members:
@@ -118,7 +128,8 @@ def bench_batch(args):
         "config": {"workload": "throttle sweep: %d rotor operating points, %d per size on %s, outlet 90-110 kPa "
                                "(seed %d), JST, RK41A, one batch" % (len(pts), per,
                                                                     ", ".join("%dx%dx%d" % z for z in sizes), args.seed),
-                   "nodes": nodes, "cases": len(pts), "stopped_cases": stopped, "stage_tiling": tiling,
+                   "nodes": nodes, "cases": len(pts), "stopped_cases": stopped, "diverged_cases": diverged,
+                   "stage_tiling": tiling,
                    "setup_s": t_setup},
         "plain_fp64_sequential_s": seq, "batched_s": wall, "speedup_vs_plain": seq / wall,
     }
@@ -299,7 +310,10 @@ def main():
     avg_ms = stage_ms / max(launches, 1)
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
     avg_pre_ms = pre_ms / max(launches, 1)
-    traffic = traffic_from_profiles("k_stage")
+    # the committed counter summary covers the default workload (FP64 JST rotor) only
+    default_case = (args.precision == "fp64" and args.size == "193x49x129" and args.scheme == "JST"
+                    and not args.viscous)
+    traffic = traffic_from_profiles("k_stage") if default_case else None
     out = {
         "metric": "cell-stage updates/s + HBM roofline %; p50/p99 time-to-converge vs plain FP64",
         "value": value,

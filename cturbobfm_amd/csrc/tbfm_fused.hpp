@@ -718,7 +718,11 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
 // plane are extrapolated from the staged nodes in LDS (blocks at a domain end only, one
 // more barrier) instead of being loaded.  Loads whose results are used at once (the
 // virtual plane NK of a non-periodic case, the end-plane K boundary fluxes, blade-row
-// sources, viscous terms) stay where they are used.
+// sources, viscous terms) stay where they are used.  The FP32 instance (ideal gas) stages
+// the plane, the column queue and the face arithmetic in float like stage_block's FP32
+// path; its ghost positions are extrapolated from the staged float values, where
+// stage_block extrapolates the FP64 state and then rounds (results differ in the last
+// float bits next to the i/j domain ends).
 // A per-plane copy of a value that the compiler may not treat as loop-invariant: what is
 // derived from it (LDS and record offsets) is recomputed each plane instead of being
 // hoisted out of the k loop into registers that the plane's arithmetic needs.
@@ -824,34 +828,43 @@ __device__ __forceinline__ unsigned slot_off(const CaseDev& c, const FaceSlot& d
 // LDS of the 3-D JST kernel: the staged plane (conserved state, velocity and specific
 // total energy, pressure, spectral factor; the density is U[0]), the I/J face fluxes of
 // the tile, the K flux of the face below each column's next node and the column's
-// residual-norm accumulators (which also hold the per-block norm tree).
-template <class TL>
-struct PlaneLdsJ {
-    double U[5][TL::NP];
-    double W[4][TL::NP];   // W[v - 1]: primitive v = 1..4
-    double P[TL::NP];
-    double RF[TL::NP];
+// residual-norm accumulators (which also hold the per-block norm tree).  The staged plane
+// is in the storage type ST (float for the FP32 JST kernel, which has registers to spare
+// for its norm accumulators and runs its norm tree in the staged plane).
+struct ColumnNorms {
+    double nsq[5][NT];
+};
+struct NoColumnNorms {};
+template <class TL, typename ST = double>
+struct PlaneLdsJ : std::conditional<sizeof(ST) == 8, ColumnNorms, NoColumnNorms>::type {
+    ST U[5][TL::NP];
+    ST W[4][TL::NP];   // W[v - 1]: primitive v = 1..4
+    ST P[TL::NP];
+    ST RF[TL::NP];
     double FI[5 * TL::TJ * (TL::TI + 1)];
     double FJ[5 * (TL::TJ + 1) * TL::TI];
     double FK[5][NT];
-    double nsq[5][NT];
+    static_assert(sizeof(ST) == 8 || 11 * TL::NP * sizeof(ST) >= 5 * NT * sizeof(double),
+                  "the norm tree fits in the staged plane");
+    __device__ __forceinline__ double (*red())[NT] { return reinterpret_cast<double (*)[NT]>(&U[0][0]); }
 };
 
 // Stores what the faces read at LDS position pos (state, primitives, pressure, spectral
 // factor); unused positions get harmless constants (commit_position for this layout).
-template <class TL, int FL>
-__device__ __forceinline__ void commit_j(const CaseDev& c, PlaneLdsJ<TL>& L, int pos, bool ok, const double Un[5]) {
+// The state is given in the storage type (converted from FP64 by the caller).
+template <class TL, int FL, typename ST>
+__device__ __forceinline__ void commit_j(const CaseDev& c, PlaneLdsJ<TL, ST>& L, int pos, bool ok, const ST Un[5]) {
     if (!ok) {
 #pragma unroll
-        for (int v = 0; v < 5; ++v) L.U[v][pos] = 1.0;
+        for (int v = 0; v < 5; ++v) L.U[v][pos] = ST(1.0);
 #pragma unroll
-        for (int v = 0; v < 4; ++v) L.W[v][pos] = 1.0;
-        L.P[pos] = 1.0;
-        L.RF[pos] = 0.0;
+        for (int v = 0; v < 4; ++v) L.W[v][pos] = ST(1.0);
+        L.P[pos] = ST(1.0);
+        L.RF[pos] = ST(0.0);
         return;
     }
-    double W[5], p, rf;
-    derive<double, FL>(Un, W, p, rf, Gas<FL>(c));
+    ST W[5], p, rf;
+    derive<ST, FL>(Un, W, p, rf, Gas<FL>(c));
 #pragma unroll
     for (int v = 0; v < 5; ++v) L.U[v][pos] = Un[v];
 #pragma unroll
@@ -862,41 +875,43 @@ __device__ __forceinline__ void commit_j(const CaseDev& c, PlaneLdsJ<TL>& L, int
 
 // JST face flux*|S| between LDS positions l and r = l + st (lds_face for this layout; st is
 // the direction's LDS stride, 1 for I faces and the halo row length for J faces: SD when
-// the caller's direction is fixed at compile time, else the argument sdr).
-template <class TL, int FL, int SD>
-__device__ __forceinline__ void lds_face_j(const CaseDev& c, const PlaneLdsJ<TL>& L, int l, double nx,
-                                           double ny, double nz, double area, double out[5], int sdr = 0) {
+// the caller's direction is fixed at compile time, else the argument sdr).  The arithmetic
+// runs in the storage type ST (the flux type of the kernel), the result is FP64.
+template <class TL, int FL, int SD, typename ST>
+__device__ __forceinline__ void lds_face_j(const CaseDev& c, const PlaneLdsJ<TL, ST>& L, int l, ST nx, ST ny,
+                                           ST nz, ST area, double out[5], int sdr = 0) {
     const int st = SD ? SD : sdr;
     const int r = l + st, ll = l - st, rr = r + st;
-    const double pll = L.P[ll], pl = L.P[l], pr = L.P[r], prr = L.P[rr];
-    const double sl = jst_sensor<double>(pll, pl, pr);
-    const double sr = jst_sensor<double>(pl, pr, prr);
-    double Wavg[5];
-    Wavg[0] = (L.U[0][l] + L.U[0][r]) * 0.5;
+    const ST pll = L.P[ll], pl = L.P[l], pr = L.P[r], prr = L.P[rr];
+    const ST sl = jst_sensor<ST>(pll, pl, pr);
+    const ST sr = jst_sensor<ST>(pl, pr, prr);
+    ST Wavg[5];
+    Wavg[0] = (L.U[0][l] + L.U[0][r]) * ST(0.5);
 #pragma unroll
-    for (int v = 1; v < 5; ++v) Wavg[v] = (L.W[v - 1][l] + L.W[v - 1][r]) * 0.5;
-    const double rr_ = smax(L.RF[l], L.RF[r]);
-    const double s = smax(sl, sr);
-    const double psi2 = 0.5 * s * rr_;
-    const double t4 = 0.02 * rr_ - 2.0 * psi2;
-    const double psi4 = (0.0 < t4) ? t4 : 0.0;
-    double F[5];
-    phys_flux_n<double, FL>(Wavg, nx, ny, nz, Gas<FL>(c), F);
+    for (int v = 1; v < 5; ++v) Wavg[v] = (L.W[v - 1][l] + L.W[v - 1][r]) * ST(0.5);
+    const ST rr_ = smax(L.RF[l], L.RF[r]);
+    const ST s = smax(sl, sr);
+    const ST psi2 = ST(0.5) * s * rr_;
+    const ST t4 = ST(0.02) * rr_ - ST(2.0) * psi2;
+    const ST psi4 = (ST(0.0) < t4) ? t4 : ST(0.0);
+    ST F[5];
+    phys_flux_n<ST, FL>(Wavg, nx, ny, nz, Gas<FL>(c), F);
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
-        const double ul = L.U[v][l], ur = L.U[v][r];
-        const double d1 = ur - ul;
-        const double d3 = ((L.U[v][rr] - ur) - d1 * 2) + (ul - L.U[v][ll]);
-        const double dis = d1 * psi2 - d3 * psi4;
-        out[v] = (F[v] - dis) * area;
+        const ST ul = L.U[v][l], ur = L.U[v][r];
+        const ST d1 = ur - ul;
+        const ST d3 = ((L.U[v][rr] - ur) - d1 * 2) + (ul - L.U[v][ll]);
+        const ST dis = d1 * psi2 - d3 * psi4;
+        out[v] = double((F[v] - dis) * area);
     }
 }
 
 // A ghost position of the staged plane (one step outside the domain in i or j), built
 // from the staged boundary node and its inner neighbour exactly as fetch_src builds it
-// from global memory (solver.cpp:1980-1993); other positions are left alone.
-template <class TL, int FL>
-__device__ __forceinline__ void ghost_from_lds(const CaseDev& c, PlaneLdsJ<TL>& L, int p, int i0, int j0) {
+// from global memory (solver.cpp:1980-1993), in the storage type; other positions are
+// left alone.
+template <class TL, int FL, typename ST>
+__device__ __forceinline__ void ghost_from_lds(const CaseDev& c, PlaneLdsJ<TL, ST>& L, int p, int i0, int j0) {
     const int hi = p % TL::HI, hj = p / TL::HI;
     const int gi = i0 - 2 + hi, gj = j0 - 2 + hj;
     const int NI = c.ni, NJ = c.nj;
@@ -907,31 +922,32 @@ __device__ __forceinline__ void ghost_from_lds(const CaseDev& c, PlaneLdsJ<TL>& 
     const bool lo = ighost ? (gi < 0) : (gj < 0);
     const int sd = ighost ? 1 : TL::HI;
     const int a = lo ? p + sd : p - sd, b = lo ? p + 2 * sd : p - 2 * sd;
-    double Un[5];
+    ST Un[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
-        const double A = L.U[v][a], B = L.U[v][b];
+        const ST A = L.U[v][a], B = L.U[v][b];
         Un[v] = lo ? A - (B - A) : A + (A - B);
     }
-    commit_j<TL, FL>(c, L, p, true, Un);
+    commit_j<TL, FL, ST>(c, L, p, true, Un);
 }
 
 // Column queue of the 3-D JST kernel along k.  At the K face of plane k it holds plane k-1
 // (state and pressure), k+1 and k+2 (state, pressure, velocity and specific total energy,
 // spectral factor) in registers; plane k itself is the column's own node of the staged
 // plane in LDS.  The ideal gas derives a plane when it enters at k+2; the table fluid
-// derives its pressure then and its primitives when it reaches k+1.
-template <int FL>
+// derives its pressure then and its primitives when it reaches k+1.  All in the storage
+// type ST of the staged plane.
+template <int FL, typename ST>
 struct KColJ {
-    double A[5];
-    double sl;   // pressure sensor of plane k (the right sensor of the previous K face)
-    double C[5], Cp, Cw[4], Crf;
-    double D[5], Dp, Dw[4], Drf;
+    ST A[5];
+    ST sl;   // pressure sensor of plane k (the right sensor of the previous K face)
+    ST C[5], Cp, Cw[4], Crf;
+    ST D[5], Dp, Dw[4], Drf;
     __device__ __forceinline__ void enter(const CaseDev& c, const double u[5]) {
-        double w[5], rf;
+        ST w[5], rf;
 #pragma unroll
-        for (int v = 0; v < 5; ++v) D[v] = u[v];
-        derive<double, FL>(D, w, Dp, rf, Gas<FL>(c));
+        for (int v = 0; v < 5; ++v) D[v] = ST(u[v]);
+        derive<ST, FL>(D, w, Dp, rf, Gas<FL>(c));
         if constexpr (FL == 0) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) Dw[v] = w[v + 1];
@@ -948,21 +964,21 @@ struct KColJ {
             for (int v = 0; v < 4; ++v) Cw[v] = Dw[v];
             Crf = Drf;
         } else {
-            double w[5], p;
-            derive<double, FL>(C, w, p, Crf, Gas<FL>(c));
+            ST w[5], p;
+            derive<ST, FL>(C, w, p, Crf, Gas<FL>(c));
 #pragma unroll
             for (int v = 0; v < 4; ++v) Cw[v] = w[v + 1];
         }
     }
     // plane k-1 from the own node of the staged plane (before it is overwritten)
     template <class TL>
-    __device__ __forceinline__ void own_to_a(const PlaneLdsJ<TL>& L, int own) {
+    __device__ __forceinline__ void own_to_a(const PlaneLdsJ<TL, ST>& L, int own) {
 #pragma unroll
         for (int v = 0; v < 5; ++v) A[v] = L.U[v][own];
     }
     // plane k+1 becomes the own node of the staged plane
     template <class TL>
-    __device__ __forceinline__ void c_to_own(PlaneLdsJ<TL>& L, int own) const {
+    __device__ __forceinline__ void c_to_own(PlaneLdsJ<TL, ST>& L, int own) const {
 #pragma unroll
         for (int v = 0; v < 5; ++v) L.U[v][own] = C[v];
 #pragma unroll
@@ -973,32 +989,32 @@ struct KColJ {
     // K face between plane k (own LDS node) and k+1 (KQueue::face with slot 1 from LDS),
     // with the left sensor sl0; returns the right sensor (the next plane's left one)
     template <class TL>
-    __device__ __forceinline__ double face(const CaseDev& c, const PlaneLdsJ<TL>& L, int own, double sl0,
-                                           const double nn[4], double out[5]) const {
-        double B[5], Bw[5];
+    __device__ __forceinline__ ST face(const CaseDev& c, const PlaneLdsJ<TL, ST>& L, int own, ST sl0,
+                                       const ST nn[4], double out[5]) const {
+        ST B[5], Bw[5];
 #pragma unroll
         for (int v = 0; v < 5; ++v) B[v] = L.U[v][own];
         Bw[0] = B[0];
 #pragma unroll
         for (int v = 0; v < 4; ++v) Bw[v + 1] = L.W[v][own];
-        const double Bp = L.P[own], Brf = L.RF[own];
-        const double sr = jst_sensor<double>(Bp, Cp, Dp);
-        const double Wr[5] = {C[0], Cw[0], Cw[1], Cw[2], Cw[3]};
-        jst_face<double, FL>(A, B, C, D, Bw, Wr, Brf, Crf, sl0, sr, nn[0], nn[1], nn[2], nn[3], Gas<FL>(c), out);
+        const ST Bp = L.P[own], Brf = L.RF[own];
+        const ST sr = jst_sensor<ST>(Bp, Cp, Dp);
+        const ST Wr[5] = {C[0], Cw[0], Cw[1], Cw[2], Cw[3]};
+        jst_face<ST, FL>(A, B, C, D, Bw, Wr, Brf, Crf, sl0, sr, nn[0], nn[1], nn[2], nn[3], Gas<FL>(c), out);
         return sr;
     }
 };
 
-template <class TL, int FL>
+template <class TL, int FL, typename ST>
 __device__ __forceinline__ void stage_block_jst3d(const CaseDev* __restrict__ cp, const Work w, const StageOps st) {
-    constexpr int SCH = 0;
-    using FT = double;
+    using FT = ST;   // flux arithmetic and face records in the storage type
     constexpr int TI = TL::TI, TJ = TL::TJ, HI = TL::HI, NH = TL::NH, NP = TL::NP;
     constexpr int NFI = TJ * (TI + 1), NFJ = (TJ + 1) * TI;
-    constexpr unsigned RS = 32u;   // FP64 face-record bytes
-    static_assert(sizeof(PlaneLdsJ<TL>) <= 80 * 1024, "two blocks per CU");
+    constexpr unsigned RS = 4u * sizeof(ST);   // face-record bytes (4 components)
+    static_assert(sizeof(PlaneLdsJ<TL, ST>) <= 80 * 1024, "two blocks per CU");
+    constexpr bool kNsqLds = sizeof(ST) == 8;   // FP64: norm accumulators in LDS
     static_assert(NH <= NT, "one halo slot per thread");
-    __shared__ PlaneLdsJ<TL> L;
+    __shared__ PlaneLdsJ<TL, ST> L;
     const CaseDev& c = *cp;
     const int tid = threadIdx.x;
     const int ti = tid % TI, tj = tid / TI;
@@ -1041,23 +1057,27 @@ __device__ __forceinline__ void stage_block_jst3d(const CaseDev* __restrict__ cp
 
     // ---- column queue warm-up (planes kc0-2 .. kc0+1; plane kc0-1 as the own node of the
     // staged plane) and the K face below plane kc0
-    KColJ<FL> q;
+    KColJ<FL, ST> q;
+    double nsq[5] = {0.0, 0.0, 0.0, 0.0, 0.0};   // FP32 plane: the column's norm accumulators
     {
         double FKl[5] = {0, 0, 0, 0, 0};
         if (act) {
             double u[5] = {1.0, 1.0, 1.0, 1.0, 1.0};
             if (kc0 - 2 >= -1) load_k(c, Uin, i, j, kc0 - 2, u);
-            double Ap;   // pressure of plane kc0-2 (left sensor of the K face below plane kc0)
+            ST Ap;   // pressure of plane kc0-2 (left sensor of the K face below plane kc0)
             {
-                double w[5], rf;
+                ST w[5], rf;
 #pragma unroll
-                for (int v = 0; v < 5; ++v) q.A[v] = u[v];
-                derive<double, FL>(q.A, w, Ap, rf, Gas<FL>(c));
+                for (int v = 0; v < 5; ++v) q.A[v] = ST(u[v]);
+                derive<ST, FL>(q.A, w, Ap, rf, Gas<FL>(c));
             }
 #pragma unroll
             for (int v = 0; v < 5; ++v) u[v] = 1.0;
             load_k(c, Uin, i, j, kc0 - 1, u);   // kc0 - 1 >= -1
-            commit_j<TL, FL>(c, L, own, true, u);
+            {
+                const ST us[5] = {ST(u[0]), ST(u[1]), ST(u[2]), ST(u[3]), ST(u[4])};
+                commit_j<TL, FL, ST>(c, L, own, true, us);
+            }
             load_k(c, Uin, i, j, kc0, u);
             q.enter(c, u);
             q.d_to_c(c);
@@ -1066,30 +1086,30 @@ __device__ __forceinline__ void stage_block_jst3d(const CaseDev* __restrict__ cp
             if (kc0 + 1 <= NK) load_k(c, Uin, i, j, kc0 + 1, u);
             q.enter(c, u);
             // left sensor of the first K face of the loop: plane kc0 (planes kc0-1 .. kc0+1)
-            q.sl = jst_sensor<double>(L.P[own], q.Cp, q.Dp);
+            q.sl = jst_sensor<ST>(L.P[own], q.Cp, q.Dp);
             if (!bladecol) {
                 if (kc0 == 0) {
 #pragma unroll
                     for (int v = 0; v < 5; ++v) FKl[v] = gld(c.bnd + c.boffK, (unsigned)((j * NI + i) * 6 + v));
                 } else {
-                    double nk[4];
+                    ST nk[4];
                     kface_normal<FT>(c, (kc0 * NJ + j) * NI + i, nk);
-                    q.face(c, L, own, jst_sensor<double>(Ap, L.P[own], q.Cp), nk, FKl);
+                    q.face(c, L, own, jst_sensor<ST>(Ap, L.P[own], q.Cp), nk, FKl);
                 }
             }
         }
 #pragma unroll
         for (int v = 0; v < 5; ++v) {
             L.FK[v][tid] = FKl[v];
-            L.nsq[v][tid] = 0.0;
+            if constexpr (kNsqLds) L.nsq[v][tid] = 0.0;
         }
     }
     // ---- loads ahead of plane kc0, in the order the loop issues them
     double pre[5], hU[5];
     load5(Uin, N, qplane(kc0 + 2) * NIJ + colc, pre);
     load5(Uin, N, kc0 * NIJ + hsrc.a, hU);
-    double nnk[4];
-    krec_load<double>(fb, krec_off(kc0), nnk);
+    ST nnk[4];
+    krec_load<ST>(fb, krec_off(kc0), nnk);
     SlotData sd[2];
 #pragma unroll
     for (int s = 0; s < 2; ++s) sd[s] = slot_load(fb, slot_off<RS>(c, fs[s], kc0));
@@ -1118,9 +1138,13 @@ __device__ __forceinline__ void stage_block_jst3d(const CaseDev* __restrict__ cp
                 q.enter(c, pre);
             }
         } else if (col) {
-            commit_j<TL, FL>(c, L, own, false, pre);
+            const ST dummy[5] = {ST(1.0), ST(1.0), ST(1.0), ST(1.0), ST(1.0)};
+            commit_j<TL, FL, ST>(c, L, own, false, dummy);
         }
-        if (hown) commit_j<TL, FL>(c, L, hpos, hsrc.kind == 1, hU);
+        if (hown) {
+            const ST hs[5] = {ST(hU[0]), ST(hU[1]), ST(hU[2]), ST(hU[3]), ST(hU[4])};
+            commit_j<TL, FL, ST>(c, L, hpos, hsrc.kind == 1, hs);
+        }
         // ---- loads for the next plane: the column's plane k+3, the halo slot's plane k+1.
         // The last plane of the piece has no next one: its loads read what this plane read
         // (cache-resident; the same loads in the same order keep every wait exact)
@@ -1129,8 +1153,8 @@ __device__ __forceinline__ void stage_block_jst3d(const CaseDev* __restrict__ cp
         load5(Uin, N, min(kq, NK - 1) * NIJ + hsrc.a, hU);
         __syncthreads();
         if (ghosts) {
-            ghost_from_lds<TL, FL>(c, L, tid, i0, j0);
-            if (tid + NT < NP) ghost_from_lds<TL, FL>(c, L, tid + NT, i0, j0);
+            ghost_from_lds<TL, FL, ST>(c, L, tid, i0, j0);
+            if (tid + NT < NP) ghost_from_lds<TL, FL, ST>(c, L, tid + NT, i0, j0);
             __syncthreads();
         }
         // ---- every I and J face of the tile once (face slots of the thread); the few
@@ -1146,11 +1170,11 @@ __device__ __forceinline__ void stage_block_jst3d(const CaseDev* __restrict__ cp
             if (f.bnd()) {
                 slot_flux(d, F);
             } else {
-                double nn[4];
-                slot_record<double>(d, nn);
-                if (s == 0) lds_face_j<TL, FL, 1>(c, L, f.l(), nn[0], nn[1], nn[2], nn[3], F);
-                else if (s == 1) lds_face_j<TL, FL, HI>(c, L, f.l(), nn[0], nn[1], nn[2], nn[3], F);
-                else lds_face_j<TL, FL, 0>(c, L, f.l(), nn[0], nn[1], nn[2], nn[3], F, f.isI() ? 1 : HI);
+                ST nn[4];
+                slot_record<ST>(d, nn);
+                if (s == 0) lds_face_j<TL, FL, 1, ST>(c, L, f.l(), nn[0], nn[1], nn[2], nn[3], F);
+                else if (s == 1) lds_face_j<TL, FL, HI, ST>(c, L, f.l(), nn[0], nn[1], nn[2], nn[3], F);
+                else lds_face_j<TL, FL, 0, ST>(c, L, f.l(), nn[0], nn[1], nn[2], nn[3], F, f.isI() ? 1 : HI);
             }
             const bool fI = (s == 0) || (s == 2 && f.isI());
             double* out = fI ? L.FI : L.FJ;
@@ -1181,7 +1205,7 @@ __device__ __forceinline__ void stage_block_jst3d(const CaseDev* __restrict__ cp
         }
         {
             const int kn = min(kq, NK - 1);
-            krec_load<double>(fb, krec_off(kq), nnk);
+            krec_load<ST>(fb, krec_off(kq), nnk);
 #pragma unroll
             for (int s = 0; s < 2; ++s) sd[s] = slot_load(fb, slot_off<RS>(c, FaceSlot{per_plane(fs[s].b0), per_plane(fs[s].info)}, kn));
         }
@@ -1230,9 +1254,13 @@ __device__ __forceinline__ void stage_block_jst3d(const CaseDev* __restrict__ cp
                 const double factor = st.alpha * dtv / volv;
 #pragma unroll
                 for (int v = 0; v < 5; ++v) gst(st.Uout + (long)v * N, (unsigned)n, u0v[v] - R[v] * factor);
-                if (st.last)
+                if (st.last) {
 #pragma unroll
-                    for (int v = 0; v < 5; ++v) L.nsq[v][tid] += R[v] * R[v];
+                    for (int v = 0; v < 5; ++v) {
+                        if constexpr (kNsqLds) L.nsq[v][tid] += R[v] * R[v];
+                        else nsq[v] += R[v] * R[v];
+                    }
+                }
             }
         }
 #pragma unroll
@@ -1247,29 +1275,45 @@ __device__ __forceinline__ void stage_block_jst3d(const CaseDev* __restrict__ cp
     }
     // ---- per-block norm partial sums: a deterministic tree over the accumulators
     if (st.last) {
+        double (*red)[NT];
+        if constexpr (kNsqLds) {
+            red = L.nsq;
+        } else {
+            // the staged plane's last readers (this plane's faces) are past the barrier
+            // before the assembly
+            red = L.red();
+#pragma unroll
+            for (int v = 0; v < 5; ++v) red[v][tid] = nsq[v];
+        }
         __syncthreads();
         const long b = ((long)w.z * c.sgy + w.y) * c.sgx + w.x;
         for (int s = NT / 2; s > 0; s >>= 1) {
             if (tid < s)
 #pragma unroll
-                for (int v = 0; v < 5; ++v) L.nsq[v][tid] += L.nsq[v][tid + s];
+                for (int v = 0; v < 5; ++v) red[v][tid] += red[v][tid + s];
             __syncthreads();
         }
-        if (tid < 5) gst(c.partial, (unsigned)(b * 5 + tid), L.nsq[tid][0]);
+        if (tid < 5) gst(c.partial, (unsigned)(b * 5 + tid), red[tid][0]);
     }
 }
 
 // Work: one entry per (i, j) tile and k chunk of a case.  The case and work tables are
 // also passed as restrict kernel parameters (read-only for the launch), which lets the
 // compiler keep case fields in scalar registers or reload them freely.
-// Waves per SIMD the register allocation must allow: three for the FP32 JST kernels
-// (their LDS plane, 41.6 KB for 28x9 tiles, fits three blocks per CU), two for the FP64
-// 3-D JST kernel of the ideal gas (its LDS plane allows two blocks per CU), otherwise no
-// bound (the other FP64 kernels need about 240 registers: two waves).
+// The 3-D JST instances with the static load schedule: FP64 (ideal gas or table fluid) and
+// FP32 (ideal gas) where the staged plane fits two blocks per CU and its halo one
+// position per thread.
 template <class TL, int SCH, bool THREED, typename FT, int FL>
-constexpr int kStageWaves = std::is_same<StageST<SCH, FT, FL>, float>::value ? 3
-                            : ((SCH == 0 && THREED && FL == 0 && sizeof(FT) == 8 &&
-                                sizeof(PlaneLdsJ<TL>) <= 80 * 1024) ? 2 : 1);
+constexpr bool kJst3d = SCH == 0 && THREED && (sizeof(FT) == 8 || FL == 0) && TL::NH <= NT &&
+                        sizeof(PlaneLdsJ<TL, StageST<SCH, FT, FL>>) <= 80 * 1024;
+// Waves per SIMD the register allocation must allow: two for the 3-D JST kernels of the
+// ideal gas with the static schedule (LDS allows two blocks per CU; a third wave would
+// spill the loads in flight, which costs far more than the wave gains), three for the
+// other FP32 JST kernels (their LDS plane, 41.6 KB for 28x9 tiles, fits three blocks per
+// CU), otherwise no bound (the other FP64 kernels need about 240 registers: two waves).
+template <class TL, int SCH, bool THREED, typename FT, int FL>
+constexpr int kStageWaves = (kJst3d<TL, SCH, THREED, FT, FL> && FL == 0) ? 2
+                            : (std::is_same<StageST<SCH, FT, FL>, float>::value ? 3 : 1);
 
 template <class TL, int SCH, bool THREED, typename FT, int FL>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kStageWaves<TL, SCH, THREED, FT, FL>)))
@@ -1285,9 +1329,8 @@ k_stage(const CaseDev* __restrict__ cases, const Work* __restrict__ work, Launch
     st.alpha = cp->alpha[La.stage];
     st.last = La.last;
     st.rmode = La.rmode;
-    // the FP64 3-D JST instances whose LDS fits two blocks per CU use the static schedule
-    if constexpr (SCH == 0 && THREED && sizeof(FT) == 8 && sizeof(PlaneLdsJ<TL>) <= 80 * 1024)
-        stage_block_jst3d<TL, FL>(cp, w, st);
+    if constexpr (kJst3d<TL, SCH, THREED, FT, FL>)
+        stage_block_jst3d<TL, FL, StageST<SCH, FT, FL>>(cp, w, st);
     else stage_block<TL, SCH, THREED, FT, FL>(cp, w, st);
 }
 

@@ -932,10 +932,12 @@ void setup(tbfm_case* cs, const GridInput& grid) {
     cs->solo.dcases = cs->cdev;
     cs->solo.dcap = 1;
     cs->solo.ownsCases = false;
-    // algorithmic bytes per node and stage: U^(k-1), U^(0), U^(k) (5 each), dt, V and one
-    // face vector per active direction
+    // algorithmic bytes per node and stage: U^(k-1), U^(0), U^(k) (5 doubles each), dt, V
+    // and one face vector of 3 components per active direction (FP32 for the FP32-flux
+    // kernels, which read FP32 face records)
     const int dirs = (cs->topo == Topology::ThreeD) ? 3 : (cs->topo == Topology::OneD ? 1 : 2);
-    cs->bytes_per_node = 8.0 * (15 + 2 + 3 * dirs);
+    const double rec = (cs->precision == Precision::Mixed) ? 4.0 : 8.0;
+    cs->bytes_per_node = 8.0 * (15 + 2) + rec * 3 * dirs;
 
     // ---- initial solution: uniform (solver.cpp:853-888) or from a restart file
     // (solver.cpp:890-1295), with the restart's body forces as the side outputs' start
@@ -1951,8 +1953,8 @@ int tbfm_gpu_profile_get(tbfm_case* c, double* ms, long* launches, double* bytes
         drainProfile(c);
         if (ms) *ms = c->prof_ms;
         if (launches) *launches = c->prof_launches;
-        // the stage kernel's contract: 208 B per node in 3-D FP64 (read U^(k-1), U^(0),
-        // dt, V and one face vector per direction, write U^(k))
+        // the stage kernel's contract: 208 B per node in 3-D FP64, 172 B with FP32 face
+        // records (read U^(k-1), U^(0), dt, V and one face vector per direction, write U^(k))
         if (bytes) *bytes = c->bytes_per_node * (double)c->N;
         return TBFM_OK;
     });

@@ -603,7 +603,12 @@ __device__ inline void bfm_source(const CT& c, const Gas<FL>& gs, long s, double
         V3T<T> sp = v3(-rel.y, rel.x, T(0.0));
         sp = vdiv(sp, vmag(sp));
         V3T<T> vin = vsub(rel, vmul(sp, vdot(rel, sp)));
-        T dv = -asin(vdot(vin, n) / vmag(vin));
+        T sd0 = vdot(vin, n) / vmag(vin);
+        // FP32: a relative flow within ~3e-4 rad of the camber normal rounds this sine
+        // past +-1 (FP64 needs ~1e-8 rad); the FP32 source keeps it in asin's domain, the
+        // FP64 one is the reference's expression as it stands
+        if constexpr (sizeof(T) == 4) sd0 = fminf(fmaxf(sd0, -1.0f), 1.0f);
+        T dv = -asin(sd0);
         // inviscid force direction (:168-199): the camber normal itself once the
         // deviation exceeds the stall deviation (STALLED_BFM_ACTIVE, :178-184)
         const bool stalled = c.stall && dv > T(c.s_stall[s]);

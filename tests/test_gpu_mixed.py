@@ -52,3 +52,35 @@ def test_mixed_against_fp64(name):
         assert rel < TURBO, nm
     # the residual history follows the FP64 one closely (same convergence behaviour)
     assert np.max(np.abs(a.history - b.history)) < 0.05
+
+
+def test_mixed_body_force_with_relative_flow_along_the_camber_normal():
+    """Where the relative flow runs along the camber normal, the sine of the deviation
+    angle is -1 and FP32 rounding puts it past -1 at about half such nodes (asin then
+    returns NaN; FP64 needs the flow within ~1e-8 rad of the normal, FP32 within ~3e-4
+    rad, which a throttled rotor's transient reaches).  The FP32 source keeps the sine in
+    asin's domain: the residual stays finite and the body force is the limit one."""
+    case = cases.rotor3d(49, 17, 33, n_iterations=100)
+    f = case["fields"]
+    om = f["rpm"] * 2.0 * np.pi / 60.0
+    y, z = case["y"], case["z"]
+    r, th = np.hypot(y, z), np.arctan2(z, y)
+    lam = 30.0
+    uax = -lam * f["normalAxial"]
+    ur = -lam * f["normalRadial"]
+    ut = om * r - lam * f["normalTangential"]
+    rho, p = 1.2, 1.0e5
+    U = np.empty((5,) + y.shape)
+    U[0] = rho
+    U[1] = rho * uax
+    U[2] = rho * (ur * np.cos(th) - ut * np.sin(th))
+    U[3] = rho * (ur * np.sin(th) + ut * np.cos(th))
+    U[4] = p / 0.4 + 0.5 * rho * (uax ** 2 + ur ** 2 + ut ** 2)
+    blade = f["bladePresent"] > 0
+    assert blade.sum() > 1000
+    s = Solver.from_case(case, {"GPU_PRECISION": "mixed"})
+    R = s.residual(U)
+    s.close()
+    print("blade nodes", int(blade.sum()), "non-finite residual nodes", int((~np.isfinite(R)).any(axis=0).sum()),
+          flush=True)
+    assert np.isfinite(R).all()
