@@ -174,7 +174,20 @@ def serve_leg(args, n_cases):
     plain = {"p50_s": float(np.percentile(ttc, 50)), "p99_s": float(np.percentile(ttc, 99)),
              "cases_per_hour": len(specs) / plain_wall * 3600.0, "wall_s": plain_wall,
              "converged": int(sum(r.converged for r in alone)), "iterations": int(sum(r.iterations for r in alone))}
-    return specs, served, plain, dm, dpr, res
+    # capacity: the same requests all arriving at once, so that neither path waits for
+    # arrivals (the Poisson stream's cases/hour are bounded by its rate); the plain path's
+    # capacity is its summed service time
+    srv = serve.Server(drop=args.drop, max_iter=args.max_iter, chunk=args.chunk, log=progress)
+    t0 = time.perf_counter()
+    res_cap = srv.run(specs, np.zeros(len(specs)))
+    wall_cap = time.perf_counter() - t0
+    busy = float(sum(a.finished - a.arrival for a in alone))
+    served_cap = serve.summary(res_cap, wall_cap)
+    capacity = {"served_cases_per_hour": served_cap["cases_per_hour"], "served_wall_s": wall_cap,
+                "served_converged": served_cap["converged"],
+                "plain_fp64_cases_per_hour": len(specs) / busy * 3600.0, "plain_fp64_busy_s": busy,
+                "cases_per_hour_ratio": served_cap["cases_per_hour"] / (len(specs) / busy * 3600.0)}
+    return specs, served, plain, dm, dpr, res, capacity
 
 
 def equal_iteration_diffs(specs, res):
@@ -201,9 +214,43 @@ def equal_iteration_diffs(specs, res):
     return dm, dpr
 
 
+def limit_cycle_diffs(specs, n=12000, window=4000):
+    """Mass flow and total-pressure ratio averaged over the stalled limit cycle (the last
+    `window` of n iterations; the cases stall about 3.5 orders down, where the inlet state
+    moves in bisection-sized steps and the instantaneous figures wander), FP32 flux
+    against FP64, for the first ideal-gas request of each grid size in the stream."""
+    from cturbobfm_amd import Solver
+    seen, rows = set(), []
+    for sp in specs:
+        if "table-fluid" in sp.name or (sp.three_d, sp.nodes) in seen:
+            continue
+        seen.add((sp.three_d, sp.nodes))
+        means = {}
+        for prec in ("fp64", "mixed"):
+            case = sp.build()
+            case["config"]["TURBO_OUTPUT"] = "yes"
+            case["config"]["TURBO_DIRECTION"] = "i"
+            ov = dict(sp.overrides)
+            ov["GPU_PRECISION"] = prec
+            s = Solver.from_case(case, ov)
+            s.set_convergence(True, 1.0e3)   # a drop no run reaches: exactly n iterations
+            while s.iterations < n:
+                s.advance(min(3000, n - s.iterations))
+            h = s.turbo_history(1, n)[-window:]
+            s.close()
+            means[prec] = (h[:, 1].mean(), h[:, 2].mean())
+        (ma, pa), (mb, pb) = means["fp64"], means["mixed"]
+        rows.append({"case": sp.name.split(" p_out")[0].split(" M_out")[0], "mass_flow_rel_diff": abs(mb - ma) / abs(ma),
+                     "prtt_rel_diff": abs(pb - pa) / abs(pa)})
+    return {"iterations": n, "window": window, "cases": rows,
+            "mass_flow_max": max(r["mass_flow_rel_diff"] for r in rows) if rows else None,
+            "prtt_max": max(r["prtt_rel_diff"] for r in rows) if rows else None}
+
+
 def bench_serve(args):
-    specs, served, plain, dm, dpr, res = serve_leg(args, args.serve_cases)
+    specs, served, plain, dm, dpr, res, capacity = serve_leg(args, args.serve_cases)
     em, epr = equal_iteration_diffs(specs, res)
+    lc = limit_cycle_diffs(specs)
     return {
         "metric": "served cases/hour; p50/p99 time-to-converge vs plain FP64",
         "value": served["cases_per_hour"], "unit": "cases/hour", "n_gpus": 1, "steps": len(specs), "warmup": 0,
@@ -213,7 +260,7 @@ def bench_serve(args):
         "config": {"workload": "%d cases, Poisson arrivals %.1f/s, retire at %.1f orders of density drop "
                                "(cap %d iterations)" % (len(specs), args.rate, args.drop, args.max_iter),
                    "chunk_iterations": args.chunk},
-        "served": served, "plain_fp64": plain,
+        "served": served, "plain_fp64": plain, "capacity": capacity,
         "table_fluid_cases": int(sum("table-fluid" in sp.name for sp in specs)),
         "converged_rel_diff_vs_plain": {"mass_flow_max": max(dm) if dm else None,
                                         "mass_flow_median": float(np.median(dm)) if dm else None,
@@ -225,6 +272,7 @@ def bench_serve(args):
                                               "prtt_max": max(epr) if epr else None,
                                               "prtt_median": float(np.median(epr)) if epr else None,
                                               "cases": len(em)},
+        "limit_cycle_mean_rel_diff_vs_fp64": lc,
         "cases_per_hour_ratio": served["cases_per_hour"] / plain["cases_per_hour"],
     }
 
@@ -348,7 +396,7 @@ def main():
     if args.serve_leg > 0:
         # the metric's second half: time-to-converge of a bounded served stream against the
         # plain FP64 path serving the same stream one case at a time
-        specs, served, plain, dm, dpr, _ = serve_leg(args, args.serve_leg)
+        specs, served, plain, dm, dpr, _, capacity = serve_leg(args, args.serve_leg)
         out["time_to_converge"] = {
             "workload": "%d served cases (bump 65x33 / 129x65, rotor 97x25x65 / 129x33x97 / 193x49x129, "
                         "5 %% table fluid), Poisson %.1f/s (seed %d), retired at a %.1f-order density drop"
@@ -358,7 +406,8 @@ def main():
             "plain_fp64_p50_s": plain["p50_s"], "plain_fp64_p99_s": plain["p99_s"],
             "plain_fp64_cases_per_hour": plain["cases_per_hour"],
             "cases_per_hour_ratio": served["cases_per_hour"] / plain["cases_per_hour"],
-            "mass_flow_rel_diff_max": max(dm) if dm else None, "prtt_rel_diff_max": max(dpr) if dpr else None}
+            "mass_flow_rel_diff_max": max(dm) if dm else None, "prtt_rel_diff_max": max(dpr) if dpr else None,
+            "capacity": capacity}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
     print(json.dumps(out), flush=True)

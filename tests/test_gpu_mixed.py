@@ -84,3 +84,37 @@ def test_mixed_body_force_with_relative_flow_along_the_camber_normal():
     print("blade nodes", int(blade.sum()), "non-finite residual nodes", int((~np.isfinite(R)).any(axis=0).sum()),
           flush=True)
     assert np.isfinite(R).all()
+
+
+@pytest.mark.parametrize("name", ["bump_65x33", "rotor_97x25x65"])
+def test_mixed_limit_cycle_means_against_fp64(name):
+    """The synthetic cases stall 3.5 orders down in a limit cycle (the reference's inlet
+    bisection stops at |residual| < 1e-4 p0, so the inlet state moves in bisection-sized
+    steps) whose wander is larger than the precision effect: instantaneous mass flow and
+    PRtt of the FP64 path itself move by up to 1e-5 over 1000 iterations.  Their means
+    over the limit cycle are the converged figures: FP32 flux against FP64, at 1e-6.
+
+    Measured (MI355X, 12000 iterations, means over the last 4000): bump 65x33 1.4e-7 /
+    2.7e-7 (mass flow / PRtt), rotor 97x25x65 3.9e-9 / 2.7e-8."""
+    n, window = 12000, 4000
+    hist = {}
+    for prec in ("fp64", "mixed"):
+        if name == "bump_65x33":
+            case = cases.bump2d(65, 33, n_iterations=n, mach_out=0.5)
+        else:
+            case = cases.rotor3d(97, 25, 65, n_iterations=n, outlet_pressure=95000.0)
+        case["config"]["TURBO_OUTPUT"] = "yes"
+        case["config"]["TURBO_DIRECTION"] = "i"
+        s = Solver.from_case(case, {"GPU_PRECISION": prec})
+        s.set_convergence(True, 1.0e3)   # a drop no run reaches: exactly n iterations
+        while s.iterations < n:
+            s.advance(min(3000, n - s.iterations))
+        hist[prec] = s.turbo_history(1, n)
+        s.close()
+    a, b = hist["fp64"], hist["mixed"]
+    for col, q in ((1, "mass flow"), (2, "PRtt")):
+        ma, mb = a[-window:, col].mean(), b[-window:, col].mean()
+        rel = abs(mb - ma) / abs(ma)
+        print("%s: limit-cycle means fp64 %.9g mixed %.9g rel %.2e (fp64 std/mean %.2e)"
+              % (q, ma, mb, rel, a[-window:, col].std() / abs(ma)), flush=True)
+        assert rel < 1e-6, q
