@@ -3,7 +3,7 @@
 Bar (FP64 path): per-call residuals and time steps agree to 1e-13 relative (bit-exact
 where no libm transcendental enters: the inlet bisection uses pow, the body force asin /
 atan2); full runs agree to <= 1e-11 relative in the per-equation log-residual history
-and in the final conserved fields, the north-star criterion.
+and in the final conserved fields, the bar of the FP64 path against the CPU solver.
 """
 import numpy as np
 import pytest
