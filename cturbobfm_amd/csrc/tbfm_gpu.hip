@@ -134,6 +134,8 @@ struct tbfm_case {
     long hist_cap = 0;
     long iter_host = 1;               // next iteration number
     int tile = 0;                     // 0: 32x8, 1: 16x16, 2: 64x4, 3: 28x9 (i x j columns)
+    int forced_tile = -1;             // GPU_STAGE_TILING: tile shape and k pieces, no autotuning
+    int forced_pieces = 0;
     bool tuned = false;
     std::string tuning;               // autotuner report
     Plan solo;                        // the case advancing alone
@@ -839,7 +841,21 @@ void setup(tbfm_case* cs, const GridInput& grid) {
     c.nstages = (int)cs->rk.size();
     for (int q = 0; q < 4; ++q) c.alpha[q] = (q < c.nstages) ? cs->rk[q] : 0.0;
     c.rper = dalloc<double>(10 * NI * NJ, own);
-    setTiling(cs, 0, (cs->topo == Topology::ThreeD) ? 8 : 1);
+    // GPU_STAGE_TILING = <TI>x<TJ>[/z<pieces>] fixes the stage kernel's tiling (one of the
+    // tile shapes, default 8 k pieces in 3-D) instead of the autotuner's choice
+    if (cfg.has("GPU_STAGE_TILING")) {
+        const std::string v = cfg.get("GPU_STAGE_TILING");
+        int ti = 0, tj = 0, z = (cs->topo == Topology::ThreeD) ? 8 : 1;
+        const int got = std::sscanf(v.c_str(), "%dx%d/z%d", &ti, &tj, &z);
+        for (int t = 0; t < kTiles && got >= 2; ++t)
+            if (tileI(t) == ti && tileJ(t) == tj) cs->forced_tile = t;
+        if (cs->forced_tile < 0 || z < 1)
+            throw std::runtime_error("Invalid value for key \"GPU_STAGE_TILING\" (32x8 | 16x16 | 64x4 | 28x9, "
+                                     "optionally /z<k pieces>)");
+        cs->forced_pieces = z;
+    }
+    if (cs->forced_tile >= 0) setTiling(cs, cs->forced_tile, cs->forced_pieces);
+    else setTiling(cs, 0, (cs->topo == Topology::ThreeD) ? 8 : 1);
     c.nparts_per = (periodic && cs->topo == Topology::ThreeD) ? (int)((NI * NJ + 255) / 256) : 0;
     // norm partials sized for the finest tiling the autotuner may pick
     long maxParts = 0;
@@ -1334,6 +1350,14 @@ void autotune(tbfm_case* cs) {
     if (cs->tuned) return;
     cs->tuned = true;
     Plan& P = cs->solo;
+    if (cs->forced_tile >= 0) {
+        setTiling(cs, cs->forced_tile, cs->forced_pieces);
+        pushCase(cs);
+        buildTables(P);
+        cs->tuning = "fixed -> " + std::to_string(tileI(cs->tile)) + "x" + std::to_string(tileJ(cs->tile)) + "/z" +
+                     std::to_string(cs->c.sgz);
+        return;
+    }
     const std::string key = std::to_string(cs->ni) + "x" + std::to_string(cs->nj) + "x" + std::to_string(cs->nk) + "/" +
                             std::to_string((int)cs->scheme) + std::to_string(cs->c.muscl) + "/" +
                             std::to_string((int)cs->topo) + "/" + std::to_string((int)cs->precision) + "/" +
