@@ -75,9 +75,10 @@ template <class TL>
 __device__ __forceinline__ int fiJ(int v, int r, int ti) { return (v * (TL::TJ + 1) + r) * TL::TI + ti; }
 
 // Reads the conserved state of LDS position `pos` of plane k: a real node or a ghost one
-// step outside the domain in i or j (linear extrapolation, solver.cpp:1980-1993); false
-// for a position no face reads.
-template <class TL>
+// step outside the domain in i or j (linear extrapolation, solver.cpp:1980-1993, formed
+// in the storage type ST from the state rounded to it, as the static-schedule kernel
+// extrapolates from its staged plane); false for a position no face reads.
+template <class TL, typename ST>
 __device__ __forceinline__ bool fetch_position(const CaseDev& c, const double* __restrict__ U, int pos, int i0,
                                                int j0, int k, double Un[5]) {
     const int hi = pos % TL::HI, hj = pos / TL::HI;
@@ -108,7 +109,10 @@ __device__ __forceinline__ bool fetch_position(const CaseDev& c, const double* _
     load5(U, N, na, A);
     load5(U, N, nb, B);
 #pragma unroll
-    for (int v = 0; v < 5; ++v) Un[v] = lo ? A[v] - (B[v] - A[v]) : A[v] + (A[v] - B[v]);
+    for (int v = 0; v < 5; ++v) {
+        const ST a = ST(A[v]), b = ST(B[v]);
+        Un[v] = double(lo ? a - (b - a) : a + (a - b));
+    }
     return true;
 }
 
@@ -147,7 +151,9 @@ __device__ __forceinline__ HaloSrc halo_src(const CaseDev& c, int pos, int i0, i
     h.kind = lo ? 2 : 3;
     return h;
 }
-// The conserved state of a resolved position on the plane starting at node `plane`.
+// The conserved state of a resolved position on the plane starting at node `plane`
+// (ghosts extrapolated in the storage type ST, as fetch_position).
+template <typename ST>
 __device__ __forceinline__ bool fetch_src(const double* __restrict__ U, int N, const HaloSrc& h, int plane,
                                           double Un[5]) {
     if (h.kind == 0) return false;
@@ -159,7 +165,10 @@ __device__ __forceinline__ bool fetch_src(const double* __restrict__ U, int N, c
     load5(U, N, plane + h.a, A);
     load5(U, N, plane + h.b, B);
 #pragma unroll
-    for (int v = 0; v < 5; ++v) Un[v] = (h.kind == 2) ? A[v] - (B[v] - A[v]) : A[v] + (A[v] - B[v]);
+    for (int v = 0; v < 5; ++v) {
+        const ST a = ST(A[v]), b = ST(B[v]);
+        Un[v] = double((h.kind == 2) ? a - (b - a) : a + (a - b));
+    }
     return true;
 }
 
@@ -196,7 +205,7 @@ template <class TL, int SCH, int FL, typename ST>
 __device__ inline void fill_position(const CaseDev& c, const double* __restrict__ U, PlaneLds<TL, ST>& L, int pos,
                                      int i0, int j0, int k) {
     double Un[5];
-    const bool ok = fetch_position<TL>(c, U, pos, i0, j0, k, Un);
+    const bool ok = fetch_position<TL, ST>(c, U, pos, i0, j0, k, Un);
     commit_position<TL, SCH, FL, ST>(c, L, pos, ok, Un);
 }
 
@@ -511,7 +520,7 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
     const HaloSrc hsrc = hown ? halo_src<TL>(c, hpos, i0, j0) : HaloSrc{0, 0, 0};
     double hU[5];
     bool hok = false;
-    if (hown) hok = fetch_src(Uin, N, hsrc, kc0 * NIJ, hU);
+    if (hown) hok = fetch_src<ST>(Uin, N, hsrc, kc0 * NIJ, hU);
 
 #pragma unroll 1
     for (int k = kc0; k < kc1; ++k) {
@@ -571,7 +580,7 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
         if (col && !act) fill_position<TL, SCH, FL, ST>(c, Uin, L, own, i0, j0, k);
         if (hown) {
             commit_position<TL, SCH, FL, ST>(c, L, hpos, hok, hU);
-            if (k + 1 < kc1) hok = fetch_src(Uin, N, hsrc, (k + 1) * NIJ, hU);
+            if (k + 1 < kc1) hok = fetch_src<ST>(Uin, N, hsrc, (k + 1) * NIJ, hU);
         }
         for (int h = tid + NT; h < NH; h += NT) fill_position<TL, SCH, FL, ST>(c, Uin, L, halo_pos<TL>(h), i0, j0, k);
         __syncthreads();
@@ -720,9 +729,8 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
 // virtual plane NK of a non-periodic case, the end-plane K boundary fluxes, blade-row
 // sources, viscous terms) stay where they are used.  The FP32 instance (ideal gas) stages
 // the plane, the column queue and the face arithmetic in float like stage_block's FP32
-// path; its ghost positions are extrapolated from the staged float values, where
-// stage_block extrapolates the FP64 state and then rounds (results differ in the last
-// float bits next to the i/j domain ends).
+// path; both extrapolate the i/j ghost positions in float from the state rounded to float
+// (here from the staged plane), so every tiling gives the same bits.
 // A per-plane copy of a value that the compiler may not treat as loop-invariant: what is
 // derived from it (LDS and record offsets) is recomputed each plane instead of being
 // hoisted out of the k loop into registers that the plane's arithmetic needs.

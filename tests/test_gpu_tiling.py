@@ -2,12 +2,9 @@
 another.  The residual of a node does not depend on how the grid is cut into tiles and
 k pieces, so every tiling gives the same bits: 32x8, 16x16 and 28x9 run the static-schedule
 3-D JST kernel, 64x4 (halo beyond one position per thread) the earlier loop, and the
-pieces change where each column's queue is warmed up.  FP64 is also held to the oracle.
-
-FP32 flux (GPU_PRECISION = mixed): the static-schedule kernel extrapolates its ghost
-positions from the staged float values and the earlier loop from the FP64 state, so 64x4
-agrees with the others bit for bit away from the i/j domain ends and to float rounding
-at the two nodes next to them."""
+pieces change where each column's queue is warmed up.  The same holds with FP32 flux
+(GPU_PRECISION = mixed), where both kernels extrapolate the i/j ghost positions in float
+from the state rounded to float.  FP64 is also held to the oracle."""
 import numpy as np
 import pytest
 
@@ -46,19 +43,12 @@ def test_tilings_agree(name, prec):
     out = {t: run(case, t, prec, U) for t in TILINGS}
     print("case", name, prec, flush=True)
     R0, V0 = out[TILINGS[0]]
-    ni, nj = R0.shape[1], R0.shape[2]
-    inner = np.zeros(R0.shape[1:], bool)
-    inner[2:ni - 2, 2:nj - 2, :] = True
     for t in TILINGS[1:]:
         R, V = out[t]
         same_r, same_v = np.array_equal(R, R0), np.array_equal(V, V0)
         print("%-9s residual %s, 20 iterations %s" % (t, "identical" if same_r else "differs",
                                                        "identical" if same_v else "differ"), flush=True)
-        if prec == "fp64" or not t.startswith("64x4"):
-            assert same_r and same_v, t
-        else:
-            assert np.array_equal(R[:, inner], R0[:, inner]), t
-            assert np.max(np.abs(R - R0)) <= 1e-5 * np.max(np.abs(R0)), t
+        assert same_r and same_v, t
     if prec == "fp64":
         oc.set_state(U)
         Ro = oc.residual(U, it=1)
