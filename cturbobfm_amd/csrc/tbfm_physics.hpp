@@ -605,9 +605,10 @@ __device__ inline void bfm_source(const CT& c, const Gas<FL>& gs, long s, double
         V3T<T> vin = vsub(rel, vmul(sp, vdot(rel, sp)));
         T sd0 = vdot(vin, n) / vmag(vin);
         // FP32: a relative flow within ~3e-4 rad of the camber normal rounds this sine
-        // past +-1 (FP64 needs ~1e-8 rad); the FP32 source keeps it in asin's domain, the
-        // FP64 one is the reference's expression as it stands
-        if constexpr (sizeof(T) == 4) sd0 = fminf(fmaxf(sd0, -1.0f), 1.0f);
+        // past +-1 (FP64 needs ~1e-8 rad); the FP32 source keeps it in asin's domain (a
+        // NaN from a vanishing relative velocity passes through, as in FP64), the FP64
+        // one is the reference's expression as it stands
+        if constexpr (sizeof(T) == 4) sd0 = (sd0 < T(-1)) ? T(-1) : ((sd0 > T(1)) ? T(1) : sd0);
         T dv = -asin(sd0);
         // inviscid force direction (:168-199): the camber normal itself once the
         // deviation exceeds the stall deviation (STALLED_BFM_ACTIVE, :178-184)

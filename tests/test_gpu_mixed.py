@@ -59,16 +59,20 @@ def test_mixed_body_force_with_relative_flow_along_the_camber_normal():
     angle is -1 and FP32 rounding puts it past -1 at about half such nodes (asin then
     returns NaN; FP64 needs the flow within ~1e-8 rad of the normal, FP32 within ~3e-4
     rad, which a throttled rotor's transient reaches).  The FP32 source keeps the sine in
-    asin's domain: the residual stays finite and the body force is the limit one."""
+    asin's domain: the residual stays finite and the body force is the limit one.  (Where
+    the camber normal has no meridional component, at the trailing edge of this rotor, such
+    a flow has no meridional velocity and the reference's blade-to-blade direction is 0/0 in
+    either precision; those nodes get an axial flow.)"""
     case = cases.rotor3d(49, 17, 33, n_iterations=100)
     f = case["fields"]
     om = f["rpm"] * 2.0 * np.pi / 60.0
     y, z = case["y"], case["z"]
     r, th = np.hypot(y, z), np.arctan2(z, y)
     lam = 30.0
-    uax = -lam * f["normalAxial"]
-    ur = -lam * f["normalRadial"]
-    ut = om * r - lam * f["normalTangential"]
+    along = np.hypot(f["normalAxial"], f["normalRadial"]) > 0.05
+    uax = np.where(along, -lam * f["normalAxial"], 100.0)
+    ur = np.where(along, -lam * f["normalRadial"], 0.0)
+    ut = np.where(along, om * r - lam * f["normalTangential"], 0.0)
     rho, p = 1.2, 1.0e5
     U = np.empty((5,) + y.shape)
     U[0] = rho
@@ -76,7 +80,7 @@ def test_mixed_body_force_with_relative_flow_along_the_camber_normal():
     U[2] = rho * (ur * np.cos(th) - ut * np.sin(th))
     U[3] = rho * (ur * np.sin(th) + ut * np.cos(th))
     U[4] = p / 0.4 + 0.5 * rho * (uax ** 2 + ur ** 2 + ut ** 2)
-    blade = f["bladePresent"] > 0
+    blade = (f["bladePresent"] > 0) & along
     assert blade.sum() > 1000
     s = Solver.from_case(case, {"GPU_PRECISION": "mixed"})
     R = s.residual(U)
