@@ -295,6 +295,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--size", default="193x49x129")
     ap.add_argument("--scheme", default="JST")
+    ap.add_argument("--limiter", default="",
+                    help="stage, Roe: MUSCL reconstruction with this limiter (e.g. 'van albada'), CFL 1.2")
     ap.add_argument("--precision", default="fp64")
     ap.add_argument("--viscous", action="store_true",
                     help="stage: the same rotor with VISCOSITY_ACTIVE (Sutherland air, walls unchanged)")
@@ -324,7 +326,10 @@ def main():
         return
 
     ni, nj, nk = (int(v) for v in args.size.split("x"))
-    case = cases.rotor3d(ni, nj, nk, n_iterations=args.warmup + 3 * args.steps + 10, scheme=args.scheme)
+    muscl = bool(args.limiter) and args.scheme.upper() == "ROE"
+    cfl = 1.2 if muscl else 2.0
+    case = cases.rotor3d(ni, nj, nk, n_iterations=args.warmup + 3 * args.steps + 10, scheme=args.scheme,
+                         muscl=muscl, limiter=args.limiter if muscl else "None", cfl=cfl)
     if args.viscous:
         case = cases.viscous(case, walls={})
     overrides = {"GPU_PRECISION": args.precision}
@@ -360,7 +365,7 @@ def main():
     avg_pre_ms = pre_ms / max(launches, 1)
     # the committed counter summary covers the default workload (FP64 JST rotor) only
     default_case = (args.precision == "fp64" and args.size == "193x49x129" and args.scheme == "JST"
-                    and not args.viscous)
+                    and not args.viscous and not muscl)
     traffic = traffic_from_profiles("k_stage") if default_case else None
     out = {
         "metric": "cell-stage updates/s + HBM roofline %; p50/p99 time-to-converge vs plain FP64",
@@ -375,8 +380,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f64-state/f32-flux",
         "data": "synthetic annular rotor grid and blade fields (cturbobfm_amd/cases.py), free-stream start",
-        "config": {"workload": "annular rotor %dx%dx%d nodes, Hall body force, %s, RK41A, CFL 2, local dt, %s%s"
-                               % (ni, nj, nk, args.scheme, args.precision,
+        "config": {"workload": "annular rotor %dx%dx%d nodes, Hall body force, %s%s, RK41A, CFL %g, local dt, %s%s"
+                               % (ni, nj, nk, args.scheme, (" + MUSCL " + args.limiter) if muscl else "", cfl,
+                                  args.precision,
                                   ", laminar viscous terms (Sutherland)" if args.viscous else ""),
                    "nodes": nodes, "stages_per_step": stages, "body_force_nodes": nbf,
                    "stage_tiling": s.tuning().split("->")[-1].strip(),
