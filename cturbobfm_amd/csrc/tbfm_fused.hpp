@@ -69,6 +69,17 @@ using StageST = typename std::conditional<(SCH == 0 && FL == 0 && sizeof(FT) == 
 template <typename FT>
 using RecT = typename std::conditional<(sizeof(FT) == 4), float, double>::type;
 
+// Source term t (0..11) of slab node sl from the pre-stage buffer, in its stored type:
+// FP64, or the float the FP32-source pre-stage stored for the FP32-flux kernels
+// (FT = float; widened where it is used, so that the load need not be waited for early).
+template <typename FT>
+using SrcT = typename std::conditional<(sizeof(FT) == 4), float, double>::type;
+template <typename FT>
+__device__ __forceinline__ SrcT<FT> src_term(const CaseDev& c, int t, int sl) {
+    if constexpr (sizeof(FT) == 4) return gld(reinterpret_cast<const float*>(c.srcbuf) + (long)t * c.ns, (unsigned)sl);
+    else return gld(c.srcbuf + (long)t * c.ns, (unsigned)sl);
+}
+
 template <class TL>
 __device__ __forceinline__ int fiI(int v, int tj, int m) { return (v * TL::TJ + tj) * (TL::TI + 1) + m; }
 template <class TL>
@@ -659,13 +670,12 @@ __device__ __forceinline__ void stage_block(const CaseDev* __restrict__ cp, cons
                 // geometric, Gong, body force, subtracted in that order (the geometric
                 // source's components 0, 1 and 4 are +0: subtracting them changes nothing)
                 const int sl = (k * NJ + j) * (c.bi1 - c.bi0) + (i - c.bi0);
-                const int ns = c.ns;
-                R[2] -= gld(c.srcbuf, (unsigned)sl);
-                R[3] -= gld(c.srcbuf + ns, (unsigned)sl);
+                R[2] -= double(src_term<FT>(c, 0, sl));
+                R[3] -= double(src_term<FT>(c, 1, sl));
 #pragma unroll
                 for (int t = 0; t < 2; ++t)
 #pragma unroll
-                    for (int v = 0; v < 5; ++v) R[v] -= gld(c.srcbuf + (long)(2 + 5 * t + v) * ns, (unsigned)sl);
+                    for (int v = 0; v < 5; ++v) R[v] -= double(src_term<FT>(c, 2 + 5 * t + v, sl));
             }
             // no-slip walls after the periodic closure (solver.cpp:1828-1831): here for
             // the nodes outside the periodic planes, in k_periodic for theirs
@@ -1205,11 +1215,11 @@ __device__ __forceinline__ void stage_block_jst3d(const CaseDev* __restrict__ cp
         // so that they arrive during the barrier; then the loads for the next plane: K
         // record and face slots
         const bool slab = act && srcOn && i >= c.bi0 && i < c.bi1;
-        double S[12];
+        SrcT<FT> S[12];
         if (slab) {
             const int sl = (k * NJ + j) * (c.bi1 - c.bi0) + (i - c.bi0);
 #pragma unroll
-            for (int t = 0; t < 12; ++t) S[t] = gld(c.srcbuf + (long)t * c.ns, (unsigned)sl);
+            for (int t = 0; t < 12; ++t) S[t] = src_term<FT>(c, t, sl);
         }
         {
             const int kn = min(kq, NK - 1);
@@ -1241,12 +1251,12 @@ __device__ __forceinline__ void stage_block_jst3d(const CaseDev* __restrict__ cp
             if (slab) {
                 // geometric (y, z components; the others are +0), Gong, body force,
                 // subtracted in that order
-                R[2] -= S[0];
-                R[3] -= S[1];
+                R[2] -= double(S[0]);
+                R[3] -= double(S[1]);
 #pragma unroll
                 for (int t = 0; t < 2; ++t)
 #pragma unroll
-                    for (int v = 0; v < 5; ++v) R[v] -= S[2 + 5 * t + v];
+                    for (int v = 0; v < 5; ++v) R[v] -= double(S[2 + 5 * t + v]);
             }
             // no-slip walls after the periodic closure (solver.cpp:1828-1831): here for
             // the nodes outside the periodic planes, in k_periodic for theirs

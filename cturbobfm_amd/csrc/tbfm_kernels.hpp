@@ -121,7 +121,8 @@ struct CaseDev {
     const double* prof_sn;
     // body force slab: nodes with i in [bi0, bi1); ns = slab nodes; srcbuf holds the
     // source terms of the current stage, [12][ns]: the geometric source's y and z
-    // components (its other three are +0), the Gong source, the body force
+    // components (its other three are +0), the Gong source, the body force; doubles, or
+    // floats for the FP32-flux kernels (fp32: their sources are FP32 arithmetic)
     int bi0, bi1, ns;
     double* srcbuf;
     const unsigned char* blade2d;      // (j*NI + i): bladePresent(i,j,0) > 0

@@ -208,6 +208,19 @@ __device__ inline void sources_block(const CaseDev& c, const Launch& La, int bx)
     // the geometric source's density, x-momentum and energy components are +0 (its
     // cylindrical source has no axial part): only its y and z components are kept
     const int ns = c.ns;
+    if (c.fp32) {
+        // FP32 flux: the sources as floats (the geometric and body-force ones are FP32
+        // values already, the Gong stencil is rounded)
+        float* sb = reinterpret_cast<float*>(c.srcbuf);
+        sb[s] = float(g[2]);
+        sb[ns + s] = float(g[3]);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+            sb[(2 + v) * ns + s] = float(go[v]);
+            sb[(7 + v) * ns + s] = float(bf[v]);
+        }
+        return;
+    }
     c.srcbuf[s] = g[2];
     c.srcbuf[ns + s] = g[3];
 #pragma unroll
